@@ -1,0 +1,348 @@
+"""MI355X engine for DPSNN's per-millisecond LIFCA simulation step.
+
+The product is the C-ABI library ``lib/libdpsnn_hip.so`` (declared in
+``include/dpsnn_hip.h``); this module is its Python host binding and mirrors
+the reference's process interface (``localNetProcess_init`` / ``_fire`` /
+``_finish``, localNetProcess.c:40-173; fire returns 0 to continue and 1 when
+the programmed time is reached).
+
+Configuration is DPSNN's own: env vars (read from the process environment,
+as getParameters does, DPSNN_getParameters.c:23-831) plus column.txt,
+connectivity.txt, bathEfficacy.txt and optional plasticity.txt in a
+directory.  ``configs/<name>/env.txt`` holds the env vars of each canonical
+workload; ``apply_env`` loads one into ``os.environ``.
+
+Loading fails loudly when the library has not been built: there is no
+fallback path.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "lib", "libdpsnn_hip.so")
+CONFIGS = os.path.join(HERE, "configs")
+
+DRIVE_REPLAY = 0
+DRIVE_PHILOX = 1
+DONE = 1
+
+_lib = None
+
+
+class DpsnnError(RuntimeError):
+    """A dpsnn_* entry point returned a negative status."""
+
+    def __init__(self, func, code, message):
+        super().__init__(f"{func} failed ({code}): {message}")
+        self.code = code
+
+
+class Options(ctypes.Structure):
+    _fields_ = [("device", ctypes.c_int), ("drive", ctypes.c_int),
+                ("philox_seed", ctypes.c_uint64), ("output_dir", ctypes.c_char_p),
+                ("drain_every_ms", ctypes.c_uint32), ("keep_spikes", ctypes.c_int)]
+
+
+_PARAM_U32 = ["globH", "locN", "globN", "D", "totalSimTime_ms", "globCFX", "globCFY", "globCFT",
+              "neuronsPerCM", "subPopNumber", "tabNeuron", "moduloSec", "ratesSampling",
+              "globalSeed", "howManyOutputFiles", "startPlasticity_ms", "stopPlasticity_ms",
+              "startStatFiles_s", "stopStatFiles_s", "startRatesPerPop_s", "stopRatesPerPop_s",
+              "stencilDim", "overallConnectivity", "delayMin", "delayMax"]
+_PARAM_F32 = ["minSynWeight_f", "factorFloat_2_weightType", "factorWeightType_2_Float",
+              "STDPmultiplier", "antiCausalCoeff", "tauSTDP_ms", "causalSTDPmaxAbsValue",
+              "antiCausalSTDPmaxAbsValue", "tauDerivativeDecay_s", "derivativeDrift"]
+
+
+class Parameters(ctypes.Structure):
+    _fields_ = ([(n, ctypes.c_uint32) for n in _PARAM_U32] +
+                [("spikes_writeDisable", ctypes.c_int32),
+                 ("statRatesPerPop_writeDisable", ctypes.c_int32)] +
+                [(n, ctypes.c_float) for n in _PARAM_F32])
+
+    def as_dict(self):
+        return {name: getattr(self, name) for name, _ in self._fields_}
+
+
+class Stats(ctypes.Structure):
+    _fields_ = [("steps", ctypes.c_uint64), ("spikes", ctypes.c_uint64),
+                ("synaptic_events", ctypes.c_uint64), ("external_events", ctypes.c_uint64),
+                ("synapses", ctypes.c_uint64), ("delivery_items", ctypes.c_uint64),
+                ("neurons", ctypes.c_uint32), ("max_in_items", ctypes.c_uint32),
+                ("init_seconds", ctypes.c_double), ("device_bytes", ctypes.c_uint64)]
+
+    def as_dict(self):
+        return {name: getattr(self, name) for name, _ in self._fields_}
+
+
+def _p(t):
+    return ctypes.POINTER(t)
+
+
+_SIGS = {
+    "dpsnn_get_parameters": (ctypes.c_int, [ctypes.c_char_p, _p(Parameters)]),
+    "dpsnn_create": (ctypes.c_int, [ctypes.c_char_p, _p(Options), _p(ctypes.c_void_p)]),
+    "dpsnn_step": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint32]),
+    "dpsnn_sync": (ctypes.c_int, [ctypes.c_void_p]),
+    "dpsnn_now_ms": (ctypes.c_uint32, [ctypes.c_void_p]),
+    "dpsnn_total_ms": (ctypes.c_uint32, [ctypes.c_void_p]),
+    "dpsnn_get_state": (ctypes.c_int, [ctypes.c_void_p] + [ctypes.c_void_p] * 5),
+    "dpsnn_last_spikes": (ctypes.c_int, [ctypes.c_void_p] + [ctypes.c_void_p] * 3),
+    "dpsnn_num_synapses": (ctypes.c_uint64, [ctypes.c_void_p]),
+    "dpsnn_get_synapses": (ctypes.c_int, [ctypes.c_void_p] + [ctypes.c_void_p] * 6),
+    "dpsnn_write_synapse_dump": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_char_p]),
+    "dpsnn_get_stats": (ctypes.c_int, [ctypes.c_void_p, _p(Stats)]),
+    "dpsnn_stream": (ctypes.c_void_p, [ctypes.c_void_p]),
+    "dpsnn_destroy": (None, [ctypes.c_void_p]),
+    "dpsnn_profile": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
+    "dpsnn_kernel_time": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, _p(ctypes.c_double),
+                                         _p(ctypes.c_uint64)]),
+    "dpsnn_host_build": (ctypes.c_int, [ctypes.c_char_p, _p(ctypes.c_void_p)]),
+    "dpsnn_host_num_neurons": (ctypes.c_uint32, [ctypes.c_void_p]),
+    "dpsnn_host_num_synapses": (ctypes.c_uint64, [ctypes.c_void_p]),
+    "dpsnn_host_num_items": (ctypes.c_uint64, [ctypes.c_void_p]),
+    "dpsnn_host_get_synapses": (ctypes.c_int, [ctypes.c_void_p] + [ctypes.c_void_p] * 4),
+    "dpsnn_host_get_inlists": (ctypes.c_int, [ctypes.c_void_p] + [ctypes.c_void_p] * 4),
+    "dpsnn_host_get_external": (ctypes.c_int, [ctypes.c_void_p] + [ctypes.c_void_p] * 2),
+    "dpsnn_host_write_synapse_dump": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_char_p]),
+    "dpsnn_host_external_step": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
+                                                ctypes.c_void_p, ctypes.c_uint64, _p(ctypes.c_uint64)]),
+    "dpsnn_host_destroy": (None, [ctypes.c_void_p]),
+    "dpsnn_device_exp": (ctypes.c_int, [ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p,
+                                        ctypes.c_uint64]),
+    "dpsnn_last_error": (ctypes.c_char_p, []),
+}
+
+
+def library():
+    """The loaded C-ABI library; raises if it has not been built."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"{LIB_PATH} is missing: build it first "
+                               "(python 201812realtimecortsim_amd/build.py)")
+        lib = ctypes.CDLL(LIB_PATH)
+        for name, (res, args) in _SIGS.items():
+            fn = getattr(lib, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = lib
+    return _lib
+
+
+def exported_symbols():
+    return sorted(_SIGS)
+
+
+def _check(func, rc):
+    if rc < 0:
+        raise DpsnnError(func, rc, library().dpsnn_last_error().decode())
+    return rc
+
+
+def _ptr(a):
+    return a.ctypes.data_as(ctypes.c_void_p) if a is not None else None
+
+
+def read_env_file(path):
+    env = {}
+    with open(path) as f:
+        for line in f:
+            line = line.strip()
+            if line and not line.startswith("#") and "=" in line:
+                k, v = line.split("=", 1)
+                env[k.strip()] = v.strip()
+    return env
+
+
+def apply_env(config_dir, **overrides):
+    """Loads <config_dir>/env.txt (and overrides) into os.environ, where the
+    engine's getParameters reads them; returns the applied dict."""
+    env = read_env_file(os.path.join(config_dir, "env.txt"))
+    env.update({k: str(v) for k, v in overrides.items()})
+    os.environ.update(env)
+    return env
+
+
+def config_dir(name):
+    return os.path.join(CONFIGS, name)
+
+
+def get_parameters(inidir):
+    out = Parameters()
+    _check("dpsnn_get_parameters", library().dpsnn_get_parameters(inidir.encode(), ctypes.byref(out)))
+    return out.as_dict()
+
+
+class Network:
+    """One local network on one GPU (a dpsnn_net)."""
+
+    def __init__(self, inidir, device=0, drive=DRIVE_REPLAY, philox_seed=1, output_dir=None,
+                 drain_every_ms=0, keep_spikes=False):
+        self._lib = library()
+        self._out = output_dir.encode() if output_dir else None
+        opt = Options(device, drive, philox_seed, self._out, drain_every_ms, int(keep_spikes))
+        h = ctypes.c_void_p()
+        _check("dpsnn_create", self._lib.dpsnn_create(inidir.encode(), ctypes.byref(opt), ctypes.byref(h)))
+        self._h = h
+        self.n = self.stats()["neurons"]
+
+    def close(self):
+        if getattr(self, "_h", None):
+            self._lib.dpsnn_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        self.close()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def step(self, nsteps=1):
+        """Advances nsteps ms; returns True once the programmed time is reached."""
+        return _check("dpsnn_step", self._lib.dpsnn_step(self._h, nsteps)) == DONE
+
+    def sync(self):
+        _check("dpsnn_sync", self._lib.dpsnn_sync(self._h))
+
+    @property
+    def now_ms(self):
+        return self._lib.dpsnn_now_ms(self._h)
+
+    @property
+    def total_ms(self):
+        return self._lib.dpsnn_total_ms(self._h)
+
+    def state(self):
+        n = self.n
+        v, c, tr, te = (np.empty(n, np.float64) for _ in range(4))
+        fired = np.empty(n, np.uint8)
+        _check("dpsnn_get_state", self._lib.dpsnn_get_state(self._h, _ptr(v), _ptr(c), _ptr(tr), _ptr(te),
+                                                            _ptr(fired)))
+        return {"v": v, "c": c, "Tr": tr, "Te": te, "fired": fired}
+
+    def last_spikes(self):
+        ids = np.empty(self.n, np.uint32)
+        te = np.empty(self.n, np.float64)
+        cnt = ctypes.c_uint32()
+        _check("dpsnn_last_spikes", self._lib.dpsnn_last_spikes(self._h, _ptr(ids), _ptr(te),
+                                                                ctypes.byref(cnt)))
+        return ids[:cnt.value].copy(), te[:cnt.value].copy()
+
+    def synapses(self):
+        s = self._lib.dpsnn_num_synapses(self._h)
+        pre, post = np.empty(s, np.uint32), np.empty(s, np.uint32)
+        delay, w = np.empty(s, np.uint8), np.empty(s, np.int16)
+        td, last = np.empty(s, np.float32), np.empty(s, np.int32)
+        _check("dpsnn_get_synapses", self._lib.dpsnn_get_synapses(self._h, _ptr(pre), _ptr(post), _ptr(delay),
+                                                                  _ptr(w), _ptr(td), _ptr(last)))
+        return {"pre": pre, "post": post, "delay": delay, "weight": w, "tderiv": td, "last": last}
+
+    def write_synapse_dump(self, path):
+        _check("dpsnn_write_synapse_dump", self._lib.dpsnn_write_synapse_dump(self._h, path.encode()))
+
+    def stats(self):
+        out = Stats()
+        _check("dpsnn_get_stats", self._lib.dpsnn_get_stats(self._h, ctypes.byref(out)))
+        return out.as_dict()
+
+    def profile(self, enable=True):
+        _check("dpsnn_profile", self._lib.dpsnn_profile(self._h, int(enable)))
+
+    def kernel_time(self, kernel=0):
+        ms, n = ctypes.c_double(), ctypes.c_uint64()
+        _check("dpsnn_kernel_time", self._lib.dpsnn_kernel_time(self._h, kernel, ctypes.byref(ms),
+                                                                ctypes.byref(n)))
+        return ms.value, n.value
+
+
+class HostConnectome:
+    """The host generator and delivery layout, without a device."""
+
+    def __init__(self, inidir):
+        self._lib = library()
+        h = ctypes.c_void_p()
+        _check("dpsnn_host_build", self._lib.dpsnn_host_build(inidir.encode(), ctypes.byref(h)))
+        self._h = h
+        self.n = self._lib.dpsnn_host_num_neurons(h)
+        self.s = self._lib.dpsnn_host_num_synapses(h)
+        self.items = self._lib.dpsnn_host_num_items(h)
+
+    def close(self):
+        if getattr(self, "_h", None):
+            self._lib.dpsnn_host_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        self.close()
+
+    def synapses(self):
+        s = self.s
+        pre, post = np.empty(s, np.uint32), np.empty(s, np.uint32)
+        delay, w = np.empty(s, np.uint8), np.empty(s, np.int16)
+        _check("dpsnn_host_get_synapses", self._lib.dpsnn_host_get_synapses(self._h, _ptr(pre), _ptr(post),
+                                                                            _ptr(delay), _ptr(w)))
+        return {"pre": pre, "post": post, "delay": delay, "weight": w}
+
+    def inlists(self):
+        off = np.empty(self.n + 1, np.uint64)
+        pd = np.empty(self.items, np.uint32)
+        w = np.empty(self.items, np.int16)
+        syn = np.empty(self.items, np.uint32)
+        _check("dpsnn_host_get_inlists", self._lib.dpsnn_host_get_inlists(self._h, _ptr(off), _ptr(pd),
+                                                                          _ptr(w), _ptr(syn)))
+        return {"off": off, "pre": pd & 0xFFFFFF, "delay": pd >> 24, "weight": w, "syn": syn}
+
+    def external_params(self):
+        inv = np.empty(self.n, np.float64)
+        mx = np.empty(self.n, np.uint32)
+        _check("dpsnn_host_get_external", self._lib.dpsnn_host_get_external(self._h, _ptr(inv), _ptr(mx)))
+        return inv, mx
+
+    def external_step(self, cap=None):
+        cap = cap or self.n * 64
+        off = np.empty(self.n + 1, np.uint32)
+        t, v = np.empty(cap, np.float64), np.empty(cap, np.float64)
+        cnt = ctypes.c_uint64()
+        _check("dpsnn_host_external_step", self._lib.dpsnn_host_external_step(
+            self._h, _ptr(off), _ptr(t), _ptr(v), cap, ctypes.byref(cnt)))
+        return off, t[:cnt.value].copy(), v[:cnt.value].copy()
+
+    def write_synapse_dump(self, path):
+        _check("dpsnn_host_write_synapse_dump", self._lib.dpsnn_host_write_synapse_dump(self._h, path.encode()))
+
+
+def device_exp(x, device=0):
+    x = np.ascontiguousarray(x, np.float64)
+    y = np.empty_like(x)
+    _check("dpsnn_device_exp", library().dpsnn_device_exp(device, _ptr(x), _ptr(y), x.size))
+    return y
+
+
+# ---- the reference's process interface (DAL shape) ------------------------
+class LocalNetProcess:
+    """Python mirror of localNetProcess_{init,fire,finish}
+    (localNetProcess.c:40-173): init builds the network from the env and the
+    ini files in ``inidir``; fire advances one ms and returns 0, or 1 once
+    env_totalSimTime_ms is reached; finish syncs and releases the device."""
+
+    def __init__(self, inidir=".", output_dir=".", device=0, drive=DRIVE_REPLAY):
+        self.inidir, self.output_dir, self.device, self.drive = inidir, output_dir, device, drive
+        self.net = None
+
+    def init(self):
+        self.net = Network(self.inidir, device=self.device, drive=self.drive, output_dir=self.output_dir)
+
+    def fire(self):
+        if self.net.now_ms >= self.net.total_ms:
+            return 1
+        self.net.step(1)
+        return 0
+
+    def finish(self):
+        self.net.sync()
+        self.net.close()

@@ -1,0 +1,995 @@
+// dpsnn_engine.hip -- MI355X engine for DPSNN's per-millisecond LIFCA step.
+//
+// One network = one process's whole local network on one GPU.  Per step t
+// (one reference completeTimeStep_ms, DPSNN_localNet_sim.c:22-307):
+//
+//   k_scan_fired   fired flags of step t-1's dynamics -> spike bitmap of
+//                  slot t, emission times into the slot's Te history, flags
+//                  cleared, per-(column, subPop) counts.  Replaces the fired
+//                  scan + compaction + world-size-1 axonal exchange + ring
+//                  insert (:103-148, :620-806, DPSNN_spike.c:209-239).
+//   k_compact      (only when a raster is wanted) ascending spike list of
+//                  slot t appended to the device raster buffer.
+//   k_integrate    one lane per target neuron: walks the neuron's in-list
+//                  (delivery insertion order), keeps the items whose source
+//                  fired in step t-delay, orders them by arrival time with
+//                  a stable per-lane bucket sort, merges the external bath
+//                  events (internal first on ties) and runs LIFCADynamic
+//                  per event in FP64.  Replaces delivery + addInputSpike +
+//                  regularDynamicTimeStep + LIFCADynamic (:837-999,
+//                  DPSNN_neuron_sim.c:289-300, :522-824).
+//
+// The device holds no per-step event queue: a target pulls its live events
+// from its static in-list and the spike history (bitmap + Te per slot), so
+// the only per-step traffic is the in-list stream, the history lookups and
+// the neuron state.  Built with -ffp-contract=off: the reference's FP64
+// update is second order and unfused, and must round where it rounds.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "dpsnn_exp.h"
+#include "dpsnn_hip.h"
+#include "dpsnn_host.h"
+
+namespace dpsnn {
+
+constexpr int kBlock = 256;      // integrate / scan block (4 waves)
+constexpr int kBuckets = 64;     // arrival-time buckets per lane
+constexpr uint32_t kPad = 0xFFFFFFFFu;
+constexpr int kExtRing = 8;      // replayed external steps in flight
+
+static thread_local std::string g_err;
+static int fail(int code, const char *fmt, ...) {
+  char buf[1024];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof buf, fmt, ap);
+  va_end(ap);
+  g_err = buf;
+  return code;
+}
+#define HIP_TRY(expr)                                                              \
+  do {                                                                             \
+    hipError_t e_ = (expr);                                                        \
+    if (e_ != hipSuccess)                                                          \
+      return fail(DPSNN_ERR_DEVICE, "%s failed: %s", #expr, hipGetErrorString(e_)); \
+  } while (0)
+
+// Per-sub-population LIFCA parameters, in this order.
+enum { kTau, kTheta, kH, kTarp, kAlphaC, kTauC, kGC, kParams = 8 };
+
+struct DevNet {
+  uint32_t N, NW, RS, nsub, npc, ncols, cap, drive;
+  float w2f;
+  uint64_t philoxSeed;
+  const uint8_t *subPop;
+  const double *sp;        // [nsub][kParams]
+  const double *lut;       // [nsub][256]
+  const double *invNuExt;
+  const uint32_t *maxExt;
+  double *v, *c, *Tr, *Te, *assigned;
+  uint64_t *philoxCtr;
+  uint8_t *fired;
+  uint64_t *bits;          // [RS][NW]
+  double *teHist;          // [RS][N]
+  const uint64_t *grpOff;  // [G+1] item offset of each 64-neuron group
+  const uint32_t *grpRows; // [G]
+  const uint32_t *inPD;    // pre | delay << 24, kPad = none
+  const int16_t *inW;
+  double *Ak, *Bk;         // per-lane scratch [N][cap]
+  float *Aw, *Bw;
+  unsigned long long *counters;  // [0] events, [1] external, [2] spikes
+};
+
+// --------------------------------------------------------------------------
+// Philox4x32-10 (Salmon et al., SC'11): counter-based stream for the
+// on-device external drive.  Key = (seed, neuron), counter = call index.
+// --------------------------------------------------------------------------
+__device__ inline void philox4x32(uint32_t c[4], uint32_t k0, uint32_t k1) {
+  for (int r = 0; r < 10; ++r) {
+    const uint64_t p0 = (uint64_t)0xD2511F53u * c[0], p1 = (uint64_t)0xCD9E8D57u * c[2];
+    const uint32_t h0 = (uint32_t)(p0 >> 32), l0 = (uint32_t)p0;
+    const uint32_t h1 = (uint32_t)(p1 >> 32), l1 = (uint32_t)p1;
+    c[0] = h1 ^ c[1] ^ k0;
+    c[1] = l1;
+    c[2] = h0 ^ c[3] ^ k1;
+    c[3] = l0;
+    k0 += 0x9E3779B9u;
+    k1 += 0xBB67AE85u;
+  }
+}
+
+struct PhiloxStream {
+  uint32_t k0, k1;
+  uint64_t ctr;
+  double buf[2];
+  int have;
+  __device__ double next() {
+    if (have == 0) {
+      uint32_t c[4] = {(uint32_t)ctr, (uint32_t)(ctr >> 32), 0u, 0x44505300u};
+      philox4x32(c, k0, k1);
+      ++ctr;
+      const uint64_t a = ((uint64_t)c[0] << 21) ^ (uint64_t)c[1];
+      const uint64_t b = ((uint64_t)c[2] << 21) ^ (uint64_t)c[3];
+      buf[0] = (double)(a & ((1ull << 53) - 1)) * 0x1p-53;
+      buf[1] = (double)(b & ((1ull << 53) - 1)) * 0x1p-53;
+      have = 2;
+    }
+    return buf[--have];
+  }
+};
+
+// --------------------------------------------------------------------------
+// LIFCADynamic, DPSNN_neuron_sim.c:664-824 (second-order, non-Perseo path).
+// --------------------------------------------------------------------------
+struct Lifca {
+  double v, c, Tr, Te;
+  double Tau, Theta, H, Tarp, AlphaC, TauC, gC;
+  bool fired;
+  __device__ inline void event(double t, double I) {
+    double deltaT = t - Tr;
+    const double TFLES = t - Te;
+    if (TFLES > Tarp) {
+      const double TrTe = Tr - Te;
+      if (TrTe < Tarp) {
+        c *= dpsnn_exp(-(deltaT - TFLES + Tarp) / TauC);
+        deltaT = TFLES - Tarp;
+      }
+      const double dtHalf = deltaT / 2;
+      const double rmHalf = dtHalf / Tau;
+      const double rc = deltaT / TauC;
+      v = v - v * rmHalf - gC * c * dtHalf;
+      v = v - v * rmHalf - gC * c * dtHalf;
+      c = c * (1 - rc + rc * rc / 2);
+      v += I;
+      if (v >= Theta) {
+        v = H;
+        c += AlphaC;
+        Te = t;
+        fired = true;
+      }
+    } else {
+      const double rc = deltaT / TauC;
+      c = c * (1 - rc + rc * rc / 2);
+    }
+    Tr = t;
+  }
+};
+
+// --------------------------------------------------------------------------
+// k_scan_fired: one thread per neuron, one 64-bit bitmap word per wave.
+// --------------------------------------------------------------------------
+__global__ __launch_bounds__(kBlock) void k_scan_fired(DevNet P, uint32_t t, uint32_t *popCnt) {
+  const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+  const bool live = i < P.N;
+  const bool f = live && P.fired[i];
+  const uint64_t mask = __ballot(f);
+  const uint32_t slot = t % P.RS;
+  if ((threadIdx.x & 63) == 0 && (i >> 6) < P.NW) {
+    P.bits[(size_t)slot * P.NW + (i >> 6)] = mask;
+    if (mask) atomicAdd(&P.counters[2], (unsigned long long)__popcll(mask));
+  }
+  if (f) {
+    P.teHist[(size_t)slot * P.N + i] = P.Te[i];
+    P.fired[i] = 0;
+    if (popCnt) atomicAdd(&popCnt[(i / P.npc) * P.nsub + P.subPop[i]], 1u);
+  }
+}
+
+// --------------------------------------------------------------------------
+// k_compact: single workgroup; ascending ids of slot t's bitmap with their
+// emission times, appended at *rasterCount.
+// --------------------------------------------------------------------------
+constexpr int kCompactThreads = 1024;
+__global__ __launch_bounds__(kCompactThreads) void k_compact(DevNet P, uint32_t t, uint32_t *ids,
+                                                             double *te, uint32_t *rasterCount,
+                                                             uint32_t *stepCount) {
+  __shared__ uint32_t part[kCompactThreads];
+  const uint32_t slot = t % P.RS;
+  const uint64_t *w = P.bits + (size_t)slot * P.NW;
+  const uint32_t per = (P.NW + kCompactThreads - 1) / kCompactThreads;
+  const uint32_t w0 = threadIdx.x * per, w1 = min(P.NW, w0 + per);
+  uint32_t n = 0;
+  for (uint32_t k = w0; k < w1; ++k) n += __popcll(w[k]);
+  part[threadIdx.x] = n;
+  __syncthreads();
+  for (int off = 1; off < kCompactThreads; off <<= 1) {  // inclusive scan
+    const uint32_t add = threadIdx.x >= (uint32_t)off ? part[threadIdx.x - off] : 0;
+    __syncthreads();
+    part[threadIdx.x] += add;
+    __syncthreads();
+  }
+  const uint32_t base = *rasterCount;
+  uint32_t pos = base + part[threadIdx.x] - n;
+  for (uint32_t k = w0; k < w1; ++k) {
+    uint64_t m = w[k];
+    while (m) {
+      const int b = __ffsll((long long)m) - 1;
+      m &= m - 1;
+      const uint32_t id = k * 64 + (uint32_t)b;
+      ids[pos] = id;
+      te[pos] = P.teHist[(size_t)slot * P.N + id];
+      ++pos;
+    }
+  }
+  __syncthreads();
+  if (threadIdx.x == kCompactThreads - 1) {
+    *rasterCount = base + part[kCompactThreads - 1];
+    *stepCount = part[kCompactThreads - 1];
+  }
+}
+
+// --------------------------------------------------------------------------
+// k_integrate: one lane per target neuron.
+// --------------------------------------------------------------------------
+struct ExtView {  // replayed external events of this step
+  const uint32_t *off;
+  const double *time, *value;
+};
+
+__global__ __launch_bounds__(kBlock) void k_integrate(DevNet P, uint32_t t, ExtView X) {
+  __shared__ uint16_t bucket[kBuckets * kBlock];
+  const uint32_t j = blockIdx.x * kBlock + threadIdx.x;
+  if (j >= P.N) return;
+  const double tF = (double)t, nextF = (double)(tF + 1.0);
+  const uint32_t tid = threadIdx.x;
+  for (int b = 0; b < kBuckets; ++b) bucket[b * kBlock + tid] = 0;
+
+  // gather: live items of this lane's in-list, in insertion order
+  const uint32_t g = j >> 6, l = j & 63;
+  const uint64_t base = P.grpOff[g];
+  const uint32_t rows = P.grpRows[g];
+  double *Ak = P.Ak + (size_t)j * P.cap;
+  float *Aw = P.Aw + (size_t)j * P.cap;
+  uint32_t n = 0;
+  for (uint32_t r = 0; r < rows; ++r) {
+    const uint64_t at = base + (uint64_t)r * 64 + l;
+    const uint32_t it = P.inPD[at];
+    if (it == kPad) break;
+    const uint32_t d = it >> 24, pre = it & 0xFFFFFFu;
+    if (d > t) continue;  // that step has not happened
+    const uint32_t s = (t - d) % P.RS;
+    const uint64_t word = P.bits[(size_t)s * P.NW + (pre >> 6)];
+    if ((word >> (pre & 63)) & 1ull) {
+      const double te = P.teHist[(size_t)s * P.N + pre];
+      const double frac = te - trunc(te);  // modf, exact
+      const double k = tF + frac;
+      Ak[n] = k;
+      Aw[n] = ((float)P.inW[at]) * P.w2f;
+      int b = (int)((k - tF) * (double)kBuckets);
+      b = b < 0 ? 0 : (b >= kBuckets ? kBuckets - 1 : b);
+      bucket[b * kBlock + tid]++;
+      ++n;
+    }
+  }
+  // stable bucket sort A -> B, then stable insertion sort inside buckets
+  double *Bk = P.Bk + (size_t)j * P.cap;
+  float *Bw = P.Bw + (size_t)j * P.cap;
+  {
+    uint32_t run = 0;
+    for (int b = 0; b < kBuckets; ++b) {
+      const uint32_t c = bucket[b * kBlock + tid];
+      bucket[b * kBlock + tid] = (uint16_t)run;
+      run += c;
+    }
+  }
+  for (uint32_t e = 0; e < n; ++e) {
+    const double k = Ak[e];
+    int b = (int)((k - tF) * (double)kBuckets);
+    b = b < 0 ? 0 : (b >= kBuckets ? kBuckets - 1 : b);
+    const uint32_t pos = bucket[b * kBlock + tid]++;
+    Bk[pos] = k;
+    Bw[pos] = Aw[e];
+  }
+  {
+    uint32_t lo = 0;
+    for (int b = 0; b < kBuckets; ++b) {
+      const uint32_t hi = bucket[b * kBlock + tid];
+      for (uint32_t e = lo + 1; e < hi; ++e) {
+        const double k = Bk[e];
+        const float w = Bw[e];
+        uint32_t q = e;
+        while (q > lo && Bk[q - 1] > k) {
+          Bk[q] = Bk[q - 1];
+          Bw[q] = Bw[q - 1];
+          --q;
+        }
+        Bk[q] = k;
+        Bw[q] = w;
+      }
+      lo = hi;
+    }
+  }
+
+  // merge with the external bath and integrate
+  const uint32_t sp = P.subPop[j];
+  const double *prm = P.sp + (size_t)sp * kParams;
+  Lifca L;
+  L.v = P.v[j]; L.c = P.c[j]; L.Tr = P.Tr[j]; L.Te = P.Te[j];
+  L.Tau = prm[kTau]; L.Theta = prm[kTheta]; L.H = prm[kH]; L.Tarp = prm[kTarp];
+  L.AlphaC = prm[kAlphaC]; L.TauC = prm[kTauC]; L.gC = prm[kGC];
+  L.fired = false;
+  uint32_t nExt = 0;
+  if (P.drive == DPSNN_DRIVE_REPLAY) {
+    uint32_t e = X.off[j];
+    const uint32_t ee = X.off[j + 1];
+    nExt = ee - e;
+    uint32_t ii = 0;
+    while (ii < n || e < ee) {
+      if (ii < n && (e >= ee || Bk[ii] <= X.time[e])) {
+        L.event(Bk[ii], (double)Bw[ii]);
+        ++ii;
+      } else {
+        L.event(X.time[e], X.value[e]);
+        ++e;
+      }
+    }
+  } else {
+    const double inv = P.invNuExt[j];
+    const uint32_t cap = P.maxExt[j];
+    const double *lut = P.lut + (size_t)sp * kLutDepth;
+    double a = P.assigned[j];
+    PhiloxStream R{(uint32_t)P.philoxSeed, (uint32_t)(P.philoxSeed >> 32) ^ j, P.philoxCtr[j], {0, 0}, 0};
+    const bool on = inv > 0;
+    uint32_t ii = 0;
+    for (;;) {
+      const bool hasE = on && a < nextF && nExt < cap;
+      if (ii < n && (!hasE || Bk[ii] <= a)) {
+        L.event(Bk[ii], (double)Bw[ii]);
+        ++ii;
+      } else if (hasE) {
+        const double val = lut[(uint32_t)(R.next() * kLutDepth)];
+        L.event(a, val);
+        ++nExt;
+        double u = 1.0 - R.next();
+        a += -logf((float)u) * inv;
+      } else {
+        break;
+      }
+    }
+    if (on && nExt >= cap && a < nextF) {
+      do {
+        double u = 1.0 - R.next();
+        a += -logf((float)u) * inv;
+      } while (a < nextF);
+    }
+    P.assigned[j] = a;
+    P.philoxCtr[j] = R.ctr;
+  }
+  P.v[j] = L.v; P.c[j] = L.c; P.Tr[j] = L.Tr; P.Te[j] = L.Te;
+  P.fired[j] = L.fired ? 1 : 0;
+  // wave-aggregated counters
+  unsigned long long ev = n, ex = nExt;
+  for (int o = 32; o > 0; o >>= 1) {
+    ev += __shfl_down(ev, o);
+    ex += __shfl_down(ex, o);
+  }
+  if ((threadIdx.x & 63) == 0) {
+    atomicAdd(&P.counters[0], ev);
+    atomicAdd(&P.counters[1], ex);
+  }
+}
+
+}  // namespace dpsnn
+
+// ============================================================================
+// Host engine
+// ============================================================================
+using namespace dpsnn;
+
+struct ExtSlot {
+  uint32_t *h_off = nullptr;
+  double *h_time = nullptr, *h_value = nullptr;
+  size_t capEv = 0;
+  uint32_t *d_off = nullptr;
+  double *d_time = nullptr, *d_value = nullptr;
+  hipEvent_t done = nullptr;
+  bool pending = false;
+};
+
+struct dpsnn_net {
+  Params par;
+  Connectome con;
+  Ran3 rng;
+  ExternalReplay replay;
+  ExternalStep extStep;
+  StatWriter writer;
+  bool writing = false;
+  dpsnn_options opt{};
+  hipStream_t stream = nullptr;
+  DevNet dev{};
+  std::vector<void *> allocs;
+  uint64_t devBytes = 0;
+  // history
+  uint32_t t = 0;
+  // external ring
+  ExtSlot ext[kExtRing];
+  uint32_t extNext = 0;
+  // raster drain
+  uint32_t *d_rasterIds = nullptr;
+  double *d_rasterTe = nullptr;
+  uint32_t *d_rasterCount = nullptr;  // [1]
+  uint32_t *d_stepCount = nullptr;    // [drain]
+  uint32_t *d_popCnt = nullptr;       // [drain][ncols*nsub]
+  uint32_t rasterCap = 0, drainEvery = 100, pendingSteps = 0, pendingFrom = 0;
+  bool wantRaster = false;
+  std::vector<uint32_t> h_ids, h_steps, h_pop;
+  std::vector<double> h_te;
+  // last spikes
+  std::vector<uint32_t> lastIds;
+  std::vector<double> lastTe;
+  uint32_t lastCount = 0;
+  double initSeconds = 0;
+  uint64_t items = 0;
+  // profiling
+  bool profile = false;
+  hipEvent_t pe0 = nullptr, pe1 = nullptr;
+  double profMs = 0;
+  uint64_t profN = 0;
+};
+
+template <class T>
+static int dev_alloc(dpsnn_net *n, T **p, size_t count) {
+  const size_t bytes = std::max<size_t>(count, 1) * sizeof(T);
+  hipError_t e = hipMalloc((void **)p, bytes);
+  if (e != hipSuccess)
+    return fail(DPSNN_ERR_MEMORY, "hipMalloc of %zu bytes failed: %s", bytes, hipGetErrorString(e));
+  n->allocs.push_back(*p);
+  n->devBytes += bytes;
+  return 0;
+}
+template <class T>
+static int upload(dpsnn_net *n, T *d, const T *h, size_t count) {
+  HIP_TRY(hipMemcpyAsync(d, h, count * sizeof(T), hipMemcpyHostToDevice, n->stream));
+  return 0;
+}
+
+static void free_all(dpsnn_net *n) {
+  if (n->stream) hipStreamSynchronize(n->stream);
+  for (void *p : n->allocs) hipFree(p);
+  n->allocs.clear();
+  for (auto &s : n->ext) {
+    if (s.h_off) hipHostFree(s.h_off);
+    if (s.h_time) hipHostFree(s.h_time);
+    if (s.h_value) hipHostFree(s.h_value);
+    if (s.d_off) hipFree(s.d_off);
+    if (s.d_time) hipFree(s.d_time);
+    if (s.d_value) hipFree(s.d_value);
+    if (s.done) hipEventDestroy(s.done);
+    s = ExtSlot{};
+  }
+  if (n->pe0) hipEventDestroy(n->pe0);
+  if (n->pe1) hipEventDestroy(n->pe1);
+  if (n->stream) hipStreamDestroy(n->stream);
+  n->stream = nullptr;
+}
+
+static int build_device(dpsnn_net *n) {
+  Connectome &c = n->con;
+  const dpsnn_parameters &q = n->par.pub;
+  DevNet &P = n->dev;
+  P.N = c.N;
+  P.NW = (c.N + 63) / 64;
+  P.RS = c.max_delay + 2;  // live slots t-maxDelay..t, plus one spare
+  P.nsub = q.subPopNumber;
+  P.npc = q.neuronsPerCM;
+  P.ncols = q.globCFT;
+  P.cap = std::max<uint32_t>(c.max_in, 1);
+  P.drive = (uint32_t)n->opt.drive;
+  P.w2f = q.factorWeightType_2_Float;
+  P.philoxSeed = n->opt.philox_seed;
+  int rc;
+  const uint32_t N = c.N;
+  // per-subPop parameters and LUT
+  std::vector<double> sp(size_t(P.nsub) * kParams, 0.0);
+  for (uint32_t s = 0; s < P.nsub; ++s) {
+    const SubPop &x = n->par.sp[s];
+    double *o = &sp[size_t(s) * kParams];
+    o[kTau] = x.Tau; o[kTheta] = x.Theta; o[kH] = x.H; o[kTarp] = x.Tarp;
+    o[kAlphaC] = x.AlphaC; o[kTauC] = x.TauC; o[kGC] = x.gC;
+  }
+  double *d_sp, *d_lut, *d_inv;
+  uint8_t *d_sub;
+  uint32_t *d_maxExt;
+  if ((rc = dev_alloc(n, &d_sp, sp.size())) || (rc = upload(n, d_sp, sp.data(), sp.size()))) return rc;
+  if ((rc = dev_alloc(n, &d_lut, c.bathLut.size())) || (rc = upload(n, d_lut, c.bathLut.data(), c.bathLut.size()))) return rc;
+  if ((rc = dev_alloc(n, &d_inv, N)) || (rc = upload(n, d_inv, c.invNuExt.data(), N))) return rc;
+  if ((rc = dev_alloc(n, &d_sub, N)) || (rc = upload(n, d_sub, c.subPop.data(), N))) return rc;
+  if ((rc = dev_alloc(n, &d_maxExt, N)) || (rc = upload(n, d_maxExt, c.maxExt.data(), N))) return rc;
+  P.sp = d_sp; P.lut = d_lut; P.invNuExt = d_inv; P.subPop = d_sub; P.maxExt = d_maxExt;
+  // neuron state, initNeuralKind DPSNN_neuron_init.c:99-115
+  std::vector<double> v(N), zeros(N, 0.0), te(N, -1000.0);
+  for (uint32_t i = 0; i < N; ++i) v[i] = n->par.sp[c.subPop[i]].H;
+  if ((rc = dev_alloc(n, &P.v, N)) || (rc = upload(n, P.v, v.data(), N))) return rc;
+  if ((rc = dev_alloc(n, &P.c, N)) || (rc = upload(n, P.c, zeros.data(), N))) return rc;
+  if ((rc = dev_alloc(n, &P.Tr, N)) || (rc = upload(n, P.Tr, zeros.data(), N))) return rc;
+  if ((rc = dev_alloc(n, &P.Te, N)) || (rc = upload(n, P.Te, te.data(), N))) return rc;
+  if ((rc = dev_alloc(n, &P.assigned, N)) || (rc = upload(n, P.assigned, zeros.data(), N))) return rc;
+  if ((rc = dev_alloc(n, &P.philoxCtr, N))) return rc;
+  HIP_TRY(hipMemsetAsync(P.philoxCtr, 0, sizeof(uint64_t) * N, n->stream));
+  if ((rc = dev_alloc(n, &P.fired, N))) return rc;
+  HIP_TRY(hipMemsetAsync(P.fired, 0, N, n->stream));
+  // history
+  if ((rc = dev_alloc(n, &P.bits, size_t(P.RS) * P.NW))) return rc;
+  HIP_TRY(hipMemsetAsync(P.bits, 0, sizeof(uint64_t) * size_t(P.RS) * P.NW, n->stream));
+  if ((rc = dev_alloc(n, &P.teHist, size_t(P.RS) * N))) return rc;
+  HIP_TRY(hipMemsetAsync(P.teHist, 0, sizeof(double) * size_t(P.RS) * N, n->stream));
+  // in-lists in 64-neuron interleaved groups
+  const uint32_t G = (N + 63) / 64;
+  std::vector<uint64_t> grpOff(G + 1, 0);
+  std::vector<uint32_t> grpRows(G, 0);
+  for (uint32_t g = 0; g < G; ++g) {
+    uint32_t rows = 0;
+    for (uint32_t l = 0; l < 64 && g * 64 + l < N; ++l) {
+      const uint32_t j = g * 64 + l;
+      rows = std::max<uint32_t>(rows, (uint32_t)(c.in_off[j + 1] - c.in_off[j]));
+    }
+    grpRows[g] = rows;
+    grpOff[g + 1] = grpOff[g] + uint64_t(rows) * 64;
+  }
+  const uint64_t items = grpOff[G];
+  n->items = c.in_off[N];
+  {
+    std::vector<uint32_t> pd(items, kPad);
+    std::vector<int16_t> w(items, 0);
+    for (uint32_t j = 0; j < N; ++j) {
+      const uint32_t g = j >> 6, l = j & 63;
+      for (uint64_t k = c.in_off[j]; k < c.in_off[j + 1]; ++k) {
+        const uint64_t at = grpOff[g] + (k - c.in_off[j]) * 64 + l;
+        pd[at] = c.in_pre_d[k];
+        w[at] = c.in_w[k];
+      }
+    }
+    uint64_t *d_go;
+    uint32_t *d_gr, *d_pd;
+    int16_t *d_w;
+    if ((rc = dev_alloc(n, &d_go, G + 1)) || (rc = upload(n, d_go, grpOff.data(), G + 1))) return rc;
+    if ((rc = dev_alloc(n, &d_gr, G)) || (rc = upload(n, d_gr, grpRows.data(), G))) return rc;
+    if ((rc = dev_alloc(n, &d_pd, items)) || (rc = upload(n, d_pd, pd.data(), items))) return rc;
+    if ((rc = dev_alloc(n, &d_w, items)) || (rc = upload(n, d_w, w.data(), items))) return rc;
+    P.grpOff = d_go; P.grpRows = d_gr; P.inPD = d_pd; P.inW = d_w;
+    HIP_TRY(hipStreamSynchronize(n->stream));  // host staging freed below
+  }
+  // per-lane scratch
+  if ((rc = dev_alloc(n, &P.Ak, size_t(N) * P.cap)) || (rc = dev_alloc(n, &P.Bk, size_t(N) * P.cap)) ||
+      (rc = dev_alloc(n, &P.Aw, size_t(N) * P.cap)) || (rc = dev_alloc(n, &P.Bw, size_t(N) * P.cap)))
+    return rc;
+  if ((rc = dev_alloc(n, &P.counters, 4))) return rc;
+  HIP_TRY(hipMemsetAsync(P.counters, 0, 4 * sizeof(unsigned long long), n->stream));
+  // raster / rates drain buffers
+  n->drainEvery = n->opt.drain_every_ms ? n->opt.drain_every_ms : 100;
+  const size_t popPer = size_t(P.ncols) * P.nsub;
+  if ((rc = dev_alloc(n, &n->d_popCnt, popPer * n->drainEvery))) return rc;
+  HIP_TRY(hipMemsetAsync(n->d_popCnt, 0, sizeof(uint32_t) * popPer * n->drainEvery, n->stream));
+  n->wantRaster = (n->writing && !q.spikes_writeDisable) || n->opt.keep_spikes;
+  if (n->wantRaster) {
+    n->rasterCap = N * std::min<uint32_t>(n->drainEvery, 1000) + 1;
+    if ((rc = dev_alloc(n, &n->d_rasterIds, n->rasterCap)) || (rc = dev_alloc(n, &n->d_rasterTe, n->rasterCap)) ||
+        (rc = dev_alloc(n, &n->d_rasterCount, 1)) || (rc = dev_alloc(n, &n->d_stepCount, n->drainEvery)))
+      return rc;
+    HIP_TRY(hipMemsetAsync(n->d_rasterCount, 0, sizeof(uint32_t), n->stream));
+  }
+  // external replay slots
+  if (P.drive == DPSNN_DRIVE_REPLAY) {
+    for (auto &s : n->ext) {
+      HIP_TRY(hipHostMalloc((void **)&s.h_off, sizeof(uint32_t) * (N + 1), 0));
+      HIP_TRY(hipMalloc((void **)&s.d_off, sizeof(uint32_t) * (N + 1)));
+      HIP_TRY(hipEventCreateWithFlags(&s.done, hipEventDisableTiming));
+    }
+  }
+  HIP_TRY(hipStreamSynchronize(n->stream));
+  return 0;
+}
+
+static int ensure_ext_cap(ExtSlot &s, size_t need) {
+  if (need <= s.capEv) return 0;
+  const size_t cap = need + need / 2 + 1024;
+  if (s.h_time) hipHostFree(s.h_time);
+  if (s.h_value) hipHostFree(s.h_value);
+  if (s.d_time) hipFree(s.d_time);
+  if (s.d_value) hipFree(s.d_value);
+  HIP_TRY(hipHostMalloc((void **)&s.h_time, sizeof(double) * cap, 0));
+  HIP_TRY(hipHostMalloc((void **)&s.h_value, sizeof(double) * cap, 0));
+  HIP_TRY(hipMalloc((void **)&s.d_time, sizeof(double) * cap));
+  HIP_TRY(hipMalloc((void **)&s.d_value, sizeof(double) * cap));
+  s.capEv = cap;
+  return 0;
+}
+
+// Copies the pending raster / rates out of device memory and writes files.
+static int drain(dpsnn_net *n) {
+  if (n->pendingSteps == 0) return 0;
+  const DevNet &P = n->dev;
+  const size_t popPer = size_t(P.ncols) * P.nsub;
+  const uint32_t steps = n->pendingSteps;
+  n->h_pop.resize(popPer * steps);
+  HIP_TRY(hipMemcpyAsync(n->h_pop.data(), n->d_popCnt, sizeof(uint32_t) * popPer * steps,
+                         hipMemcpyDeviceToHost, n->stream));
+  uint32_t total = 0;
+  if (n->wantRaster) {
+    n->h_steps.resize(steps);
+    HIP_TRY(hipMemcpyAsync(n->h_steps.data(), n->d_stepCount, sizeof(uint32_t) * steps,
+                           hipMemcpyDeviceToHost, n->stream));
+    HIP_TRY(hipMemcpyAsync(&total, n->d_rasterCount, sizeof(uint32_t), hipMemcpyDeviceToHost, n->stream));
+    HIP_TRY(hipStreamSynchronize(n->stream));
+    n->h_ids.resize(total);
+    n->h_te.resize(total);
+    if (total) {
+      HIP_TRY(hipMemcpyAsync(n->h_ids.data(), n->d_rasterIds, sizeof(uint32_t) * total,
+                             hipMemcpyDeviceToHost, n->stream));
+      HIP_TRY(hipMemcpyAsync(n->h_te.data(), n->d_rasterTe, sizeof(double) * total,
+                             hipMemcpyDeviceToHost, n->stream));
+    }
+    HIP_TRY(hipMemsetAsync(n->d_rasterCount, 0, sizeof(uint32_t), n->stream));
+  }
+  HIP_TRY(hipMemsetAsync(n->d_popCnt, 0, sizeof(uint32_t) * popPer * steps, n->stream));
+  HIP_TRY(hipStreamSynchronize(n->stream));
+  uint32_t pos = 0;
+  for (uint32_t k = 0; k < steps; ++k) {
+    const uint32_t cnt = n->wantRaster ? n->h_steps[k] : 0;
+    if (n->writing &&
+        !n->writer.step(n->pendingFrom + k, n->h_ids.data() + pos, n->h_te.data() + pos, cnt,
+                        n->h_pop.data() + popPer * k))
+      return fail(DPSNN_ERR_IO, "writing output files in %s failed", n->opt.output_dir);
+    if (k == steps - 1 && n->opt.keep_spikes) {
+      n->lastIds.assign(n->h_ids.begin() + pos, n->h_ids.begin() + pos + cnt);
+      n->lastTe.assign(n->h_te.begin() + pos, n->h_te.begin() + pos + cnt);
+      n->lastCount = cnt;
+    }
+    pos += cnt;
+  }
+  n->pendingSteps = 0;
+  return 0;
+}
+
+static int one_step(dpsnn_net *n) {
+  DevNet &P = n->dev;
+  const uint32_t t = n->t;
+  const uint32_t blocks = (P.N + kBlock - 1) / kBlock;
+  const size_t popPer = size_t(P.ncols) * P.nsub;
+  if (n->pendingSteps == 0) n->pendingFrom = t;
+  uint32_t *pop = n->d_popCnt + popPer * n->pendingSteps;
+  hipLaunchKernelGGL(k_scan_fired, dim3(blocks), dim3(kBlock), 0, n->stream, P, t, pop);
+  if (n->wantRaster)
+    hipLaunchKernelGGL(k_compact, dim3(1), dim3(kCompactThreads), 0, n->stream, P, t, n->d_rasterIds,
+                       n->d_rasterTe, n->d_rasterCount, n->d_stepCount + n->pendingSteps);
+  ExtView X{nullptr, nullptr, nullptr};
+  if (P.drive == DPSNN_DRIVE_REPLAY) {
+    ExtSlot &s = n->ext[n->extNext];
+    n->extNext = (n->extNext + 1) % kExtRing;
+    if (s.pending) HIP_TRY(hipEventSynchronize(s.done));
+    n->replay.generate(t, n->extStep);
+    const size_t ne = n->extStep.time.size();
+    int rc = ensure_ext_cap(s, ne);
+    if (rc) return rc;
+    std::memcpy(s.h_off, n->extStep.off.data(), sizeof(uint32_t) * (P.N + 1));
+    if (ne) {
+      std::memcpy(s.h_time, n->extStep.time.data(), sizeof(double) * ne);
+      std::memcpy(s.h_value, n->extStep.value.data(), sizeof(double) * ne);
+    }
+    HIP_TRY(hipMemcpyAsync(s.d_off, s.h_off, sizeof(uint32_t) * (P.N + 1), hipMemcpyHostToDevice, n->stream));
+    if (ne) {
+      HIP_TRY(hipMemcpyAsync(s.d_time, s.h_time, sizeof(double) * ne, hipMemcpyHostToDevice, n->stream));
+      HIP_TRY(hipMemcpyAsync(s.d_value, s.h_value, sizeof(double) * ne, hipMemcpyHostToDevice, n->stream));
+    }
+    X = ExtView{s.d_off, s.d_time, s.d_value};
+    if (n->profile) HIP_TRY(hipEventRecord(n->pe0, n->stream));
+    hipLaunchKernelGGL(k_integrate, dim3(blocks), dim3(kBlock), 0, n->stream, P, t, X);
+    if (n->profile) HIP_TRY(hipEventRecord(n->pe1, n->stream));
+    HIP_TRY(hipEventRecord(s.done, n->stream));
+    s.pending = true;
+  } else {
+    if (n->profile) HIP_TRY(hipEventRecord(n->pe0, n->stream));
+    hipLaunchKernelGGL(k_integrate, dim3(blocks), dim3(kBlock), 0, n->stream, P, t, X);
+    if (n->profile) HIP_TRY(hipEventRecord(n->pe1, n->stream));
+  }
+  HIP_TRY(hipGetLastError());
+  if (n->profile) {
+    HIP_TRY(hipEventSynchronize(n->pe1));
+    float ms = 0;
+    HIP_TRY(hipEventElapsedTime(&ms, n->pe0, n->pe1));
+    n->profMs += ms;
+    n->profN++;
+  }
+  n->pendingSteps++;
+  n->t = t + 1;
+  if (n->pendingSteps == n->drainEvery || n->opt.keep_spikes) return drain(n);
+  return 0;
+}
+
+// ============================================================================
+// C ABI
+// ============================================================================
+extern "C" {
+
+const char *dpsnn_last_error(void) { return g_err.c_str(); }
+
+int dpsnn_get_parameters(const char *inidir, dpsnn_parameters *out) {
+  if (!inidir || !out) return fail(DPSNN_ERR_ARG, "null argument");
+  Params p;
+  std::string err;
+  if (!read_params(inidir, p, err)) return fail(DPSNN_ERR_CONFIG, "%s", err.c_str());
+  *out = p.pub;
+  return 0;
+}
+
+int dpsnn_create(const char *inidir, const dpsnn_options *opt, dpsnn_net **out) {
+  if (!inidir || !out) return fail(DPSNN_ERR_ARG, "null argument");
+  *out = nullptr;
+  auto t0 = std::chrono::steady_clock::now();
+  std::unique_ptr<dpsnn_net> n(new dpsnn_net());
+  if (opt) n->opt = *opt;
+  if (n->opt.drive != DPSNN_DRIVE_REPLAY && n->opt.drive != DPSNN_DRIVE_PHILOX)
+    return fail(DPSNN_ERR_ARG, "unknown drive mode %d", n->opt.drive);
+  std::string err;
+  if (!read_params(inidir, n->par, err)) return fail(DPSNN_ERR_CONFIG, "%s", err.c_str());
+  const dpsnn_parameters &q = n->par.pub;
+  if (q.startPlasticity_ms <= q.stopPlasticity_ms && q.startPlasticity_ms < q.totalSimTime_ms)
+    return fail(DPSNN_ERR_CONFIG, "plasticity window overlaps the run; STDP is not enabled in this build");
+  // localNet init: seed with env_globalSeed (DPSNN_localNet_init.c:119),
+  // generate, then continue the stream from SetRandomSeed(loc_h=0) (:283)
+  n->rng.seed((int32_t)q.globalSeed);
+  if (!generate_connectome(n->par, n->rng, n->con, err)) return fail(DPSNN_ERR_CONFIG, "%s", err.c_str());
+  build_delivery(n->par, n->con);
+  n->rng.seed(0);
+  if (n->opt.drive == DPSNN_DRIVE_REPLAY) n->replay.init(n->con, &n->rng);
+  n->writing = n->opt.output_dir && n->opt.output_dir[0];
+  if (n->writing) n->writer.init(n->par, n->opt.output_dir);
+  HIP_TRY(hipSetDevice(n->opt.device));
+  HIP_TRY(hipStreamCreateWithFlags(&n->stream, hipStreamNonBlocking));
+  HIP_TRY(hipEventCreate(&n->pe0));
+  HIP_TRY(hipEventCreate(&n->pe1));
+  int rc = build_device(n.get());
+  if (rc) {
+    free_all(n.get());
+    return rc;
+  }
+  // the device holds the in-lists; the host keeps the backward list for
+  // dumps and drops the delivery staging
+  std::vector<uint32_t>().swap(n->con.in_pre_d);
+  std::vector<int16_t>().swap(n->con.in_w);
+  std::vector<uint32_t>().swap(n->con.in_syn);
+  n->initSeconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+  *out = n.release();
+  return 0;
+}
+
+int dpsnn_step(dpsnn_net *n, uint32_t nsteps) {
+  if (!n) return fail(DPSNN_ERR_ARG, "null network");
+  HIP_TRY(hipSetDevice(n->opt.device));
+  for (uint32_t k = 0; k < nsteps; ++k) {
+    if (n->t >= n->par.pub.totalSimTime_ms) {
+      int rc = drain(n);
+      return rc ? rc : DPSNN_DONE;
+    }
+    int rc = one_step(n);
+    if (rc) return rc;
+  }
+  return 0;
+}
+
+int dpsnn_sync(dpsnn_net *n) {
+  if (!n) return fail(DPSNN_ERR_ARG, "null network");
+  HIP_TRY(hipStreamSynchronize(n->stream));
+  return drain(n);
+}
+
+uint32_t dpsnn_now_ms(const dpsnn_net *n) { return n ? n->t : 0; }
+uint32_t dpsnn_total_ms(const dpsnn_net *n) { return n ? n->par.pub.totalSimTime_ms : 0; }
+
+int dpsnn_get_state(dpsnn_net *n, double *v, double *c, double *Tr, double *Te, uint8_t *fired) {
+  if (!n) return fail(DPSNN_ERR_ARG, "null network");
+  const size_t N = n->dev.N;
+  HIP_TRY(hipStreamSynchronize(n->stream));
+  if (v) HIP_TRY(hipMemcpy(v, n->dev.v, N * 8, hipMemcpyDeviceToHost));
+  if (c) HIP_TRY(hipMemcpy(c, n->dev.c, N * 8, hipMemcpyDeviceToHost));
+  if (Tr) HIP_TRY(hipMemcpy(Tr, n->dev.Tr, N * 8, hipMemcpyDeviceToHost));
+  if (Te) HIP_TRY(hipMemcpy(Te, n->dev.Te, N * 8, hipMemcpyDeviceToHost));
+  if (fired) HIP_TRY(hipMemcpy(fired, n->dev.fired, N, hipMemcpyDeviceToHost));
+  return 0;
+}
+
+int dpsnn_last_spikes(dpsnn_net *n, uint32_t *ids, double *te, uint32_t *count) {
+  if (!n) return fail(DPSNN_ERR_ARG, "null network");
+  if (!n->opt.keep_spikes) return fail(DPSNN_ERR_ARG, "network created without keep_spikes");
+  int rc = dpsnn_sync(n);
+  if (rc) return rc;
+  if (count) *count = n->lastCount;
+  if (ids) std::memcpy(ids, n->lastIds.data(), sizeof(uint32_t) * n->lastCount);
+  if (te) std::memcpy(te, n->lastTe.data(), sizeof(double) * n->lastCount);
+  return 0;
+}
+
+uint64_t dpsnn_num_synapses(const dpsnn_net *n) { return n ? n->con.S : 0; }
+
+int dpsnn_get_synapses(dpsnn_net *n, uint32_t *pre, uint32_t *post, uint8_t *delay, int16_t *weight,
+                       float *tderiv, int32_t *last_act) {
+  if (!n) return fail(DPSNN_ERR_ARG, "null network");
+  const Connectome &c = n->con;
+  if (pre) std::memcpy(pre, c.pre.data(), 4 * c.S);
+  if (post) std::memcpy(post, c.post.data(), 4 * c.S);
+  if (delay) std::memcpy(delay, c.delay.data(), c.S);
+  if (weight) std::memcpy(weight, c.weight.data(), 2 * c.S);
+  if (tderiv) std::memset(tderiv, 0, 4 * c.S);
+  if (last_act) std::memset(last_act, 0, 4 * c.S);
+  return 0;
+}
+
+int dpsnn_write_synapse_dump(dpsnn_net *n, const char *path);
+int dpsnn_get_stats(dpsnn_net *n, dpsnn_stats *out) {
+  if (!n || !out) return fail(DPSNN_ERR_ARG, "null argument");
+  unsigned long long cnt[4] = {0, 0, 0, 0};
+  HIP_TRY(hipStreamSynchronize(n->stream));
+  HIP_TRY(hipMemcpy(cnt, n->dev.counters, sizeof cnt, hipMemcpyDeviceToHost));
+  out->steps = n->t;
+  out->synaptic_events = cnt[0];
+  out->external_events = cnt[1];
+  out->spikes = cnt[2];
+  out->synapses = n->con.S;
+  out->delivery_items = n->items;
+  out->neurons = n->con.N;
+  out->max_in_items = n->con.max_in;
+  out->init_seconds = n->initSeconds;
+  out->device_bytes = n->devBytes;
+  return 0;
+}
+
+void *dpsnn_stream(dpsnn_net *n) { return n ? (void *)n->stream : nullptr; }
+
+int dpsnn_profile(dpsnn_net *n, int enable) {
+  if (!n) return fail(DPSNN_ERR_ARG, "null network");
+  n->profile = enable != 0;
+  n->profMs = 0;
+  n->profN = 0;
+  return 0;
+}
+
+int dpsnn_kernel_time(dpsnn_net *n, int kernel, double *avg_ms, uint64_t *launches) {
+  if (!n) return fail(DPSNN_ERR_ARG, "null network");
+  (void)kernel;
+  if (avg_ms) *avg_ms = n->profN ? n->profMs / (double)n->profN : 0.0;
+  if (launches) *launches = n->profN;
+  n->profMs = 0;
+  n->profN = 0;
+  return 0;
+}
+
+void dpsnn_destroy(dpsnn_net *n) {
+  if (!n) return;
+  drain(n);
+  n->writer.close();
+  free_all(n);
+  delete n;
+}
+
+}  // extern "C"
+
+// ============================================================================
+// Host-only access and diagnostics
+// ============================================================================
+struct dpsnn_host_net {
+  Params par;
+  Connectome con;
+  Ran3 rng;
+  ExternalReplay replay;
+  ExternalStep step;
+  uint32_t t = 0;
+};
+
+namespace {
+int write_dump(const Connectome &c, float w2f, const char *path) {
+  FILE *f = std::fopen(path, "w");
+  if (!f) return fail(DPSNN_ERR_IO, "cannot open %s", path);
+  std::vector<char> line(256);
+  for (uint64_t s = 0; s < c.S; ++s) {
+    const float w = (float)(c.weight[s] * w2f);
+    const int len = std::snprintf(line.data(), line.size(), "%d %d %d %d %d %f %f\n", (int)c.pre[s],
+                                  (int)c.post[s], (int)(c.post[s] / c.N), (int)c.delay[s], (int)c.kind[s],
+                                  w, 0.0f);
+    std::fwrite(line.data(), 1, (size_t)len, f);
+  }
+  std::fclose(f);
+  return 0;
+}
+__global__ void k_exp(const double *x, double *y, uint64_t n) {
+  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) y[i] = dpsnn_exp(x[i]);
+}
+}  // namespace
+
+extern "C" {
+
+int dpsnn_host_build(const char *inidir, dpsnn_host_net **out) {
+  if (!inidir || !out) return fail(DPSNN_ERR_ARG, "null argument");
+  *out = nullptr;
+  std::unique_ptr<dpsnn_host_net> h(new dpsnn_host_net());
+  std::string err;
+  if (!read_params(inidir, h->par, err)) return fail(DPSNN_ERR_CONFIG, "%s", err.c_str());
+  h->rng.seed((int32_t)h->par.pub.globalSeed);
+  if (!generate_connectome(h->par, h->rng, h->con, err)) return fail(DPSNN_ERR_CONFIG, "%s", err.c_str());
+  build_delivery(h->par, h->con);
+  h->rng.seed(0);
+  h->replay.init(h->con, &h->rng);
+  *out = h.release();
+  return 0;
+}
+uint32_t dpsnn_host_num_neurons(const dpsnn_host_net *h) { return h ? h->con.N : 0; }
+uint64_t dpsnn_host_num_synapses(const dpsnn_host_net *h) { return h ? h->con.S : 0; }
+uint64_t dpsnn_host_num_items(const dpsnn_host_net *h) { return h ? h->con.in_off[h->con.N] : 0; }
+int dpsnn_host_get_synapses(const dpsnn_host_net *h, uint32_t *pre, uint32_t *post, uint8_t *delay,
+                            int16_t *weight) {
+  if (!h) return fail(DPSNN_ERR_ARG, "null argument");
+  const Connectome &c = h->con;
+  if (pre) std::memcpy(pre, c.pre.data(), 4 * c.S);
+  if (post) std::memcpy(post, c.post.data(), 4 * c.S);
+  if (delay) std::memcpy(delay, c.delay.data(), c.S);
+  if (weight) std::memcpy(weight, c.weight.data(), 2 * c.S);
+  return 0;
+}
+int dpsnn_host_get_inlists(const dpsnn_host_net *h, uint64_t *off, uint32_t *pre_delay, int16_t *weight,
+                           uint32_t *syn) {
+  if (!h) return fail(DPSNN_ERR_ARG, "null argument");
+  const Connectome &c = h->con;
+  const uint64_t items = c.in_off[c.N];
+  if (off) std::memcpy(off, c.in_off.data(), 8 * (c.N + 1));
+  if (pre_delay) std::memcpy(pre_delay, c.in_pre_d.data(), 4 * items);
+  if (weight) std::memcpy(weight, c.in_w.data(), 2 * items);
+  if (syn) std::memcpy(syn, c.in_syn.data(), 4 * items);
+  return 0;
+}
+int dpsnn_host_get_external(const dpsnn_host_net *h, double *invNuExt, uint32_t *maxCount) {
+  if (!h) return fail(DPSNN_ERR_ARG, "null argument");
+  if (invNuExt) std::memcpy(invNuExt, h->con.invNuExt.data(), 8 * h->con.N);
+  if (maxCount) std::memcpy(maxCount, h->con.maxExt.data(), 4 * h->con.N);
+  return 0;
+}
+int dpsnn_host_write_synapse_dump(const dpsnn_host_net *h, const char *path) {
+  if (!h || !path) return fail(DPSNN_ERR_ARG, "null argument");
+  return write_dump(h->con, h->par.pub.factorWeightType_2_Float, path);
+}
+int dpsnn_host_external_step(dpsnn_host_net *h, uint32_t *off, double *time, double *value, uint64_t cap,
+                             uint64_t *count) {
+  if (!h) return fail(DPSNN_ERR_ARG, "null argument");
+  h->replay.generate(h->t++, h->step);
+  const uint64_t ne = h->step.time.size();
+  if (count) *count = ne;
+  if (off) std::memcpy(off, h->step.off.data(), 4 * (size_t(h->con.N) + 1));
+  if (ne > cap) return fail(DPSNN_ERR_ARG, "external buffer holds %llu events, %llu needed",
+                            (unsigned long long)cap, (unsigned long long)ne);
+  if (time) std::memcpy(time, h->step.time.data(), 8 * ne);
+  if (value) std::memcpy(value, h->step.value.data(), 8 * ne);
+  return 0;
+}
+void dpsnn_host_destroy(dpsnn_host_net *h) { delete h; }
+
+int dpsnn_write_synapse_dump(dpsnn_net *n, const char *path) {
+  if (!n || !path) return fail(DPSNN_ERR_ARG, "null argument");
+  return write_dump(n->con, n->par.pub.factorWeightType_2_Float, path);
+}
+
+int dpsnn_device_exp(int device, const double *x, double *y, uint64_t n) {
+  if (!x || !y) return fail(DPSNN_ERR_ARG, "null argument");
+  HIP_TRY(hipSetDevice(device));
+  double *dx = nullptr, *dy = nullptr;
+  HIP_TRY(hipMalloc((void **)&dx, 8 * std::max<uint64_t>(n, 1)));
+  HIP_TRY(hipMalloc((void **)&dy, 8 * std::max<uint64_t>(n, 1)));
+  HIP_TRY(hipMemcpy(dx, x, 8 * n, hipMemcpyHostToDevice));
+  const uint64_t blocks = (n + 255) / 256;
+  if (n) hipLaunchKernelGGL(k_exp, dim3((uint32_t)blocks), dim3(256), 0, 0, dx, dy, n);
+  HIP_TRY(hipGetLastError());
+  HIP_TRY(hipMemcpy(y, dy, 8 * n, hipMemcpyDeviceToHost));
+  HIP_TRY(hipFree(dx));
+  HIP_TRY(hipFree(dy));
+  return 0;
+}
+
+}  // extern "C"

@@ -1,0 +1,748 @@
+// dpsnn_host.cpp -- host side of the engine (see dpsnn_host.h).
+//
+// Numeric types follow the reference declaration by declaration, because
+// the connectome and the replayed drive must come out bit-identical:
+//   * ini values are parsed into float and widened into double fields;
+//   * weights are float until quantized to int16 (weightType);
+//   * delays are computed in float and rounded with roundr2i.
+// This file is compiled without FP contraction.
+#include "dpsnn_host.h"
+
+#include <algorithm>
+#include <cerrno>
+#include <cmath>
+#include <cstdarg>
+#include <cstdlib>
+#include <cstring>
+#include <sys/stat.h>
+
+namespace dpsnn {
+
+// --------------------------------------------------------------------------
+// ran3 / NormDev.  randdev.c:116-189 (rand3 with the author's |mj| fix),
+// :477-502 (SetRandomSeed: idum = -seed then one draw, so seed 0 after the
+// first initialisation only advances the stream), :584-606 (polar
+// Box-Muller whose second value is cached across reseeds).
+// --------------------------------------------------------------------------
+double Ran3::next() {
+  constexpr int32_t kBig = 1000000000, kSeed = 161803398;
+  if (idum_ < 0 || iff_ == 0) {
+    iff_ = 1;
+    int32_t mj = kSeed - (idum_ < 0 ? -idum_ : idum_);
+    mj = mj < 0 ? -mj : mj;
+    mj %= kBig;
+    ma_[55] = mj;
+    int32_t mk = 1;
+    for (int32_t i = 1; i <= 54; ++i) {
+      const int32_t ii = (21 * i) % 55;
+      ma_[ii] = mk;
+      mk = mj - mk;
+      if (mk < 0) mk += kBig;
+      mj = ma_[ii];
+    }
+    for (int round = 0; round < 4; ++round)
+      for (int32_t i = 1; i <= 55; ++i) {
+        ma_[i] -= ma_[1 + (i + 30) % 55];
+        if (ma_[i] < 0) ma_[i] += kBig;
+      }
+    inext_ = 0;
+    inextp_ = 31;
+    idum_ = 1;
+  }
+  if (++inext_ == 56) inext_ = 1;
+  if (++inextp_ == 56) inextp_ = 1;
+  int32_t mj = ma_[inext_] - ma_[inextp_];
+  if (mj < 0) mj += kBig;
+  ma_[inext_] = mj;
+  ++count_;
+  return mj * (1.0 / (double)kBig);
+}
+
+void Ran3::seed(int32_t s) {
+  idum_ = -s;
+  (void)next();
+}
+
+double Ran3::norm() {
+  if (set_) {
+    set_ = 0;
+    return cached_;
+  }
+  double v1, v2, r;
+  do {
+    v1 = (double)(2.0 * next() - 1.0);
+    v2 = (double)(2.0 * next() - 1.0);
+    r = v1 * v1 + v2 * v2;
+  } while (r >= 1.0 || r == 0.0);
+  const double fac = (double)std::sqrt(-2.0 * std::log(r) / r);
+  cached_ = v1 * fac;
+  set_ = 1;
+  return v2 * fac;
+}
+
+// --------------------------------------------------------------------------
+// erflib pieces: roundr2i (erflib.c:172-181) and the truncated Gaussian
+// LUT of the bath (erflib.c:41-169).
+// --------------------------------------------------------------------------
+static uint32_t round_half_up(double r) {
+  const double f = std::floor(r);
+  return (f + 0.5 > r) ? (uint32_t)f : (uint32_t)(f + 1.0);
+}
+
+static double erfc_nr(double x) {  // erfcc, Numerical Recipes
+  const double z = std::fabs(x), t = 1.0 / (1.0 + 0.5 * z);
+  const double ans = t * std::exp(-z * z - 1.26551223 + t * (1.00002368 + t * (0.37409196 +
+                     t * (0.09678418 + t * (-0.18628806 + t * (0.27886807 + t * (-1.13520398 +
+                     t * (1.48851587 + t * (-0.82215223 + t * 0.17087277)))))))));
+  return x >= 0.0 ? ans : 2.0 - ans;
+}
+static double cdf(double x, double mu, double sigma) {
+  return 1 - 0.5 * erfc_nr((x - mu) / (std::sqrt(2.0) * sigma));
+}
+static double bisect_quantile(double p, double mu, double sigma, double a, double b) {
+  if (p <= 0.0) return a;
+  if (p >= 1.0) return b;
+  double lo = a, hi = b;
+  const double c0 = cdf(a, mu, sigma);
+  for (int it = 0; (hi - lo) > 1e-6 && it < 256; ++it) {
+    const double x = (hi + lo) * 0.5;
+    if ((cdf(x, mu, sigma) - c0) / (cdf(b, mu, sigma) - c0) < p) lo = x; else hi = x;
+  }
+  return (hi + lo) * 0.5;
+}
+static void gaussian_lut(double *tab, int n, double mu, double sigma, double xmin, double xmax) {
+  if (!(sigma > 0.0)) {
+    for (int i = 0; i < n; ++i) tab[i] = mu;
+    return;
+  }
+  const double dp = 1.0 / (double)n;
+  double x1 = xmin;
+  int i = 0;
+  for (double p = dp; p <= 1.0; p += dp, ++i) {
+    const double x0 = x1;
+    x1 = bisect_quantile(p, mu, sigma, xmin, xmax);
+    tab[i] = (x1 + x0) * 0.5;
+  }
+}
+
+// --------------------------------------------------------------------------
+// Parameters: DPSNN_getParameters.c:23-831 (env), :834-898 (column.txt
+// sizes), :900-1053 (plasticity.txt) and DPSNN_LIFCAconnectome.cpp:16-131
+// (connectivity.txt), :240-300 (bathEfficacy.txt), :302-398 (column.txt
+// values), :400-487 (delay range).
+// --------------------------------------------------------------------------
+namespace {
+
+struct EnvReader {
+  std::string &err;
+  bool ok = true;
+  uint32_t u32(const char *name) {
+    const char *s = std::getenv(name);
+    if (!s) {
+      if (ok) err = std::string("missing env var ") + name;
+      ok = false;
+      return 0;
+    }
+    return (uint32_t)std::atoi(s);
+  }
+  int32_t flag(const char *name) {
+    const uint32_t v = u32(name);
+    if (ok && v > 1) {
+      err = std::string("invalid value of ") + name;
+      ok = false;
+    }
+    return (int32_t)v;
+  }
+};
+
+// One ini line: whitespace tokens until a '#'-token, each through atof into
+// float (the readers' float initValue[]).  Returns the token count.
+int ini_tokens(char *line, std::vector<float> &vals) {
+  int n = 0;
+  for (char *tok = std::strtok(line, " \t\n"); tok; tok = std::strtok(nullptr, " \t\n")) {
+    if (tok[0] == '#') break;
+    if ((size_t)n < vals.size()) vals[n] = (float)std::atof(tok);
+    ++n;
+  }
+  return n;
+}
+
+FILE *open_ini(const std::string &dir, const char *name) {
+  return std::fopen((dir + "/" + name).c_str(), "r");
+}
+
+bool read_columns(const std::string &dir, Params &p, std::string &err) {
+  FILE *f = open_ini(dir, "column.txt");
+  if (!f) { err = "cannot open column.txt"; return false; }
+  std::vector<char> line(1 << 12);
+  std::vector<float> v(12);
+  uint32_t npc = 0, nsp = 0;
+  while (std::fgets(line.data(), (int)line.size(), f)) {
+    std::fill(v.begin(), v.end(), 0.0f);
+    const int n = ini_tokens(line.data(), v);
+    if (n <= 0) continue;
+    if (n < 3) { err = "column.txt: a line needs at least 3 values"; std::fclose(f); return false; }
+    const uint32_t s = (uint32_t)v[0];
+    if (s >= kMaxSubPop) { err = "column.txt: sub-population id out of range"; std::fclose(f); return false; }
+    SubPop &q = p.sp[s];
+    q.count = (uint32_t)v[1];
+    q.offset = s == 0 ? 0 : p.sp[s - 1].offset + p.sp[s - 1].count;
+    q.JExt = v[2]; q.DJExt = v[3]; q.NuExt = v[4]; q.Tau = v[5]; q.Theta = v[6];
+    q.H = v[7]; q.Tarp = v[8]; q.AlphaC = v[9]; q.TauC = v[10]; q.gC = v[11];
+    npc += (uint32_t)v[1];
+    ++nsp;
+  }
+  std::fclose(f);
+  if (npc == 0) { err = "column.txt: neuronsPerCM <= 0"; return false; }
+  p.pub.subPopNumber = nsp;
+  p.pub.neuronsPerCM = npc;
+  return true;
+}
+
+bool read_connectivity(const std::string &dir, Params &p, std::string &err) {
+  FILE *f = open_ini(dir, "connectivity.txt");
+  if (!f) { err = "cannot open connectivity.txt"; return false; }
+  const uint32_t mx = p.maxModDeltaX, my = p.maxModDeltaY;
+  const uint32_t cells = (mx + 1) * (my + 1);
+  std::vector<char> line(10 * kMaxStencil * kMaxStencil + 64);
+  std::vector<float> v(6 + kMaxStencil * kMaxStencil);
+  std::vector<float> spec(size_t(kMaxSubPop) * kMaxSubPop * kMaxStencil * kMaxStencil, 0.0f);
+  auto at = [&](uint32_t s, uint32_t t, uint32_t x, uint32_t y) -> float & {
+    return spec[((size_t(s) * kMaxSubPop + t) * kMaxStencil + x) * kMaxStencil + y];
+  };
+  while (std::fgets(line.data(), (int)line.size(), f)) {
+    std::fill(v.begin(), v.end(), 0.0f);
+    const int n = ini_tokens(line.data(), v);
+    if (n <= 0) continue;
+    if ((uint32_t)(n - 6) != cells) {
+      err = "connectivity.txt: number of stencil values does not match env_stencilDim";
+      std::fclose(f);
+      return false;
+    }
+    const uint32_t s = (uint32_t)v[0], t = (uint32_t)v[1];
+    if (s >= p.pub.subPopNumber || t >= p.pub.subPopNumber) {
+      err = "connectivity.txt: sub-population id out of range";
+      std::fclose(f);
+      return false;
+    }
+    p.sp[s].J[t] = v[2]; p.sp[s].DJ[t] = v[3]; p.sp[s].DMin[t] = v[4]; p.sp[s].DMax[t] = v[5];
+    uint32_t k = 6;
+    for (uint32_t y = 0; y <= my; ++y)
+      for (uint32_t x = 0; x <= mx; ++x) at(s, t, x, y) = v[k++];
+  }
+  std::fclose(f);
+  p.prob.assign(size_t(kMaxSubPop) * kMaxSubPop * kMaxStencil * kMaxStencil, 0.0f);
+  auto pr = [&](uint32_t s, uint32_t t, uint32_t x, uint32_t y) -> float & {
+    return p.prob[((size_t(s) * kMaxSubPop + t) * kMaxStencil + x) * kMaxStencil + y];
+  };
+  for (uint32_t s = 0; s < p.pub.subPopNumber; ++s)
+    for (uint32_t t = 0; t < p.pub.subPopNumber; ++t) {
+      if (p.pub.overallConnectivity == 0) {  // explicit stencil: mirror quadrant
+        for (uint32_t x = 0; x <= mx; ++x)
+          for (uint32_t y = 0; y <= my; ++y) {
+            const float q = at(s, t, x, y);
+            pr(s, t, mx + x, my + y) = q;
+            pr(s, t, mx - x, my + y) = q;
+            pr(s, t, mx - x, my - y) = q;
+            pr(s, t, mx + x, my - y) = q;
+          }
+      } else {  // homogeneous
+        for (uint32_t x = 0; x < p.stencilX; ++x)
+          for (uint32_t y = 0; y < p.stencilY; ++y) pr(s, t, x, y) = at(s, t, 0, 0);
+      }
+    }
+  return true;
+}
+
+bool read_bath(const std::string &dir, Params &p, std::string &err) {
+  FILE *f = open_ini(dir, "bathEfficacy.txt");
+  if (!f) { err = "cannot open bathEfficacy.txt"; return false; }
+  p.bath.assign(size_t(kMaxStencil) * kMaxStencil, 0);
+  std::vector<char> line(100000);
+  uint32_t y = 0;
+  while (std::fgets(line.data(), (int)line.size(), f)) {
+    uint32_t x = 0, n = 0;
+    for (char *tok = std::strtok(line.data(), " \t\n"); tok; tok = std::strtok(nullptr, " \t\n")) {
+      if (tok[0] == '#') break;
+      if (x < kMaxStencil && y < kMaxStencil) p.bath[size_t(x) * kMaxStencil + y] = (uint32_t)std::atoi(tok);
+      ++x;
+      ++n;
+    }
+    if (n == 0) continue;
+    if (n != p.pub.stencilDim) {
+      err = "bathEfficacy.txt: row length does not match env_stencilDim";
+      std::fclose(f);
+      return false;
+    }
+    ++y;
+  }
+  std::fclose(f);
+  return true;
+}
+
+bool read_plasticity(const std::string &dir, Params &p, std::string &err) {
+  dpsnn_parameters &q = p.pub;
+  FILE *f = open_ini(dir, "plasticity.txt");
+  if (!f) {  // defaults, DPSNN_getParameters.c:916-928
+    q.antiCausalCoeff = -1.0f; q.STDPmultiplier = 0.001f; q.tauSTDP_ms = 20.0f;
+    q.causalSTDPmaxAbsValue = 0.1f; q.antiCausalSTDPmaxAbsValue = 0.12f;
+    q.tauDerivativeDecay_s = 1.0f; q.derivativeDrift = 0.0f;
+    return true;
+  }
+  std::vector<char> line(200);
+  std::vector<float> v(20, 0.0f);
+  int n = 0;
+  while (std::fgets(line.data(), (int)line.size(), f)) n = ini_tokens(line.data(), v);  // last line wins
+  std::fclose(f);
+  if (n == 0 || n >= 20) { err = "plasticity.txt: empty or too many values"; return false; }
+  if (v[0] != 1.0f) { err = "plasticity.txt: unknown plasticity algorithm"; return false; }
+  if (n != 8) { err = "plasticity.txt: expected 8 values"; return false; }
+  q.antiCausalCoeff = v[1]; q.STDPmultiplier = v[2]; q.tauSTDP_ms = v[3];
+  q.causalSTDPmaxAbsValue = v[4]; q.antiCausalSTDPmaxAbsValue = v[5];
+  q.tauDerivativeDecay_s = v[6]; q.derivativeDrift = v[7];
+  return true;
+}
+
+}  // namespace
+
+bool read_params(const std::string &dir, Params &p, std::string &err) {
+  dpsnn_parameters &q = p.pub;
+  q = dpsnn_parameters{};
+  EnvReader env{err};
+  q.overallConnectivity = env.u32("env_overallConnectivity");
+  if (env.ok && q.overallConnectivity > 1) { err = "invalid env_overallConnectivity"; return false; }
+  q.stencilDim = env.u32("env_stencilDim");
+  if (!env.ok) return false;
+  if (q.stencilDim == 0 || q.stencilDim > kMaxStencil) { err = "env_stencilDim out of range"; return false; }
+  if (q.overallConnectivity == 0) {
+    p.maxModDeltaX = p.maxModDeltaY = q.stencilDim - 1;
+    p.stencilX = p.stencilY = 2 * p.maxModDeltaX + 1;
+  } else {
+    p.maxModDeltaX = p.maxModDeltaY = 0;
+    p.stencilX = p.stencilY = q.stencilDim;
+  }
+  if (p.stencilX > kMaxStencil) { err = "stencil larger than 96 columns"; return false; }
+  q.globH = env.u32("env_globH");
+  if (env.ok && q.globH != 1) { err = "env_globH must be 1: one process's whole local network per device"; return false; }
+  const uint32_t synGen = env.u32("env_synGen");
+  if (env.ok && synGen != 2) { err = "unsupported env_synGen (only 2, simpleCorticalModule)"; return false; }
+  if (!env.ok) return false;
+  if (!read_columns(dir, p, err)) return false;
+  q.globCFX = env.u32("env_CFX");
+  q.globCFY = env.u32("env_CFY");
+  if (!env.ok) return false;
+  q.globCFT = q.globCFX * q.globCFY;
+  if (q.globCFT == 0 || q.globCFT > 96 * 96 + 1) { err = "globCFT out of range"; return false; }
+  q.globN = q.locN = q.neuronsPerCM * q.globCFT;
+  if (q.globN >= (1u << 24)) { err = "more than 2^24 neurons per device network"; return false; }
+  const char *ws = std::getenv("env_minSynWeight_f");
+  if (!ws) { err = "missing env var env_minSynWeight_f"; return false; }
+  q.minSynWeight_f = (float)std::atof(ws);
+  if (q.minSynWeight_f >= 0 || q.minSynWeight_f < -6.0) { err = "env_minSynWeight_f out of range"; return false; }
+  q.factorFloat_2_weightType = -32767 / q.minSynWeight_f;
+  q.factorWeightType_2_Float = q.minSynWeight_f / -32767;
+  q.tabNeuron = q.globN / q.globCFT;  // makeMaxCMFract = 1
+  q.D = env.u32("env_D");
+  if (env.ok && (q.D == 0 || q.D > kDelaySlots)) { err = "env_D out of range (1..80)"; return false; }
+  q.totalSimTime_ms = env.u32("env_totalSimTime_ms");
+  q.startPlasticity_ms = env.u32("env_startPlasticity_ms");
+  q.stopPlasticity_ms = env.u32("env_stopPlasticity_ms");
+  q.moduloSec = env.u32("env_moduloSec");
+  q.startStatFiles_s = env.u32("env_startStatFiles_s");
+  q.stopStatFiles_s = env.u32("env_stopStatFiles_s");
+  q.spikes_writeDisable = env.flag("env_spikes_writeDisable");
+  (void)env.flag("env_statSTDPevent_writeDisable");
+  (void)env.flag("env_statSynPeriodicProbe_writeDisable");
+  q.statRatesPerPop_writeDisable = env.flag("env_statRatesPerPop_writeDisable");
+  q.startRatesPerPop_s = env.u32("env_startRatesPerPop_s");
+  q.stopRatesPerPop_s = env.u32("env_stopRatesPerPop_s");
+  (void)env.u32("env_startSynPeriodicProbe_s");
+  (void)env.u32("env_stopSynPeriodicProbe_s");
+  (void)env.u32("env_startSTDPevent_s");
+  (void)env.u32("env_stopSTDPevent_s");
+  q.ratesSampling = env.u32("env_ratesSampling");
+  if (!env.ok) return false;
+  if (q.moduloSec == 0) { err = "env_moduloSec must be > 0"; return false; }
+  if (q.ratesSampling == 0) { err = "env_ratesSampling must be > 0"; return false; }
+  const char *gs = std::getenv("env_globalSeed");
+  if (!gs) { err = "missing env var env_globalSeed"; return false; }
+  if (std::atoi(gs) < 0) { err = "env_globalSeed < 0 asks for a time-based seed; give a fixed seed"; return false; }
+  q.globalSeed = (uint32_t)std::atoi(gs);
+  const char *hm = std::getenv("env_howManyOutputFiles");
+  q.howManyOutputFiles = hm ? (uint32_t)std::atoi(hm) : 0;
+  if (q.howManyOutputFiles > 1) q.howManyOutputFiles = 0;
+  if (!read_plasticity(dir, p, err)) return false;
+  if (q.tauSTDP_ms * 3 > kStdpSpan || q.tauSTDP_ms <= 0 || q.STDPmultiplier < 0) {
+    err = "plasticity.txt: tauSTDP_ms must be in (0, 128/3] and STDPmultiplier >= 0";
+    return false;
+  }
+  (void)env.u32("env_collectivesAlgo");
+  (void)env.u32("env_pktLength");
+  if (!env.ok) return false;
+  if (!read_bath(dir, p, err) || !read_connectivity(dir, p, err)) return false;
+  double dmin = 1e36, dmax = 0;
+  for (uint32_t s = 0; s < q.subPopNumber; ++s)
+    for (uint32_t t = 0; t < q.subPopNumber; ++t) {
+      dmax = std::max(dmax, p.sp[s].DMax[t]);
+      dmin = std::min(dmin, p.sp[s].DMin[t]);
+    }
+  q.delayMin = (uint32_t)dmin;
+  q.delayMax = (uint32_t)dmax;
+  return true;
+}
+
+// --------------------------------------------------------------------------
+// Connectome generator: simpleCM_invokeIndivSynGenerators
+// (DPSNN_LIFCAcorticalModuleGen.cpp:62-220) with generateTargetNeuList's
+// Bernoulli branch (DPSNN_LIFCAconnectome.cpp:513-823, non-periodic edges),
+// getEmptyRandomSynapses (:1079-1133), getRandomDelay_UNI (:299-338),
+// the truncated-normal weight draw (:158-183) and generateSourceNeuCExt
+// (:871-887).  The random stream is reseeded with the column index at each
+// column's first neuron; the NormDev cache and the seed-0 quirk carry over.
+// --------------------------------------------------------------------------
+namespace {
+
+struct Target {
+  uint32_t id;
+  uint8_t sub;
+};
+
+uint32_t sub_pop_in_column(const Params &p, uint32_t inCM) {
+  for (uint32_t s = 0; s < p.pub.subPopNumber; ++s)
+    if (inCM >= p.sp[s].offset && inCM < p.sp[s].count + p.sp[s].offset) return s;
+  return 0;
+}
+
+uint32_t geometric_gap(Ran3 &rng, float prob, uint32_t maxN) {
+  const double q = (double)prob, r = rng.next();
+  double P = q, C = P;
+  uint32_t n = 1;
+  while (C < r && n <= maxN) {
+    ++n;
+    P *= 1 - q;
+    C += P;
+  }
+  return n;
+}
+
+uint32_t uniform_delay(const Params &p, Ran3 &rng, uint32_t s, uint32_t t) {
+  float DMin = (float)p.sp[s].DMin[t], DMax = (float)p.sp[s].DMax[t];
+  float lo = (float)p.pub.delayMin, hi = (float)p.pub.delayMax;
+  const float levels = (float)kDelaySlots;
+  const float step = (hi - lo) / levels;
+  hi -= step / 2;
+  lo += step / 2;
+  DMax = round_half_up((DMax - lo) / step) * step + lo;
+  DMin = round_half_up((DMin - lo) / step) * step + lo;
+  const float level = ((rng.next() * (DMax - DMin) + DMin - lo) / step);
+  return round_half_up(level * step);
+}
+
+}  // namespace
+
+bool generate_connectome(const Params &p, Ran3 &rng, Connectome &c, std::string &err) {
+  const dpsnn_parameters &q = p.pub;
+  const uint32_t N = q.globN, npc = q.neuronsPerCM;
+  c.N = N;
+  c.subPop.assign(N, 0);
+  c.invNuExt.assign(N, 0.0);
+  c.maxExt.assign(N, 0);
+  c.bathLut.assign(size_t(q.subPopNumber) * kLutDepth, 0.0);
+  for (uint32_t s = 0; s < q.subPopNumber; ++s) {  // initLUT, bath part
+    const float J = (float)p.sp[s].JExt, DJ = (float)p.sp[s].DJExt;
+    gaussian_lut(&c.bathLut[size_t(s) * kLutDepth], kLutDepth, J, J * DJ, 0.0, 2.0 * J);
+  }
+  // generation-order arrays (pre is implicit: neuron i's synapses are
+  // contiguous); delay, post and weight per synapse
+  std::vector<uint64_t> gen_off(N + 1, 0);
+  std::vector<uint32_t> g_post;
+  std::vector<uint8_t> g_delay;
+  std::vector<int16_t> g_w;
+  const size_t guess = size_t(N) * 1200;
+  g_post.reserve(guess); g_delay.reserve(guess); g_w.reserve(guess);
+  std::vector<Target> targets;
+  targets.reserve(kMaxOutDegree);
+  for (uint32_t i = 0; i < N; ++i) {
+    const uint32_t col = i / npc, cfx = col % q.globCFX, cfy = col / q.globCFX, inCM = i % npc;
+    const uint32_t src = sub_pop_in_column(p, inCM);
+    c.subPop[i] = (uint8_t)src;
+    if (i % q.tabNeuron == 0) rng.seed((int32_t)(i / q.tabNeuron));
+    targets.clear();
+    for (uint32_t ts = 0; ts < q.subPopNumber; ++ts)
+      for (uint32_t x = 0; x < p.stencilX; ++x)
+        for (uint32_t y = 0; y < p.stencilY; ++y) {
+          int32_t tx, ty;
+          if (q.overallConnectivity == 0) {
+            tx = (int32_t)cfx + (int32_t)x - (int32_t)p.maxModDeltaX;
+            ty = (int32_t)cfy + (int32_t)y - (int32_t)p.maxModDeltaY;
+          } else {
+            tx = (int32_t)x;
+            ty = (int32_t)y;
+          }
+          if (tx < 0 || tx >= (int32_t)q.globCFX || ty < 0 || ty >= (int32_t)q.globCFY) continue;
+          const float prob = p.prob_at(src, ts, x, y);
+          if (!(prob > 0)) continue;
+          if (!(prob > 0.00001)) {
+            err = "connection probabilities in (0, 1e-5] take a branch the reference marks as incorrect; not supported";
+            return false;
+          }
+          const uint32_t first = p.sp[ts].offset, end = first + p.sp[ts].count;
+          const uint32_t colBase = (uint32_t)tx * npc + (uint32_t)ty * npc * q.globCFX;
+          uint32_t cur = first - 1;  // wraps like the reference's uint32 arithmetic
+          const uint32_t span = end - cur;
+          while ((cur += geometric_gap(rng, prob, span)) < end) {
+            if (targets.size() + 1 >= kMaxOutDegree) {
+              err = "a neuron would get 7500 or more synapses (DSD__maxM)";
+              return false;
+            }
+            targets.push_back({colBase + cur, (uint8_t)sub_pop_in_column(p, cur)});
+          }
+        }
+    for (const Target &tg : targets) {
+      const uint32_t d = uniform_delay(p, rng, src, tg.sub);
+      const float mean = (float)p.sp[src].J[tg.sub];
+      const float sigma = (float)(p.sp[src].DJ[tg.sub] * mean);
+      float w;
+      if (mean > 0.0) {
+        do { w = (float)(mean + rng.norm() * sigma); } while (w < 0.0 || w >= 2.0 * mean);
+      } else {
+        do { w = (float)(mean + rng.norm() * sigma); } while (w <= 2.0 * mean || w > 0);
+      }
+      if (w < q.minSynWeight_f || w > -q.minSynWeight_f) {
+        err = "synaptic weight outside [env_minSynWeight_f, -env_minSynWeight_f]";
+        return false;
+      }
+      if (d >= kDelaySlots || d >= q.D) {
+        err = "a stored delay is not below env_D";
+        return false;
+      }
+      g_post.push_back(tg.id);
+      g_delay.push_back((uint8_t)d);
+      g_w.push_back((int16_t)(w * q.factorFloat_2_weightType));
+    }
+    gen_off[i + 1] = g_post.size();
+    // external bath of this neuron
+    const float nu = (float)p.sp[src].NuExt;
+    uint32_t bx = cfx, by = cfy;
+    if (bx >= q.globCFX / 2) bx = -1 * (bx - q.globCFX + 1);
+    if (by >= q.globCFY / 2) by = -1 * (by - q.globCFY + 1);
+    if (bx >= q.stencilDim) bx = q.stencilDim - 1;
+    if (by >= q.stencilDim) by = q.stencilDim - 1;
+    const float cext = (float)p.bath[size_t(bx) * kMaxStencil + by];
+    if (cext != 0) {
+      c.invNuExt[i] = 1000.0 / (nu * cext);
+      c.maxExt[i] = (uint32_t)std::ceil(20 / c.invNuExt[i]);
+    }
+  }
+  // inflate: stable counting sort by delay into the backward list
+  const uint64_t S = g_post.size();
+  c.S = S;
+  uint64_t bucket[kDelaySlots + 1] = {};
+  for (uint64_t s = 0; s < S; ++s) ++bucket[g_delay[s] + 1];
+  for (uint32_t d = 0; d < kDelaySlots; ++d) bucket[d + 1] += bucket[d];
+  c.pre.resize(S); c.post.resize(S); c.delay.resize(S); c.kind.resize(S); c.weight.resize(S);
+  uint32_t maxd = 0;
+  for (uint32_t i = 0; i < N; ++i)
+    for (uint64_t s = gen_off[i]; s < gen_off[i + 1]; ++s) {
+      const uint64_t b = bucket[g_delay[s]]++;
+      c.pre[b] = i;
+      c.post[b] = g_post[s];
+      c.delay[b] = g_delay[s];
+      c.kind[b] = c.subPop[i];
+      c.weight[b] = g_w[s];
+      maxd = std::max<uint32_t>(maxd, g_delay[s]);
+    }
+  c.max_delay = maxd;
+  return true;
+}
+
+// --------------------------------------------------------------------------
+// Delivery layout.  The reference delivers, at step t, for d = D-1 down to
+// 0, the spikes of the step t-d in ascending id, each through the run of its
+// synapses that starts at offsetByDelay[pre][d] (DPSNN_localNet_init.c:
+// 702-723) and continues while the source id matches -- findAndGetNextSyn
+// (DPSNN_synapse.c:48-136) reads one entry past its bucket, so a run that
+// ends a bucket continues into the next bucket when that bucket starts with
+// the same source.  Events reach each target's input list in that order,
+// which is also the tie order of its stable time sort.  Here each target
+// gets the list of (delay of the run, source, synapse) in exactly that
+// order; at step t an item is live when its source fired in step t-delay.
+// --------------------------------------------------------------------------
+void build_delivery(const Params &p, Connectome &c) {
+  const uint32_t N = c.N, D = p.pub.D;
+  const uint64_t S = c.S;
+  // bucket boundaries
+  std::vector<uint64_t> bstart(kDelaySlots + 1, 0);
+  for (uint64_t s = 0; s < S; ++s) ++bstart[c.delay[s] + 1];
+  for (uint32_t d = 0; d < kDelaySlots; ++d) bstart[d + 1] += bstart[d];
+  // pass 1: count items per target; pass 2: fill in delivery order
+  std::vector<uint64_t> cnt(N + 1, 0);
+  for (int pass = 0; pass < 2; ++pass) {
+    if (pass == 1) {
+      c.in_off.assign(N + 1, 0);
+      for (uint32_t j = 0; j < N; ++j) c.in_off[j + 1] = c.in_off[j] + cnt[j];
+      const uint64_t items = c.in_off[N];
+      c.in_pre_d.resize(items); c.in_w.resize(items); c.in_syn.resize(items);
+      std::fill(cnt.begin(), cnt.end(), 0);
+    }
+    for (uint32_t d = std::min<uint32_t>(D, kDelaySlots); d-- > 0;) {
+      uint64_t s = bstart[d];
+      while (s < bstart[d + 1]) {
+        const uint32_t src = c.pre[s];
+        uint64_t e = s;
+        while (e < S && c.pre[e] == src) {  // the run, possibly spilling over
+          const uint32_t tgt = c.post[e];
+          if (pass == 0) {
+            ++cnt[tgt];
+          } else {
+            const uint64_t at = c.in_off[tgt] + cnt[tgt]++;
+            c.in_pre_d[at] = src | (d << 24);
+            c.in_w[at] = c.weight[e];
+            c.in_syn[at] = (uint32_t)e;
+          }
+          ++e;
+        }
+        // next source of this bucket
+        s = e < bstart[d + 1] ? e : bstart[d + 1];
+        while (s < bstart[d + 1] && c.pre[s] == src) ++s;
+      }
+    }
+  }
+  c.max_in = 0;
+  for (uint32_t j = 0; j < N; ++j) c.max_in = std::max<uint32_t>(c.max_in, (uint32_t)cnt[j]);
+  // incoming synapses per target in backward-list order (LTP list,
+  // DPSNN_localNet_init.c:225-237)
+  c.inc_off.assign(N + 1, 0);
+  for (uint64_t s = 0; s < S; ++s) ++c.inc_off[c.post[s] + 1];
+  for (uint32_t j = 0; j < N; ++j) c.inc_off[j + 1] += c.inc_off[j];
+  c.inc_syn.resize(S);
+  std::vector<uint64_t> fill(c.inc_off.begin(), c.inc_off.end() - 1);
+  for (uint64_t s = 0; s < S; ++s) c.inc_syn[fill[c.post[s]]++] = (uint32_t)s;
+}
+
+// --------------------------------------------------------------------------
+// External drive replay: regularDynamicTimeStep's bath loop
+// (DPSNN_neuron_sim.c:604-640) with getPoissonLapse (:844-851) -- a LUT
+// index draw then a lapse draw per event, lapses-only past the per-ms cap --
+// in neuron order, every step, from the network's single ran3 stream.
+// --------------------------------------------------------------------------
+void ExternalReplay::init(const Connectome &c, Ran3 *rng) {
+  c_ = &c;
+  rng_ = rng;
+  assigned_.assign(c.N, 0.0);
+}
+
+void ExternalReplay::generate(uint32_t t, ExternalStep &out) {
+  const Connectome &c = *c_;
+  const double next = (double)((double)t + 1.0);
+  out.off.resize(c.N + 1);
+  out.time.clear();
+  out.value.clear();
+  out.off[0] = 0;
+  for (uint32_t i = 0; i < c.N; ++i) {
+    const double inv = c.invNuExt[i];
+    if (inv > 0 && assigned_[i] < next) {
+      const double *lut = &c.bathLut[size_t(c.subPop[i]) * kLutDepth];
+      const uint32_t cap = c.maxExt[i];
+      uint32_t n = 0;
+      double a = assigned_[i];
+      auto lapse = [&]() {
+        double u = (double)1.0 - rng_->next();
+        if (u < 0) u = 0;
+        return (-logf((float)u) * inv);
+      };
+      do {
+        out.time.push_back(a);
+        out.value.push_back(lut[(uint32_t)(rng_->next() * kLutDepth)]);
+        ++n;
+        a += lapse();
+      } while (a < next && n < cap);
+      if (n >= cap && a < next) {
+        do { a += lapse(); } while (a < next);
+      }
+      assigned_[i] = a;
+    }
+    out.off[i + 1] = (uint32_t)out.time.size();
+  }
+}
+
+// --------------------------------------------------------------------------
+// Output files.  Names as statBasicClass::openFile (DPSNN_stat.c:55-107);
+// raster lines "%d %.7g" (DPSNN_stat.c:275-296); ratesPerPop lines as
+// statSpikingRatesPerPopClass::write (DPSNN_stat.c:225-273), with the
+// per-population counting of DPSNN_localNet_sim.c:227-253.
+// --------------------------------------------------------------------------
+bool StatWriter::init(const Params &p, const std::string &dir) {
+  p_ = &p;
+  dir_ = dir;
+  perPop_.assign(size_t(p.pub.globCFT) * p.pub.subPopNumber, 0);
+  buf_.resize(1 << 16);
+  return true;
+}
+
+bool StatWriter::window(uint32_t sec) const {
+  const dpsnn_parameters &q = p_->pub;
+  return sec >= q.startStatFiles_s && sec <= q.stopStatFiles_s && sec % q.moduloSec == 0;
+}
+
+FILE *StatWriter::open_file(const char *prefix, uint32_t sec) {
+  const dpsnn_parameters &q = p_->pub;
+  char name[512];
+  if (q.howManyOutputFiles == 1)
+    std::snprintf(name, sizeof name, "%s-sec%u-C%u-H%u-h%03u.dat", prefix, sec, q.globCFT, q.globH, 0u);
+  else if (sec == 0)
+    std::snprintf(name, sizeof name, "%s-start-C%u-H%u-h%u.dat", prefix, q.globCFT, q.globH, 0u);
+  else if (sec % 2 == 0)
+    std::snprintf(name, sizeof name, "%s-lasteven-C%u-H%u-h%u.dat", prefix, q.globCFT, q.globH, 0u);
+  else
+    std::snprintf(name, sizeof name, "%s-lastodd-C%u-H%u-h%u.dat", prefix, q.globCFT, q.globH, 0u);
+  return std::fopen((dir_ + "/" + name).c_str(), "w");
+}
+
+bool StatWriter::step(uint32_t t, const uint32_t *ids, const double *te, uint32_t n,
+                      const uint32_t *popCounts) {
+  const dpsnn_parameters &q = p_->pub;
+  const uint32_t sec = t / 1000;
+  const bool inWindow = window(sec);
+  const bool ratesSec = sec >= q.startRatesPerPop_s && sec <= q.stopRatesPerPop_s;
+  if (t % 1000 == 0 && inWindow && !dir_.empty()) {
+    if (!q.spikes_writeDisable) {
+      if (spikes_) std::fclose(spikes_);
+      if (!(spikes_ = open_file("spikes", sec))) return false;
+    }
+    if (!q.statRatesPerPop_writeDisable && ratesSec) {
+      if (rates_) std::fclose(rates_);
+      if (!(rates_ = open_file("ratesPerPop", sec))) return false;
+    }
+  }
+  if (spikes_ && inWindow) {
+    for (uint32_t k = 0; k < n; ++k) {
+      const int len = std::snprintf(buf_.data(), buf_.size(), "%d %.7g\n", (int)ids[k], te[k]);
+      std::fwrite(buf_.data(), 1, (size_t)len, spikes_);
+    }
+  }
+  for (size_t k = 0; k < perPop_.size(); ++k) perPop_[k] += popCounts[k];
+  if (rates_ && inWindow && ratesSec && t % q.ratesSampling == 0) {
+    std::fprintf(rates_, "%d ", (int)(t - q.ratesSampling));
+    for (uint32_t col = 0; col < q.globCFT; ++col)
+      for (uint32_t s = 0; s < q.subPopNumber; ++s)
+        std::fprintf(rates_, "%.7g ",
+                     ((float)(perPop_[size_t(col) * q.subPopNumber + s]) * 1000.0) /
+                         ((float)(p_->sp[s].count) * (float)(q.ratesSampling)));
+    std::fprintf(rates_, " \n");
+    std::fill(perPop_.begin(), perPop_.end(), 0);
+  }
+  if (t % 1000 == 999 && inWindow) {
+    if (spikes_) { std::fclose(spikes_); spikes_ = nullptr; }
+    if (rates_ && ratesSec) { std::fclose(rates_); rates_ = nullptr; }
+  }
+  return true;
+}
+
+void StatWriter::close() {
+  if (spikes_) std::fclose(spikes_);
+  if (rates_) std::fclose(rates_);
+  spikes_ = rates_ = nullptr;
+}
+
+}  // namespace dpsnn

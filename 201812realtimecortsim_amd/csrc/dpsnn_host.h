@@ -1,0 +1,130 @@
+// dpsnn_host.h -- host side of the engine: DPSNN's parameter/ini interface,
+// its ran3 random stream, the LIFCA connectome generator, the delivery
+// layout the device consumes, the replayed external drive and the
+// raster/rates writers.  Everything here runs once at network creation or
+// streams alongside the device step; nothing here is on the device path.
+#pragma once
+#include <cstdint>
+#include <cstdio>
+#include <string>
+#include <vector>
+
+#include "dpsnn_hip.h"
+
+namespace dpsnn {
+
+constexpr uint32_t kMaxSubPop = 27;      // DSD__subPopTot
+constexpr uint32_t kMaxStencil = 96;     // makeStencil_Max
+constexpr uint32_t kDelaySlots = 80;     // DSD__maxD: stored delays < 80
+constexpr uint32_t kLutDepth = 256;      // ANALOG_DEPTH
+constexpr uint32_t kStdpSpan = 128;      // DSD__maxSTDPspan_ms
+constexpr uint32_t kMaxOutDegree = 7500; // DSD__maxM (miniM)
+
+// One sub-population's parameters (neuSubPopParamStruct,
+// DPSNN_parameters.h:177-194).  Values pass through float on the way in.
+struct SubPop {
+  uint32_t count = 0, offset = 0;
+  double DMin[kMaxSubPop] = {}, DMax[kMaxSubPop] = {}, J[kMaxSubPop] = {},
+         DJ[kMaxSubPop] = {};
+  double JExt = 0, DJExt = 0, NuExt = 0, Tau = 0, Theta = 0, H = 0, Tarp = 0,
+         AlphaC = 0, TauC = 0, gC = 0;
+};
+
+struct Params {
+  dpsnn_parameters pub{};
+  uint32_t maxModDeltaX = 0, maxModDeltaY = 0, stencilX = 0, stencilY = 0;
+  SubPop sp[kMaxSubPop];
+  std::vector<float> prob;        // [src][tgt][x][y] over kMaxStencil^2
+  std::vector<uint32_t> bath;     // [x][y]
+  float prob_at(uint32_t s, uint32_t t, uint32_t x, uint32_t y) const {
+    return prob[((size_t(s) * kMaxSubPop + t) * kMaxStencil + x) * kMaxStencil + y];
+  }
+};
+
+// Reads env vars + ini files exactly as getParameters and
+// describeConnectome do.  Returns false and sets err on a config error.
+bool read_params(const std::string &dir, Params &p, std::string &err);
+
+// ran3 + NormDev with the reference's persistent-state semantics
+// (randdev.c:116-189, :477-502, :584-606), one instance per network.
+class Ran3 {
+ public:
+  double next();
+  void seed(int32_t s);
+  double norm();
+  uint64_t draws() const { return count_; }
+ private:
+  int32_t idum_ = -77531, inext_ = 0, inextp_ = 0, ma_[56] = {}, iff_ = 0;
+  int32_t set_ = 0;
+  double cached_ = 0;
+  uint64_t count_ = 0;
+};
+
+// The connectome in the reference's two orders plus what the device needs.
+struct Connectome {
+  uint32_t N = 0;
+  uint64_t S = 0;
+  // backward (delivery-ordered) list: stable sort of generation order by
+  // delay (inflateIntermToBackwardSynList, DPSNN_localNet_init.c:631-740)
+  std::vector<uint32_t> pre, post;
+  std::vector<uint8_t> delay, kind;
+  std::vector<int16_t> weight;
+  // per-neuron
+  std::vector<uint8_t> subPop;
+  std::vector<double> invNuExt;
+  std::vector<uint32_t> maxExt;
+  std::vector<double> bathLut;    // [subPop][256]
+  // per-target in-lists in delivery insertion order:
+  // (delay desc, pre asc, position in the pre's run asc)
+  std::vector<uint64_t> in_off;   // [N+1]
+  std::vector<uint32_t> in_pre_d; // pre | delay << 24
+  std::vector<int16_t> in_w;
+  std::vector<uint32_t> in_syn;   // backward-list index of the synapse
+  uint32_t max_in = 0;
+  uint32_t max_delay = 0;         // largest stored delay
+  // per-target incoming synapses (backward-list order), for LTP
+  std::vector<uint64_t> inc_off;
+  std::vector<uint32_t> inc_syn;
+};
+
+bool generate_connectome(const Params &p, Ran3 &rng, Connectome &c, std::string &err);
+void build_delivery(const Params &p, Connectome &c);
+
+// Host-side replay of the external Poisson drive (DPSNN_neuron_sim.c:
+// 604-640, :844-865): the reference's ran3 draws in neuron order, step by
+// step, recorded as per-neuron (time, value) lists.
+struct ExternalStep {
+  std::vector<uint32_t> off;   // [N+1]
+  std::vector<double> time, value;
+};
+class ExternalReplay {
+ public:
+  void init(const Connectome &c, Ran3 *rng);
+  void generate(uint32_t t, ExternalStep &out);
+ private:
+  const Connectome *c_ = nullptr;
+  Ran3 *rng_ = nullptr;
+  std::vector<double> assigned_;
+};
+
+// Raster and per-population rate files (DPSNN_stat.c:55-107, :225-296).
+class StatWriter {
+ public:
+  bool init(const Params &p, const std::string &dir);
+  // Called once per step, in order, with that step's compacted spikes and
+  // per-(column, subPop) counts.
+  bool step(uint32_t t, const uint32_t *ids, const double *te, uint32_t n,
+            const uint32_t *popCounts);
+  void close();
+  ~StatWriter() { close(); }
+ private:
+  FILE *open_file(const char *prefix, uint32_t sec);
+  bool window(uint32_t sec) const;
+  const Params *p_ = nullptr;
+  std::string dir_;
+  FILE *spikes_ = nullptr, *rates_ = nullptr;
+  std::vector<uint32_t> perPop_;
+  std::vector<char> buf_;
+};
+
+}  // namespace dpsnn

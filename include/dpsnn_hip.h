@@ -1,0 +1,178 @@
+/* dpsnn_hip.h -- C ABI of the MI355X engine for DPSNN's per-millisecond
+ * LIFCA simulation step (one process's whole local network on one GPU).
+ *
+ * The engine replaces, behind this boundary, the reference's step path:
+ *   localNetClass::completeTimeStep_ms   DPSNN_localNet_sim.c:22-307
+ *   (fired scan, compaction, axonal spike exchange at world size 1, delay
+ *   ring, delivery, per-neuron event ordering, external Poisson drive,
+ *   LIFCA dynamics, STDP, raster and per-population rates)
+ * while keeping its parameter interface (env vars + column.txt,
+ * connectivity.txt, bathEfficacy.txt, plasticity.txt read by
+ *   getParameters                        DPSNN_getParameters.c:23-831)
+ * and its output files (spikes-*.dat, ratesPerPop-*.dat written as
+ *   statSpikesClass::write               DPSNN_stat.c:275-296
+ *   statSpikingRatesPerPopClass::write   DPSNN_stat.c:225-273).
+ *
+ * All entry points return 0 on success and a negative dpsnn_status on
+ * failure (the reference prints and calls exit(0) instead); the message of
+ * the last failure on the calling thread is dpsnn_last_error().
+ * Plain pointers and sizes only: host buffers are caller-owned.
+ */
+#ifndef DPSNN_HIP_H
+#define DPSNN_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum {
+  DPSNN_OK = 0,
+  DPSNN_DONE = 1,            /* dpsnn_step: programmed time reached */
+  DPSNN_ERR_CONFIG = -1,     /* bad env var / ini file (reference: exit(0)) */
+  DPSNN_ERR_DEVICE = -2,     /* HIP runtime error */
+  DPSNN_ERR_MEMORY = -3,     /* host or device allocation failed */
+  DPSNN_ERR_ARG = -4,        /* invalid argument */
+  DPSNN_ERR_OVERFLOW = -5,   /* a per-step bound was exceeded on device */
+  DPSNN_ERR_IO = -6          /* output file could not be written */
+} dpsnn_status;
+
+/* External Poisson drive of each neuron (DPSNN_neuron_sim.c:604-640). */
+typedef enum {
+  DPSNN_DRIVE_REPLAY = 0,    /* the reference's own ran3 stream, generated on
+                                the host in neuron order and replayed on the
+                                device: bit-exact with the reference */
+  DPSNN_DRIVE_PHILOX = 1     /* same rates and amplitudes from an on-device
+                                counter-based Philox4x32-10 stream:
+                                statistically equivalent, not bit-exact */
+} dpsnn_drive;
+
+typedef struct {
+  int device;                /* HIP device ordinal */
+  int drive;                 /* dpsnn_drive */
+  uint64_t philox_seed;      /* seed of the Philox drive */
+  const char *output_dir;    /* where spikes-*.dat / ratesPerPop-*.dat go;
+                                NULL or "" = no files */
+  uint32_t drain_every_ms;   /* raster/rates device->host drain period
+                                (0 = default 100 ms) */
+  int keep_spikes;           /* 1 = keep each step's spike list readable
+                                through dpsnn_last_spikes */
+} dpsnn_options;
+
+/* Parameters as getParameters returns them (the subset this path reads;
+ * field names follow struct DPSNN_parameters, DPSNN_parameters.h:72-146). */
+typedef struct {
+  uint32_t globH, locN, globN, D, totalSimTime_ms, globCFX, globCFY, globCFT,
+      neuronsPerCM, subPopNumber, tabNeuron, moduloSec, ratesSampling,
+      globalSeed, howManyOutputFiles, startPlasticity_ms, stopPlasticity_ms,
+      startStatFiles_s, stopStatFiles_s, startRatesPerPop_s, stopRatesPerPop_s,
+      stencilDim, overallConnectivity, delayMin, delayMax;
+  int32_t spikes_writeDisable, statRatesPerPop_writeDisable;
+  float minSynWeight_f, factorFloat_2_weightType, factorWeightType_2_Float;
+  float STDPmultiplier, antiCausalCoeff, tauSTDP_ms, causalSTDPmaxAbsValue,
+      antiCausalSTDPmaxAbsValue, tauDerivativeDecay_s, derivativeDrift;
+} dpsnn_parameters;
+
+typedef struct dpsnn_net dpsnn_net;
+
+/* Counters of what the device did (cumulative since creation). */
+typedef struct {
+  uint64_t steps;            /* simulated ms */
+  uint64_t spikes;           /* emitted spikes */
+  uint64_t synaptic_events;  /* delivered synaptic events (sum of runs) */
+  uint64_t external_events;  /* external bath events integrated */
+  uint64_t synapses;         /* synapses in the connectome */
+  uint64_t delivery_items;   /* in-list items (synapses + spill-over) */
+  uint32_t neurons;
+  uint32_t max_in_items;     /* longest per-neuron in-list */
+  double init_seconds;       /* connectome generation + upload */
+  uint64_t device_bytes;     /* device memory held by this network */
+} dpsnn_stats;
+
+/* getParameters equivalent: reads the env vars of the calling process and
+ * the ini files in `inidir`.  Replaces DPSNN_getParameters.c:23-831. */
+int dpsnn_get_parameters(const char *inidir, dpsnn_parameters *out);
+
+/* Builds a network: reads parameters, runs the reference connectome
+ * generator on the host (DPSNN_LIFCAcorticalModuleGen.cpp:62-220), builds
+ * the delivery layout (DPSNN_localNet_init.c:631-740) and uploads state and
+ * per-neuron in-lists to the device.  Replaces localNetClass::init
+ * (DPSNN_localNet_init.c:43-307) at world size 1. */
+int dpsnn_create(const char *inidir, const dpsnn_options *opt, dpsnn_net **out);
+
+/* Advances `nsteps` milliseconds (each = one completeTimeStep_ms).  Returns
+ * DPSNN_DONE once totalSimTime_ms has been reached (the reference's fire()
+ * returning 1, localNetProcess.c:138-157).  Work is queued on the
+ * network's stream; files are written at each drain. */
+int dpsnn_step(dpsnn_net *net, uint32_t nsteps);
+/* Waits for queued work, drains raster/rates to files. */
+int dpsnn_sync(dpsnn_net *net);
+uint32_t dpsnn_now_ms(const dpsnn_net *net);
+uint32_t dpsnn_total_ms(const dpsnn_net *net);
+
+/* Device state after the last completed step (neuron order). */
+int dpsnn_get_state(dpsnn_net *net, double *v, double *c, double *Tr,
+                    double *Te, uint8_t *fired);
+/* Spikes compacted by the last step (requires keep_spikes). */
+int dpsnn_last_spikes(dpsnn_net *net, uint32_t *ids, double *te,
+                      uint32_t *count);
+/* The delivery-ordered (backward) synapse list the device holds:
+ * pre, post, stored delay, int16 weight; plus STDP state. */
+uint64_t dpsnn_num_synapses(const dpsnn_net *net);
+int dpsnn_get_synapses(dpsnn_net *net, uint32_t *pre, uint32_t *post,
+                       uint8_t *delay, int16_t *weight, float *tderiv,
+                       int32_t *last_act);
+/* Writes the reference's synapse dump text
+ * (localNet_h0_N<locN>_SynapticList_BACKWARD_R<rep>.dat format,
+ * DPSNN_synapse.h:63-91) of the device's current synapses. */
+int dpsnn_write_synapse_dump(dpsnn_net *net, const char *path);
+int dpsnn_get_stats(dpsnn_net *net, dpsnn_stats *out);
+/* HIP stream the network runs on (hipStream_t as void*). */
+void *dpsnn_stream(dpsnn_net *net);
+void dpsnn_destroy(dpsnn_net *net);
+
+/* Average device time (ms) of the integration kernel over the launches
+ * since the last call, measured with HIP events on the network's stream;
+ * *launches receives the launch count.  Requires dpsnn_profile(net, 1). */
+int dpsnn_profile(dpsnn_net *net, int enable);
+int dpsnn_kernel_time(dpsnn_net *net, int kernel, double *avg_ms,
+                      uint64_t *launches);
+
+/* ---- host-only access (no device needed) ------------------------------
+ * The connectome generator and delivery layout as the device receives
+ * them, and the replayed external drive, for inspection and CPU checks. */
+typedef struct dpsnn_host_net dpsnn_host_net;
+int dpsnn_host_build(const char *inidir, dpsnn_host_net **out);
+uint32_t dpsnn_host_num_neurons(const dpsnn_host_net *h);
+uint64_t dpsnn_host_num_synapses(const dpsnn_host_net *h);
+uint64_t dpsnn_host_num_items(const dpsnn_host_net *h);
+/* backward (delivery-ordered) list */
+int dpsnn_host_get_synapses(const dpsnn_host_net *h, uint32_t *pre,
+                            uint32_t *post, uint8_t *delay, int16_t *weight);
+/* per-target in-lists: off[N+1], items (pre | delay << 24), weights and
+ * backward-list indices, in delivery insertion order */
+int dpsnn_host_get_inlists(const dpsnn_host_net *h, uint64_t *off,
+                           uint32_t *pre_delay, int16_t *weight,
+                           uint32_t *syn);
+/* per-neuron external bath: 1000/(NuExt*CExt) and the per-ms cap */
+int dpsnn_host_get_external(const dpsnn_host_net *h, double *invNuExt,
+                            uint32_t *maxCount);
+int dpsnn_host_write_synapse_dump(const dpsnn_host_net *h, const char *path);
+/* Draws the next step's replayed external events: off[N+1] per neuron,
+ * (time, value) pairs into buffers of `cap` entries; *count = total. */
+int dpsnn_host_external_step(dpsnn_host_net *h, uint32_t *off, double *time,
+                             double *value, uint64_t cap, uint64_t *count);
+void dpsnn_host_destroy(dpsnn_host_net *h);
+
+/* Device exp of x[0..n) into y (diagnostic: the device's glibc-exact exp
+ * checked against the host libm). */
+int dpsnn_device_exp(int device, const double *x, double *y, uint64_t n);
+
+const char *dpsnn_last_error(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
