@@ -432,9 +432,9 @@ struct dpsnn_net {
   uint64_t items = 0;
   // profiling
   bool profile = false;
-  hipEvent_t pe0 = nullptr, pe1 = nullptr;
-  double profMs = 0;
-  uint64_t profN = 0;
+  std::vector<std::pair<hipEvent_t, hipEvent_t>> profEv;  // integrate launches
+  size_t profUsed = 0;
+  bool collect = false;  // raster / rates are being produced
 };
 
 template <class T>
@@ -467,8 +467,11 @@ static void free_all(dpsnn_net *n) {
     if (s.done) hipEventDestroy(s.done);
     s = ExtSlot{};
   }
-  if (n->pe0) hipEventDestroy(n->pe0);
-  if (n->pe1) hipEventDestroy(n->pe1);
+  for (auto &e : n->profEv) {
+    hipEventDestroy(e.first);
+    hipEventDestroy(e.second);
+  }
+  n->profEv.clear();
   if (n->stream) hipStreamDestroy(n->stream);
   n->stream = nullptr;
 }
@@ -571,6 +574,7 @@ static int build_device(dpsnn_net *n) {
   if ((rc = dev_alloc(n, &n->d_popCnt, popPer * n->drainEvery))) return rc;
   HIP_TRY(hipMemsetAsync(n->d_popCnt, 0, sizeof(uint32_t) * popPer * n->drainEvery, n->stream));
   n->wantRaster = (n->writing && !q.spikes_writeDisable) || n->opt.keep_spikes;
+  n->collect = n->writing || n->opt.keep_spikes;
   if (n->wantRaster) {
     n->rasterCap = N * std::min<uint32_t>(n->drainEvery, 1000) + 1;
     if ((rc = dev_alloc(n, &n->d_rasterIds, n->rasterCap)) || (rc = dev_alloc(n, &n->d_rasterTe, n->rasterCap)) ||
@@ -651,26 +655,46 @@ static int drain(dpsnn_net *n) {
   return 0;
 }
 
+static int prof_begin(dpsnn_net *n) {
+  if (!n->profile) return 0;
+  if (n->profUsed == n->profEv.size()) {
+    hipEvent_t a, b;
+    HIP_TRY(hipEventCreate(&a));
+    HIP_TRY(hipEventCreate(&b));
+    n->profEv.emplace_back(a, b);
+  }
+  HIP_TRY(hipEventRecord(n->profEv[n->profUsed].first, n->stream));
+  return 0;
+}
+static int prof_end(dpsnn_net *n) {
+  if (!n->profile) return 0;
+  HIP_TRY(hipEventRecord(n->profEv[n->profUsed].second, n->stream));
+  n->profUsed++;
+  return 0;
+}
+
 static int one_step(dpsnn_net *n) {
   DevNet &P = n->dev;
   const uint32_t t = n->t;
   const uint32_t blocks = (P.N + kBlock - 1) / kBlock;
   const size_t popPer = size_t(P.ncols) * P.nsub;
-  if (n->pendingSteps == 0) n->pendingFrom = t;
-  uint32_t *pop = n->d_popCnt + popPer * n->pendingSteps;
+  int rc;
+  if (n->collect && n->pendingSteps == 0) n->pendingFrom = t;
+  uint32_t *pop = n->collect ? n->d_popCnt + popPer * n->pendingSteps : nullptr;
   hipLaunchKernelGGL(k_scan_fired, dim3(blocks), dim3(kBlock), 0, n->stream, P, t, pop);
   if (n->wantRaster)
     hipLaunchKernelGGL(k_compact, dim3(1), dim3(kCompactThreads), 0, n->stream, P, t, n->d_rasterIds,
                        n->d_rasterTe, n->d_rasterCount, n->d_stepCount + n->pendingSteps);
   ExtView X{nullptr, nullptr, nullptr};
+  ExtSlot *slot = nullptr;
   if (P.drive == DPSNN_DRIVE_REPLAY) {
     ExtSlot &s = n->ext[n->extNext];
+    slot = &s;
     n->extNext = (n->extNext + 1) % kExtRing;
     if (s.pending) HIP_TRY(hipEventSynchronize(s.done));
     n->replay.generate(t, n->extStep);
     const size_t ne = n->extStep.time.size();
-    int rc = ensure_ext_cap(s, ne);
-    if (rc) return rc;
+    if ((rc = ensure_ext_cap(s, ne))) return rc;
     std::memcpy(s.h_off, n->extStep.off.data(), sizeof(uint32_t) * (P.N + 1));
     if (ne) {
       std::memcpy(s.h_time, n->extStep.time.data(), sizeof(double) * ne);
@@ -682,26 +706,18 @@ static int one_step(dpsnn_net *n) {
       HIP_TRY(hipMemcpyAsync(s.d_value, s.h_value, sizeof(double) * ne, hipMemcpyHostToDevice, n->stream));
     }
     X = ExtView{s.d_off, s.d_time, s.d_value};
-    if (n->profile) HIP_TRY(hipEventRecord(n->pe0, n->stream));
-    hipLaunchKernelGGL(k_integrate, dim3(blocks), dim3(kBlock), 0, n->stream, P, t, X);
-    if (n->profile) HIP_TRY(hipEventRecord(n->pe1, n->stream));
-    HIP_TRY(hipEventRecord(s.done, n->stream));
-    s.pending = true;
-  } else {
-    if (n->profile) HIP_TRY(hipEventRecord(n->pe0, n->stream));
-    hipLaunchKernelGGL(k_integrate, dim3(blocks), dim3(kBlock), 0, n->stream, P, t, X);
-    if (n->profile) HIP_TRY(hipEventRecord(n->pe1, n->stream));
+  }
+  if ((rc = prof_begin(n))) return rc;
+  hipLaunchKernelGGL(k_integrate, dim3(blocks), dim3(kBlock), 0, n->stream, P, t, X);
+  if ((rc = prof_end(n))) return rc;
+  if (slot) {
+    HIP_TRY(hipEventRecord(slot->done, n->stream));
+    slot->pending = true;
   }
   HIP_TRY(hipGetLastError());
-  if (n->profile) {
-    HIP_TRY(hipEventSynchronize(n->pe1));
-    float ms = 0;
-    HIP_TRY(hipEventElapsedTime(&ms, n->pe0, n->pe1));
-    n->profMs += ms;
-    n->profN++;
-  }
-  n->pendingSteps++;
   n->t = t + 1;
+  if (!n->collect) return 0;
+  n->pendingSteps++;
   if (n->pendingSteps == n->drainEvery || n->opt.keep_spikes) return drain(n);
   return 0;
 }
@@ -746,8 +762,6 @@ int dpsnn_create(const char *inidir, const dpsnn_options *opt, dpsnn_net **out) 
   if (n->writing) n->writer.init(n->par, n->opt.output_dir);
   HIP_TRY(hipSetDevice(n->opt.device));
   HIP_TRY(hipStreamCreateWithFlags(&n->stream, hipStreamNonBlocking));
-  HIP_TRY(hipEventCreate(&n->pe0));
-  HIP_TRY(hipEventCreate(&n->pe1));
   int rc = build_device(n.get());
   if (rc) {
     free_all(n.get());
@@ -847,19 +861,25 @@ void *dpsnn_stream(dpsnn_net *n) { return n ? (void *)n->stream : nullptr; }
 
 int dpsnn_profile(dpsnn_net *n, int enable) {
   if (!n) return fail(DPSNN_ERR_ARG, "null network");
+  HIP_TRY(hipStreamSynchronize(n->stream));
   n->profile = enable != 0;
-  n->profMs = 0;
-  n->profN = 0;
+  n->profUsed = 0;
   return 0;
 }
 
 int dpsnn_kernel_time(dpsnn_net *n, int kernel, double *avg_ms, uint64_t *launches) {
   if (!n) return fail(DPSNN_ERR_ARG, "null network");
-  (void)kernel;
-  if (avg_ms) *avg_ms = n->profN ? n->profMs / (double)n->profN : 0.0;
-  if (launches) *launches = n->profN;
-  n->profMs = 0;
-  n->profN = 0;
+  if (kernel != 0) return fail(DPSNN_ERR_ARG, "kernel 0 (k_integrate) is the only timed kernel");
+  HIP_TRY(hipStreamSynchronize(n->stream));
+  double total = 0;
+  for (size_t k = 0; k < n->profUsed; ++k) {
+    float ms = 0;
+    HIP_TRY(hipEventElapsedTime(&ms, n->profEv[k].first, n->profEv[k].second));
+    total += ms;
+  }
+  if (avg_ms) *avg_ms = n->profUsed ? total / (double)n->profUsed : 0.0;
+  if (launches) *launches = n->profUsed;
+  n->profUsed = 0;
   return 0;
 }
 

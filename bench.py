@@ -1,0 +1,144 @@
+"""Benchmark of the MI355X engine on DPSNN's per-millisecond LIFCA step.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--config c1]
+
+One "step" = one simulated millisecond of the whole local network on one GPU
+(the reference's completeTimeStep_ms).  Default workload: configs[1], a
+24x24 grid of 1250-neuron columns (720,000 neurons, ~795 M synapses),
+on-device Philox external drive, no raster files; synthetic network from
+the reference generator (no external data).  Prints one JSON line:
+  value         synaptic events delivered per wall second over the K timed
+                steps (inputs resident in HBM; host I/O not in the region)
+  ms_per_step   wall ms per simulated ms = real-time factor x 1000
+  roofline      the integration kernel (k_integrate) against HBM bandwidth:
+                algorithmic bytes = 25 B per synaptic event + 72 B per neuron
+                (+16 B per replayed external event), divided by its average
+                launch time from HIP events on the network's stream
+  cpu_baseline  the oracle (scalar C restatement of DPSNN) on the host, on a
+                bounded sample (configs[0]'s network, a few tens of ms)
+"""
+import argparse
+import ctypes
+import importlib
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+
+
+def cpu_baseline(pkg, sample_ms):
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    from conftest import Oracle
+    from helpers import OracleNet
+    cfg = pkg.config_dir("c0")
+    pkg.apply_env(cfg)
+    ora = OracleNet(Oracle(), cfg)
+    # synaptic events = sum over delivered runs; count them from the spikes
+    # and the out-degree per delay, exactly as delivered
+    import numpy as np
+    b = ora.backward()
+    D = 20
+    deg = np.zeros((ora.n, D), np.int64)
+    np.add.at(deg, (b["pre"], b["delay"]), 1)
+    hist = []
+    events = 0
+    t0 = time.perf_counter()
+    for t in range(sample_ms):
+        ora.fire()
+        ids, _ = ora.last_spikes()
+        hist.append(ids)
+        for d in range(min(D, t + 1)):
+            events += int(deg[hist[t - d], d].sum())
+    el = time.perf_counter() - t0
+    ora.close()
+    return {"value": events / el, "unit": "synaptic events/s", "cores": 1, "kind": "port",
+            "sample": f"oracle (scalar C port of DPSNN) on configs[0]'s 5000-neuron network, "
+                      f"{sample_ms} ms simulated after init, {el:.1f} s wall"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--config", default="c1")
+    ap.add_argument("--drive", default="philox", choices=["philox", "replay"])
+    ap.add_argument("--cpu-sample-ms", type=int, default=40)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py: this engine runs one network per GPU and this build measures one GPU; "
+              "--gpus must be 1 (multi-GPU sharding is a design note, not a runnable mode)",
+              file=sys.stderr)
+        return 2
+    pkg = importlib.import_module("201812realtimecortsim_amd")
+    cfg = pkg.config_dir(args.config)
+    total = args.warmup + args.steps
+    pkg.apply_env(cfg, env_totalSimTime_ms=max(total, 1), env_spikes_writeDisable=1)
+    drive = pkg.DRIVE_PHILOX if args.drive == "philox" else pkg.DRIVE_REPLAY
+    print(f"[bench] building {args.config} ...", file=sys.stderr, flush=True)
+    t0 = time.perf_counter()
+    net = pkg.Network(cfg, drive=drive, philox_seed=20181201)
+    init_s = time.perf_counter() - t0
+    st0 = net.stats()
+    print(f"[bench] init {init_s:.1f} s: {st0['neurons']} neurons, {st0['synapses']} synapses, "
+          f"{st0['device_bytes'] / 1e9:.1f} GB on device", file=sys.stderr, flush=True)
+    net.step(args.warmup)
+    net.sync()
+    a = net.stats()
+    net.profile(True)
+    t0 = time.perf_counter()
+    net.step(args.steps)
+    net.sync()
+    wall = time.perf_counter() - t0
+    b = net.stats()
+    kms, launches = net.kernel_time(0)
+    net.close()
+    events = b["synaptic_events"] - a["synaptic_events"]
+    ext = b["external_events"] - a["external_events"]
+    spikes = b["spikes"] - a["spikes"]
+    neurons = st0["neurons"]
+    per_launch = (25.0 * events + 72.0 * neurons * args.steps +
+                  (16.0 * ext if drive == pkg.DRIVE_REPLAY else 0.0)) / max(args.steps, 1)
+    achieved = per_launch / (kms * 1e-3) / 1e9 if kms > 0 else 0.0
+    line = {
+        "metric": "synaptic events/s + real-time factor on 1 GPU; batched/served vs plain",
+        "value": events / wall,
+        "unit": "synaptic events/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": 1000.0 * wall / args.steps,
+        "real_time_factor": (wall / (args.steps / 1000.0)),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (reference generator, seeded)",
+        "config": {"workload": f"configs[{args.config[1:]}] plain LIFCA step, "
+                               f"{st0['neurons']} neurons, {st0['synapses']} synapses, "
+                               f"{args.drive} external drive",
+                   "neurons": neurons, "synapses": st0["synapses"], "drive": args.drive,
+                   "spikes_per_step": spikes / args.steps,
+                   "mean_rate_hz": spikes / args.steps * 1000.0 / neurons,
+                   "init_seconds": init_s},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "kernel": "k_integrate", "kernel_ms": kms, "launches": launches,
+                     "algorithmic_bytes_per_launch": per_launch},
+        "cpu_baseline": None,
+    }
+    if not args.no_cpu_baseline:
+        print("[bench] cpu baseline ...", file=sys.stderr, flush=True)
+        line["cpu_baseline"] = cpu_baseline(pkg, args.cpu_sample_ms)
+    print(json.dumps(line), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

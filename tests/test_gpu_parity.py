@@ -1,0 +1,136 @@
+"""GPU parity: the HIP engine, through the C ABI, against the reference's
+own outputs (golden fixtures) and against the oracle, bit-exact."""
+import math
+import os
+
+import numpy as np
+import pytest
+
+from helpers import GOLDEN, OracleNet, golden, prefix_sha, sha256_file
+
+pytestmark = pytest.mark.gpu
+
+
+def _tiny_config(tmp, pkg, counts=(50, 50, 25), grid=2, nu_ext=3.0):
+    """configs[0]'s parameters with smaller columns (the stencil keeps the
+    same exp(-d) shape and 0.9 total connection probability)."""
+    src = pkg.config_dir("c0")
+    d = tmp / "tiny"
+    d.mkdir(exist_ok=True)
+    e, e2, i = counts
+    (d / "column.txt").write_text(
+        f"0 {e} 0.5 0.0 {nu_ext} 20 20 15 2 1 500 0.05\n"
+        f"1 {e2} 0.5 0.0 {nu_ext} 20 20 15 2 1 500 0.05\n"
+        f"2 {i} 0.5 0.0 {nu_ext} 10 20 15 2 0 500 0\n")
+    for f in ("connectivity.txt", "bathEfficacy.txt"):
+        (d / f).write_text(open(os.path.join(src, f)).read())
+    pkg.apply_env(src, env_CFX=grid, env_CFY=grid)
+    return str(d)
+
+
+def test_device_exp_matches_host_libm(pkg):
+    rng = np.random.default_rng(7)
+    x = np.concatenate([-rng.random(300000) * 0.01, (rng.random(300000) * 2 - 1) * 700,
+                        [0.0, -0.0, 1e-300, -1e-300, -708.5, 709.7]])
+    y = pkg.device_exp(x)
+    ref = np.array([math.exp(v) for v in x])
+    assert np.array_equal(y.view(np.uint64), ref.view(np.uint64))
+
+
+def _run_c0(pkg, out, ms, **env):
+    cfg = pkg.config_dir("c0")
+    pkg.apply_env(cfg, env_totalSimTime_ms=ms, **env)
+    with pkg.Network(cfg, output_dir=str(out)) as net:
+        while not net.step(50):
+            pass
+        net.sync()
+        return net.stats()
+
+
+def test_c0_raster_first_500ms_identical_to_reference(pkg, tmp_path):
+    g = golden("c0_raster.json")
+    st = _run_c0(pkg, tmp_path, 500)
+    path = tmp_path / "spikes-sec0-C4-H1-h000.dat"
+    with open(path, "rb") as f:
+        nlines = f.read().count(b"\n")
+    assert nlines == g["prefix"]["500"]["lines"]
+    for cp in ("30", "100", "200", "500"):
+        assert prefix_sha(path, g["prefix"][cp]["lines"]) == g["prefix"][cp]["sha256"], cp
+    rates = open(tmp_path / "ratesPerPop-sec0-C4-H1-h000.dat").read().splitlines()
+    ref = open(os.path.join(GOLDEN, "c0_ratesPerPop-sec0.dat")).read().splitlines()
+    assert rates == ref[:100]
+    assert st["spikes"] == g["prefix"]["500"]["lines"]
+
+
+def test_c0_full_second_identical_to_reference(pkg, tmp_path):
+    g = golden("c0_raster.json")
+    st = _run_c0(pkg, tmp_path, 1000)
+    assert sha256_file(tmp_path / "spikes-sec0-C4-H1-h000.dat") == g["sha256"]
+    assert (open(tmp_path / "ratesPerPop-sec0-C4-H1-h000.dat").read() ==
+            open(os.path.join(GOLDEN, "c0_ratesPerPop-sec0.dat")).read())
+    # synaptic events = sum over spikes of the out-degree delivered
+    assert st["steps"] == 1000 and st["spikes"] == g["lines"]
+
+
+def test_c0_with_minimal_env_D_is_identical(pkg, tmp_path):
+    # delays are stored 0..9: env_D = 10 covers them; the result cannot change
+    g = golden("c0_raster.json")
+    _run_c0(pkg, tmp_path, 100, env_D=10)
+    assert prefix_sha(tmp_path / "spikes-sec0-C4-H1-h000.dat", g["prefix"]["100"]["lines"]) == \
+        g["prefix"]["100"]["sha256"]
+
+
+def test_state_trace_bit_exact_vs_oracle(pkg, oracle, tmp_path):
+    """v, c, Tr, Te and fired flags of every neuron after every step."""
+    cfg = _tiny_config(tmp_path, pkg)
+    pkg.apply_env(pkg.config_dir("c0"), env_CFX=2, env_CFY=2, env_totalSimTime_ms=300)
+    ora = OracleNet(oracle, cfg)
+    with pkg.Network(cfg, keep_spikes=True) as net:
+        for step in range(300):
+            ora.fire()
+            net.step(1)
+            a, b = net.state(), ora.state()
+            for k in ("v", "c", "Tr", "Te"):
+                assert np.array_equal(a[k].view(np.uint64), b[k].view(np.uint64)), (step, k)
+            assert np.array_equal(a["fired"], b["fired"]), step
+            ids, te = net.last_spikes()
+            oids, ote = ora.last_spikes()
+            assert np.array_equal(ids, oids) and np.array_equal(te, ote), step
+    ora.close()
+
+
+def test_silent_network_without_drive(pkg, tmp_path):
+    # NuExt = 0: no external events; nothing can ever fire
+    cfg = _tiny_config(tmp_path, pkg, nu_ext=0.0)
+    pkg.apply_env(pkg.config_dir("c0"), env_CFX=2, env_CFY=2, env_totalSimTime_ms=50)
+    with pkg.Network(cfg) as net:
+        net.step(50)
+        st = net.stats()
+        s = net.state()
+    assert st["spikes"] == 0 and st["external_events"] == 0 and st["synaptic_events"] == 0
+    assert np.all(s["v"] == 15.0) and np.all(s["Te"] == -1000.0)
+
+
+def test_philox_drive_statistics(pkg, tmp_path):
+    """The on-device Philox drive has the replayed drive's statistics."""
+    cfg = pkg.config_dir("c0")
+    pkg.apply_env(cfg, env_totalSimTime_ms=400)
+    res = {}
+    for drive in (pkg.DRIVE_REPLAY, pkg.DRIVE_PHILOX):
+        with pkg.Network(cfg, drive=drive, philox_seed=12345) as net:
+            net.step(400)
+            res[drive] = net.stats()
+    a, b = res[pkg.DRIVE_REPLAY], res[pkg.DRIVE_PHILOX]
+    # 2.4 kHz per neuron bath over 400 ms, Poisson counts
+    assert abs(b["external_events"] / a["external_events"] - 1) < 0.01
+    assert abs(b["spikes"] / a["spikes"] - 1) < 0.05
+
+
+def test_unsupported_plasticity_window_is_rejected(pkg):
+    cfg = pkg.config_dir("c2_small")
+    pkg.apply_env(cfg)
+    try:
+        with pytest.raises(pkg.DpsnnError):
+            pkg.Network(cfg)
+    finally:
+        pkg.apply_env(pkg.config_dir("c0"))

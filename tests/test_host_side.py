@@ -1,0 +1,170 @@
+"""The product's host side (generator, delivery layout, replayed drive,
+parameter interface) and the C ABI surface -- no device needed."""
+import ctypes
+import os
+import re
+
+import numpy as np
+import pytest
+
+from helpers import OracleNet, golden, sha256_file
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+@pytest.fixture(scope="module")
+def c0_pair(pkg, oracle):
+    cfg = pkg.config_dir("c0")
+    pkg.apply_env(cfg)
+    host = pkg.HostConnectome(cfg)
+    ora = OracleNet(oracle, cfg)
+    yield host, ora
+    host.close()
+    ora.close()
+
+
+def test_library_exports_every_declared_symbol(pkg):
+    header = open(os.path.join(ROOT, "include", "dpsnn_hip.h")).read()
+    declared = set(re.findall(r"\b(dpsnn_\w+)\s*\(", header))
+    lib = ctypes.CDLL(pkg.LIB_PATH)
+    missing = [name for name in sorted(declared) if not hasattr(lib, name)]
+    assert not missing, missing
+    assert declared == set(pkg.exported_symbols())
+
+
+def test_generator_dump_matches_reference(c0_pair, tmp_path):
+    host, _ = c0_pair
+    g = golden("c0_synapses.json")
+    path = str(tmp_path / "dump.dat")
+    host.write_synapse_dump(path)
+    assert sha256_file(path) == g["sha256"]
+
+
+def test_backward_list_matches_oracle(c0_pair):
+    host, ora = c0_pair
+    a, b = host.synapses(), ora.backward()
+    for k in ("pre", "post", "delay", "weight"):
+        assert np.array_equal(a[k], b[k]), k
+
+
+def _reference_delivery_order(b, D):
+    """Per target, the (delay, pre, synapse) items in the order the
+    reference's delivery loop (DPSNN_localNet_sim.c:858-979) appends them
+    when every source fires: delays D-1..0, sources ascending, each run from
+    offsetByDelay continuing while the source id matches."""
+    pre, post, delay = b["pre"], b["post"], b["delay"]
+    S = len(pre)
+    items = []
+    bstart = np.searchsorted(delay, np.arange(81))
+    for d in range(D - 1, -1, -1):
+        s, end = bstart[d], bstart[d + 1]
+        while s < end:
+            src = pre[s]
+            e = s
+            while e < S and pre[e] == src:
+                items.append((post[e], d, src, e))
+                e += 1
+            s = e if e < end else end
+    items.sort(key=lambda x: x[0])  # stable: keeps the append order per target
+    return items
+
+
+def test_inlists_follow_reference_delivery_order(c0_pair, pkg):
+    host, ora = c0_pair
+    b = ora.backward()
+    D = pkg.get_parameters(pkg.config_dir("c0"))["D"]
+    want = _reference_delivery_order(b, D)
+    il = host.inlists()
+    off = il["off"]
+    assert int(off[-1]) == len(want)
+    tgt = np.repeat(np.arange(host.n), np.diff(off).astype(np.int64))
+    got_t = tgt
+    want = np.array(want, dtype=np.int64)
+    assert np.array_equal(got_t, want[:, 0])
+    assert np.array_equal(il["delay"].astype(np.int64), want[:, 1])
+    assert np.array_equal(il["pre"].astype(np.int64), want[:, 2])
+    assert np.array_equal(il["syn"].astype(np.int64), want[:, 3])
+    assert np.array_equal(il["weight"], b["weight"][il["syn"]])
+
+
+def test_external_parameters_match_oracle(c0_pair):
+    host, ora = c0_pair
+    a_inv, a_max = host.external_params()
+    b_inv, b_max = ora.external_params()
+    assert np.array_equal(a_inv, b_inv) and np.array_equal(a_max, b_max)
+    # 1000 / (NuExt * CExt) = 1000 / (3 * 800) ms; cap = ceil(20 / invNuExt)
+    assert np.all(a_inv == 1000.0 / float(np.float32(3.0) * np.float32(800.0))) and np.all(a_max == 48)
+
+
+def test_replayed_external_stream_matches_oracle(pkg, oracle):
+    cfg = pkg.config_dir("c0")
+    pkg.apply_env(cfg)
+    host = pkg.HostConnectome(cfg)
+    ora = OracleNet(oracle, cfg)
+    ora.record_external(True)
+    for _ in range(8):
+        off, t, v = host.external_step()
+        ora.fire()
+        cnt, ot, ov = ora.last_external()
+        assert np.array_equal(np.diff(off.astype(np.int64)), cnt.astype(np.int64))
+        assert np.array_equal(t, ot) and np.array_equal(v, ov)
+    host.close()
+    ora.close()
+
+
+@pytest.mark.parametrize("var", ["env_globalSeed", "env_D", "env_CFX", "env_minSynWeight_f",
+                                 "env_totalSimTime_ms", "env_spikes_writeDisable"])
+def test_missing_env_var_is_a_config_error(pkg, var):
+    cfg = pkg.config_dir("c0")
+    pkg.apply_env(cfg)
+    old = os.environ.pop(var)
+    try:
+        with pytest.raises(pkg.DpsnnError) as e:
+            pkg.get_parameters(cfg)
+        assert e.value.code == -1 and var in str(e.value)
+    finally:
+        os.environ[var] = old
+
+
+def test_bad_ini_files_are_config_errors(pkg, tmp_path):
+    cfg = pkg.config_dir("c0")
+    pkg.apply_env(cfg)
+    for name in ("column.txt", "connectivity.txt", "bathEfficacy.txt"):
+        d = tmp_path / name.split(".")[0]
+        d.mkdir()
+        for f in ("column.txt", "connectivity.txt", "bathEfficacy.txt"):
+            if f != name:
+                (d / f).write_text(open(os.path.join(cfg, f)).read())
+        with pytest.raises(pkg.DpsnnError):
+            pkg.HostConnectome(str(d))
+    # a stencil row of the wrong length
+    d = tmp_path / "badconn"
+    d.mkdir()
+    for f in ("column.txt", "bathEfficacy.txt"):
+        (d / f).write_text(open(os.path.join(cfg, f)).read())
+    (d / "connectivity.txt").write_text("0 0 0.6 0.1 1 10 0.5 0.2\n")
+    with pytest.raises(pkg.DpsnnError) as e:
+        pkg.HostConnectome(str(d))
+    assert "stencil" in str(e.value)
+
+
+def test_world_size_above_one_is_rejected(pkg):
+    cfg = pkg.config_dir("c0")
+    pkg.apply_env(cfg, env_globH=2)
+    try:
+        with pytest.raises(pkg.DpsnnError) as e:
+            pkg.get_parameters(cfg)
+        assert "env_globH" in str(e.value)
+    finally:
+        pkg.apply_env(cfg)
+
+
+def test_parameters_mirror_get_parameters(pkg):
+    cfg = pkg.config_dir("c0")
+    pkg.apply_env(cfg)
+    p = pkg.get_parameters(cfg)
+    assert p["globN"] == 5000 and p["neuronsPerCM"] == 1250 and p["subPopNumber"] == 3
+    assert p["tabNeuron"] == 1250 and p["D"] == 20 and p["delayMin"] == 1 and p["delayMax"] == 10
+    # factors of DPSNN_getParameters.c:378-379, computed in float
+    assert p["factorFloat_2_weightType"] == np.float32(-32767) / np.float32(-6.0)
+    assert p["factorWeightType_2_Float"] == np.float32(-6.0) / np.float32(-32767)

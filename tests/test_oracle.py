@@ -1,0 +1,125 @@
+"""The oracle (CPU restatement, oracle/dpsnn_oracle.c) against the
+reference's own outputs and against what the mathematics fixes."""
+import math
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+from helpers import GOLDEN, OracleNet, golden, prefix_sha, sha256_file
+
+
+@pytest.fixture()
+def c0(pkg, oracle):
+    cfg = pkg.config_dir("c0")
+    pkg.apply_env(cfg)
+    net = OracleNet(oracle, cfg)
+    yield net
+    net.close()
+
+
+def test_synapse_dump_matches_reference(c0, tmp_path):
+    g = golden("c0_synapses.json")
+    assert c0.s == g["lines"]
+    path = str(tmp_path / "dump.dat")
+    c0.write_synapse_dump(path)
+    assert sha256_file(path) == g["sha256"]
+
+
+def test_delay_histogram_and_order(c0):
+    g = golden("c0_synapses.json")
+    b = c0.backward()
+    counts = np.bincount(b["delay"], minlength=10)
+    assert {str(d): int(c) for d, c in enumerate(counts) if c} == g["per_delay"]
+    # delivery order: delay-major, then pre ascending within each delay bucket
+    key = b["delay"].astype(np.int64) * (1 << 32) + b["pre"]
+    assert np.all(np.diff(key) >= 0)
+
+
+def test_raster_first_30ms_matches_reference(c0, tmp_path):
+    g = golden("c0_raster.json")
+    c0.set_output_dir(str(tmp_path))
+    for _ in range(30):
+        c0.fire()
+    c0.close()  # flushes the files
+    path = tmp_path / "spikes-sec0-C4-H1-h000.dat"
+    assert prefix_sha(path, g["prefix"]["30"]["lines"]) == g["prefix"]["30"]["sha256"]
+    with open(path) as f:
+        assert sum(1 for _ in f) == g["prefix"]["30"]["lines"]
+    rates = open(tmp_path / "ratesPerPop-sec0-C4-H1-h000.dat").read().splitlines()
+    ref = open(os.path.join(GOLDEN, "c0_ratesPerPop-sec0.dat")).read().splitlines()
+    assert rates == ref[:len(rates)] and len(rates) == 6  # written at t = 0, 5, ..., 25
+
+
+def _ran3_stream(seed, n):
+    """Knuth's subtractive generator (NR ran3) as published, for the check
+    below; independent of the C code."""
+    MBIG, MSEED = 1000000000, 161803398
+    ma = [0] * 56
+    mj = abs(MSEED - abs(seed)) % MBIG
+    ma[55] = mj
+    mk = 1
+    for i in range(1, 55):
+        ii = (21 * i) % 55
+        ma[ii] = mk
+        mk = mj - mk
+        if mk < 0:
+            mk += MBIG
+        mj = ma[ii]
+    for _ in range(4):
+        for i in range(1, 56):
+            ma[i] -= ma[1 + (i + 30) % 55]
+            if ma[i] < 0:
+                ma[i] += MBIG
+    inext, inextp = 0, 31
+    out = []
+    for _ in range(n):
+        inext = 1 if inext + 1 == 56 else inext + 1
+        inextp = 1 if inextp + 1 == 56 else inextp + 1
+        mj = ma[inext] - ma[inextp]
+        if mj < 0:
+            mj += MBIG
+        ma[inext] = mj
+        out.append(mj)
+    return out
+
+
+def test_ran3_lagged_fibonacci_invariant():
+    # the subtractive generator satisfies x[n] = x[n-55] - x[n-24] (mod 1e9)
+    xs = _ran3_stream(-1, 2000)
+    for n in range(55, 2000):
+        assert xs[n] == (xs[n - 55] - xs[n - 24]) % 1000000000
+
+
+def test_stdp_tables_closed_form(pkg):
+    # causal[i] = A+ m (1-1/tau)^i for i >= 1, causal[0] = A+ m; anti-causal alike
+    # (DPSNN_neuron_sim.c:74-91); checked on the float recurrence bounds
+    Ap, Am, m, tau = 0.1, 0.12, 1.2, 20.0
+    c = [np.float32(Ap) * np.float32(m)]
+    for i in range(1, 128):
+        c.append(np.float32(np.float32(1) - np.float32(1) / np.float32(tau)) * c[-1])
+    for i in range(1, 128):
+        assert abs(float(c[i]) - Ap * m * (1 - 1 / tau) ** i) <= 1e-6 * Ap * m
+
+
+def test_device_exp_source_matches_host_libm(tmp_path):
+    """dpsnn_exp.h compiled for the host against glibc exp: bit-exact."""
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    src = tmp_path / "t.c"
+    src.write_text(r'''
+#include <stdio.h>
+#include <stdlib.h>
+#include "dpsnn_exp.h"
+int main(void){ long bad=0,n=0; srand48(3);
+  for(long i=0;i<3000000;i++){ double x=(i%3==0)? -drand48()*0.01 : (drand48()*2-1)*760;
+    double a=exp(x), b=dpsnn_exp(x); n++; if (dpsnn_asu64(a)!=dpsnn_asu64(b)) bad++; }
+  double sp[]={0.0,-0.0,1e-300,-1e-300,709.78,709.79,-745.1,-745.2,-708.5,1.0/0.0,-1.0/0.0};
+  for(int i=0;i<11;i++) if (dpsnn_asu64(exp(sp[i]))!=dpsnn_asu64(dpsnn_exp(sp[i]))) bad++;
+  printf("%ld %ld\n",bad,n); return bad!=0; }''')
+    exe = tmp_path / "t"
+    subprocess.run(["gcc", "-O2", "-ffp-contract=off", "-I",
+                    os.path.join(root, "201812realtimecortsim_amd", "csrc"), str(src), "-o", str(exe), "-lm"],
+                   check=True)
+    out = subprocess.run([str(exe)], capture_output=True, text=True)
+    assert out.returncode == 0, out.stdout
