@@ -10,12 +10,13 @@
 //                  insert (:103-148, :620-806, DPSNN_spike.c:209-239).
 //   k_compact      (only when a raster is wanted) ascending spike list of
 //                  slot t appended to the device raster buffer.
-//   k_integrate    one lane per target neuron: walks the neuron's in-list
-//                  (delivery insertion order), keeps the items whose source
-//                  fired in step t-delay, orders them by arrival time with
-//                  a stable per-lane bucket sort, merges the external bath
-//                  events (internal first on ties) and runs LIFCADynamic
-//                  per event in FP64.  Replaces delivery + addInputSpike +
+//   k_step         per target neuron: streams the neuron's in-list (delivery
+//                  insertion order), keeps the items whose source fired in
+//                  step t-delay, orders them by arrival time (wave-wide
+//                  bucket sort in LDS on exact 64-bit keys), merges the
+//                  external bath (internal first on ties) and runs
+//                  LIFCADynamic per event in FP64, one lane per neuron.
+//                  Replaces delivery + addInputSpike +
 //                  regularDynamicTimeStep + LIFCADynamic (:837-999,
 //                  DPSNN_neuron_sim.c:289-300, :522-824).
 //
@@ -46,6 +47,7 @@ constexpr int kBlock = 256;      // integrate / scan block (4 waves)
 constexpr int kBuckets = 64;     // arrival-time buckets per lane
 constexpr uint32_t kPad = 0xFFFFFFFFu;
 constexpr int kExtRing = 8;      // replayed external steps in flight
+constexpr uint32_t kMaxInList = 3900;  // per-neuron in-list (12-bit key field, LDS)
 
 static thread_local std::string g_err;
 static int fail(int code, const char *fmt, ...) {
@@ -81,12 +83,12 @@ struct DevNet {
   uint8_t *fired;
   uint64_t *bits;          // [RS][NW]
   double *teHist;          // [RS][N]
-  const uint64_t *grpOff;  // [G+1] item offset of each 64-neuron group
-  const uint32_t *grpRows; // [G]
-  const uint32_t *inPD;    // pre | delay << 24, kPad = none
+  const uint64_t *inOff;   // [N+1] start of each neuron's in-list
+  const uint32_t *inPD;    // pre | delay << 24, in delivery insertion order
   const int16_t *inW;
-  double *Ak, *Bk;         // per-lane scratch [N][cap]
-  float *Aw, *Bw;
+  uint64_t *keys;          // [grid][tile][cap] sorted event keys
+  uint32_t *keyCount;      // [grid][tile]
+  uint32_t tile;           // neurons per wave
   unsigned long long *counters;  // [0] events, [1] external, [2] spikes
 };
 
@@ -229,153 +231,222 @@ __global__ __launch_bounds__(kCompactThreads) void k_compact(DevNet P, uint32_t 
 }
 
 // --------------------------------------------------------------------------
-// k_integrate: one lane per target neuron.
+// k_step: the delivery + ordering + dynamics of one step.
+//
+// Event key.  An event of step t arrives at k = t + frac(Te_pre) (the
+// reference's double, DPSNN_neuron_sim.c:297, :579) and the reference's
+// stable merge sort orders equal k by insertion order, which is the order
+// of the neuron's in-list.  For t >= 1 every k lies in [t, 2t], so
+// tick = bits(k) - bits(t) is exact and order-preserving (< 2^52), and
+//     key = tick << 12 | position-in-in-list
+// is unique and sorts exactly as (k, insertion order).  The key also
+// recovers k (bits(t) + tick) and the synapse (position).
+//
+// Phase 1, per neuron of the wave's tile, all 64 lanes: stream the in-list
+// (coalesced), keep live items (source fired in step t - delay: one bit of
+// the spike history, then its Te), compact the keys into LDS in insertion
+// order, bucket them by the top 8 bits of frac (LDS counters), sort each
+// bucket by insertion, and store the sorted keys to the wave's scratch.
+// Phase 2, one lane per neuron of the tile: merge the sorted internal
+// events with the external bath (internal first on equal time) and run
+// LIFCADynamic for each, in FP64.
 // --------------------------------------------------------------------------
 struct ExtView {  // replayed external events of this step
   const uint32_t *off;
   const double *time, *value;
 };
 
-__global__ __launch_bounds__(kBlock) void k_integrate(DevNet P, uint32_t t, ExtView X) {
-  __shared__ uint16_t bucket[kBuckets * kBlock];
-  const uint32_t j = blockIdx.x * kBlock + threadIdx.x;
-  if (j >= P.N) return;
+constexpr int kWave = 64;
+constexpr int kKeyBuckets = 256;
+
+__device__ inline uint64_t lane_mask_lt() {
+  const uint32_t lane = threadIdx.x & 63;
+  return lane ? (~0ull >> (64 - lane)) : 0ull;
+}
+
+__global__ __launch_bounds__(kWave) void k_step(DevNet P, uint32_t t, ExtView X) {
+  extern __shared__ uint64_t lds[];
+  uint64_t *live = lds;                  // [cap] keys in insertion order
+  uint64_t *sorted = lds + P.cap;        // [cap]
+  uint32_t *bcnt = (uint32_t *)(lds + 2 * (size_t)P.cap);  // [kKeyBuckets + 1]
+  const uint32_t lane = threadIdx.x;
   const double tF = (double)t, nextF = (double)(tF + 1.0);
-  const uint32_t tid = threadIdx.x;
-  for (int b = 0; b < kBuckets; ++b) bucket[b * kBlock + tid] = 0;
+  const uint64_t tBits = dpsnn_asu64(tF);
+  // bucket = top 8 bits of frac: tick >> (52 - e - 8), e = exponent of t
+  const int e = t ? (31 - __clz((int)t)) : 0;
+  const int shift = 44 - e;
+  const uint32_t tiles = (P.N + P.tile - 1) / P.tile;
+  uint64_t *wkeys = P.keys + (size_t)blockIdx.x * P.tile * P.cap;
+  uint32_t *wcnt = P.keyCount + (size_t)blockIdx.x * P.tile;
+  unsigned long long evSum = 0, exSum = 0;
 
-  // gather: live items of this lane's in-list, in insertion order
-  const uint32_t g = j >> 6, l = j & 63;
-  const uint64_t base = P.grpOff[g];
-  const uint32_t rows = P.grpRows[g];
-  double *Ak = P.Ak + (size_t)j * P.cap;
-  float *Aw = P.Aw + (size_t)j * P.cap;
-  uint32_t n = 0;
-  for (uint32_t r = 0; r < rows; ++r) {
-    const uint64_t at = base + (uint64_t)r * 64 + l;
-    const uint32_t it = P.inPD[at];
-    if (it == kPad) break;
-    const uint32_t d = it >> 24, pre = it & 0xFFFFFFu;
-    if (d > t) continue;  // that step has not happened
-    const uint32_t s = (t - d) % P.RS;
-    const uint64_t word = P.bits[(size_t)s * P.NW + (pre >> 6)];
-    if ((word >> (pre & 63)) & 1ull) {
-      const double te = P.teHist[(size_t)s * P.N + pre];
-      const double frac = te - trunc(te);  // modf, exact
-      const double k = tF + frac;
-      Ak[n] = k;
-      Aw[n] = ((float)P.inW[at]) * P.w2f;
-      int b = (int)((k - tF) * (double)kBuckets);
-      b = b < 0 ? 0 : (b >= kBuckets ? kBuckets - 1 : b);
-      bucket[b * kBlock + tid]++;
-      ++n;
-    }
-  }
-  // stable bucket sort A -> B, then stable insertion sort inside buckets
-  double *Bk = P.Bk + (size_t)j * P.cap;
-  float *Bw = P.Bw + (size_t)j * P.cap;
-  {
-    uint32_t run = 0;
-    for (int b = 0; b < kBuckets; ++b) {
-      const uint32_t c = bucket[b * kBlock + tid];
-      bucket[b * kBlock + tid] = (uint16_t)run;
-      run += c;
-    }
-  }
-  for (uint32_t e = 0; e < n; ++e) {
-    const double k = Ak[e];
-    int b = (int)((k - tF) * (double)kBuckets);
-    b = b < 0 ? 0 : (b >= kBuckets ? kBuckets - 1 : b);
-    const uint32_t pos = bucket[b * kBlock + tid]++;
-    Bk[pos] = k;
-    Bw[pos] = Aw[e];
-  }
-  {
-    uint32_t lo = 0;
-    for (int b = 0; b < kBuckets; ++b) {
-      const uint32_t hi = bucket[b * kBlock + tid];
-      for (uint32_t e = lo + 1; e < hi; ++e) {
-        const double k = Bk[e];
-        const float w = Bw[e];
-        uint32_t q = e;
-        while (q > lo && Bk[q - 1] > k) {
-          Bk[q] = Bk[q - 1];
-          Bw[q] = Bw[q - 1];
-          --q;
+  for (uint32_t tile = blockIdx.x; tile < tiles; tile += gridDim.x) {
+    const uint32_t j0 = tile * P.tile;
+    // ---------------- phase 1 ----------------
+    for (uint32_t q = 0; q < P.tile; ++q) {
+      const uint32_t j = j0 + q;
+      if (j >= P.N) break;
+      const uint64_t beg = P.inOff[j];
+      const uint32_t len = (uint32_t)(P.inOff[j + 1] - beg);
+      for (uint32_t b = lane; b <= kKeyBuckets; b += kWave) bcnt[b] = 0;
+      __syncthreads();
+      uint32_t n = 0;
+      if (t > 0) {
+        for (uint32_t c0 = 0; c0 < len; c0 += kWave) {
+          const uint32_t idx = c0 + lane;
+          bool on = false;
+          uint64_t key = 0;
+          if (idx < len) {
+            const uint32_t it = P.inPD[beg + idx];
+            const uint32_t d = it >> 24, pre = it & 0xFFFFFFu;
+            if (d <= t) {
+              const uint32_t s = (t - d) % P.RS;
+              const uint64_t word = P.bits[(size_t)s * P.NW + (pre >> 6)];
+              if ((word >> (pre & 63)) & 1ull) {
+                const double te = P.teHist[(size_t)s * P.N + pre];
+                const double k = tF + (te - trunc(te));  // modf, exact
+                const uint64_t tick = dpsnn_asu64(k) - tBits;
+                key = (tick << 12) | idx;
+                on = true;
+              }
+            }
+          }
+          const uint64_t m = __ballot(on);
+          if (on) {
+            live[n + __popcll(m & lane_mask_lt())] = key;
+            uint32_t b = (uint32_t)((key >> 12) >> shift);
+            b = b < kKeyBuckets ? b : kKeyBuckets - 1;
+            atomicAdd(&bcnt[b], 1u);
+          }
+          n += (uint32_t)__popcll(m);
         }
-        Bk[q] = k;
-        Bw[q] = w;
       }
-      lo = hi;
+      __syncthreads();
+      // exclusive scan of the bucket counters (4 per lane)
+      {
+        uint32_t c[4], sum = 0;
+        for (int r = 0; r < 4; ++r) { c[r] = bcnt[lane * 4 + r]; sum += c[r]; }
+        uint32_t incl = sum;
+        for (int o = 1; o < kWave; o <<= 1) {
+          const uint32_t v = __shfl_up(incl, o);
+          if (lane >= (uint32_t)o) incl += v;
+        }
+        uint32_t run = incl - sum;
+        __syncthreads();
+        for (int r = 0; r < 4; ++r) { bcnt[lane * 4 + r] = run; run += c[r]; }
+        if (lane == kWave - 1) bcnt[kKeyBuckets] = run;
+      }
+      __syncthreads();
+      // scatter into buckets (order inside a bucket is fixed below)
+      for (uint32_t i = lane; i < n; i += kWave) {
+        const uint64_t key = live[i];
+        uint32_t b = (uint32_t)((key >> 12) >> shift);
+        b = b < kKeyBuckets ? b : kKeyBuckets - 1;
+        sorted[atomicAdd(&bcnt[b], 1u)] = key;
+      }
+      __syncthreads();
+      // bcnt[b] now holds the end of bucket b; sort buckets by insertion
+      for (int r = 0; r < 4; ++r) {
+        const uint32_t b = lane * 4 + r;
+        const uint32_t hi = bcnt[b], lo = b ? bcnt[b - 1] : 0;
+        for (uint32_t x = lo + 1; x < hi; ++x) {
+          const uint64_t key = sorted[x];
+          uint32_t y = x;
+          while (y > lo && sorted[y - 1] > key) {
+            sorted[y] = sorted[y - 1];
+            --y;
+          }
+          sorted[y] = key;
+        }
+      }
+      __syncthreads();
+      uint64_t *out = wkeys + (size_t)q * P.cap;
+      for (uint32_t i = lane; i < n; i += kWave) out[i] = sorted[i];
+      if (lane == 0) wcnt[q] = n;
+      evSum += n;
+      __syncthreads();
     }
-  }
-
-  // merge with the external bath and integrate
-  const uint32_t sp = P.subPop[j];
-  const double *prm = P.sp + (size_t)sp * kParams;
-  Lifca L;
-  L.v = P.v[j]; L.c = P.c[j]; L.Tr = P.Tr[j]; L.Te = P.Te[j];
-  L.Tau = prm[kTau]; L.Theta = prm[kTheta]; L.H = prm[kH]; L.Tarp = prm[kTarp];
-  L.AlphaC = prm[kAlphaC]; L.TauC = prm[kTauC]; L.gC = prm[kGC];
-  L.fired = false;
-  uint32_t nExt = 0;
-  if (P.drive == DPSNN_DRIVE_REPLAY) {
-    uint32_t e = X.off[j];
-    const uint32_t ee = X.off[j + 1];
-    nExt = ee - e;
-    uint32_t ii = 0;
-    while (ii < n || e < ee) {
-      if (ii < n && (e >= ee || Bk[ii] <= X.time[e])) {
-        L.event(Bk[ii], (double)Bw[ii]);
-        ++ii;
+    __threadfence_block();
+    __syncthreads();
+    // ---------------- phase 2 ----------------
+    const uint32_t j = j0 + lane;
+    if (lane < P.tile && j < P.N) {
+      const uint32_t n = wcnt[lane];
+      const uint64_t *ks = wkeys + (size_t)lane * P.cap;
+      const uint64_t beg = P.inOff[j];
+      const uint32_t sp = P.subPop[j];
+      const double *prm = P.sp + (size_t)sp * kParams;
+      Lifca L;
+      L.v = P.v[j]; L.c = P.c[j]; L.Tr = P.Tr[j]; L.Te = P.Te[j];
+      L.Tau = prm[kTau]; L.Theta = prm[kTheta]; L.H = prm[kH]; L.Tarp = prm[kTarp];
+      L.AlphaC = prm[kAlphaC]; L.TauC = prm[kTauC]; L.gC = prm[kGC];
+      L.fired = false;
+      uint32_t nExt = 0, ii = 0;
+      double kI = 0;
+      float wI = 0;
+      auto load_internal = [&](uint32_t i) {
+        const uint64_t key = ks[i];
+        kI = dpsnn_asf64(tBits + (key >> 12));
+        wI = ((float)P.inW[beg + (key & 0xFFFu)]) * P.w2f;
+      };
+      if (n) load_internal(0);
+      if (P.drive == DPSNN_DRIVE_REPLAY) {
+        uint32_t x = X.off[j];
+        const uint32_t xe = X.off[j + 1];
+        nExt = xe - x;
+        while (ii < n || x < xe) {
+          if (ii < n && (x >= xe || kI <= X.time[x])) {
+            L.event(kI, (double)wI);
+            if (++ii < n) load_internal(ii);
+          } else {
+            L.event(X.time[x], X.value[x]);
+            ++x;
+          }
+        }
       } else {
-        L.event(X.time[e], X.value[e]);
-        ++e;
+        const double inv = P.invNuExt[j];
+        const uint32_t cap = P.maxExt[j];
+        const double *lut = P.lut + (size_t)sp * kLutDepth;
+        double a = P.assigned[j];
+        PhiloxStream R{(uint32_t)P.philoxSeed, (uint32_t)(P.philoxSeed >> 32) ^ j, P.philoxCtr[j], {0, 0}, 0};
+        const bool onE = inv > 0;
+        for (;;) {
+          // the reference's do-while: the first due event is always drawn,
+          // the following ones while fewer than the per-ms cap
+          const bool hasE = onE && a < nextF && (nExt == 0 || nExt < cap);
+          if (ii < n && (!hasE || kI <= a)) {
+            L.event(kI, (double)wI);
+            if (++ii < n) load_internal(ii);
+          } else if (hasE) {
+            const double val = lut[(uint32_t)(R.next() * kLutDepth)];
+            L.event(a, val);
+            ++nExt;
+            const double u = 1.0 - R.next();
+            a += -logf((float)u) * inv;
+          } else {
+            break;
+          }
+        }
+        if (onE && nExt >= cap && a < nextF) {
+          do {
+            const double u = 1.0 - R.next();
+            a += -logf((float)u) * inv;
+          } while (a < nextF);
+        }
+        P.assigned[j] = a;
+        P.philoxCtr[j] = R.ctr;
       }
+      P.v[j] = L.v; P.c[j] = L.c; P.Tr[j] = L.Tr; P.Te[j] = L.Te;
+      P.fired[j] = L.fired ? 1 : 0;
+      exSum += nExt;
     }
-  } else {
-    const double inv = P.invNuExt[j];
-    const uint32_t cap = P.maxExt[j];
-    const double *lut = P.lut + (size_t)sp * kLutDepth;
-    double a = P.assigned[j];
-    PhiloxStream R{(uint32_t)P.philoxSeed, (uint32_t)(P.philoxSeed >> 32) ^ j, P.philoxCtr[j], {0, 0}, 0};
-    const bool on = inv > 0;
-    uint32_t ii = 0;
-    for (;;) {
-      const bool hasE = on && a < nextF && nExt < cap;
-      if (ii < n && (!hasE || Bk[ii] <= a)) {
-        L.event(Bk[ii], (double)Bw[ii]);
-        ++ii;
-      } else if (hasE) {
-        const double val = lut[(uint32_t)(R.next() * kLutDepth)];
-        L.event(a, val);
-        ++nExt;
-        double u = 1.0 - R.next();
-        a += -logf((float)u) * inv;
-      } else {
-        break;
-      }
-    }
-    if (on && nExt >= cap && a < nextF) {
-      do {
-        double u = 1.0 - R.next();
-        a += -logf((float)u) * inv;
-      } while (a < nextF);
-    }
-    P.assigned[j] = a;
-    P.philoxCtr[j] = R.ctr;
+    __syncthreads();
   }
-  P.v[j] = L.v; P.c[j] = L.c; P.Tr[j] = L.Tr; P.Te[j] = L.Te;
-  P.fired[j] = L.fired ? 1 : 0;
-  // wave-aggregated counters
-  unsigned long long ev = n, ex = nExt;
-  for (int o = 32; o > 0; o >>= 1) {
-    ev += __shfl_down(ev, o);
-    ex += __shfl_down(ex, o);
-  }
-  if ((threadIdx.x & 63) == 0) {
-    atomicAdd(&P.counters[0], ev);
-    atomicAdd(&P.counters[1], ex);
+  // wave-aggregated counters (evSum is uniform; exSum per lane)
+  for (int o = 32; o > 0; o >>= 1) exSum += __shfl_down(exSum, o);
+  if (lane == 0) {
+    atomicAdd(&P.counters[0], evSum);
+    atomicAdd(&P.counters[1], exSum);
   }
 }
 
@@ -430,6 +501,8 @@ struct dpsnn_net {
   uint32_t lastCount = 0;
   double initSeconds = 0;
   uint64_t items = 0;
+  uint32_t grid = 0;
+  size_t ldsBytes = 0;
   // profiling
   bool profile = false;
   std::vector<std::pair<hipEvent_t, hipEvent_t>> profEv;  // integrate launches
@@ -526,45 +599,40 @@ static int build_device(dpsnn_net *n) {
   HIP_TRY(hipMemsetAsync(P.bits, 0, sizeof(uint64_t) * size_t(P.RS) * P.NW, n->stream));
   if ((rc = dev_alloc(n, &P.teHist, size_t(P.RS) * N))) return rc;
   HIP_TRY(hipMemsetAsync(P.teHist, 0, sizeof(double) * size_t(P.RS) * N, n->stream));
-  // in-lists in 64-neuron interleaved groups
-  const uint32_t G = (N + 63) / 64;
-  std::vector<uint64_t> grpOff(G + 1, 0);
-  std::vector<uint32_t> grpRows(G, 0);
-  for (uint32_t g = 0; g < G; ++g) {
-    uint32_t rows = 0;
-    for (uint32_t l = 0; l < 64 && g * 64 + l < N; ++l) {
-      const uint32_t j = g * 64 + l;
-      rows = std::max<uint32_t>(rows, (uint32_t)(c.in_off[j + 1] - c.in_off[j]));
-    }
-    grpRows[g] = rows;
-    grpOff[g + 1] = grpOff[g] + uint64_t(rows) * 64;
-  }
-  const uint64_t items = grpOff[G];
+  // in-lists (CSR in delivery insertion order)
+  if (c.max_in >= kMaxInList)
+    return fail(DPSNN_ERR_CONFIG, "a neuron has %u in-list items; this build handles up to %u", c.max_in,
+                kMaxInList - 1);
   n->items = c.in_off[N];
   {
-    std::vector<uint32_t> pd(items, kPad);
-    std::vector<int16_t> w(items, 0);
-    for (uint32_t j = 0; j < N; ++j) {
-      const uint32_t g = j >> 6, l = j & 63;
-      for (uint64_t k = c.in_off[j]; k < c.in_off[j + 1]; ++k) {
-        const uint64_t at = grpOff[g] + (k - c.in_off[j]) * 64 + l;
-        pd[at] = c.in_pre_d[k];
-        w[at] = c.in_w[k];
-      }
-    }
-    uint64_t *d_go;
-    uint32_t *d_gr, *d_pd;
+    uint64_t *d_off;
+    uint32_t *d_pd;
     int16_t *d_w;
-    if ((rc = dev_alloc(n, &d_go, G + 1)) || (rc = upload(n, d_go, grpOff.data(), G + 1))) return rc;
-    if ((rc = dev_alloc(n, &d_gr, G)) || (rc = upload(n, d_gr, grpRows.data(), G))) return rc;
-    if ((rc = dev_alloc(n, &d_pd, items)) || (rc = upload(n, d_pd, pd.data(), items))) return rc;
-    if ((rc = dev_alloc(n, &d_w, items)) || (rc = upload(n, d_w, w.data(), items))) return rc;
-    P.grpOff = d_go; P.grpRows = d_gr; P.inPD = d_pd; P.inW = d_w;
-    HIP_TRY(hipStreamSynchronize(n->stream));  // host staging freed below
+    const uint64_t items = c.in_off[N];
+    if ((rc = dev_alloc(n, &d_off, N + 1)) || (rc = upload(n, d_off, c.in_off.data(), N + 1))) return rc;
+    if ((rc = dev_alloc(n, &d_pd, items)) || (rc = upload(n, d_pd, c.in_pre_d.data(), items))) return rc;
+    if ((rc = dev_alloc(n, &d_w, items)) || (rc = upload(n, d_w, c.in_w.data(), items))) return rc;
+    P.inOff = d_off; P.inPD = d_pd; P.inW = d_w;
+    HIP_TRY(hipStreamSynchronize(n->stream));  // host staging is dropped after create
   }
-  // per-lane scratch
-  if ((rc = dev_alloc(n, &P.Ak, size_t(N) * P.cap)) || (rc = dev_alloc(n, &P.Bk, size_t(N) * P.cap)) ||
-      (rc = dev_alloc(n, &P.Aw, size_t(N) * P.cap)) || (rc = dev_alloc(n, &P.Bw, size_t(N) * P.cap)))
+  // tiling: neurons per wave and the persistent grid
+  int cus = 0;
+  HIP_TRY(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, n->opt.device));
+  uint32_t tile = 64;
+  if (const char *e = std::getenv("DPSNN_TILE")) tile = (uint32_t)std::atoi(e);
+  else {
+    const uint64_t perWave = uint64_t(N) / (uint64_t(std::max(cus, 1)) * 16);
+    tile = 1;
+    while (tile * 2 <= perWave && tile < 64) tile *= 2;
+  }
+  if (tile < 1 || tile > 64) return fail(DPSNN_ERR_ARG, "DPSNN_TILE must be in 1..64");
+  P.tile = tile;
+  const uint32_t tiles = (N + tile - 1) / tile;
+  n->ldsBytes = (size_t)2 * P.cap * sizeof(uint64_t) + (kKeyBuckets + 1) * sizeof(uint32_t);
+  const uint32_t wavesPerCu = std::max<uint32_t>(1, std::min<uint32_t>(16, (uint32_t)((160u << 10) / n->ldsBytes)));
+  n->grid = std::min<uint32_t>(tiles, (uint32_t)std::max(cus, 1) * wavesPerCu);
+  if ((rc = dev_alloc(n, &P.keys, size_t(n->grid) * tile * P.cap)) ||
+      (rc = dev_alloc(n, &P.keyCount, size_t(n->grid) * tile)))
     return rc;
   if ((rc = dev_alloc(n, &P.counters, 4))) return rc;
   HIP_TRY(hipMemsetAsync(P.counters, 0, 4 * sizeof(unsigned long long), n->stream));
@@ -708,7 +776,7 @@ static int one_step(dpsnn_net *n) {
     X = ExtView{s.d_off, s.d_time, s.d_value};
   }
   if ((rc = prof_begin(n))) return rc;
-  hipLaunchKernelGGL(k_integrate, dim3(blocks), dim3(kBlock), 0, n->stream, P, t, X);
+  hipLaunchKernelGGL(k_step, dim3(n->grid), dim3(kWave), n->ldsBytes, n->stream, P, t, X);
   if ((rc = prof_end(n))) return rc;
   if (slot) {
     HIP_TRY(hipEventRecord(slot->done, n->stream));
@@ -755,7 +823,7 @@ int dpsnn_create(const char *inidir, const dpsnn_options *opt, dpsnn_net **out) 
   // generate, then continue the stream from SetRandomSeed(loc_h=0) (:283)
   n->rng.seed((int32_t)q.globalSeed);
   if (!generate_connectome(n->par, n->rng, n->con, err)) return fail(DPSNN_ERR_CONFIG, "%s", err.c_str());
-  build_delivery(n->par, n->con);
+  build_delivery(n->par, n->con, /*with_incoming=*/false);
   n->rng.seed(0);
   if (n->opt.drive == DPSNN_DRIVE_REPLAY) n->replay.init(n->con, &n->rng);
   n->writing = n->opt.output_dir && n->opt.output_dir[0];
@@ -869,7 +937,7 @@ int dpsnn_profile(dpsnn_net *n, int enable) {
 
 int dpsnn_kernel_time(dpsnn_net *n, int kernel, double *avg_ms, uint64_t *launches) {
   if (!n) return fail(DPSNN_ERR_ARG, "null network");
-  if (kernel != 0) return fail(DPSNN_ERR_ARG, "kernel 0 (k_integrate) is the only timed kernel");
+  if (kernel != 0) return fail(DPSNN_ERR_ARG, "kernel 0 (k_step) is the only timed kernel");
   HIP_TRY(hipStreamSynchronize(n->stream));
   double total = 0;
   for (size_t k = 0; k < n->profUsed; ++k) {
@@ -935,8 +1003,15 @@ int dpsnn_host_build(const char *inidir, dpsnn_host_net **out) {
   std::string err;
   if (!read_params(inidir, h->par, err)) return fail(DPSNN_ERR_CONFIG, "%s", err.c_str());
   h->rng.seed((int32_t)h->par.pub.globalSeed);
+  const bool verbose = std::getenv("DPSNN_VERBOSE") != nullptr;
+  auto t0 = std::chrono::steady_clock::now();
   if (!generate_connectome(h->par, h->rng, h->con, err)) return fail(DPSNN_ERR_CONFIG, "%s", err.c_str());
+  auto t1 = std::chrono::steady_clock::now();
   build_delivery(h->par, h->con);
+  auto t2 = std::chrono::steady_clock::now();
+  if (verbose)
+    std::fprintf(stderr, "[dpsnn] generate %.2f s, delivery layout %.2f s\n",
+                 std::chrono::duration<double>(t1 - t0).count(), std::chrono::duration<double>(t2 - t1).count());
   h->rng.seed(0);
   h->replay.init(h->con, &h->rng);
   *out = h.release();

@@ -9,6 +9,8 @@
 #include "dpsnn_host.h"
 
 #include <algorithm>
+#include <atomic>
+#include <thread>
 #include <cerrno>
 #include <cmath>
 #include <cstdarg>
@@ -440,33 +442,42 @@ uint32_t uniform_delay(const Params &p, Ran3 &rng, uint32_t s, uint32_t t) {
 
 }  // namespace
 
-bool generate_connectome(const Params &p, Ran3 &rng, Connectome &c, std::string &err) {
+// Synapses of one column in generation order, with what the column-parallel
+// resolution needs: the NormDev state at exit and, when the column was
+// generated assuming a cached Gaussian on entry, which weight consumed it.
+struct ColumnOut {
+  std::vector<uint32_t> count;  // synapses per neuron of the column
+  std::vector<uint32_t> post;
+  std::vector<uint8_t> delay;
+  std::vector<int16_t> w;
+  int32_t exitSet = 0;
+  double exitCached = 0;
+  bool usedNorm = false;        // any NormDev call in the column
+  int64_t patchAt = -1;         // synapse whose weight used the entry cache
+  float patchMean = 0, patchSigma = 0;
+  bool ok = true;
+  std::string err;
+  Ran3 exitRng;                 // the stream where the column ended
+};
+
+namespace {
+
+// Generates the neurons of column `col` from `rng` (already positioned at the
+// column's first draw).  If `placeholderEntry`, the first NormDev value is
+// the cached entry value, unknown yet: it is assumed to pass the weight
+// bounds (it does unless |x| >= 1/DJ) and the weight is patched later.
+void generate_column(const Params &p, uint32_t col, Ran3 &rng, bool placeholderEntry, Connectome &c,
+                     ColumnOut &out, bool writeNeuronInfo) {
   const dpsnn_parameters &q = p.pub;
-  const uint32_t N = q.globN, npc = q.neuronsPerCM;
-  c.N = N;
-  c.subPop.assign(N, 0);
-  c.invNuExt.assign(N, 0.0);
-  c.maxExt.assign(N, 0);
-  c.bathLut.assign(size_t(q.subPopNumber) * kLutDepth, 0.0);
-  for (uint32_t s = 0; s < q.subPopNumber; ++s) {  // initLUT, bath part
-    const float J = (float)p.sp[s].JExt, DJ = (float)p.sp[s].DJExt;
-    gaussian_lut(&c.bathLut[size_t(s) * kLutDepth], kLutDepth, J, J * DJ, 0.0, 2.0 * J);
-  }
-  // generation-order arrays (pre is implicit: neuron i's synapses are
-  // contiguous); delay, post and weight per synapse
-  std::vector<uint64_t> gen_off(N + 1, 0);
-  std::vector<uint32_t> g_post;
-  std::vector<uint8_t> g_delay;
-  std::vector<int16_t> g_w;
-  const size_t guess = size_t(N) * 1200;
-  g_post.reserve(guess); g_delay.reserve(guess); g_w.reserve(guess);
+  const uint32_t npc = q.neuronsPerCM, cfx = col % q.globCFX, cfy = col / q.globCFX;
   std::vector<Target> targets;
   targets.reserve(kMaxOutDegree);
-  for (uint32_t i = 0; i < N; ++i) {
-    const uint32_t col = i / npc, cfx = col % q.globCFX, cfy = col / q.globCFX, inCM = i % npc;
+  out.count.assign(npc, 0);
+  bool pendingPlaceholder = placeholderEntry;
+  for (uint32_t inCM = 0; inCM < npc; ++inCM) {
+    const uint32_t i = col * npc + inCM;
     const uint32_t src = sub_pop_in_column(p, inCM);
-    c.subPop[i] = (uint8_t)src;
-    if (i % q.tabNeuron == 0) rng.seed((int32_t)(i / q.tabNeuron));
+    if (writeNeuronInfo) c.subPop[i] = (uint8_t)src;
     targets.clear();
     for (uint32_t ts = 0; ts < q.subPopNumber; ++ts)
       for (uint32_t x = 0; x < p.stencilX; ++x)
@@ -483,8 +494,9 @@ bool generate_connectome(const Params &p, Ran3 &rng, Connectome &c, std::string 
           const float prob = p.prob_at(src, ts, x, y);
           if (!(prob > 0)) continue;
           if (!(prob > 0.00001)) {
-            err = "connection probabilities in (0, 1e-5] take a branch the reference marks as incorrect; not supported";
-            return false;
+            out.ok = false;
+            out.err = "connection probabilities in (0, 1e-5] take a branch the reference marks as incorrect; not supported";
+            return;
           }
           const uint32_t first = p.sp[ts].offset, end = first + p.sp[ts].count;
           const uint32_t colBase = (uint32_t)tx * npc + (uint32_t)ty * npc * q.globCFX;
@@ -492,8 +504,9 @@ bool generate_connectome(const Params &p, Ran3 &rng, Connectome &c, std::string 
           const uint32_t span = end - cur;
           while ((cur += geometric_gap(rng, prob, span)) < end) {
             if (targets.size() + 1 >= kMaxOutDegree) {
-              err = "a neuron would get 7500 or more synapses (DSD__maxM)";
-              return false;
+              out.ok = false;
+              out.err = "a neuron would get 7500 or more synapses (DSD__maxM)";
+              return;
             }
             targets.push_back({colBase + cur, (uint8_t)sub_pop_in_column(p, cur)});
           }
@@ -503,25 +516,37 @@ bool generate_connectome(const Params &p, Ran3 &rng, Connectome &c, std::string 
       const float mean = (float)p.sp[src].J[tg.sub];
       const float sigma = (float)(p.sp[src].DJ[tg.sub] * mean);
       float w;
-      if (mean > 0.0) {
+      if (pendingPlaceholder) {
+        // consumes the cached entry value: no draw; weight patched later
+        pendingPlaceholder = false;
+        rng.set_norm_state(0, 0.0);
+        out.patchAt = (int64_t)out.post.size();
+        out.patchMean = mean;
+        out.patchSigma = sigma;
+        w = mean;
+      } else if (mean > 0.0) {
         do { w = (float)(mean + rng.norm() * sigma); } while (w < 0.0 || w >= 2.0 * mean);
       } else {
         do { w = (float)(mean + rng.norm() * sigma); } while (w <= 2.0 * mean || w > 0);
       }
+      out.usedNorm = true;
       if (w < q.minSynWeight_f || w > -q.minSynWeight_f) {
-        err = "synaptic weight outside [env_minSynWeight_f, -env_minSynWeight_f]";
-        return false;
+        out.ok = false;
+        out.err = "synaptic weight outside [env_minSynWeight_f, -env_minSynWeight_f]";
+        return;
       }
       if (d >= kDelaySlots || d >= q.D) {
-        err = "a stored delay is not below env_D";
-        return false;
+        out.ok = false;
+        out.err = "a stored delay is not below env_D";
+        return;
       }
-      g_post.push_back(tg.id);
-      g_delay.push_back((uint8_t)d);
-      g_w.push_back((int16_t)(w * q.factorFloat_2_weightType));
+      out.post.push_back(tg.id);
+      out.delay.push_back((uint8_t)d);
+      out.w.push_back((int16_t)(w * q.factorFloat_2_weightType));
+      ++out.count[inCM];
     }
-    gen_off[i + 1] = g_post.size();
-    // external bath of this neuron
+    if (!writeNeuronInfo) continue;
+    // external bath of this neuron (generateSourceNeuCExt's mirrored index)
     const float nu = (float)p.sp[src].NuExt;
     uint32_t bx = cfx, by = cfy;
     if (bx >= q.globCFX / 2) bx = -1 * (bx - q.globCFX + 1);
@@ -532,26 +557,167 @@ bool generate_connectome(const Params &p, Ran3 &rng, Connectome &c, std::string 
     if (cext != 0) {
       c.invNuExt[i] = 1000.0 / (nu * cext);
       c.maxExt[i] = (uint32_t)std::ceil(20 / c.invNuExt[i]);
+    } else {
+      c.invNuExt[i] = 0;
+      c.maxExt[i] = 0;
     }
   }
-  // inflate: stable counting sort by delay into the backward list
+  out.exitSet = rng.norm_set();
+  out.exitCached = rng.norm_cached();
+  out.exitRng = rng;
+}
+
+}  // namespace
+
+bool generate_connectome(const Params &p, Ran3 &rng, Connectome &c, std::string &err, unsigned threads) {
+  const dpsnn_parameters &q = p.pub;
+  const uint32_t N = q.globN, npc = q.neuronsPerCM, cols = q.globCFT;
+  c.N = N;
+  c.subPop.assign(N, 0);
+  c.invNuExt.assign(N, 0.0);
+  c.maxExt.assign(N, 0);
+  c.bathLut.assign(size_t(q.subPopNumber) * kLutDepth, 0.0);
+  for (uint32_t s = 0; s < q.subPopNumber; ++s) {  // initLUT, bath part
+    const float J = (float)p.sp[s].JExt, DJ = (float)p.sp[s].DJExt;
+    gaussian_lut(&c.bathLut[size_t(s) * kLutDepth], kLutDepth, J, J * DJ, 0.0, 2.0 * J);
+  }
+  if (q.tabNeuron != npc) {
+    err = "reseeding period (tabNeuron) differs from the column size";
+    return false;
+  }
+  if (threads == 0) threads = std::max(1u, std::thread::hardware_concurrency());
+  // column 0 continues the stream seeded with env_globalSeed: SetRandomSeed(0)
+  // only advances it (DPSNN_LIFCAcorticalModuleGen.cpp:96-107, randdev.c:477-489)
+  std::vector<uint64_t> gen_off(N + 1, 0);
+  std::vector<uint32_t> g_post;
+  std::vector<uint8_t> g_delay;
+  std::vector<int16_t> g_w;
+  const size_t guess = size_t(N) * 1200;
+  g_post.reserve(guess); g_delay.reserve(guess); g_w.reserve(guess);
+  auto append = [&](uint32_t col, const ColumnOut &o) {
+    g_post.insert(g_post.end(), o.post.begin(), o.post.end());
+    g_delay.insert(g_delay.end(), o.delay.begin(), o.delay.end());
+    g_w.insert(g_w.end(), o.w.begin(), o.w.end());
+    for (uint32_t k = 0; k < npc; ++k) {
+      const uint32_t i = col * npc + k;
+      gen_off[i + 1] = gen_off[i] + o.count[k];
+    }
+  };
+  {
+    ColumnOut o;
+    rng.seed(0);
+    generate_column(p, 0, rng, false, c, o, true);
+    if (!o.ok) { err = o.err; return false; }
+    append(0, o);
+  }
+  int32_t set = rng.norm_set();
+  double cached = rng.norm_cached();
+  // columns 1..: each restarts ran3 from its own index; two variants per
+  // column (entry cache empty / full), chosen and patched in column order
+  const uint32_t batch = std::max<uint32_t>(threads, 8);
+  for (uint32_t c0 = 1; c0 < cols; c0 += batch) {
+    const uint32_t c1 = std::min(cols, c0 + batch);
+    const uint32_t nb = c1 - c0;
+    std::vector<ColumnOut> outA(nb), outB(nb);
+    std::vector<std::thread> pool;
+    std::atomic<uint32_t> nextJob{0};
+    auto worker = [&]() {
+      for (;;) {
+        const uint32_t job = nextJob.fetch_add(1);
+        if (job >= 2 * nb) return;
+        const uint32_t k = job / 2, col = c0 + k;
+        const bool variantB = job & 1;
+        Ran3 r;
+        r.seed((int32_t)col);
+        r.set_norm_state(0, 0.0);
+        generate_column(p, col, r, variantB, c, variantB ? outB[k] : outA[k], !variantB);
+      }
+    };
+    for (unsigned th = 0; th < std::min<unsigned>(threads, 2 * nb); ++th) pool.emplace_back(worker);
+    for (auto &th : pool) th.join();
+    for (uint32_t k = 0; k < nb; ++k) {
+      const uint32_t col = c0 + k;
+      ColumnOut *o = set ? &outB[k] : &outA[k];
+      if (set && o->ok) {
+        if (!o->usedNorm) {
+          // nothing drew a Gaussian: the cache passes through unchanged
+          o->exitSet = 1;
+          o->exitCached = cached;
+        } else {
+          const float w = (float)(o->patchMean + cached * o->patchSigma);
+          const bool inRange = o->patchMean > 0.0 ? !(w < 0.0 || w >= 2.0 * o->patchMean)
+                                                  : !(w <= 2.0 * o->patchMean || w > 0);
+          if (inRange && !(w < q.minSynWeight_f || w > -q.minSynWeight_f)) {
+            o->w[o->patchAt] = (int16_t)(w * q.factorFloat_2_weightType);
+          } else {
+            // the cached value is rejected by the weight bounds: regenerate
+            // the column sequentially from the true entry state
+            ColumnOut redo;
+            Ran3 r;
+            r.seed((int32_t)col);
+            r.set_norm_state(1, cached);
+            generate_column(p, col, r, false, c, redo, false);
+            outB[k] = std::move(redo);
+            o = &outB[k];
+          }
+        }
+      }
+      if (!o->ok) { err = o->err; return false; }
+      append(col, *o);
+      set = o->exitSet;
+      cached = o->exitCached;
+      if (col == cols - 1) {
+        // the simulation stream continues from where the last column ended
+        rng = o->exitRng;
+        rng.set_norm_state(set, cached);
+      }
+    }
+  }
+  // inflate: stable counting sort by delay into the backward list,
+  // DPSNN_localNet_init.c:651-680 (neuron chunks in parallel, each writing
+  // its own sub-range of every delay bucket, so the order stays stable)
   const uint64_t S = g_post.size();
   c.S = S;
-  uint64_t bucket[kDelaySlots + 1] = {};
-  for (uint64_t s = 0; s < S; ++s) ++bucket[g_delay[s] + 1];
-  for (uint32_t d = 0; d < kDelaySlots; ++d) bucket[d + 1] += bucket[d];
   c.pre.resize(S); c.post.resize(S); c.delay.resize(S); c.kind.resize(S); c.weight.resize(S);
+  const unsigned chunks = std::max(1u, std::min<unsigned>(threads, (N + 4095) / 4096));
+  std::vector<uint64_t> cnt(size_t(chunks) * kDelaySlots, 0);
+  auto chunk_lo = [&](unsigned k) { return (uint32_t)((uint64_t)N * k / chunks); };
+  {
+    std::vector<std::thread> pool;
+    for (unsigned k = 0; k < chunks; ++k)
+      pool.emplace_back([&, k]() {
+        uint64_t *cc = &cnt[size_t(k) * kDelaySlots];
+        for (uint64_t s = gen_off[chunk_lo(k)]; s < gen_off[chunk_lo(k + 1)]; ++s) ++cc[g_delay[s]];
+      });
+    for (auto &t : pool) t.join();
+  }
+  // bucket-major, chunk-minor offsets
+  uint64_t run = 0;
   uint32_t maxd = 0;
-  for (uint32_t i = 0; i < N; ++i)
-    for (uint64_t s = gen_off[i]; s < gen_off[i + 1]; ++s) {
-      const uint64_t b = bucket[g_delay[s]]++;
-      c.pre[b] = i;
-      c.post[b] = g_post[s];
-      c.delay[b] = g_delay[s];
-      c.kind[b] = c.subPop[i];
-      c.weight[b] = g_w[s];
-      maxd = std::max<uint32_t>(maxd, g_delay[s]);
+  for (uint32_t d = 0; d < kDelaySlots; ++d)
+    for (unsigned k = 0; k < chunks; ++k) {
+      const uint64_t v = cnt[size_t(k) * kDelaySlots + d];
+      if (v) maxd = d;
+      cnt[size_t(k) * kDelaySlots + d] = run;
+      run += v;
     }
+  {
+    std::vector<std::thread> pool;
+    for (unsigned k = 0; k < chunks; ++k)
+      pool.emplace_back([&, k]() {
+        uint64_t *pos = &cnt[size_t(k) * kDelaySlots];
+        for (uint32_t i = chunk_lo(k); i < chunk_lo(k + 1); ++i)
+          for (uint64_t s = gen_off[i]; s < gen_off[i + 1]; ++s) {
+            const uint64_t b = pos[g_delay[s]]++;
+            c.pre[b] = i;
+            c.post[b] = g_post[s];
+            c.delay[b] = g_delay[s];
+            c.kind[b] = c.subPop[i];
+            c.weight[b] = g_w[s];
+          }
+      });
+    for (auto &t : pool) t.join();
+  }
   c.max_delay = maxd;
   return true;
 }
@@ -568,48 +734,60 @@ bool generate_connectome(const Params &p, Ran3 &rng, Connectome &c, std::string 
 // gets the list of (delay of the run, source, synapse) in exactly that
 // order; at step t an item is live when its source fired in step t-delay.
 // --------------------------------------------------------------------------
-void build_delivery(const Params &p, Connectome &c) {
-  const uint32_t N = c.N, D = p.pub.D;
+void build_delivery(const Params &p, Connectome &c, bool with_incoming, unsigned threads) {
+  const uint32_t N = c.N, D = std::min<uint32_t>(p.pub.D, kDelaySlots);
   const uint64_t S = c.S;
-  // bucket boundaries
+  if (threads == 0) threads = std::max(1u, std::thread::hardware_concurrency());
+  threads = std::max(1u, std::min<unsigned>(threads, (N + 1023) / 1024));
+  // bucket boundaries of the backward list
   std::vector<uint64_t> bstart(kDelaySlots + 1, 0);
   for (uint64_t s = 0; s < S; ++s) ++bstart[c.delay[s] + 1];
   for (uint32_t d = 0; d < kDelaySlots; ++d) bstart[d + 1] += bstart[d];
-  // pass 1: count items per target; pass 2: fill in delivery order
-  std::vector<uint64_t> cnt(N + 1, 0);
-  for (int pass = 0; pass < 2; ++pass) {
-    if (pass == 1) {
-      c.in_off.assign(N + 1, 0);
-      for (uint32_t j = 0; j < N; ++j) c.in_off[j + 1] = c.in_off[j] + cnt[j];
-      const uint64_t items = c.in_off[N];
-      c.in_pre_d.resize(items); c.in_w.resize(items); c.in_syn.resize(items);
-      std::fill(cnt.begin(), cnt.end(), 0);
-    }
-    for (uint32_t d = std::min<uint32_t>(D, kDelaySlots); d-- > 0;) {
+  // Visits every delivered (run delay, source, synapse) in the reference's
+  // order, calling f for those whose target lies in [lo, hi).
+  auto visit = [&](uint32_t lo, uint32_t hi, auto &&f) {
+    for (uint32_t d = D; d-- > 0;) {
       uint64_t s = bstart[d];
       while (s < bstart[d + 1]) {
         const uint32_t src = c.pre[s];
         uint64_t e = s;
         while (e < S && c.pre[e] == src) {  // the run, possibly spilling over
           const uint32_t tgt = c.post[e];
-          if (pass == 0) {
-            ++cnt[tgt];
-          } else {
-            const uint64_t at = c.in_off[tgt] + cnt[tgt]++;
-            c.in_pre_d[at] = src | (d << 24);
-            c.in_w[at] = c.weight[e];
-            c.in_syn[at] = (uint32_t)e;
-          }
+          if (tgt >= lo && tgt < hi) f(d, src, e, tgt);
           ++e;
         }
-        // next source of this bucket
         s = e < bstart[d + 1] ? e : bstart[d + 1];
-        while (s < bstart[d + 1] && c.pre[s] == src) ++s;
       }
     }
-  }
+  };
+  std::vector<uint64_t> cnt(N + 1, 0);
+  auto parallel = [&](auto &&body) {
+    std::vector<std::thread> pool;
+    for (unsigned th = 0; th < threads; ++th) {
+      const uint32_t lo = (uint32_t)((uint64_t)N * th / threads), hi = (uint32_t)((uint64_t)N * (th + 1) / threads);
+      pool.emplace_back([&, lo, hi]() { body(lo, hi); });
+    }
+    for (auto &t : pool) t.join();
+  };
+  parallel([&](uint32_t lo, uint32_t hi) {
+    visit(lo, hi, [&](uint32_t, uint32_t, uint64_t, uint32_t tgt) { ++cnt[tgt]; });
+  });
+  c.in_off.assign(N + 1, 0);
+  for (uint32_t j = 0; j < N; ++j) c.in_off[j + 1] = c.in_off[j] + cnt[j];
   c.max_in = 0;
   for (uint32_t j = 0; j < N; ++j) c.max_in = std::max<uint32_t>(c.max_in, (uint32_t)cnt[j]);
+  const uint64_t items = c.in_off[N];
+  c.in_pre_d.resize(items); c.in_w.resize(items); c.in_syn.resize(items);
+  std::fill(cnt.begin(), cnt.end(), 0);
+  parallel([&](uint32_t lo, uint32_t hi) {
+    visit(lo, hi, [&](uint32_t d, uint32_t src, uint64_t e, uint32_t tgt) {
+      const uint64_t at = c.in_off[tgt] + cnt[tgt]++;
+      c.in_pre_d[at] = src | (d << 24);
+      c.in_w[at] = c.weight[e];
+      c.in_syn[at] = (uint32_t)e;
+    });
+  });
+  if (!with_incoming) return;
   // incoming synapses per target in backward-list order (LTP list,
   // DPSNN_localNet_init.c:225-237)
   c.inc_off.assign(N + 1, 0);

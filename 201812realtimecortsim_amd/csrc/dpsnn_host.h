@@ -53,6 +53,10 @@ class Ran3 {
   void seed(int32_t s);
   double norm();
   uint64_t draws() const { return count_; }
+  // NormDev's cached second value (set, value)
+  int32_t norm_set() const { return set_; }
+  double norm_cached() const { return cached_; }
+  void set_norm_state(int32_t set, double cached) { set_ = set; cached_ = cached; }
  private:
   int32_t idum_ = -77531, inext_ = 0, inextp_ = 0, ma_[56] = {}, iff_ = 0;
   int32_t set_ = 0;
@@ -87,8 +91,11 @@ struct Connectome {
   std::vector<uint32_t> inc_syn;
 };
 
-bool generate_connectome(const Params &p, Ran3 &rng, Connectome &c, std::string &err);
-void build_delivery(const Params &p, Connectome &c);
+// threads = 0: all hardware threads.
+bool generate_connectome(const Params &p, Ran3 &rng, Connectome &c, std::string &err,
+                         unsigned threads = 0);
+// with_incoming: also build the per-target incoming synapse lists (STDP).
+void build_delivery(const Params &p, Connectome &c, bool with_incoming = true, unsigned threads = 0);
 
 // Host-side replay of the external Poisson drive (DPSNN_neuron_sim.c:
 // 604-640, :844-865): the reference's ran3 draws in neuron order, step by

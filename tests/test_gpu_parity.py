@@ -99,16 +99,24 @@ def test_state_trace_bit_exact_vs_oracle(pkg, oracle, tmp_path):
     ora.close()
 
 
-def test_silent_network_without_drive(pkg, tmp_path):
-    # NuExt = 0: no external events; nothing can ever fire
+def test_zero_bath_rate_matches_oracle(pkg, oracle, tmp_path):
+    """NuExt = 0 makes invNuExt infinite and the per-ms cap 0; the
+    reference's do-while still emits one bath event per neuron at t = 0
+    (DPSNN_neuron_sim.c:606-622) and never another."""
     cfg = _tiny_config(tmp_path, pkg, nu_ext=0.0)
     pkg.apply_env(pkg.config_dir("c0"), env_CFX=2, env_CFY=2, env_totalSimTime_ms=50)
+    ora = OracleNet(oracle, cfg)
     with pkg.Network(cfg) as net:
         net.step(50)
         st = net.stats()
         s = net.state()
-    assert st["spikes"] == 0 and st["external_events"] == 0 and st["synaptic_events"] == 0
-    assert np.all(s["v"] == 15.0) and np.all(s["Te"] == -1000.0)
+    for _ in range(50):
+        ora.fire()
+    o = ora.state()
+    ora.close()
+    assert st["external_events"] == st["neurons"] and st["spikes"] == 0
+    for k in ("v", "c", "Tr", "Te"):
+        assert np.array_equal(s[k].view(np.uint64), o[k].view(np.uint64)), k
 
 
 def test_philox_drive_statistics(pkg, tmp_path):
