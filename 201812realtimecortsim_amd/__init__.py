@@ -59,7 +59,9 @@ class Parameters(ctypes.Structure):
     _fields_ = ([(n, ctypes.c_uint32) for n in _PARAM_U32] +
                 [("spikes_writeDisable", ctypes.c_int32),
                  ("statRatesPerPop_writeDisable", ctypes.c_int32)] +
-                [(n, ctypes.c_float) for n in _PARAM_F32])
+                [(n, ctypes.c_float) for n in _PARAM_F32] +
+                [("statSynPeriodicProbe_writeDisable", ctypes.c_int32),
+                 ("startSynPeriodicProbe_s", ctypes.c_uint32), ("stopSynPeriodicProbe_s", ctypes.c_uint32)])
 
     def as_dict(self):
         return {name: getattr(self, name) for name, _ in self._fields_}

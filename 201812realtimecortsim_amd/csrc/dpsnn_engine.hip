@@ -66,6 +66,9 @@ static int fail(int code, const char *fmt, ...) {
       return fail(DPSNN_ERR_DEVICE, "%s failed: %s", #expr, hipGetErrorString(e_)); \
   } while (0)
 
+constexpr int kWave = 64;
+constexpr int kWaveLTP = 256;    // STDP kernels: 4 waves per block
+
 // Per-sub-population LIFCA parameters, in this order.
 enum { kTau, kTheta, kH, kTarp, kAlphaC, kTauC, kGC, kParams = 8 };
 
@@ -90,6 +93,20 @@ struct DevNet {
   uint32_t *keyCount;      // [grid][tile]
   uint32_t tile;           // neurons per wave
   unsigned long long *counters;  // [0] events, [1] external, [2] spikes
+  // STDP (null when plasticity never runs).  Synapse state is kept in
+  // target-major order: synapse k of neuron j's incoming list lives at
+  // incOff[j] + k; in-list items name it by k (inSyn).
+  int32_t *lastSpike;      // [N] last emission step (ms) of each neuron
+  const uint64_t *incOff;  // [N+1]
+  int16_t *wT;
+  float *tdT, *tdPre;      // timeDerivative; tdPre: before the decay (probes)
+  int32_t *lastT;          // lastActivationTime
+  const uint8_t *kindT;    // presynaptic sub-population
+  const uint16_t *inSyn;   // per in-list item
+  const float *causal, *anti;        // [128] each
+  const float *maxSyn, *revMax;      // [nsub][nsub]
+  float f2w, drift, decayMul;
+  int decayReset;
 };
 
 // --------------------------------------------------------------------------
@@ -183,7 +200,89 @@ __global__ __launch_bounds__(kBlock) void k_scan_fired(DevNet P, uint32_t t, uin
   if (f) {
     P.teHist[(size_t)slot * P.N + i] = P.Te[i];
     P.fired[i] = 0;
+    if (P.lastSpike) P.lastSpike[i] = (int32_t)t;  // setLastEmittedSpikeTime_ms
     if (popCnt) atomicAdd(&popCnt[(i / P.npc) * P.nsub + P.subPop[i]], 1u);
+  }
+}
+
+// --------------------------------------------------------------------------
+// STDP.  LTP (causalSTDP_ms_ofBackwardSynapses, DPSNN_neuron_sim.c:94-177):
+// when neuron j fires in step t, each incoming synapse with
+// lastActivationTime != 0 gets timeDerivative += causal[t - 1 - last]
+// (0 past the 128 ms table).  One wave per firing neuron, its incoming
+// synapses contiguous.
+// --------------------------------------------------------------------------
+__global__ __launch_bounds__(kWaveLTP) void k_ltp(DevNet P, uint32_t t) {
+  const uint32_t lane = threadIdx.x & 63;
+  const uint32_t wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  const uint32_t waves = (gridDim.x * blockDim.x) >> 6;
+  const uint32_t slot = t % P.RS;
+  for (uint32_t j = wave; j < P.N; j += waves) {
+    const uint64_t word = P.bits[(size_t)slot * P.NW + (j >> 6)];
+    if (!((word >> (j & 63)) & 1ull)) continue;
+    for (uint64_t s = P.incOff[j] + lane; s < P.incOff[j + 1]; s += 64) {
+      const int32_t last = P.lastT[s];
+      float f = 0.0f;
+      if (last != 0) {
+        const int dt = (int)t - 1 - last;
+        if (dt < kStdpSpan) f = P.causal[dt];
+      }
+      P.tdT[s] = P.tdT[s] + f;
+    }
+  }
+}
+
+// longTermPlasticity (DPSNN_localNet_sim.c:312-414), once per simulated
+// second at ms % 1000 == 999: exc->exc synapses take w + td (+drift),
+// clipped to [0, 2J] in float and re-quantized to int16; td then decays
+// (or resets when tauDerivativeDecay_s == 1).
+__global__ __launch_bounds__(kWaveLTP) void k_commit(DevNet P) {
+  const uint32_t lane = threadIdx.x & 63;
+  const uint32_t wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  const uint32_t waves = (gridDim.x * blockDim.x) >> 6;
+  for (uint32_t j = wave; j < P.N; j += waves) {
+    const uint32_t tgt = P.subPop[j];
+    for (uint64_t s = P.incOff[j] + lane; s < P.incOff[j + 1]; s += 64) {
+      const uint32_t src = P.kindT[s];
+      const float maxSyn = P.maxSyn[src * P.nsub + tgt];
+      const float rev = P.revMax[src * P.nsub + tgt];
+      float clamp = 0.0f;
+      if (maxSyn > 0.0 && rev > 0.0) {
+        float w = (float)(P.wT[s] * P.w2f);
+        float td = P.tdT[s];
+        td += P.drift;
+        w = w + td;
+        if (w > maxSyn) {
+          clamp = (w - maxSyn);
+          td = td - clamp;
+          w = maxSyn;
+        }
+        if (w < 0.0) {
+          clamp = w;
+          td = td - clamp;
+          w = 0.0;
+        }
+        P.wT[s] = (int16_t)(w * P.f2w);
+        P.tdT[s] = td;
+      }
+      if (P.tdPre) P.tdPre[s] = P.tdT[s];
+      if (P.decayReset)
+        P.tdT[s] = 0.0;
+      else
+        P.tdT[s] = (P.tdT[s] - P.drift + clamp) * P.decayMul;
+    }
+  }
+}
+
+// After a commit the in-list items pick up their synapse's new weight.
+__global__ __launch_bounds__(kWaveLTP) void k_refresh_weights(DevNet P) {
+  const uint32_t lane = threadIdx.x & 63;
+  const uint32_t wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  const uint32_t waves = (gridDim.x * blockDim.x) >> 6;
+  for (uint32_t j = wave; j < P.N; j += waves) {
+    const uint64_t base = P.incOff[j];
+    for (uint64_t it = P.inOff[j] + lane; it < P.inOff[j + 1]; it += 64)
+      const_cast<int16_t *>(P.inW)[it] = P.wT[base + P.inSyn[it]];
   }
 }
 
@@ -256,7 +355,6 @@ struct ExtView {  // replayed external events of this step
   const double *time, *value;
 };
 
-constexpr int kWave = 64;
 constexpr int kKeyBuckets = 256;
 
 __device__ inline uint64_t lane_mask_lt() {
@@ -264,7 +362,7 @@ __device__ inline uint64_t lane_mask_lt() {
   return lane ? (~0ull >> (64 - lane)) : 0ull;
 }
 
-__global__ __launch_bounds__(kWave) void k_step(DevNet P, uint32_t t, ExtView X) {
+__global__ __launch_bounds__(kWave) void k_step(DevNet P, uint32_t t, ExtView X, int plastic) {
   extern __shared__ uint64_t lds[];
   uint64_t *live = lds;                  // [cap] keys in insertion order
   uint64_t *sorted = lds + P.cap;        // [cap]
@@ -308,6 +406,14 @@ __global__ __launch_bounds__(kWave) void k_step(DevNet P, uint32_t t, ExtView X)
                 const uint64_t tick = dpsnn_asu64(k) - tBits;
                 key = (tick << 12) | idx;
                 on = true;
+                if (plastic) {
+                  // LTD, synapseSetLastActiveTime_antiCausalSTDP
+                  // (DPSNN_neuron_sim.c:207-277): last = t, td += anti[t - lastSpike]
+                  const uint64_t sy = P.incOff[j] + P.inSyn[beg + idx];
+                  P.lastT[sy] = (int32_t)t;
+                  const int dt = (int)t - P.lastSpike[j];
+                  P.tdT[sy] = P.tdT[sy] + (dt < kStdpSpan ? P.anti[dt] : 0.0f);
+                }
               }
             }
           }
@@ -503,6 +609,9 @@ struct dpsnn_net {
   uint64_t items = 0;
   uint32_t grid = 0;
   size_t ldsBytes = 0;
+  bool stdp = false;         // plasticity runs at some step of [0, total)
+  uint32_t stdpBlocks = 0;
+  std::vector<uint16_t> inSynLocal;
   // profiling
   bool profile = false;
   std::vector<std::pair<hipEvent_t, hipEvent_t>> profEv;  // integrate launches
@@ -615,6 +724,58 @@ static int build_device(dpsnn_net *n) {
     P.inOff = d_off; P.inPD = d_pd; P.inW = d_w;
     HIP_TRY(hipStreamSynchronize(n->stream));  // host staging is dropped after create
   }
+  // STDP state, target-major (see DevNet)
+  if (n->stdp) {
+    const uint64_t S = c.S;
+    std::vector<int16_t> wT(S);
+    std::vector<uint8_t> kindT(S);
+    for (uint64_t k = 0; k < S; ++k) {
+      wT[k] = c.weight[c.inc_syn[k]];
+      kindT[k] = c.kind[c.inc_syn[k]];
+    }
+    uint64_t *d_incOff;
+    uint8_t *d_kind;
+    uint16_t *d_inSyn;
+    float *d_causal, *d_anti, *d_max, *d_rev;
+    if ((rc = dev_alloc(n, &d_incOff, N + 1)) || (rc = upload(n, d_incOff, c.inc_off.data(), N + 1))) return rc;
+    if ((rc = dev_alloc(n, &P.wT, S)) || (rc = upload(n, P.wT, wT.data(), S))) return rc;
+    if ((rc = dev_alloc(n, &d_kind, S)) || (rc = upload(n, d_kind, kindT.data(), S))) return rc;
+    if ((rc = dev_alloc(n, &d_inSyn, c.in_off[N])) ||
+        (rc = upload(n, d_inSyn, n->inSynLocal.data(), c.in_off[N])))
+      return rc;
+    if ((rc = dev_alloc(n, &P.tdT, S)) || (rc = dev_alloc(n, &P.lastT, S))) return rc;
+    HIP_TRY(hipMemsetAsync(P.tdT, 0, 4 * S, n->stream));
+    HIP_TRY(hipMemsetAsync(P.lastT, 0, 4 * S, n->stream));
+    if (!q.statSynPeriodicProbe_writeDisable) {
+      if ((rc = dev_alloc(n, &P.tdPre, S))) return rc;
+    }
+    std::vector<int32_t> lastSpike(N, -100000);  // DPSNN_localNet_init.c:217
+    if ((rc = dev_alloc(n, &P.lastSpike, N)) || (rc = upload(n, P.lastSpike, lastSpike.data(), N))) return rc;
+    // tables, initLongTermPlasticity DPSNN_neuron_sim.c:45-92
+    std::vector<float> causal(kStdpSpan), anti(kStdpSpan);
+    causal[0] = q.causalSTDPmaxAbsValue * q.STDPmultiplier;
+    causal[1] = causal[0];
+    anti[0] = q.antiCausalCoeff * q.antiCausalSTDPmaxAbsValue * q.STDPmultiplier;
+    for (uint32_t i = 1; i < kStdpSpan; ++i) causal[i] = (1 - 1 / q.tauSTDP_ms) * causal[i - 1];
+    for (uint32_t i = 1; i < kStdpSpan; ++i) anti[i] = (1 - 1 / q.tauSTDP_ms) * anti[i - 1];
+    std::vector<float> maxSyn(size_t(P.nsub) * P.nsub), rev(size_t(P.nsub) * P.nsub);
+    for (uint32_t a = 0; a < P.nsub; ++a)
+      for (uint32_t b = 0; b < P.nsub; ++b) {
+        maxSyn[a * P.nsub + b] = 2.0 * n->par.sp[a].J[b];  // source a -> target b
+        rev[a * P.nsub + b] = 2.0 * n->par.sp[b].J[a];
+      }
+    if ((rc = dev_alloc(n, &d_causal, kStdpSpan)) || (rc = upload(n, d_causal, causal.data(), kStdpSpan))) return rc;
+    if ((rc = dev_alloc(n, &d_anti, kStdpSpan)) || (rc = upload(n, d_anti, anti.data(), kStdpSpan))) return rc;
+    if ((rc = dev_alloc(n, &d_max, maxSyn.size())) || (rc = upload(n, d_max, maxSyn.data(), maxSyn.size()))) return rc;
+    if ((rc = dev_alloc(n, &d_rev, rev.size())) || (rc = upload(n, d_rev, rev.data(), rev.size()))) return rc;
+    P.incOff = d_incOff; P.kindT = d_kind; P.inSyn = d_inSyn;
+    P.causal = d_causal; P.anti = d_anti; P.maxSyn = d_max; P.revMax = d_rev;
+    P.f2w = q.factorFloat_2_weightType;
+    P.drift = q.derivativeDrift;
+    P.decayReset = q.tauDerivativeDecay_s == 1.0 ? 1 : 0;
+    P.decayMul = (1 - 1 / q.tauDerivativeDecay_s);
+    HIP_TRY(hipStreamSynchronize(n->stream));
+  }
   // tiling: neurons per wave and the persistent grid
   int cus = 0;
   HIP_TRY(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, n->opt.device));
@@ -631,6 +792,7 @@ static int build_device(dpsnn_net *n) {
   n->ldsBytes = (size_t)2 * P.cap * sizeof(uint64_t) + (kKeyBuckets + 1) * sizeof(uint32_t);
   const uint32_t wavesPerCu = std::max<uint32_t>(1, std::min<uint32_t>(16, (uint32_t)((160u << 10) / n->ldsBytes)));
   n->grid = std::min<uint32_t>(tiles, (uint32_t)std::max(cus, 1) * wavesPerCu);
+  n->stdpBlocks = std::min<uint32_t>((N + 3) / 4, (uint32_t)std::max(cus, 1) * 8);
   if ((rc = dev_alloc(n, &P.keys, size_t(n->grid) * tile * P.cap)) ||
       (rc = dev_alloc(n, &P.keyCount, size_t(n->grid) * tile)))
     return rc;
@@ -723,6 +885,73 @@ static int drain(dpsnn_net *n) {
   return 0;
 }
 
+// Downloads the target-major STDP state into backward-list order.
+static int synapse_state(dpsnn_net *n, int16_t *w, float *td, int32_t *last, const float *tdSrc = nullptr) {
+  const Connectome &c = n->con;
+  const uint64_t S = c.S;
+  std::vector<int16_t> wT(S);
+  std::vector<float> tdT(S);
+  std::vector<int32_t> lastT(S);
+  HIP_TRY(hipStreamSynchronize(n->stream));
+  HIP_TRY(hipMemcpy(wT.data(), n->dev.wT, 2 * S, hipMemcpyDeviceToHost));
+  HIP_TRY(hipMemcpy(tdT.data(), tdSrc ? tdSrc : n->dev.tdT, 4 * S, hipMemcpyDeviceToHost));
+  HIP_TRY(hipMemcpy(lastT.data(), n->dev.lastT, 4 * S, hipMemcpyDeviceToHost));
+  for (uint64_t k = 0; k < S; ++k) {
+    const uint32_t b = c.inc_syn[k];
+    if (w) w[b] = wT[k];
+    if (td) td[b] = tdT[k];
+    if (last) last[b] = lastT[k];
+  }
+  return 0;
+}
+
+// synPeriodicProbe_A/B (DPSNN_localNet_sim.c:361-364, :394-397 with the
+// record format of DPSNN_synapse.h:63-91): every synapse before and after
+// the commit, backward-list order.
+static int write_probe(dpsnn_net *n, const char *prefix, uint32_t sec, const float *tdSrc) {
+  const Connectome &c = n->con;
+  const dpsnn_parameters &q = n->par.pub;
+  std::vector<int16_t> w(c.S);
+  std::vector<float> td(c.S);
+  int rc = synapse_state(n, w.data(), td.data(), nullptr, tdSrc);
+  if (rc) return rc;
+  char name[512];
+  if (q.howManyOutputFiles == 1)
+    std::snprintf(name, sizeof name, "%s/%s-sec%u-C%u-H%u-h%03u.dat", n->opt.output_dir, prefix, sec, q.globCFT,
+                  q.globH, 0u);
+  else if (sec == 0)
+    std::snprintf(name, sizeof name, "%s/%s-start-C%u-H%u-h%u.dat", n->opt.output_dir, prefix, q.globCFT, q.globH, 0u);
+  else
+    std::snprintf(name, sizeof name, "%s/%s-%s-C%u-H%u-h%u.dat", n->opt.output_dir, prefix,
+                  sec % 2 == 0 ? "lasteven" : "lastodd", q.globCFT, q.globH, 0u);
+  FILE *f = std::fopen(name, "w");
+  if (!f) return fail(DPSNN_ERR_IO, "cannot open %s", name);
+  std::vector<char> line(256);
+  for (uint64_t s = 0; s < c.S; ++s) {
+    const float wf = (float)(w[s] * q.factorWeightType_2_Float);
+    const int len = std::snprintf(line.data(), line.size(), "%d %d %d %d %d %f %f\n", (int)c.pre[s], (int)c.post[s],
+                                  (int)(c.post[s] / c.N), (int)c.delay[s], (int)c.kind[s], wf, td[s]);
+    std::fwrite(line.data(), 1, (size_t)len, f);
+  }
+  std::fclose(f);
+  return 0;
+}
+
+static int commit(dpsnn_net *n, uint32_t t) {
+  const dpsnn_parameters &q = n->par.pub;
+  const uint32_t sec = t / 1000;
+  const bool probes = n->writing && !q.statSynPeriodicProbe_writeDisable && sec >= q.startStatFiles_s &&
+                      sec <= q.stopStatFiles_s && sec % q.moduloSec == 0 && sec >= q.startSynPeriodicProbe_s &&
+                      sec <= q.stopSynPeriodicProbe_s;
+  int rc;
+  if (probes && (rc = write_probe(n, "synPeriodicProbe_A", sec, nullptr))) return rc;
+  hipLaunchKernelGGL(k_commit, dim3(n->stdpBlocks), dim3(kWaveLTP), 0, n->stream, n->dev);
+  hipLaunchKernelGGL(k_refresh_weights, dim3(n->stdpBlocks), dim3(kWaveLTP), 0, n->stream, n->dev);
+  HIP_TRY(hipGetLastError());
+  if (probes && (rc = write_probe(n, "synPeriodicProbe_B", sec, n->dev.tdPre))) return rc;
+  return 0;
+}
+
 static int prof_begin(dpsnn_net *n) {
   if (!n->profile) return 0;
   if (n->profUsed == n->profEv.size()) {
@@ -749,7 +978,11 @@ static int one_step(dpsnn_net *n) {
   int rc;
   if (n->collect && n->pendingSteps == 0) n->pendingFrom = t;
   uint32_t *pop = n->collect ? n->d_popCnt + popPer * n->pendingSteps : nullptr;
+  const dpsnn_parameters &q = n->par.pub;
+  const int plastic = n->stdp && t >= q.startPlasticity_ms && t <= q.stopPlasticity_ms;
   hipLaunchKernelGGL(k_scan_fired, dim3(blocks), dim3(kBlock), 0, n->stream, P, t, pop);
+  if (plastic)
+    hipLaunchKernelGGL(k_ltp, dim3(n->stdpBlocks), dim3(kWaveLTP), 0, n->stream, P, t);
   if (n->wantRaster)
     hipLaunchKernelGGL(k_compact, dim3(1), dim3(kCompactThreads), 0, n->stream, P, t, n->d_rasterIds,
                        n->d_rasterTe, n->d_rasterCount, n->d_stepCount + n->pendingSteps);
@@ -776,8 +1009,9 @@ static int one_step(dpsnn_net *n) {
     X = ExtView{s.d_off, s.d_time, s.d_value};
   }
   if ((rc = prof_begin(n))) return rc;
-  hipLaunchKernelGGL(k_step, dim3(n->grid), dim3(kWave), n->ldsBytes, n->stream, P, t, X);
+  hipLaunchKernelGGL(k_step, dim3(n->grid), dim3(kWave), n->ldsBytes, n->stream, P, t, X, plastic);
   if ((rc = prof_end(n))) return rc;
+  if (plastic && t % 1000 == 999 && (rc = commit(n, t))) return rc;
   if (slot) {
     HIP_TRY(hipEventRecord(slot->done, n->stream));
     slot->pending = true;
@@ -817,14 +1051,32 @@ int dpsnn_create(const char *inidir, const dpsnn_options *opt, dpsnn_net **out) 
   std::string err;
   if (!read_params(inidir, n->par, err)) return fail(DPSNN_ERR_CONFIG, "%s", err.c_str());
   const dpsnn_parameters &q = n->par.pub;
-  if (q.startPlasticity_ms <= q.stopPlasticity_ms && q.startPlasticity_ms < q.totalSimTime_ms)
-    return fail(DPSNN_ERR_CONFIG, "plasticity window overlaps the run; STDP is not enabled in this build");
+  n->stdp = q.startPlasticity_ms <= q.stopPlasticity_ms && q.startPlasticity_ms < q.totalSimTime_ms;
+  if (n->stdp)
+    for (uint32_t s = 0; s < q.subPopNumber; ++s)
+      if (!(n->par.sp[s].Tarp >= 1.0))
+        // a source could then fire in consecutive steps and one synapse be
+        // delivered twice in a step (own bucket + spill-over run)
+        return fail(DPSNN_ERR_CONFIG, "STDP on device needs refractory periods >= 1 ms");
   // localNet init: seed with env_globalSeed (DPSNN_localNet_init.c:119),
   // generate, then continue the stream from SetRandomSeed(loc_h=0) (:283)
   n->rng.seed((int32_t)q.globalSeed);
   if (!generate_connectome(n->par, n->rng, n->con, err)) return fail(DPSNN_ERR_CONFIG, "%s", err.c_str());
-  build_delivery(n->par, n->con, /*with_incoming=*/false);
+  build_delivery(n->par, n->con, /*with_incoming=*/n->stdp);
   n->rng.seed(0);
+  if (n->stdp) {
+    // each in-list item names its synapse by its rank in the target's
+    // incoming list (backward-list order)
+    const Connectome &c = n->con;
+    std::vector<uint32_t> rank(c.S);
+    for (uint32_t j = 0; j < c.N; ++j) {
+      if (c.inc_off[j + 1] - c.inc_off[j] > 65535)
+        return fail(DPSNN_ERR_CONFIG, "neuron %u has more than 65535 incoming synapses", j);
+      for (uint64_t k = c.inc_off[j]; k < c.inc_off[j + 1]; ++k) rank[c.inc_syn[k]] = (uint32_t)(k - c.inc_off[j]);
+    }
+    n->inSynLocal.resize(c.in_off[c.N]);
+    for (uint64_t it = 0; it < c.in_off[c.N]; ++it) n->inSynLocal[it] = (uint16_t)rank[c.in_syn[it]];
+  }
   if (n->opt.drive == DPSNN_DRIVE_REPLAY) n->replay.init(n->con, &n->rng);
   n->writing = n->opt.output_dir && n->opt.output_dir[0];
   if (n->writing) n->writer.init(n->par, n->opt.output_dir);
@@ -840,6 +1092,7 @@ int dpsnn_create(const char *inidir, const dpsnn_options *opt, dpsnn_net **out) 
   std::vector<uint32_t>().swap(n->con.in_pre_d);
   std::vector<int16_t>().swap(n->con.in_w);
   std::vector<uint32_t>().swap(n->con.in_syn);
+  std::vector<uint16_t>().swap(n->inSynLocal);
   n->initSeconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
   *out = n.release();
   return 0;
@@ -900,6 +1153,7 @@ int dpsnn_get_synapses(dpsnn_net *n, uint32_t *pre, uint32_t *post, uint8_t *del
   if (pre) std::memcpy(pre, c.pre.data(), 4 * c.S);
   if (post) std::memcpy(post, c.post.data(), 4 * c.S);
   if (delay) std::memcpy(delay, c.delay.data(), c.S);
+  if (n->stdp) return synapse_state(n, weight, tderiv, last_act);
   if (weight) std::memcpy(weight, c.weight.data(), 2 * c.S);
   if (tderiv) std::memset(tderiv, 0, 4 * c.S);
   if (last_act) std::memset(last_act, 0, 4 * c.S);
@@ -1068,7 +1322,24 @@ void dpsnn_host_destroy(dpsnn_host_net *h) { delete h; }
 
 int dpsnn_write_synapse_dump(dpsnn_net *n, const char *path) {
   if (!n || !path) return fail(DPSNN_ERR_ARG, "null argument");
-  return write_dump(n->con, n->par.pub.factorWeightType_2_Float, path);
+  if (!n->stdp) return write_dump(n->con, n->par.pub.factorWeightType_2_Float, path);
+  // plastic network: the device's current weights and derivatives
+  const Connectome &c = n->con;
+  std::vector<int16_t> w(c.S);
+  std::vector<float> td(c.S);
+  int rc = synapse_state(n, w.data(), td.data(), nullptr);
+  if (rc) return rc;
+  FILE *f = std::fopen(path, "w");
+  if (!f) return fail(DPSNN_ERR_IO, "cannot open %s", path);
+  std::vector<char> line(256);
+  const float w2f = n->par.pub.factorWeightType_2_Float;
+  for (uint64_t s = 0; s < c.S; ++s) {
+    const int len = std::snprintf(line.data(), line.size(), "%d %d %d %d %d %f %f\n", (int)c.pre[s], (int)c.post[s],
+                                  (int)(c.post[s] / c.N), (int)c.delay[s], (int)c.kind[s], (float)(w[s] * w2f), td[s]);
+    std::fwrite(line.data(), 1, (size_t)len, f);
+  }
+  std::fclose(f);
+  return 0;
 }
 
 int dpsnn_device_exp(int device, const double *x, double *y, uint64_t n) {

@@ -354,12 +354,12 @@ bool read_params(const std::string &dir, Params &p, std::string &err) {
   q.stopStatFiles_s = env.u32("env_stopStatFiles_s");
   q.spikes_writeDisable = env.flag("env_spikes_writeDisable");
   (void)env.flag("env_statSTDPevent_writeDisable");
-  (void)env.flag("env_statSynPeriodicProbe_writeDisable");
+  q.statSynPeriodicProbe_writeDisable = env.flag("env_statSynPeriodicProbe_writeDisable");
   q.statRatesPerPop_writeDisable = env.flag("env_statRatesPerPop_writeDisable");
   q.startRatesPerPop_s = env.u32("env_startRatesPerPop_s");
   q.stopRatesPerPop_s = env.u32("env_stopRatesPerPop_s");
-  (void)env.u32("env_startSynPeriodicProbe_s");
-  (void)env.u32("env_stopSynPeriodicProbe_s");
+  q.startSynPeriodicProbe_s = env.u32("env_startSynPeriodicProbe_s");
+  q.stopSynPeriodicProbe_s = env.u32("env_stopSynPeriodicProbe_s");
   (void)env.u32("env_startSTDPevent_s");
   (void)env.u32("env_stopSTDPevent_s");
   q.ratesSampling = env.u32("env_ratesSampling");

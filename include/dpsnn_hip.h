@@ -73,6 +73,8 @@ typedef struct {
   float minSynWeight_f, factorFloat_2_weightType, factorWeightType_2_Float;
   float STDPmultiplier, antiCausalCoeff, tauSTDP_ms, causalSTDPmaxAbsValue,
       antiCausalSTDPmaxAbsValue, tauDerivativeDecay_s, derivativeDrift;
+  int32_t statSynPeriodicProbe_writeDisable;
+  uint32_t startSynPeriodicProbe_s, stopSynPeriodicProbe_s;
 } dpsnn_parameters;
 
 typedef struct dpsnn_net dpsnn_net;

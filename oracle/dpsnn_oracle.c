@@ -225,7 +225,9 @@ struct ora_net {
   uint32_t *firingsPerPop;
   /* output */
   char outdir[1024];
-  FILE *fp_spikes, *fp_rates;
+  FILE *fp_spikes, *fp_rates, *fp_probeA, *fp_probeB;
+  int probe_enable;
+  uint32_t probe_start_s, probe_stop_s;
   /* scratch */
   double *arr_t, *arr_v, *tmp_t, *tmp_v;
   uint32_t arr_cap;
@@ -471,15 +473,16 @@ static int get_parameters(ora_net *n, const char *dir, char *err, int errlen) {
   if (env_u32("env_stopStatFiles_s", &p->stopStatFiles_s, err, errlen)) return -1;
   if (env_bool("env_spikes_writeDisable", &p->spikes_writeDisable, err, errlen)) return -1;
   {
-    int dummy;
+    int dummy, probeDisable;
     if (env_bool("env_statSTDPevent_writeDisable", &dummy, err, errlen)) return -1;
-    if (env_bool("env_statSynPeriodicProbe_writeDisable", &dummy, err, errlen)) return -1;
+    if (env_bool("env_statSynPeriodicProbe_writeDisable", &probeDisable, err, errlen)) return -1;
+    n->probe_enable = !probeDisable;
   }
   if (env_bool("env_statRatesPerPop_writeDisable", &p->ratesPerPop_writeDisable, err, errlen)) return -1;
   if (env_u32("env_startRatesPerPop_s", &p->startRatesPerPop_s, err, errlen)) return -1;
   if (env_u32("env_stopRatesPerPop_s", &p->stopRatesPerPop_s, err, errlen)) return -1;
-  if (env_u32("env_startSynPeriodicProbe_s", &u, err, errlen)) return -1;
-  if (env_u32("env_stopSynPeriodicProbe_s", &u, err, errlen)) return -1;
+  if (env_u32("env_startSynPeriodicProbe_s", &n->probe_start_s, err, errlen)) return -1;
+  if (env_u32("env_stopSynPeriodicProbe_s", &n->probe_stop_s, err, errlen)) return -1;
   if (env_u32("env_startSTDPevent_s", &u, err, errlen)) return -1;
   if (env_u32("env_stopSTDPevent_s", &u, err, errlen)) return -1;
   if (env_u32("env_ratesSampling", &p->ratesSampling, err, errlen)) return -1;
@@ -773,6 +776,8 @@ void ora_destroy(ora_net *n) {
   if (!n) return;
   if (n->fp_spikes) fclose(n->fp_spikes);
   if (n->fp_rates) fclose(n->fp_rates);
+  if (n->fp_probeA) fclose(n->fp_probeA);
+  if (n->fp_probeB) fclose(n->fp_probeB);
   free(n->g_pre); free(n->g_post); free(n->g_delay); free(n->g_w);
   free(n->b_pre); free(n->b_post); free(n->b_delay); free(n->b_kind); free(n->b_w);
   free(n->b_td); free(n->b_last); free(n->offsetByDelay);
@@ -1008,6 +1013,16 @@ static void ltd(ora_net *n, uint32_t s, uint32_t post, uint32_t t) {
   n->b_td[s] = n->b_td[s] + f;
 }
 
+/* statSynPeriodicProbeClass::write DPSNN_stat.c:429-451 (the synapse
+ * record of instrumentedSynapse::report). */
+static void probe_line(FILE *fp, const ora_net *n, uint64_t i) {
+  float w = (float)(n->b_w[i] * n->p.w2f);
+  fprintf(fp, "%d %d %d %d %d %f ", (int)n->b_pre[i], (int)n->b_post[i], (int)(n->b_post[i] / n->p.N),
+          (int)n->b_delay[i], (int)n->b_kind[i], w);
+  fprintf(fp, "%f", n->b_td[i]);
+  fprintf(fp, "\n");
+}
+
 /* longTermPlasticity DPSNN_localNet_sim.c:312-414. */
 static void commit_plasticity(ora_net *n) {
   uint64_t i;
@@ -1017,6 +1032,7 @@ static void commit_plasticity(ora_net *n) {
     float maxSyn = 2.0 * n->sp[src].J[tgt];
     float revMax = 2.0 * n->sp[tgt].J[src];
     float clamp = 0.0f, wF, tdF;
+    if (n->fp_probeA) probe_line(n->fp_probeA, n, i);
     if (maxSyn > 0.0 && revMax > 0.0) {
       wF = (float)(n->b_w[i] * p->w2f);
       tdF = n->b_td[i];
@@ -1027,6 +1043,7 @@ static void commit_plasticity(ora_net *n) {
       n->b_w[i] = (int16_t)(wF * p->f2w);
       n->b_td[i] = tdF;
     }
+    if (n->fp_probeB) probe_line(n->fp_probeB, n, i);
     if (p->stdp.tauDerivativeDecay_s == 1.0)
       n->b_td[i] = 0.0;
     else
@@ -1049,6 +1066,10 @@ int ora_fire(ora_net *n) {
     if (!p->spikes_writeDisable) n->fp_spikes = open_stat(n, "spikes", sec);
     if (!p->ratesPerPop_writeDisable && sec >= p->startRatesPerPop_s && sec <= p->stopRatesPerPop_s)
       n->fp_rates = open_stat(n, "ratesPerPop", sec);
+    if (n->probe_enable && sec >= n->probe_start_s && sec <= n->probe_stop_s) {
+      n->fp_probeA = open_stat(n, "synPeriodicProbe_A", sec);
+      n->fp_probeB = open_stat(n, "synPeriodicProbe_B", sec);
+    }
   }
   /* fired scan + compaction (:103-132, :620-664) and the axonal ring insert
    * (DPSNN_spike.c:209-239): slot t % 80 gets this step's ascending list. */
@@ -1131,6 +1152,8 @@ int ora_fire(ora_net *n) {
     if (plastic) commit_plasticity(n);
     if (n->fp_spikes && stat_window(p, sec)) { fclose(n->fp_spikes); n->fp_spikes = NULL; }
     if (n->fp_rates && stat_window(p, sec)) { fclose(n->fp_rates); n->fp_rates = NULL; }
+    if (n->fp_probeA) { fclose(n->fp_probeA); n->fp_probeA = NULL; }
+    if (n->fp_probeB) { fclose(n->fp_probeB); n->fp_probeB = NULL; }
   }
   if (n->fp_spikes) fflush(n->fp_spikes);
   if (n->fp_rates) fflush(n->fp_rates);

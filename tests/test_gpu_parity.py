@@ -134,9 +134,36 @@ def test_philox_drive_statistics(pkg, tmp_path):
     assert abs(b["spikes"] / a["spikes"] - 1) < 0.05
 
 
-def test_unsupported_plasticity_window_is_rejected(pkg):
+def test_stdp_c2small_identical_to_reference(pkg, tmp_path):
+    """configs[2]'s STDP at configs[0]'s size, 1.2 s: raster and rates of
+    both seconds and the synapse probes around the t = 999 ms weight commit
+    are the reference's, byte for byte."""
+    g = golden("c2small_golden.json")
     cfg = pkg.config_dir("c2_small")
-    pkg.apply_env(cfg)
+    pkg.apply_env(cfg, env_statSynPeriodicProbe_writeDisable=0, env_startSynPeriodicProbe_s=0,
+                  env_stopSynPeriodicProbe_s=0)
+    try:
+        with pkg.Network(cfg, output_dir=str(tmp_path)) as net:
+            while not net.step(100):
+                pass
+            net.sync()
+    finally:
+        pkg.apply_env(pkg.config_dir("c0"))
+    for sec in (0, 1):
+        assert sha256_file(tmp_path / f"spikes-sec{sec}-C4-H1-h000.dat") == g[f"spikes-sec{sec}"]["sha256"], sec
+        assert (open(tmp_path / f"ratesPerPop-sec{sec}-C4-H1-h000.dat").read() ==
+                open(os.path.join(GOLDEN, f"c2small_ratesPerPop-sec{sec}.dat")).read()), sec
+    for probe in ("A", "B"):
+        assert (sha256_file(tmp_path / f"synPeriodicProbe_{probe}-sec0-C4-H1-h000.dat") ==
+                g[f"synPeriodicProbe_{probe}"]["sha256"]), probe
+
+
+def test_stdp_rejects_sub_ms_refractory(pkg, tmp_path):
+    cfg = _tiny_config(tmp_path, pkg)
+    col = (tmp_path / "tiny" / "column.txt")
+    col.write_text(col.read_text().replace(" 2 1 500", " 0.5 1 500"))
+    pkg.apply_env(pkg.config_dir("c0"), env_CFX=2, env_CFY=2, env_startPlasticity_ms=0,
+                  env_stopPlasticity_ms=100000)
     try:
         with pytest.raises(pkg.DpsnnError):
             pkg.Network(cfg)
