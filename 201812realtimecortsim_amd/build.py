@@ -61,7 +61,7 @@ def build(verbose=False):
         if src.endswith(".hip"):
             _run([hipcc, *HIP_FLAGS, *inc, "-c", path, "-o", obj])
         else:
-            _run(["g++", *HOST_FLAGS, *inc, "-I", "/opt/rocm/include", "-D__HIP_PLATFORM_AMD__", "-c", path, "-o", obj])
+            _run(["g++", *HOST_FLAGS, *inc, "-c", path, "-o", obj])
         objs.append(obj)
     tmp = LIB + ".tmp"
     _run([hipcc, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp, *objs])

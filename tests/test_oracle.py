@@ -123,3 +123,24 @@ int main(void){ long bad=0,n=0; srand48(3);
                    check=True)
     out = subprocess.run([str(exe)], capture_output=True, text=True)
     assert out.returncode == 0, out.stdout
+
+
+@pytest.mark.skipif(not os.environ.get("DPSNN_SLOW"), reason="5 minutes of CPU; set DPSNN_SLOW=1")
+def test_oracle_stdp_c2small_identical_to_reference(pkg, oracle, tmp_path):
+    """configs[2]'s STDP at configs[0]'s size, 1.2 s, with the synapse
+    probes on: every output file equals the reference's."""
+    g = golden("c2small_golden.json")
+    cfg = pkg.config_dir("c2_small")
+    pkg.apply_env(cfg, env_statSynPeriodicProbe_writeDisable=0, env_startSynPeriodicProbe_s=0,
+                  env_stopSynPeriodicProbe_s=0)
+    net = OracleNet(oracle, cfg)
+    net.set_output_dir(str(tmp_path))
+    while net.fire() == 0:
+        pass
+    net.close()
+    pkg.apply_env(pkg.config_dir("c0"))
+    for sec in (0, 1):
+        assert sha256_file(tmp_path / f"spikes-sec{sec}-C4-H1-h000.dat") == g[f"spikes-sec{sec}"]["sha256"]
+    for probe in ("A", "B"):
+        assert (sha256_file(tmp_path / f"synPeriodicProbe_{probe}-sec0-C4-H1-h000.dat") ==
+                g[f"synPeriodicProbe_{probe}"]["sha256"])
