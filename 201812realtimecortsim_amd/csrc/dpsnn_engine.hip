@@ -130,20 +130,21 @@ __device__ inline void philox4x32(uint32_t c[4], uint32_t k0, uint32_t k1) {
 struct PhiloxStream {
   uint32_t k0, k1;
   uint64_t ctr;
-  double buf[2];
+  double spare;  // second uniform of the last block
   int have;
   __device__ double next() {
-    if (have == 0) {
-      uint32_t c[4] = {(uint32_t)ctr, (uint32_t)(ctr >> 32), 0u, 0x44505300u};
-      philox4x32(c, k0, k1);
-      ++ctr;
-      const uint64_t a = ((uint64_t)c[0] << 21) ^ (uint64_t)c[1];
-      const uint64_t b = ((uint64_t)c[2] << 21) ^ (uint64_t)c[3];
-      buf[0] = (double)(a & ((1ull << 53) - 1)) * 0x1p-53;
-      buf[1] = (double)(b & ((1ull << 53) - 1)) * 0x1p-53;
-      have = 2;
+    if (have) {
+      have = 0;
+      return spare;
     }
-    return buf[--have];
+    uint32_t c[4] = {(uint32_t)ctr, (uint32_t)(ctr >> 32), 0u, 0x44505300u};
+    philox4x32(c, k0, k1);
+    ++ctr;
+    const uint64_t a = ((uint64_t)c[0] << 21) ^ (uint64_t)c[1];
+    const uint64_t b = ((uint64_t)c[2] << 21) ^ (uint64_t)c[3];
+    spare = (double)(b & ((1ull << 53) - 1)) * 0x1p-53;
+    have = 1;
+    return (double)(a & ((1ull << 53) - 1)) * 0x1p-53;
   }
 };
 
@@ -362,190 +363,246 @@ __device__ inline uint64_t lane_mask_lt() {
   return lane ? (~0ull >> (64 - lane)) : 0ull;
 }
 
-__global__ __launch_bounds__(kWave) void k_step(DevNet P, uint32_t t, ExtView X, int plastic) {
-  extern __shared__ uint64_t lds[];
-  uint64_t *live = lds;                  // [cap] keys in insertion order
-  uint64_t *sorted = lds + P.cap;        // [cap]
-  uint32_t *bcnt = (uint32_t *)(lds + 2 * (size_t)P.cap);  // [kKeyBuckets + 1]
-  const uint32_t lane = threadIdx.x;
+// Phase 2: merge the sorted internal events of neuron j (keys ks[0..n))
+// with its external bath and run LIFCADynamic for each; returns the number
+// of external events.
+__device__ inline uint32_t integrate_neuron(const DevNet &P, uint32_t t, const ExtView &X, uint32_t j,
+                                            const uint64_t *ks, uint32_t n) {
   const double tF = (double)t, nextF = (double)(tF + 1.0);
   const uint64_t tBits = dpsnn_asu64(tF);
+  const uint64_t beg = P.inOff[j];
+  const uint32_t sp = P.subPop[j];
+  const double *prm = P.sp + (size_t)sp * kParams;
+  Lifca L;
+  L.v = P.v[j]; L.c = P.c[j]; L.Tr = P.Tr[j]; L.Te = P.Te[j];
+  L.Tau = prm[kTau]; L.Theta = prm[kTheta]; L.H = prm[kH]; L.Tarp = prm[kTarp];
+  L.AlphaC = prm[kAlphaC]; L.TauC = prm[kTauC]; L.gC = prm[kGC];
+  L.fired = false;
+  uint32_t nExt = 0, ii = 0;
+  double kI = 0;
+  float wI = 0;
+  auto load_internal = [&](uint32_t i) {
+    const uint64_t key = ks[i];
+    kI = dpsnn_asf64(tBits + (key >> 12));
+    wI = ((float)P.inW[beg + (key & 0xFFFu)]) * P.w2f;
+  };
+  if (n) load_internal(0);
+  if (P.drive == DPSNN_DRIVE_REPLAY) {
+    uint32_t x = X.off[j];
+    const uint32_t xe = X.off[j + 1];
+    nExt = xe - x;
+    while (ii < n || x < xe) {
+      if (ii < n && (x >= xe || kI <= X.time[x])) {
+        L.event(kI, (double)wI);
+        if (++ii < n) load_internal(ii);
+      } else {
+        L.event(X.time[x], X.value[x]);
+        ++x;
+      }
+    }
+  } else {
+    const double inv = P.invNuExt[j];
+    const uint32_t cap = P.maxExt[j];
+    const double *lut = P.lut + (size_t)sp * kLutDepth;
+    double a = P.assigned[j];
+    PhiloxStream R{(uint32_t)P.philoxSeed, (uint32_t)(P.philoxSeed >> 32) ^ j, P.philoxCtr[j], 0.0, 0};
+    const bool onE = inv > 0;
+    for (;;) {
+      // the reference's do-while: the first due event is always drawn,
+      // the following ones while fewer than the per-ms cap
+      const bool hasE = onE && a < nextF && (nExt == 0 || nExt < cap);
+      if (ii < n && (!hasE || kI <= a)) {
+        L.event(kI, (double)wI);
+        if (++ii < n) load_internal(ii);
+      } else if (hasE) {
+        const double val = lut[(uint32_t)(R.next() * kLutDepth)];
+        L.event(a, val);
+        ++nExt;
+        const double u = 1.0 - R.next();
+        a += -logf((float)u) * inv;
+      } else {
+        break;
+      }
+    }
+    if (onE && nExt >= cap && a < nextF) {
+      do {
+        const double u = 1.0 - R.next();
+        a += -logf((float)u) * inv;
+      } while (a < nextF);
+    }
+    P.assigned[j] = a;
+    P.philoxCtr[j] = R.ctr;
+  }
+  P.v[j] = L.v; P.c[j] = L.c; P.Tr[j] = L.Tr; P.Te[j] = L.Te;
+  P.fired[j] = L.fired ? 1 : 0;
+  return nExt;
+}
+
+constexpr uint64_t kNoKey = ~0ull;
+
+// Keys of in-list items base + k*64 + lane (k < K) of neuron j; kNoKey for
+// items that are absent or not live.  With `first`, also counts the live
+// keys per bucket (LDS) and applies LTD to their synapses.  All K chunks'
+// loads are independent: the in-list words, then the spike-history words,
+// then the emission times, so a wave keeps 3K loads in flight.
+template <int K>
+__device__ inline uint32_t gather_keys(const DevNet &P, uint32_t t, uint32_t j, uint64_t beg, uint32_t len,
+                                       uint32_t base, uint64_t (&key)[K], uint32_t *bcnt, int shift,
+                                       uint64_t tBits, bool first, int plastic) {
+  const uint32_t lane = threadIdx.x & 63;
+  const double tF = (double)t;
+  uint32_t it[K];
+#pragma unroll
+  for (int k = 0; k < K; ++k) {
+    const uint32_t idx = base + k * kWave + lane;
+    it[k] = idx < len ? P.inPD[beg + idx] : kPad;
+  }
+#pragma unroll
+  for (int k = 0; k < K; ++k) {
+    key[k] = kNoKey;
+    const uint32_t d = it[k] >> 24, pre = it[k] & 0xFFFFFFu;
+    if (it[k] != kPad && d <= t) {
+      const uint32_t s = (t - d) % P.RS;
+      const uint64_t word = P.bits[(size_t)s * P.NW + (pre >> 6)];
+      if ((word >> (pre & 63)) & 1ull) key[k] = s;  // live: remember the slot for now
+    }
+  }
+  uint32_t n = 0;
+#pragma unroll
+  for (int k = 0; k < K; ++k) {
+    const bool on = key[k] != kNoKey;
+    if (on) {
+      const uint32_t s = (uint32_t)key[k], pre = it[k] & 0xFFFFFFu;
+      const uint32_t idx = base + k * kWave + lane;
+      const double te = P.teHist[(size_t)s * P.N + pre];
+      const double kk = tF + (te - trunc(te));  // modf, exact
+      const uint64_t tick = dpsnn_asu64(kk) - tBits;
+      key[k] = (tick << 12) | idx;
+      if (first) {
+        uint32_t b = (uint32_t)(tick >> shift);
+        b = b < kKeyBuckets ? b : kKeyBuckets - 1;
+        atomicAdd(&bcnt[b], 1u);
+        if (plastic) {
+          // LTD, synapseSetLastActiveTime_antiCausalSTDP
+          // (DPSNN_neuron_sim.c:207-277): last = t, td += anti[t - lastSpike]
+          const uint64_t sy = P.incOff[j] + P.inSyn[beg + idx];
+          P.lastT[sy] = (int32_t)t;
+          const int dt = (int)t - P.lastSpike[j];
+          P.tdT[sy] = P.tdT[sy] + (dt < kStdpSpan ? P.anti[dt] : 0.0f);
+        }
+      }
+    }
+    n += (uint32_t)__popcll(__ballot(on));
+  }
+  return n;
+}
+
+template <int K>
+__device__ inline void scatter_keys(const uint64_t (&key)[K], uint32_t *bcnt, uint64_t *sorted, int shift) {
+#pragma unroll
+  for (int k = 0; k < K; ++k)
+    if (key[k] != kNoKey) {
+      uint32_t b = (uint32_t)((key[k] >> 12) >> shift);
+      b = b < kKeyBuckets ? b : kKeyBuckets - 1;
+      sorted[atomicAdd(&bcnt[b], 1u)] = key[k];
+    }
+}
+
+// Phase 1 for neuron j, whole wave: live keys of its in-list, bucket-sorted
+// in LDS (sorted[cap], bcnt[257]) and stored to out[0..n); returns n.
+template <int K>
+__device__ inline uint32_t gather_sort_neuron(const DevNet &P, uint32_t t, uint32_t j, int plastic,
+                                              uint64_t *sorted, uint32_t *bcnt, uint64_t *out) {
+  const uint32_t lane = threadIdx.x & 63;
+  const uint64_t tBits = dpsnn_asu64((double)t);
   // bucket = top 8 bits of frac: tick >> (52 - e - 8), e = exponent of t
   const int e = t ? (31 - __clz((int)t)) : 0;
   const int shift = 44 - e;
+  constexpr uint32_t kSuper = (uint32_t)K * kWave;
+  const uint64_t beg = P.inOff[j];
+  const uint32_t len = (uint32_t)(P.inOff[j + 1] - beg);
+  for (uint32_t b = lane; b <= kKeyBuckets; b += kWave) bcnt[b] = 0;
+  __syncthreads();
+  uint32_t n = 0;
+  uint64_t key[K];
+  if (t > 0)
+    for (uint32_t base = 0; base < len; base += kSuper)
+      n += gather_keys<K>(P, t, j, beg, len, base, key, bcnt, shift, tBits, true, plastic);
+  __syncthreads();
+  // exclusive scan of the bucket counters (4 per lane)
+  {
+    uint32_t c[4], sum = 0;
+    for (int r = 0; r < 4; ++r) { c[r] = bcnt[lane * 4 + r]; sum += c[r]; }
+    uint32_t incl = sum;
+    for (int o = 1; o < kWave; o <<= 1) {
+      const uint32_t v = __shfl_up(incl, o);
+      if (lane >= (uint32_t)o) incl += v;
+    }
+    uint32_t run = incl - sum;
+    __syncthreads();
+    for (int r = 0; r < 4; ++r) { bcnt[lane * 4 + r] = run; run += c[r]; }
+    if (lane == kWave - 1) bcnt[kKeyBuckets] = run;
+  }
+  __syncthreads();
+  // scatter into buckets (order inside a bucket is fixed below); a list
+  // longer than one register batch is gathered again (L2 hits)
+  if (t > 0) {
+    if (len <= kSuper) {
+      scatter_keys<K>(key, bcnt, sorted, shift);
+    } else {
+      for (uint32_t base = 0; base < len; base += kSuper) {
+        gather_keys<K>(P, t, j, beg, len, base, key, bcnt, shift, tBits, false, 0);
+        scatter_keys<K>(key, bcnt, sorted, shift);
+      }
+    }
+  }
+  __syncthreads();
+  // bcnt[b] now holds the end of bucket b; sort buckets by insertion
+  for (int r = 0; r < 4; ++r) {
+    const uint32_t b = lane * 4 + r;
+    const uint32_t hi = bcnt[b], lo = b ? bcnt[b - 1] : 0;
+    for (uint32_t x = lo + 1; x < hi; ++x) {
+      const uint64_t kx = sorted[x];
+      uint32_t y = x;
+      while (y > lo && sorted[y - 1] > kx) {
+        sorted[y] = sorted[y - 1];
+        --y;
+      }
+      sorted[y] = kx;
+    }
+  }
+  __syncthreads();
+  for (uint32_t i = lane; i < n; i += kWave) out[i] = sorted[i];
+  __syncthreads();
+  return n;
+}
+
+// Fused step: each wave owns a tile of neurons; phase 1 for each (whole
+// wave), then phase 2 one lane per neuron from the wave's scratch.
+template <int K>
+__global__ __launch_bounds__(kWave) void k_step(DevNet P, uint32_t t, ExtView X, int plastic) {
+  extern __shared__ uint64_t lds[];
+  uint64_t *sorted = lds;                              // [cap]
+  uint32_t *bcnt = (uint32_t *)(lds + (size_t)P.cap);  // [kKeyBuckets + 1]
+  const uint32_t lane = threadIdx.x;
   const uint32_t tiles = (P.N + P.tile - 1) / P.tile;
   uint64_t *wkeys = P.keys + (size_t)blockIdx.x * P.tile * P.cap;
   uint32_t *wcnt = P.keyCount + (size_t)blockIdx.x * P.tile;
   unsigned long long evSum = 0, exSum = 0;
-
   for (uint32_t tile = blockIdx.x; tile < tiles; tile += gridDim.x) {
     const uint32_t j0 = tile * P.tile;
-    // ---------------- phase 1 ----------------
     for (uint32_t q = 0; q < P.tile; ++q) {
       const uint32_t j = j0 + q;
       if (j >= P.N) break;
-      const uint64_t beg = P.inOff[j];
-      const uint32_t len = (uint32_t)(P.inOff[j + 1] - beg);
-      for (uint32_t b = lane; b <= kKeyBuckets; b += kWave) bcnt[b] = 0;
-      __syncthreads();
-      uint32_t n = 0;
-      if (t > 0) {
-        for (uint32_t c0 = 0; c0 < len; c0 += kWave) {
-          const uint32_t idx = c0 + lane;
-          bool on = false;
-          uint64_t key = 0;
-          if (idx < len) {
-            const uint32_t it = P.inPD[beg + idx];
-            const uint32_t d = it >> 24, pre = it & 0xFFFFFFu;
-            if (d <= t) {
-              const uint32_t s = (t - d) % P.RS;
-              const uint64_t word = P.bits[(size_t)s * P.NW + (pre >> 6)];
-              if ((word >> (pre & 63)) & 1ull) {
-                const double te = P.teHist[(size_t)s * P.N + pre];
-                const double k = tF + (te - trunc(te));  // modf, exact
-                const uint64_t tick = dpsnn_asu64(k) - tBits;
-                key = (tick << 12) | idx;
-                on = true;
-                if (plastic) {
-                  // LTD, synapseSetLastActiveTime_antiCausalSTDP
-                  // (DPSNN_neuron_sim.c:207-277): last = t, td += anti[t - lastSpike]
-                  const uint64_t sy = P.incOff[j] + P.inSyn[beg + idx];
-                  P.lastT[sy] = (int32_t)t;
-                  const int dt = (int)t - P.lastSpike[j];
-                  P.tdT[sy] = P.tdT[sy] + (dt < kStdpSpan ? P.anti[dt] : 0.0f);
-                }
-              }
-            }
-          }
-          const uint64_t m = __ballot(on);
-          if (on) {
-            live[n + __popcll(m & lane_mask_lt())] = key;
-            uint32_t b = (uint32_t)((key >> 12) >> shift);
-            b = b < kKeyBuckets ? b : kKeyBuckets - 1;
-            atomicAdd(&bcnt[b], 1u);
-          }
-          n += (uint32_t)__popcll(m);
-        }
-      }
-      __syncthreads();
-      // exclusive scan of the bucket counters (4 per lane)
-      {
-        uint32_t c[4], sum = 0;
-        for (int r = 0; r < 4; ++r) { c[r] = bcnt[lane * 4 + r]; sum += c[r]; }
-        uint32_t incl = sum;
-        for (int o = 1; o < kWave; o <<= 1) {
-          const uint32_t v = __shfl_up(incl, o);
-          if (lane >= (uint32_t)o) incl += v;
-        }
-        uint32_t run = incl - sum;
-        __syncthreads();
-        for (int r = 0; r < 4; ++r) { bcnt[lane * 4 + r] = run; run += c[r]; }
-        if (lane == kWave - 1) bcnt[kKeyBuckets] = run;
-      }
-      __syncthreads();
-      // scatter into buckets (order inside a bucket is fixed below)
-      for (uint32_t i = lane; i < n; i += kWave) {
-        const uint64_t key = live[i];
-        uint32_t b = (uint32_t)((key >> 12) >> shift);
-        b = b < kKeyBuckets ? b : kKeyBuckets - 1;
-        sorted[atomicAdd(&bcnt[b], 1u)] = key;
-      }
-      __syncthreads();
-      // bcnt[b] now holds the end of bucket b; sort buckets by insertion
-      for (int r = 0; r < 4; ++r) {
-        const uint32_t b = lane * 4 + r;
-        const uint32_t hi = bcnt[b], lo = b ? bcnt[b - 1] : 0;
-        for (uint32_t x = lo + 1; x < hi; ++x) {
-          const uint64_t key = sorted[x];
-          uint32_t y = x;
-          while (y > lo && sorted[y - 1] > key) {
-            sorted[y] = sorted[y - 1];
-            --y;
-          }
-          sorted[y] = key;
-        }
-      }
-      __syncthreads();
-      uint64_t *out = wkeys + (size_t)q * P.cap;
-      for (uint32_t i = lane; i < n; i += kWave) out[i] = sorted[i];
+      const uint32_t n = gather_sort_neuron<K>(P, t, j, plastic, sorted, bcnt, wkeys + (size_t)q * P.cap);
       if (lane == 0) wcnt[q] = n;
       evSum += n;
-      __syncthreads();
     }
     __threadfence_block();
     __syncthreads();
     // ---------------- phase 2 ----------------
     const uint32_t j = j0 + lane;
-    if (lane < P.tile && j < P.N) {
-      const uint32_t n = wcnt[lane];
-      const uint64_t *ks = wkeys + (size_t)lane * P.cap;
-      const uint64_t beg = P.inOff[j];
-      const uint32_t sp = P.subPop[j];
-      const double *prm = P.sp + (size_t)sp * kParams;
-      Lifca L;
-      L.v = P.v[j]; L.c = P.c[j]; L.Tr = P.Tr[j]; L.Te = P.Te[j];
-      L.Tau = prm[kTau]; L.Theta = prm[kTheta]; L.H = prm[kH]; L.Tarp = prm[kTarp];
-      L.AlphaC = prm[kAlphaC]; L.TauC = prm[kTauC]; L.gC = prm[kGC];
-      L.fired = false;
-      uint32_t nExt = 0, ii = 0;
-      double kI = 0;
-      float wI = 0;
-      auto load_internal = [&](uint32_t i) {
-        const uint64_t key = ks[i];
-        kI = dpsnn_asf64(tBits + (key >> 12));
-        wI = ((float)P.inW[beg + (key & 0xFFFu)]) * P.w2f;
-      };
-      if (n) load_internal(0);
-      if (P.drive == DPSNN_DRIVE_REPLAY) {
-        uint32_t x = X.off[j];
-        const uint32_t xe = X.off[j + 1];
-        nExt = xe - x;
-        while (ii < n || x < xe) {
-          if (ii < n && (x >= xe || kI <= X.time[x])) {
-            L.event(kI, (double)wI);
-            if (++ii < n) load_internal(ii);
-          } else {
-            L.event(X.time[x], X.value[x]);
-            ++x;
-          }
-        }
-      } else {
-        const double inv = P.invNuExt[j];
-        const uint32_t cap = P.maxExt[j];
-        const double *lut = P.lut + (size_t)sp * kLutDepth;
-        double a = P.assigned[j];
-        PhiloxStream R{(uint32_t)P.philoxSeed, (uint32_t)(P.philoxSeed >> 32) ^ j, P.philoxCtr[j], {0, 0}, 0};
-        const bool onE = inv > 0;
-        for (;;) {
-          // the reference's do-while: the first due event is always drawn,
-          // the following ones while fewer than the per-ms cap
-          const bool hasE = onE && a < nextF && (nExt == 0 || nExt < cap);
-          if (ii < n && (!hasE || kI <= a)) {
-            L.event(kI, (double)wI);
-            if (++ii < n) load_internal(ii);
-          } else if (hasE) {
-            const double val = lut[(uint32_t)(R.next() * kLutDepth)];
-            L.event(a, val);
-            ++nExt;
-            const double u = 1.0 - R.next();
-            a += -logf((float)u) * inv;
-          } else {
-            break;
-          }
-        }
-        if (onE && nExt >= cap && a < nextF) {
-          do {
-            const double u = 1.0 - R.next();
-            a += -logf((float)u) * inv;
-          } while (a < nextF);
-        }
-        P.assigned[j] = a;
-        P.philoxCtr[j] = R.ctr;
-      }
-      P.v[j] = L.v; P.c[j] = L.c; P.Tr[j] = L.Tr; P.Te[j] = L.Te;
-      P.fired[j] = L.fired ? 1 : 0;
-      exSum += nExt;
-    }
+    if (lane < P.tile && j < P.N) exSum += integrate_neuron(P, t, X, j, wkeys + (size_t)lane * P.cap, wcnt[lane]);
     __syncthreads();
   }
   // wave-aggregated counters (evSum is uniform; exSum per lane)
@@ -554,6 +611,31 @@ __global__ __launch_bounds__(kWave) void k_step(DevNet P, uint32_t t, ExtView X,
     atomicAdd(&P.counters[0], evSum);
     atomicAdd(&P.counters[1], exSum);
   }
+}
+
+// Split form: k_gather sorts every neuron's live events into its own slot
+// of a network-wide key buffer (one wave per neuron), then k_integrate runs
+// phase 2 one lane per neuron.  Each kernel gets its own register budget.
+template <int K>
+__global__ __launch_bounds__(kWave) void k_gather(DevNet P, uint32_t t, int plastic) {
+  extern __shared__ uint64_t lds[];
+  uint64_t *sorted = lds;
+  uint32_t *bcnt = (uint32_t *)(lds + (size_t)P.cap);
+  unsigned long long evSum = 0;
+  for (uint32_t j = blockIdx.x; j < P.N; j += gridDim.x) {
+    const uint32_t n = gather_sort_neuron<K>(P, t, j, plastic, sorted, bcnt, P.keys + (size_t)j * P.cap);
+    if (threadIdx.x == 0) P.keyCount[j] = n;
+    evSum += n;
+  }
+  if (threadIdx.x == 0) atomicAdd(&P.counters[0], evSum);
+}
+
+__global__ __launch_bounds__(kBlock) void k_integrate(DevNet P, uint32_t t, ExtView X) {
+  const uint32_t j = blockIdx.x * kBlock + threadIdx.x;
+  unsigned long long ex = 0;
+  if (j < P.N) ex = integrate_neuron(P, t, X, j, P.keys + (size_t)j * P.cap, P.keyCount[j]);
+  for (int o = 32; o > 0; o >>= 1) ex += __shfl_down(ex, o);
+  if ((threadIdx.x & 63) == 0 && ex) atomicAdd(&P.counters[1], ex);
 }
 
 }  // namespace dpsnn
@@ -611,6 +693,9 @@ struct dpsnn_net {
   size_t ldsBytes = 0;
   bool stdp = false;         // plasticity runs at some step of [0, total)
   uint32_t stdpBlocks = 0;
+  int regChunks = 24;        // in-list chunks of 64 held in registers per pass
+  bool split = false;        // k_gather + k_integrate instead of the fused k_step
+  uint32_t gatherGrid = 0;
   std::vector<uint16_t> inSynLocal;
   // profiling
   bool profile = false;
@@ -789,12 +874,18 @@ static int build_device(dpsnn_net *n) {
   if (tile < 1 || tile > 64) return fail(DPSNN_ERR_ARG, "DPSNN_TILE must be in 1..64");
   P.tile = tile;
   const uint32_t tiles = (N + tile - 1) / tile;
-  n->ldsBytes = (size_t)2 * P.cap * sizeof(uint64_t) + (kKeyBuckets + 1) * sizeof(uint32_t);
+  n->ldsBytes = (size_t)P.cap * sizeof(uint64_t) + (kKeyBuckets + 1) * sizeof(uint32_t);
+  n->regChunks = c.max_in <= 8 * 64 ? 8 : (c.max_in <= 16 * 64 ? 16 : 24);
+  if (const char *e = std::getenv("DPSNN_CHUNKS")) n->regChunks = std::atoi(e);
+  if (n->regChunks != 8 && n->regChunks != 16 && n->regChunks != 24)
+    return fail(DPSNN_ERR_ARG, "DPSNN_CHUNKS must be 8, 16 or 24");
   const uint32_t wavesPerCu = std::max<uint32_t>(1, std::min<uint32_t>(16, (uint32_t)((160u << 10) / n->ldsBytes)));
   n->grid = std::min<uint32_t>(tiles, (uint32_t)std::max(cus, 1) * wavesPerCu);
   n->stdpBlocks = std::min<uint32_t>((N + 3) / 4, (uint32_t)std::max(cus, 1) * 8);
-  if ((rc = dev_alloc(n, &P.keys, size_t(n->grid) * tile * P.cap)) ||
-      (rc = dev_alloc(n, &P.keyCount, size_t(n->grid) * tile)))
+  if (const char *e = std::getenv("DPSNN_SPLIT")) n->split = std::atoi(e) != 0;
+  const size_t keySlots = n->split ? size_t(N) : size_t(n->grid) * tile;
+  n->gatherGrid = std::min<uint32_t>(N, (uint32_t)std::max(cus, 1) * wavesPerCu * 4);
+  if ((rc = dev_alloc(n, &P.keys, keySlots * P.cap)) || (rc = dev_alloc(n, &P.keyCount, keySlots)))
     return rc;
   if ((rc = dev_alloc(n, &P.counters, 4))) return rc;
   HIP_TRY(hipMemsetAsync(P.counters, 0, 4 * sizeof(unsigned long long), n->stream));
@@ -1009,7 +1100,21 @@ static int one_step(dpsnn_net *n) {
     X = ExtView{s.d_off, s.d_time, s.d_value};
   }
   if ((rc = prof_begin(n))) return rc;
-  hipLaunchKernelGGL(k_step, dim3(n->grid), dim3(kWave), n->ldsBytes, n->stream, P, t, X, plastic);
+  if (n->split) {
+    const dim3 gg(n->gatherGrid), gb(kWave);
+    switch (n->regChunks) {
+      case 8: hipLaunchKernelGGL(k_gather<8>, gg, gb, n->ldsBytes, n->stream, P, t, plastic); break;
+      case 16: hipLaunchKernelGGL(k_gather<16>, gg, gb, n->ldsBytes, n->stream, P, t, plastic); break;
+      default: hipLaunchKernelGGL(k_gather<24>, gg, gb, n->ldsBytes, n->stream, P, t, plastic); break;
+    }
+    hipLaunchKernelGGL(k_integrate, dim3(blocks), dim3(kBlock), 0, n->stream, P, t, X);
+  } else {
+    switch (n->regChunks) {
+      case 8: hipLaunchKernelGGL(k_step<8>, dim3(n->grid), dim3(kWave), n->ldsBytes, n->stream, P, t, X, plastic); break;
+      case 16: hipLaunchKernelGGL(k_step<16>, dim3(n->grid), dim3(kWave), n->ldsBytes, n->stream, P, t, X, plastic); break;
+      default: hipLaunchKernelGGL(k_step<24>, dim3(n->grid), dim3(kWave), n->ldsBytes, n->stream, P, t, X, plastic); break;
+    }
+  }
   if ((rc = prof_end(n))) return rc;
   if (plastic && t % 1000 == 999 && (rc = commit(n, t))) return rc;
   if (slot) {
