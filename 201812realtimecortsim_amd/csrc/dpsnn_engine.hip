@@ -1,15 +1,8 @@
 // dpsnn_engine.hip -- MI355X engine for DPSNN's per-millisecond LIFCA step.
 //
-// One network = one process's whole local network on one GPU.  Per step t
-// (one reference completeTimeStep_ms, DPSNN_localNet_sim.c:22-307):
+// One network = one process's whole local network on one GPU.  Step t (one
+// reference completeTimeStep_ms, DPSNN_localNet_sim.c:22-307) is ONE kernel:
 //
-//   k_scan_fired   fired flags of step t-1's dynamics -> spike bitmap of
-//                  slot t, emission times into the slot's Te history, flags
-//                  cleared, per-(column, subPop) counts.  Replaces the fired
-//                  scan + compaction + world-size-1 axonal exchange + ring
-//                  insert (:103-148, :620-806, DPSNN_spike.c:209-239).
-//   k_compact      (only when a raster is wanted) ascending spike list of
-//                  slot t appended to the device raster buffer.
 //   k_step         per target neuron: streams the neuron's in-list (delivery
 //                  insertion order), keeps the items whose source fired in
 //                  step t-delay, orders them by arrival time (wave-wide
@@ -19,6 +12,18 @@
 //                  Replaces delivery + addInputSpike +
 //                  regularDynamicTimeStep + LIFCADynamic (:837-999,
 //                  DPSNN_neuron_sim.c:289-300, :522-824).
+//                  Its tail is the fired scan of step t+1: each wave ballots
+//                  its 64 neurons' spikes into one word of the spike bitmap
+//                  of slot t+1, stores their emission times in the slot's Te
+//                  history, the last-spike ms and the per-(column, subPop)
+//                  counts, and applies STDP's LTP to the incoming synapses of
+//                  the neurons that fired.  That replaces the fired scan +
+//                  compaction + world-size-1 axonal exchange + ring insert
+//                  (:103-148, :620-806, DPSNN_spike.c:209-239) and
+//                  causalSTDP_ms (DPSNN_neuron_sim.c:94-177).
+//   k_compact      (only when a raster is wanted) ascending spike list of
+//                  slot t+1 appended to the device raster buffer.
+//   k_commit       (STDP, ms % 1000 == 999) the per-second weight commit.
 //
 // The device holds no per-step event queue: a target pulls its live events
 // from its static in-list and the spike history (bitmap + Te per slot), so
@@ -44,7 +49,6 @@
 namespace dpsnn {
 
 constexpr int kBlock = 256;      // integrate / scan block (4 waves)
-constexpr int kBuckets = 64;     // arrival-time buckets per lane
 constexpr uint32_t kPad = 0xFFFFFFFFu;
 constexpr int kExtRing = 8;      // replayed external steps in flight
 constexpr uint32_t kMaxInList = 3900;  // per-neuron in-list (12-bit key field, LDS)
@@ -83,15 +87,13 @@ struct DevNet {
   const uint32_t *maxExt;
   double *v, *c, *Tr, *Te, *assigned;
   uint64_t *philoxCtr;
-  uint8_t *fired;
   uint64_t *bits;          // [RS][NW]
   double *teHist;          // [RS][N]
   const uint64_t *inOff;   // [N+1] start of each neuron's in-list
   const uint32_t *inPD;    // pre | delay << 24, in delivery insertion order
   const int16_t *inW;
-  uint64_t *keys;          // [grid][tile][cap] sorted event keys
-  uint32_t *keyCount;      // [grid][tile]
-  uint32_t tile;           // neurons per wave
+  uint64_t *keys;          // sorted event keys: [grid][64][cap] (fused) or [N][cap] (split)
+  uint32_t *keyCount;      // [grid][64] or [N]
   unsigned long long *counters;  // [0] events, [1] external, [2] spikes
   // STDP (null when plasticity never runs).  Synapse state is kept in
   // target-major order: synapse k of neuron j's incoming list lives at
@@ -186,51 +188,65 @@ struct Lifca {
 };
 
 // --------------------------------------------------------------------------
-// k_scan_fired: one thread per neuron, one 64-bit bitmap word per wave.
+// Fired scan of step t1 = t + 1, run by the wave that integrated 64
+// consecutive neurons j0..j0+63 (j0 % 64 == 0) in step t: one bitmap word,
+// the emission times of the neurons that fired, their last-spike ms
+// (setLastEmittedSpikeTime_ms, DPSNN_neuron_init.c:200-202) and the
+// per-(column, subPop) counts.  Every word of the slot is written by
+// exactly one wave, so the slot needs no clearing.  Whole wave, uniform.
 // --------------------------------------------------------------------------
-__global__ __launch_bounds__(kBlock) void k_scan_fired(DevNet P, uint32_t t, uint32_t *popCnt) {
-  const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
-  const bool live = i < P.N;
-  const bool f = live && P.fired[i];
+__device__ inline uint64_t publish_spikes(const DevNet &P, uint32_t t1, uint32_t j, bool f, double te,
+                                          uint32_t *popCnt, int count) {
   const uint64_t mask = __ballot(f);
-  const uint32_t slot = t % P.RS;
-  if ((threadIdx.x & 63) == 0 && (i >> 6) < P.NW) {
-    P.bits[(size_t)slot * P.NW + (i >> 6)] = mask;
-    if (mask) atomicAdd(&P.counters[2], (unsigned long long)__popcll(mask));
+  const uint32_t slot = t1 % P.RS;
+  if ((threadIdx.x & 63) == 0 && (j >> 6) < P.NW) {
+    P.bits[(size_t)slot * P.NW + (j >> 6)] = mask;
+    if (mask && count) atomicAdd(&P.counters[2], (unsigned long long)__popcll(mask));
   }
   if (f) {
-    P.teHist[(size_t)slot * P.N + i] = P.Te[i];
-    P.fired[i] = 0;
-    if (P.lastSpike) P.lastSpike[i] = (int32_t)t;  // setLastEmittedSpikeTime_ms
-    if (popCnt) atomicAdd(&popCnt[(i / P.npc) * P.nsub + P.subPop[i]], 1u);
+    P.teHist[(size_t)slot * P.N + j] = te;
+    if (P.lastSpike) P.lastSpike[j] = (int32_t)t1;
+    if (popCnt) atomicAdd(&popCnt[(j / P.npc) * P.nsub + P.subPop[j]], 1u);
   }
+  return mask;
 }
 
 // --------------------------------------------------------------------------
 // STDP.  LTP (causalSTDP_ms_ofBackwardSynapses, DPSNN_neuron_sim.c:94-177):
-// when neuron j fires in step t, each incoming synapse with
-// lastActivationTime != 0 gets timeDerivative += causal[t - 1 - last]
-// (0 past the 128 ms table).  One wave per firing neuron, its incoming
-// synapses contiguous.
+// when the scan of step t finds that neuron j fired (in step t-1's
+// dynamics), each incoming synapse with lastActivationTime != 0 gets
+// timeDerivative += causal[t - 1 - last] (0 past the 128 ms table).  One
+// wave per firing neuron, its incoming synapses contiguous.
 // --------------------------------------------------------------------------
-__global__ __launch_bounds__(kWaveLTP) void k_ltp(DevNet P, uint32_t t) {
+__device__ inline void ltp_neuron(const DevNet &P, uint32_t t, uint32_t j) {
   const uint32_t lane = threadIdx.x & 63;
+  for (uint64_t s = P.incOff[j] + lane; s < P.incOff[j + 1]; s += 64) {
+    const int32_t last = P.lastT[s];
+    float f = 0.0f;
+    if (last != 0) {
+      const int dt = (int)t - 1 - last;
+      if (dt < kStdpSpan) f = P.causal[dt];
+    }
+    P.tdT[s] = P.tdT[s] + f;
+  }
+}
+
+// LTP of the scan at step t for the neurons of one bitmap word (whole wave).
+__device__ inline void ltp_word(const DevNet &P, uint32_t t, uint64_t mask, uint32_t j0) {
+  while (mask) {
+    const uint32_t b = (uint32_t)__ffsll((long long)mask) - 1;
+    mask &= mask - 1;
+    ltp_neuron(P, t, j0 + b);
+  }
+}
+
+// Stand-alone LTP of the scan at step t (after a commit, which must come
+// first).
+__global__ __launch_bounds__(kWaveLTP) void k_ltp(DevNet P, uint32_t t) {
   const uint32_t wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
   const uint32_t waves = (gridDim.x * blockDim.x) >> 6;
   const uint32_t slot = t % P.RS;
-  for (uint32_t j = wave; j < P.N; j += waves) {
-    const uint64_t word = P.bits[(size_t)slot * P.NW + (j >> 6)];
-    if (!((word >> (j & 63)) & 1ull)) continue;
-    for (uint64_t s = P.incOff[j] + lane; s < P.incOff[j + 1]; s += 64) {
-      const int32_t last = P.lastT[s];
-      float f = 0.0f;
-      if (last != 0) {
-        const int dt = (int)t - 1 - last;
-        if (dt < kStdpSpan) f = P.causal[dt];
-      }
-      P.tdT[s] = P.tdT[s] + f;
-    }
-  }
+  for (uint32_t w = wave; w < P.NW; w += waves) ltp_word(P, t, P.bits[(size_t)slot * P.NW + w], w * 64);
 }
 
 // longTermPlasticity (DPSNN_localNet_sim.c:312-414), once per simulated
@@ -342,7 +358,7 @@ __global__ __launch_bounds__(kCompactThreads) void k_compact(DevNet P, uint32_t 
 // is unique and sorts exactly as (k, insertion order).  The key also
 // recovers k (bits(t) + tick) and the synapse (position).
 //
-// Phase 1, per neuron of the wave's tile, all 64 lanes: stream the in-list
+// Phase 1, per neuron of the wave's tile of 64, all 64 lanes: stream the in-list
 // (coalesced), keep live items (source fired in step t - delay: one bit of
 // the spike history, then its Te), compact the keys into LDS in insertion
 // order, bucket them by the top 8 bits of frac (LDS counters), sort each
@@ -356,7 +372,6 @@ struct ExtView {  // replayed external events of this step
   const double *time, *value;
 };
 
-constexpr int kKeyBuckets = 256;
 
 __device__ inline uint64_t lane_mask_lt() {
   const uint32_t lane = threadIdx.x & 63;
@@ -365,9 +380,9 @@ __device__ inline uint64_t lane_mask_lt() {
 
 // Phase 2: merge the sorted internal events of neuron j (keys ks[0..n))
 // with its external bath and run LIFCADynamic for each; returns the number
-// of external events.
+// of external events and whether (and when) the neuron fired.
 __device__ inline uint32_t integrate_neuron(const DevNet &P, uint32_t t, const ExtView &X, uint32_t j,
-                                            const uint64_t *ks, uint32_t n) {
+                                            const uint64_t *ks, uint32_t n, bool &fired, double &te) {
   const double tF = (double)t, nextF = (double)(tF + 1.0);
   const uint64_t tBits = dpsnn_asu64(tF);
   const uint64_t beg = P.inOff[j];
@@ -434,18 +449,27 @@ __device__ inline uint32_t integrate_neuron(const DevNet &P, uint32_t t, const E
     P.philoxCtr[j] = R.ctr;
   }
   P.v[j] = L.v; P.c[j] = L.c; P.Tr[j] = L.Tr; P.Te[j] = L.Te;
-  P.fired[j] = L.fired ? 1 : 0;
+  fired = L.fired;
+  te = L.Te;
   return nExt;
 }
 
 constexpr uint64_t kNoKey = ~0ull;
+
+// Bucket of a key: the top log2(B) bits of its tick (tick < 2^(52 - e),
+// e = floor(log2 t)), clamped.
+template <int B>
+__device__ inline uint32_t key_bucket(uint64_t tick, int shift) {
+  const uint64_t b = tick >> shift;
+  return b < (uint64_t)B ? (uint32_t)b : (uint32_t)(B - 1);
+}
 
 // Keys of in-list items base + k*64 + lane (k < K) of neuron j; kNoKey for
 // items that are absent or not live.  With `first`, also counts the live
 // keys per bucket (LDS) and applies LTD to their synapses.  All K chunks'
 // loads are independent: the in-list words, then the spike-history words,
 // then the emission times, so a wave keeps 3K loads in flight.
-template <int K>
+template <int K, int B>
 __device__ inline uint32_t gather_keys(const DevNet &P, uint32_t t, uint32_t j, uint64_t beg, uint32_t len,
                                        uint32_t base, uint64_t (&key)[K], uint32_t *bcnt, int shift,
                                        uint64_t tBits, bool first, int plastic) {
@@ -479,9 +503,7 @@ __device__ inline uint32_t gather_keys(const DevNet &P, uint32_t t, uint32_t j, 
       const uint64_t tick = dpsnn_asu64(kk) - tBits;
       key[k] = (tick << 12) | idx;
       if (first) {
-        uint32_t b = (uint32_t)(tick >> shift);
-        b = b < kKeyBuckets ? b : kKeyBuckets - 1;
-        atomicAdd(&bcnt[b], 1u);
+        atomicAdd(&bcnt[key_bucket<B>(tick, shift)], 1u);
         if (plastic) {
           // LTD, synapseSetLastActiveTime_antiCausalSTDP
           // (DPSNN_neuron_sim.c:207-277): last = t, td += anti[t - lastSpike]
@@ -497,42 +519,41 @@ __device__ inline uint32_t gather_keys(const DevNet &P, uint32_t t, uint32_t j, 
   return n;
 }
 
-template <int K>
+template <int K, int B>
 __device__ inline void scatter_keys(const uint64_t (&key)[K], uint32_t *bcnt, uint64_t *sorted, int shift) {
 #pragma unroll
   for (int k = 0; k < K; ++k)
-    if (key[k] != kNoKey) {
-      uint32_t b = (uint32_t)((key[k] >> 12) >> shift);
-      b = b < kKeyBuckets ? b : kKeyBuckets - 1;
-      sorted[atomicAdd(&bcnt[b], 1u)] = key[k];
-    }
+    if (key[k] != kNoKey) sorted[atomicAdd(&bcnt[key_bucket<B>(key[k] >> 12, shift)], 1u)] = key[k];
 }
 
 // Phase 1 for neuron j, whole wave: live keys of its in-list, bucket-sorted
-// in LDS (sorted[cap], bcnt[257]) and stored to out[0..n); returns n.
-template <int K>
+// in LDS (sorted[len], bcnt[B + 1]) and stored to out[0..n); returns n.
+template <int K, int B>
 __device__ inline uint32_t gather_sort_neuron(const DevNet &P, uint32_t t, uint32_t j, int plastic,
                                               uint64_t *sorted, uint32_t *bcnt, uint64_t *out) {
+  static_assert(B % kWave == 0, "bucket count must be a multiple of 64");
+  constexpr int R = B / kWave;  // buckets per lane
+  constexpr int LOG2B = B == 64 ? 6 : B == 128 ? 7 : B == 256 ? 8 : B == 512 ? 9 : 10;
   const uint32_t lane = threadIdx.x & 63;
   const uint64_t tBits = dpsnn_asu64((double)t);
-  // bucket = top 8 bits of frac: tick >> (52 - e - 8), e = exponent of t
   const int e = t ? (31 - __clz((int)t)) : 0;
-  const int shift = 44 - e;
+  const int shift = 52 - LOG2B - e;
   constexpr uint32_t kSuper = (uint32_t)K * kWave;
   const uint64_t beg = P.inOff[j];
   const uint32_t len = (uint32_t)(P.inOff[j + 1] - beg);
-  for (uint32_t b = lane; b <= kKeyBuckets; b += kWave) bcnt[b] = 0;
+  for (uint32_t b = lane; b <= (uint32_t)B; b += kWave) bcnt[b] = 0;
   __syncthreads();
   uint32_t n = 0;
   uint64_t key[K];
   if (t > 0)
     for (uint32_t base = 0; base < len; base += kSuper)
-      n += gather_keys<K>(P, t, j, beg, len, base, key, bcnt, shift, tBits, true, plastic);
+      n += gather_keys<K, B>(P, t, j, beg, len, base, key, bcnt, shift, tBits, true, plastic);
   __syncthreads();
-  // exclusive scan of the bucket counters (4 per lane)
+  // exclusive scan of the bucket counters (R per lane)
   {
-    uint32_t c[4], sum = 0;
-    for (int r = 0; r < 4; ++r) { c[r] = bcnt[lane * 4 + r]; sum += c[r]; }
+    uint32_t c[R], sum = 0;
+#pragma unroll
+    for (int r = 0; r < R; ++r) { c[r] = bcnt[lane * R + r]; sum += c[r]; }
     uint32_t incl = sum;
     for (int o = 1; o < kWave; o <<= 1) {
       const uint32_t v = __shfl_up(incl, o);
@@ -540,26 +561,27 @@ __device__ inline uint32_t gather_sort_neuron(const DevNet &P, uint32_t t, uint3
     }
     uint32_t run = incl - sum;
     __syncthreads();
-    for (int r = 0; r < 4; ++r) { bcnt[lane * 4 + r] = run; run += c[r]; }
-    if (lane == kWave - 1) bcnt[kKeyBuckets] = run;
+#pragma unroll
+    for (int r = 0; r < R; ++r) { bcnt[lane * R + r] = run; run += c[r]; }
+    if (lane == kWave - 1) bcnt[B] = run;
   }
   __syncthreads();
   // scatter into buckets (order inside a bucket is fixed below); a list
   // longer than one register batch is gathered again (L2 hits)
   if (t > 0) {
     if (len <= kSuper) {
-      scatter_keys<K>(key, bcnt, sorted, shift);
+      scatter_keys<K, B>(key, bcnt, sorted, shift);
     } else {
       for (uint32_t base = 0; base < len; base += kSuper) {
-        gather_keys<K>(P, t, j, beg, len, base, key, bcnt, shift, tBits, false, 0);
-        scatter_keys<K>(key, bcnt, sorted, shift);
+        gather_keys<K, B>(P, t, j, beg, len, base, key, bcnt, shift, tBits, false, 0);
+        scatter_keys<K, B>(key, bcnt, sorted, shift);
       }
     }
   }
   __syncthreads();
   // bcnt[b] now holds the end of bucket b; sort buckets by insertion
-  for (int r = 0; r < 4; ++r) {
-    const uint32_t b = lane * 4 + r;
+  for (int r = 0; r < R; ++r) {
+    const uint32_t b = lane * R + r;
     const uint32_t hi = bcnt[b], lo = b ? bcnt[b - 1] : 0;
     for (uint32_t x = lo + 1; x < hi; ++x) {
       const uint64_t kx = sorted[x];
@@ -577,66 +599,166 @@ __device__ inline uint32_t gather_sort_neuron(const DevNet &P, uint32_t t, uint3
   return n;
 }
 
-// Fused step: each wave owns a tile of neurons; phase 1 for each (whole
-// wave), then phase 2 one lane per neuron from the wave's scratch.
-template <int K>
-__global__ __launch_bounds__(kWave) void k_step(DevNet P, uint32_t t, ExtView X, int plastic) {
+// What the tail of step t does for the scan of step t+1: where its counts
+// go (null: not collected) and whether LTP runs there.
+struct NextScan {
+  uint32_t *popCnt;
+  int ltp;
+  int count;  // step t+1 is simulated: its scan counts the spikes
+};
+
+// One tile (neurons 64*tile .. 64*tile+63) of step t, one wave: phase 1 for
+// each neuron into the wave's scratch wkeys[64][cap] / wcnt[64], phase 2 one
+// lane per neuron, then the fired scan of step t+1.  Returns the tile's
+// delivered events (uniform) and adds external events to *exSum (per lane).
+template <int K, int B>
+__device__ inline unsigned long long step_tile(const DevNet &P, uint32_t t, const ExtView &X, int ltd,
+                                               const NextScan &S, uint32_t tile, uint32_t cap, uint64_t *sorted,
+                                               uint32_t *bcnt, uint64_t *wkeys, uint32_t *wcnt,
+                                               unsigned long long &exSum) {
+  const uint32_t lane = threadIdx.x & 63;
+  const uint32_t j0 = tile * kWave;
+  const uint32_t m = min((uint32_t)kWave, P.N - j0);
+  unsigned long long evSum = 0;
+  for (uint32_t q = 0; q < m; ++q) {
+    const uint32_t n = gather_sort_neuron<K, B>(P, t, j0 + q, ltd, sorted, bcnt, wkeys + (size_t)q * cap);
+    if (lane == 0) wcnt[q] = n;
+    evSum += n;
+  }
+  __threadfence_block();
+  __syncthreads();
+  // ---------------- phase 2 ----------------
+  bool f = false;
+  double te = 0.0;
+  if (lane < m) exSum += integrate_neuron(P, t, X, j0 + lane, wkeys + (size_t)lane * cap, wcnt[lane], f, te);
+  // ---------------- fired scan of step t+1 ----------------
+  const uint64_t mask = publish_spikes(P, t + 1, j0 + lane, f, te, S.popCnt, S.count);
+  if (S.ltp) ltp_word(P, t + 1, mask, j0);
+  __syncthreads();
+  return evSum;
+}
+
+// Fused step of one network: each wave owns tiles blockIdx.x, +gridDim.x, ...
+template <int K, int B>
+__global__ __launch_bounds__(kWave) void k_step(DevNet P, uint32_t t, ExtView X, int ltd, NextScan S) {
   extern __shared__ uint64_t lds[];
   uint64_t *sorted = lds;                              // [cap]
-  uint32_t *bcnt = (uint32_t *)(lds + (size_t)P.cap);  // [kKeyBuckets + 1]
-  const uint32_t lane = threadIdx.x;
-  const uint32_t tiles = (P.N + P.tile - 1) / P.tile;
-  uint64_t *wkeys = P.keys + (size_t)blockIdx.x * P.tile * P.cap;
-  uint32_t *wcnt = P.keyCount + (size_t)blockIdx.x * P.tile;
+  uint32_t *bcnt = (uint32_t *)(lds + (size_t)P.cap);  // [B + 1]
+  uint64_t *wkeys = P.keys + (size_t)blockIdx.x * kWave * P.cap;
+  uint32_t *wcnt = P.keyCount + (size_t)blockIdx.x * kWave;
   unsigned long long evSum = 0, exSum = 0;
-  for (uint32_t tile = blockIdx.x; tile < tiles; tile += gridDim.x) {
-    const uint32_t j0 = tile * P.tile;
-    for (uint32_t q = 0; q < P.tile; ++q) {
-      const uint32_t j = j0 + q;
-      if (j >= P.N) break;
-      const uint32_t n = gather_sort_neuron<K>(P, t, j, plastic, sorted, bcnt, wkeys + (size_t)q * P.cap);
-      if (lane == 0) wcnt[q] = n;
-      evSum += n;
-    }
-    __threadfence_block();
-    __syncthreads();
-    // ---------------- phase 2 ----------------
-    const uint32_t j = j0 + lane;
-    if (lane < P.tile && j < P.N) exSum += integrate_neuron(P, t, X, j, wkeys + (size_t)lane * P.cap, wcnt[lane]);
-    __syncthreads();
-  }
-  // wave-aggregated counters (evSum is uniform; exSum per lane)
+  for (uint32_t tile = blockIdx.x; tile < P.NW; tile += gridDim.x)
+    evSum += step_tile<K, B>(P, t, X, ltd, S, tile, P.cap, sorted, bcnt, wkeys, wcnt, exSum);
   for (int o = 32; o > 0; o >>= 1) exSum += __shfl_down(exSum, o);
-  if (lane == 0) {
+  if (threadIdx.x == 0) {
     atomicAdd(&P.counters[0], evSum);
     atomicAdd(&P.counters[1], exSum);
   }
 }
 
+// Per-network, per-step arguments of a batched launch.
+struct BatchItem {
+  uint32_t t;
+  int ltd;
+  NextScan S;
+  ExtView X;
+  uint32_t tileBegin;  // first global tile of this network
+};
+
+// Batched step: many independent networks (each at its own ms) in one
+// persistent launch.  Global tile g belongs to the network whose
+// [tileBegin, tileBegin + NW) holds it; each wave walks g = blockIdx.x,
+// +gridDim.x, ... with scratch sized for the batch's longest in-list.
+template <int K, int B>
+__global__ __launch_bounds__(kWave) void k_batch_step(const DevNet *nets, const BatchItem *items, uint32_t count,
+                                                      uint32_t totalTiles, uint32_t cap, uint64_t *keys,
+                                                      uint32_t *keyCount) {
+  extern __shared__ uint64_t lds[];
+  uint64_t *sorted = lds;
+  uint32_t *bcnt = (uint32_t *)(lds + (size_t)cap);
+  uint64_t *wkeys = keys + (size_t)blockIdx.x * kWave * cap;
+  uint32_t *wcnt = keyCount + (size_t)blockIdx.x * kWave;
+  for (uint32_t g = blockIdx.x; g < totalTiles; g += gridDim.x) {
+    uint32_t lo = 0, hi = count;  // last item with tileBegin <= g
+    while (hi - lo > 1) {
+      const uint32_t mid = (lo + hi) / 2;
+      if (items[mid].tileBegin <= g) lo = mid; else hi = mid;
+    }
+    const DevNet &P = nets[lo];
+    const BatchItem &it = items[lo];
+    unsigned long long exSum = 0;
+    const unsigned long long ev =
+        step_tile<K, B>(P, it.t, it.X, it.ltd, it.S, g - it.tileBegin, cap, sorted, bcnt, wkeys, wcnt, exSum);
+    for (int o = 32; o > 0; o >>= 1) exSum += __shfl_down(exSum, o);
+    if (threadIdx.x == 0) {
+      atomicAdd(&P.counters[0], ev);
+      if (exSum) atomicAdd(&P.counters[1], exSum);
+    }
+  }
+}
+
 // Split form: k_gather sorts every neuron's live events into its own slot
 // of a network-wide key buffer (one wave per neuron), then k_integrate runs
-// phase 2 one lane per neuron.  Each kernel gets its own register budget.
-template <int K>
-__global__ __launch_bounds__(kWave) void k_gather(DevNet P, uint32_t t, int plastic) {
+// phase 2 one lane per neuron and the fired scan of step t+1.  Each kernel
+// gets its own register budget.
+template <int K, int B>
+__global__ __launch_bounds__(kWave) void k_gather(DevNet P, uint32_t t, int ltd) {
   extern __shared__ uint64_t lds[];
   uint64_t *sorted = lds;
   uint32_t *bcnt = (uint32_t *)(lds + (size_t)P.cap);
   unsigned long long evSum = 0;
   for (uint32_t j = blockIdx.x; j < P.N; j += gridDim.x) {
-    const uint32_t n = gather_sort_neuron<K>(P, t, j, plastic, sorted, bcnt, P.keys + (size_t)j * P.cap);
+    const uint32_t n = gather_sort_neuron<K, B>(P, t, j, ltd, sorted, bcnt, P.keys + (size_t)j * P.cap);
     if (threadIdx.x == 0) P.keyCount[j] = n;
     evSum += n;
   }
   if (threadIdx.x == 0) atomicAdd(&P.counters[0], evSum);
 }
 
-__global__ __launch_bounds__(kBlock) void k_integrate(DevNet P, uint32_t t, ExtView X) {
+// Blocks of kBlock threads: each wave holds 64 consecutive neurons starting
+// at a multiple of 64, as publish_spikes requires.
+__global__ __launch_bounds__(kBlock) void k_integrate(DevNet P, uint32_t t, ExtView X, NextScan S) {
   const uint32_t j = blockIdx.x * kBlock + threadIdx.x;
   unsigned long long ex = 0;
-  if (j < P.N) ex = integrate_neuron(P, t, X, j, P.keys + (size_t)j * P.cap, P.keyCount[j]);
+  bool f = false;
+  double te = 0.0;
+  if (j < P.N) ex = integrate_neuron(P, t, X, j, P.keys + (size_t)j * P.cap, P.keyCount[j], f, te);
+  const uint64_t mask = publish_spikes(P, t + 1, j, f, te, S.popCnt, S.count);
+  if (S.ltp) ltp_word(P, t + 1, mask, j & ~63u);
   for (int o = 32; o > 0; o >>= 1) ex += __shfl_down(ex, o);
   if ((threadIdx.x & 63) == 0 && ex) atomicAdd(&P.counters[1], ex);
 }
+
+// Launch shapes: K = in-list chunks of 64 items held in registers per
+// pass, B = arrival-time buckets of the LDS sort.  Every shape computes the
+// same result bit for bit; they differ only in speed.
+using StepFn = void (*)(DevNet, uint32_t, ExtView, int, NextScan);
+using GatherFn = void (*)(DevNet, uint32_t, int);
+using BatchFn = void (*)(const DevNet *, const BatchItem *, uint32_t, uint32_t, uint32_t, uint64_t *, uint32_t *);
+struct KernelShape {
+  int K, B;
+  StepFn step;
+  GatherFn gather;
+  BatchFn batch;
+};
+#define DPSNN_SHAPE(k, b) {k, b, k_step<k, b>, k_gather<k, b>, k_batch_step<k, b>}
+static const KernelShape kShapes[] = {
+    DPSNN_SHAPE(8, 64),  DPSNN_SHAPE(8, 256),  DPSNN_SHAPE(8, 1024),
+    DPSNN_SHAPE(16, 64), DPSNN_SHAPE(16, 256), DPSNN_SHAPE(16, 1024),
+    DPSNN_SHAPE(24, 64), DPSNN_SHAPE(24, 256), DPSNN_SHAPE(24, 1024),
+};
+#undef DPSNN_SHAPE
+constexpr int kNumShapes = (int)(sizeof(kShapes) / sizeof(kShapes[0]));
+
+static int find_shape(int K, int B) {
+  for (int i = 0; i < kNumShapes; ++i)
+    if (kShapes[i].K == K && kShapes[i].B == B) return i;
+  return -1;
+}
+
+// LDS of one phase-1 wave: the sort buffer for the longest in-list and the
+// bucket counters.
+static size_t lds_bytes(uint32_t cap, int B) { return (size_t)cap * sizeof(uint64_t) + (B + 1) * sizeof(uint32_t); }
 
 }  // namespace dpsnn
 
@@ -684,16 +806,18 @@ struct dpsnn_net {
   std::vector<uint32_t> h_ids, h_steps, h_pop;
   std::vector<double> h_te;
   // last spikes
-  std::vector<uint32_t> lastIds;
-  std::vector<double> lastTe;
-  uint32_t lastCount = 0;
+  std::vector<uint32_t> lastIds, aheadIds;
+  std::vector<double> lastTe, aheadTe;
+  uint32_t aheadMs = UINT32_MAX;
   double initSeconds = 0;
   uint64_t items = 0;
   uint32_t grid = 0;
   size_t ldsBytes = 0;
   bool stdp = false;         // plasticity runs at some step of [0, total)
   uint32_t stdpBlocks = 0;
-  int regChunks = 24;        // in-list chunks of 64 held in registers per pass
+  int shape = 0;             // kShapes index
+  struct dpsnn_batch *batch = nullptr;  // stepped by this batch (its stream)
+  hipStream_t ownStream = nullptr;
   bool split = false;        // k_gather + k_integrate instead of the fused k_step
   uint32_t gatherGrid = 0;
   std::vector<uint16_t> inSynLocal;
@@ -786,8 +910,6 @@ static int build_device(dpsnn_net *n) {
   if ((rc = dev_alloc(n, &P.assigned, N)) || (rc = upload(n, P.assigned, zeros.data(), N))) return rc;
   if ((rc = dev_alloc(n, &P.philoxCtr, N))) return rc;
   HIP_TRY(hipMemsetAsync(P.philoxCtr, 0, sizeof(uint64_t) * N, n->stream));
-  if ((rc = dev_alloc(n, &P.fired, N))) return rc;
-  HIP_TRY(hipMemsetAsync(P.fired, 0, N, n->stream));
   // history
   if ((rc = dev_alloc(n, &P.bits, size_t(P.RS) * P.NW))) return rc;
   HIP_TRY(hipMemsetAsync(P.bits, 0, sizeof(uint64_t) * size_t(P.RS) * P.NW, n->stream));
@@ -861,30 +983,26 @@ static int build_device(dpsnn_net *n) {
     P.decayMul = (1 - 1 / q.tauDerivativeDecay_s);
     HIP_TRY(hipStreamSynchronize(n->stream));
   }
-  // tiling: neurons per wave and the persistent grid
+  // launch shapes.  The fused k_step gives each wave whole tiles of 64
+  // neurons; a network with fewer tiles than ~4 per CU runs the split form
+  // (one wave per neuron in phase 1) instead.
   int cus = 0;
   HIP_TRY(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, n->opt.device));
-  uint32_t tile = 64;
-  if (const char *e = std::getenv("DPSNN_TILE")) tile = (uint32_t)std::atoi(e);
-  else {
-    const uint64_t perWave = uint64_t(N) / (uint64_t(std::max(cus, 1)) * 16);
-    tile = 1;
-    while (tile * 2 <= perWave && tile < 64) tile *= 2;
-  }
-  if (tile < 1 || tile > 64) return fail(DPSNN_ERR_ARG, "DPSNN_TILE must be in 1..64");
-  P.tile = tile;
-  const uint32_t tiles = (N + tile - 1) / tile;
-  n->ldsBytes = (size_t)P.cap * sizeof(uint64_t) + (kKeyBuckets + 1) * sizeof(uint32_t);
-  n->regChunks = c.max_in <= 8 * 64 ? 8 : (c.max_in <= 16 * 64 ? 16 : 24);
-  if (const char *e = std::getenv("DPSNN_CHUNKS")) n->regChunks = std::atoi(e);
-  if (n->regChunks != 8 && n->regChunks != 16 && n->regChunks != 24)
-    return fail(DPSNN_ERR_ARG, "DPSNN_CHUNKS must be 8, 16 or 24");
+  cus = std::max(cus, 1);
+  const uint32_t tiles = P.NW;
+  int K = c.max_in <= 8 * 64 ? 8 : (c.max_in <= 16 * 64 ? 16 : 24), B = 256;
+  if (const char *e = std::getenv("DPSNN_CHUNKS")) K = std::atoi(e);
+  if (const char *e = std::getenv("DPSNN_BUCKETS")) B = std::atoi(e);
+  n->shape = find_shape(K, B);
+  if (n->shape < 0) return fail(DPSNN_ERR_ARG, "no kernel shape with %d register chunks and %d buckets", K, B);
+  n->ldsBytes = lds_bytes(P.cap, B);
   const uint32_t wavesPerCu = std::max<uint32_t>(1, std::min<uint32_t>(16, (uint32_t)((160u << 10) / n->ldsBytes)));
-  n->grid = std::min<uint32_t>(tiles, (uint32_t)std::max(cus, 1) * wavesPerCu);
-  n->stdpBlocks = std::min<uint32_t>((N + 3) / 4, (uint32_t)std::max(cus, 1) * 8);
+  n->grid = std::min<uint32_t>(tiles, (uint32_t)cus * wavesPerCu);
+  n->stdpBlocks = std::min<uint32_t>((P.NW + 3) / 4, (uint32_t)cus * 8);
+  n->split = tiles < (uint32_t)cus * 4;
   if (const char *e = std::getenv("DPSNN_SPLIT")) n->split = std::atoi(e) != 0;
-  const size_t keySlots = n->split ? size_t(N) : size_t(n->grid) * tile;
-  n->gatherGrid = std::min<uint32_t>(N, (uint32_t)std::max(cus, 1) * wavesPerCu * 4);
+  const size_t keySlots = n->split ? size_t(N) : size_t(n->grid) * kWave;
+  n->gatherGrid = std::min<uint32_t>(N, (uint32_t)cus * wavesPerCu * 4);
   if ((rc = dev_alloc(n, &P.keys, keySlots * P.cap)) || (rc = dev_alloc(n, &P.keyCount, keySlots)))
     return rc;
   if ((rc = dev_alloc(n, &P.counters, 4))) return rc;
@@ -932,6 +1050,11 @@ static int ensure_ext_cap(ExtSlot &s, size_t need) {
 
 // Copies the pending raster / rates out of device memory and writes files.
 static int drain(dpsnn_net *n) {
+  if (n->opt.keep_spikes && n->aheadMs != UINT32_MAX && n->aheadMs + 1 == n->t) {
+    n->lastIds.swap(n->aheadIds);
+    n->lastTe.swap(n->aheadTe);
+    n->aheadMs = UINT32_MAX;
+  }
   if (n->pendingSteps == 0) return 0;
   const DevNet &P = n->dev;
   const size_t popPer = size_t(P.ncols) * P.nsub;
@@ -965,10 +1088,15 @@ static int drain(dpsnn_net *n) {
         !n->writer.step(n->pendingFrom + k, n->h_ids.data() + pos, n->h_te.data() + pos, cnt,
                         n->h_pop.data() + popPer * k))
       return fail(DPSNN_ERR_IO, "writing output files in %s failed", n->opt.output_dir);
-    if (k == steps - 1 && n->opt.keep_spikes) {
-      n->lastIds.assign(n->h_ids.begin() + pos, n->h_ids.begin() + pos + cnt);
-      n->lastTe.assign(n->h_te.begin() + pos, n->h_te.begin() + pos + cnt);
-      n->lastCount = cnt;
+    // last_spikes reports the scan of the last completed step (ms t - 1);
+    // the record of ms t, already produced, waits for the next step
+    const uint32_t m = n->pendingFrom + k;
+    if (n->opt.keep_spikes && (m + 1 == n->t || m == n->t)) {
+      std::vector<uint32_t> &ids = m + 1 == n->t ? n->lastIds : n->aheadIds;
+      std::vector<double> &te = m + 1 == n->t ? n->lastTe : n->aheadTe;
+      ids.assign(n->h_ids.begin() + pos, n->h_ids.begin() + pos + cnt);
+      te.assign(n->h_te.begin() + pos, n->h_te.begin() + pos + cnt);
+      if (m == n->t) n->aheadMs = m;
     }
     pos += cnt;
   }
@@ -1061,22 +1189,50 @@ static int prof_end(dpsnn_net *n) {
   return 0;
 }
 
+// Opens the raster/rates record of ms m (records are contiguous) and returns
+// its index in the drain buffers; drains first when they are full.
+static int open_record(dpsnn_net *n, uint32_t m, uint32_t *idx) {
+  int rc;
+  if (n->pendingSteps == n->drainEvery && (rc = drain(n))) return rc;
+  if (n->pendingSteps == 0) n->pendingFrom = m;
+  *idx = n->pendingSteps++;
+  return 0;
+}
+
+static int compact(dpsnn_net *n, uint32_t m, uint32_t rec) {
+  if (n->wantRaster)
+    hipLaunchKernelGGL(k_compact, dim3(1), dim3(kCompactThreads), 0, n->stream, n->dev, m, n->d_rasterIds,
+                       n->d_rasterTe, n->d_rasterCount, n->d_stepCount + rec);
+  return 0;
+}
+
+static bool plastic_at(const dpsnn_net *n, uint32_t t) {
+  const dpsnn_parameters &q = n->par.pub;
+  return n->stdp && t >= q.startPlasticity_ms && t <= q.stopPlasticity_ms;
+}
+
+// Step t: delivery + dynamics of t, and the fired scan of t+1 (which the
+// reference runs at the head of step t+1; nothing reads the scan's outputs
+// in between).  The scan of step 0 finds no spikes.
 static int one_step(dpsnn_net *n) {
   DevNet &P = n->dev;
-  const uint32_t t = n->t;
-  const uint32_t blocks = (P.N + kBlock - 1) / kBlock;
+  const uint32_t t = n->t, next = t + 1;
+  const uint32_t T = n->par.pub.totalSimTime_ms;
   const size_t popPer = size_t(P.ncols) * P.nsub;
   int rc;
-  if (n->collect && n->pendingSteps == 0) n->pendingFrom = t;
-  uint32_t *pop = n->collect ? n->d_popCnt + popPer * n->pendingSteps : nullptr;
-  const dpsnn_parameters &q = n->par.pub;
-  const int plastic = n->stdp && t >= q.startPlasticity_ms && t <= q.stopPlasticity_ms;
-  hipLaunchKernelGGL(k_scan_fired, dim3(blocks), dim3(kBlock), 0, n->stream, P, t, pop);
-  if (plastic)
-    hipLaunchKernelGGL(k_ltp, dim3(n->stdpBlocks), dim3(kWaveLTP), 0, n->stream, P, t);
-  if (n->wantRaster)
-    hipLaunchKernelGGL(k_compact, dim3(1), dim3(kCompactThreads), 0, n->stream, P, t, n->d_rasterIds,
-                       n->d_rasterTe, n->d_rasterCount, n->d_stepCount + n->pendingSteps);
+  uint32_t rec;
+  if (n->collect && t == 0) {
+    if ((rc = open_record(n, 0, &rec)) || (rc = compact(n, 0, rec))) return rc;
+  }
+  const int ltd = plastic_at(n, t);
+  const bool commitNow = ltd && t % 1000 == 999;
+  const bool ltpNext = next < T && plastic_at(n, next);
+  const bool recordNext = n->collect && next < T;
+  NextScan S{nullptr, ltpNext && !commitNow, next < T};
+  if (recordNext) {
+    if ((rc = open_record(n, next, &rec))) return rc;
+    S.popCnt = n->d_popCnt + popPer * rec;
+  }
   ExtView X{nullptr, nullptr, nullptr};
   ExtSlot *slot = nullptr;
   if (P.drive == DPSNN_DRIVE_REPLAY) {
@@ -1100,32 +1256,28 @@ static int one_step(dpsnn_net *n) {
     X = ExtView{s.d_off, s.d_time, s.d_value};
   }
   if ((rc = prof_begin(n))) return rc;
+  const KernelShape &K = kShapes[n->shape];
   if (n->split) {
-    const dim3 gg(n->gatherGrid), gb(kWave);
-    switch (n->regChunks) {
-      case 8: hipLaunchKernelGGL(k_gather<8>, gg, gb, n->ldsBytes, n->stream, P, t, plastic); break;
-      case 16: hipLaunchKernelGGL(k_gather<16>, gg, gb, n->ldsBytes, n->stream, P, t, plastic); break;
-      default: hipLaunchKernelGGL(k_gather<24>, gg, gb, n->ldsBytes, n->stream, P, t, plastic); break;
-    }
-    hipLaunchKernelGGL(k_integrate, dim3(blocks), dim3(kBlock), 0, n->stream, P, t, X);
+    hipLaunchKernelGGL(K.gather, dim3(n->gatherGrid), dim3(kWave), n->ldsBytes, n->stream, P, t, ltd);
+    const uint32_t blocks = (P.N + kBlock - 1) / kBlock;
+    hipLaunchKernelGGL(k_integrate, dim3(blocks), dim3(kBlock), 0, n->stream, P, t, X, S);
   } else {
-    switch (n->regChunks) {
-      case 8: hipLaunchKernelGGL(k_step<8>, dim3(n->grid), dim3(kWave), n->ldsBytes, n->stream, P, t, X, plastic); break;
-      case 16: hipLaunchKernelGGL(k_step<16>, dim3(n->grid), dim3(kWave), n->ldsBytes, n->stream, P, t, X, plastic); break;
-      default: hipLaunchKernelGGL(k_step<24>, dim3(n->grid), dim3(kWave), n->ldsBytes, n->stream, P, t, X, plastic); break;
-    }
+    hipLaunchKernelGGL(K.step, dim3(n->grid), dim3(kWave), n->ldsBytes, n->stream, P, t, X, ltd, S);
   }
   if ((rc = prof_end(n))) return rc;
-  if (plastic && t % 1000 == 999 && (rc = commit(n, t))) return rc;
+  if (commitNow) {
+    if ((rc = commit(n, t))) return rc;
+    // the scan of step t+1 (with its LTP) follows the commit
+    if (ltpNext) hipLaunchKernelGGL(k_ltp, dim3(n->stdpBlocks), dim3(kWaveLTP), 0, n->stream, P, next);
+  }
+  if (recordNext && (rc = compact(n, next, rec))) return rc;
   if (slot) {
     HIP_TRY(hipEventRecord(slot->done, n->stream));
     slot->pending = true;
   }
   HIP_TRY(hipGetLastError());
-  n->t = t + 1;
-  if (!n->collect) return 0;
-  n->pendingSteps++;
-  if (n->pendingSteps == n->drainEvery || n->opt.keep_spikes) return drain(n);
+  n->t = next;
+  if (n->opt.keep_spikes) return drain(n);
   return 0;
 }
 
@@ -1154,7 +1306,10 @@ int dpsnn_create(const char *inidir, const dpsnn_options *opt, dpsnn_net **out) 
   if (n->opt.drive != DPSNN_DRIVE_REPLAY && n->opt.drive != DPSNN_DRIVE_PHILOX)
     return fail(DPSNN_ERR_ARG, "unknown drive mode %d", n->opt.drive);
   std::string err;
-  if (!read_params(inidir, n->par, err)) return fail(DPSNN_ERR_CONFIG, "%s", err.c_str());
+  EnvMap over;
+  if (!parse_env_overrides(n->opt.env, over, err) || !read_params(inidir, n->par, err, &over))
+    return fail(DPSNN_ERR_CONFIG, "%s", err.c_str());
+  n->opt.env = nullptr;  // not kept past the call
   const dpsnn_parameters &q = n->par.pub;
   n->stdp = q.startPlasticity_ms <= q.stopPlasticity_ms && q.startPlasticity_ms < q.totalSimTime_ms;
   if (n->stdp)
@@ -1234,7 +1389,13 @@ int dpsnn_get_state(dpsnn_net *n, double *v, double *c, double *Tr, double *Te, 
   if (c) HIP_TRY(hipMemcpy(c, n->dev.c, N * 8, hipMemcpyDeviceToHost));
   if (Tr) HIP_TRY(hipMemcpy(Tr, n->dev.Tr, N * 8, hipMemcpyDeviceToHost));
   if (Te) HIP_TRY(hipMemcpy(Te, n->dev.Te, N * 8, hipMemcpyDeviceToHost));
-  if (fired) HIP_TRY(hipMemcpy(fired, n->dev.fired, N, hipMemcpyDeviceToHost));
+  if (fired) {
+    // neurons that fired in the last completed step: bitmap of slot t
+    std::vector<uint64_t> w(n->dev.NW);
+    const size_t slot = n->t % n->dev.RS;
+    HIP_TRY(hipMemcpy(w.data(), n->dev.bits + slot * n->dev.NW, 8 * w.size(), hipMemcpyDeviceToHost));
+    for (size_t i = 0; i < N; ++i) fired[i] = (uint8_t)((w[i >> 6] >> (i & 63)) & 1u);
+  }
   return 0;
 }
 
@@ -1243,9 +1404,10 @@ int dpsnn_last_spikes(dpsnn_net *n, uint32_t *ids, double *te, uint32_t *count) 
   if (!n->opt.keep_spikes) return fail(DPSNN_ERR_ARG, "network created without keep_spikes");
   int rc = dpsnn_sync(n);
   if (rc) return rc;
-  if (count) *count = n->lastCount;
-  if (ids) std::memcpy(ids, n->lastIds.data(), sizeof(uint32_t) * n->lastCount);
-  if (te) std::memcpy(te, n->lastTe.data(), sizeof(double) * n->lastCount);
+  const uint32_t cnt = (uint32_t)n->lastIds.size();
+  if (count) *count = cnt;
+  if (ids) std::memcpy(ids, n->lastIds.data(), sizeof(uint32_t) * cnt);
+  if (te) std::memcpy(te, n->lastTe.data(), sizeof(double) * cnt);
   return 0;
 }
 

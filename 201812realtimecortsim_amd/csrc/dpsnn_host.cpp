@@ -137,9 +137,17 @@ namespace {
 
 struct EnvReader {
   std::string &err;
+  const EnvMap *over;
   bool ok = true;
+  const char *get(const char *name) const {
+    if (over) {
+      auto it = over->find(name);
+      if (it != over->end()) return it->second.c_str();
+    }
+    return std::getenv(name);
+  }
   uint32_t u32(const char *name) {
-    const char *s = std::getenv(name);
+    const char *s = get(name);
     if (!s) {
       if (ok) err = std::string("missing env var ") + name;
       ok = false;
@@ -307,10 +315,34 @@ bool read_plasticity(const std::string &dir, Params &p, std::string &err) {
 
 }  // namespace
 
-bool read_params(const std::string &dir, Params &p, std::string &err) {
+bool parse_env_overrides(const char *text, EnvMap &out, std::string &err) {
+  out.clear();
+  if (!text) return true;
+  std::string s(text);
+  size_t pos = 0;
+  while (pos <= s.size()) {
+    size_t end = s.find_first_of(";\n", pos);
+    if (end == std::string::npos) end = s.size();
+    std::string item = s.substr(pos, end - pos);
+    const size_t a = item.find_first_not_of(" \t\r");
+    if (a != std::string::npos) {
+      item = item.substr(a, item.find_last_not_of(" \t\r") - a + 1);
+      const size_t eq = item.find('=');
+      if (eq == std::string::npos || eq == 0) {
+        err = "environment override without name=value: " + item;
+        return false;
+      }
+      out[item.substr(0, eq)] = item.substr(eq + 1);
+    }
+    pos = end + 1;
+  }
+  return true;
+}
+
+bool read_params(const std::string &dir, Params &p, std::string &err, const EnvMap *over) {
   dpsnn_parameters &q = p.pub;
   q = dpsnn_parameters{};
-  EnvReader env{err};
+  EnvReader env{err, over};
   q.overallConnectivity = env.u32("env_overallConnectivity");
   if (env.ok && q.overallConnectivity > 1) { err = "invalid env_overallConnectivity"; return false; }
   q.stencilDim = env.u32("env_stencilDim");
@@ -337,7 +369,7 @@ bool read_params(const std::string &dir, Params &p, std::string &err) {
   if (q.globCFT == 0 || q.globCFT > 96 * 96 + 1) { err = "globCFT out of range"; return false; }
   q.globN = q.locN = q.neuronsPerCM * q.globCFT;
   if (q.globN >= (1u << 24)) { err = "more than 2^24 neurons per device network"; return false; }
-  const char *ws = std::getenv("env_minSynWeight_f");
+  const char *ws = env.get("env_minSynWeight_f");
   if (!ws) { err = "missing env var env_minSynWeight_f"; return false; }
   q.minSynWeight_f = (float)std::atof(ws);
   if (q.minSynWeight_f >= 0 || q.minSynWeight_f < -6.0) { err = "env_minSynWeight_f out of range"; return false; }
@@ -366,11 +398,11 @@ bool read_params(const std::string &dir, Params &p, std::string &err) {
   if (!env.ok) return false;
   if (q.moduloSec == 0) { err = "env_moduloSec must be > 0"; return false; }
   if (q.ratesSampling == 0) { err = "env_ratesSampling must be > 0"; return false; }
-  const char *gs = std::getenv("env_globalSeed");
+  const char *gs = env.get("env_globalSeed");
   if (!gs) { err = "missing env var env_globalSeed"; return false; }
   if (std::atoi(gs) < 0) { err = "env_globalSeed < 0 asks for a time-based seed; give a fixed seed"; return false; }
   q.globalSeed = (uint32_t)std::atoi(gs);
-  const char *hm = std::getenv("env_howManyOutputFiles");
+  const char *hm = env.get("env_howManyOutputFiles");
   q.howManyOutputFiles = hm ? (uint32_t)std::atoi(hm) : 0;
   if (q.howManyOutputFiles > 1) q.howManyOutputFiles = 0;
   if (!read_plasticity(dir, p, err)) return false;

@@ -6,6 +6,7 @@
 #pragma once
 #include <cstdint>
 #include <cstdio>
+#include <map>
 #include <string>
 #include <vector>
 
@@ -41,9 +42,15 @@ struct Params {
   }
 };
 
+// name -> value pairs that take precedence over the process environment
+// (several networks of one process, each with its own env_* settings).
+using EnvMap = std::map<std::string, std::string>;
+// "name=value" items separated by ';' or newlines.
+bool parse_env_overrides(const char *text, EnvMap &out, std::string &err);
+
 // Reads env vars + ini files exactly as getParameters and
 // describeConnectome do.  Returns false and sets err on a config error.
-bool read_params(const std::string &dir, Params &p, std::string &err);
+bool read_params(const std::string &dir, Params &p, std::string &err, const EnvMap *over = nullptr);
 
 // ran3 + NormDev with the reference's persistent-state semantics
 // (randdev.c:116-189, :477-502, :584-606), one instance per network.

@@ -59,6 +59,11 @@ typedef struct {
                                 (0 = default 100 ms) */
   int keep_spikes;           /* 1 = keep each step's spike list readable
                                 through dpsnn_last_spikes */
+  const char *env;           /* "env_name=value" items separated by ';' or
+                                newlines that take precedence over the
+                                process environment (several networks with
+                                their own settings in one process); NULL =
+                                the environment only */
 } dpsnn_options;
 
 /* Parameters as getParameters returns them (the subset this path reads;
@@ -82,7 +87,8 @@ typedef struct dpsnn_net dpsnn_net;
 /* Counters of what the device did (cumulative since creation). */
 typedef struct {
   uint64_t steps;            /* simulated ms */
-  uint64_t spikes;           /* emitted spikes */
+  uint64_t spikes;           /* spikes found by the fired scans so far
+                                (the raster's spike count) */
   uint64_t synaptic_events;  /* delivered synaptic events (sum of runs) */
   uint64_t external_events;  /* external bath events integrated */
   uint64_t synapses;         /* synapses in the connectome */
