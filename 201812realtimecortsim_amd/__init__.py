@@ -83,6 +83,14 @@ def _p(t):
     return ctypes.POINTER(t)
 
 
+class BatchShape(ctypes.Structure):
+    _fields_ = [("chunks", ctypes.c_int), ("buckets", ctypes.c_int), ("waves_per_cu", ctypes.c_int),
+                ("ms_per_step", ctypes.c_double)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
 _SIGS = {
     "dpsnn_get_parameters": (ctypes.c_int, [ctypes.c_char_p, _p(Parameters)]),
     "dpsnn_create": (ctypes.c_int, [ctypes.c_char_p, _p(Options), _p(ctypes.c_void_p)]),
@@ -114,6 +122,18 @@ _SIGS = {
     "dpsnn_host_destroy": (None, [ctypes.c_void_p]),
     "dpsnn_device_exp": (ctypes.c_int, [ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p,
                                         ctypes.c_uint64]),
+    "dpsnn_batch_create": (ctypes.c_int, [ctypes.c_int, _p(ctypes.c_void_p)]),
+    "dpsnn_batch_add": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p]),
+    "dpsnn_batch_remove": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p]),
+    "dpsnn_batch_size": (ctypes.c_uint32, [ctypes.c_void_p]),
+    "dpsnn_batch_step": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint32]),
+    "dpsnn_batch_sync": (ctypes.c_int, [ctypes.c_void_p]),
+    "dpsnn_batch_tune": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint32, _p(BatchShape)]),
+    "dpsnn_batch_get_shape": (ctypes.c_int, [ctypes.c_void_p, _p(BatchShape)]),
+    "dpsnn_batch_set_shape": (ctypes.c_int, [ctypes.c_void_p, _p(BatchShape)]),
+    "dpsnn_batch_profile": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
+    "dpsnn_batch_kernel_time": (ctypes.c_int, [ctypes.c_void_p, _p(ctypes.c_double), _p(ctypes.c_uint64)]),
+    "dpsnn_batch_destroy": (None, [ctypes.c_void_p]),
     "dpsnn_last_error": (ctypes.c_char_p, []),
 }
 
@@ -327,6 +347,78 @@ def device_exp(x, device=0):
     y = np.empty_like(x)
     _check("dpsnn_device_exp", library().dpsnn_device_exp(device, _ptr(x), _ptr(y), x.size))
     return y
+
+
+class Batch:
+    """Independent networks advancing in one persistent launch per step (a
+    dpsnn_batch).  Members are Network objects; while they belong to the
+    batch they are stepped only through it."""
+
+    def __init__(self, device=0):
+        self._lib = library()
+        h = ctypes.c_void_p()
+        _check("dpsnn_batch_create", self._lib.dpsnn_batch_create(device, ctypes.byref(h)))
+        self._h = h
+        self.members = []
+
+    def add(self, net):
+        _check("dpsnn_batch_add", self._lib.dpsnn_batch_add(self._h, net._h))
+        self.members.append(net)
+
+    def remove(self, net):
+        _check("dpsnn_batch_remove", self._lib.dpsnn_batch_remove(self._h, net._h))
+        self.members.remove(net)
+
+    def __len__(self):
+        return int(self._lib.dpsnn_batch_size(self._h))
+
+    def step(self, nsteps=1):
+        """Returns True once no member has steps left."""
+        rc = self._lib.dpsnn_batch_step(self._h, nsteps)
+        _check("dpsnn_batch_step", rc)
+        return rc == 1
+
+    def sync(self):
+        _check("dpsnn_batch_sync", self._lib.dpsnn_batch_sync(self._h))
+
+    def tune(self, trial_steps=2):
+        s = BatchShape()
+        _check("dpsnn_batch_tune", self._lib.dpsnn_batch_tune(self._h, trial_steps, ctypes.byref(s)))
+        return s.as_dict()
+
+    def shape(self):
+        s = BatchShape()
+        _check("dpsnn_batch_get_shape", self._lib.dpsnn_batch_get_shape(self._h, ctypes.byref(s)))
+        return s.as_dict()
+
+    def set_shape(self, chunks, buckets, waves_per_cu=0):
+        s = BatchShape(chunks, buckets, waves_per_cu, 0.0)
+        _check("dpsnn_batch_set_shape", self._lib.dpsnn_batch_set_shape(self._h, ctypes.byref(s)))
+
+    def profile(self, enable=True):
+        _check("dpsnn_batch_profile", self._lib.dpsnn_batch_profile(self._h, int(enable)))
+
+    def kernel_time(self):
+        ms = ctypes.c_double()
+        n = ctypes.c_uint64()
+        _check("dpsnn_batch_kernel_time", self._lib.dpsnn_batch_kernel_time(self._h, ctypes.byref(ms),
+                                                                            ctypes.byref(n)))
+        return ms.value, n.value
+
+    def close(self):
+        if getattr(self, "_h", None):
+            self._lib.dpsnn_batch_destroy(self._h)
+            self._h = None
+            self.members = []
+
+    def __del__(self):
+        self.close()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
 
 
 # ---- the reference's process interface (DAL shape) ------------------------

@@ -71,6 +71,15 @@ static int fail(int code, const char *fmt, ...) {
   } while (0)
 
 constexpr int kWave = 64;
+constexpr double kNotFired = -1.0;  // Te history entry of a neuron that did not fire (spike times are >= 0)
+
+// LDS written by some lanes of a wave and read by others: order the wave's
+// accesses around this point.
+__device__ inline void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
 constexpr int kWaveLTP = 256;    // STDP kernels: 4 waves per block
 
 // Per-sub-population LIFCA parameters, in this order.
@@ -88,11 +97,12 @@ struct DevNet {
   double *v, *c, *Tr, *Te, *assigned;
   uint64_t *philoxCtr;
   uint64_t *bits;          // [RS][NW]
-  double *teHist;          // [RS][N]
+  double *teHist;          // [RS][N] emission time, or kNotFired
   const uint64_t *inOff;   // [N+1] start of each neuron's in-list
   const uint32_t *inPD;    // pre | delay << 24, in delivery insertion order
   const int16_t *inW;
-  uint64_t *keys;          // sorted event keys: [grid][64][cap] (fused) or [N][cap] (split)
+  uint64_t *keys;          // sorted event ticks: [grid][64][cap] (fused) or [N][cap] (split)
+  int16_t *keyW;           // their weights, same layout
   uint32_t *keyCount;      // [grid][64] or [N]
   unsigned long long *counters;  // [0] events, [1] external, [2] spikes
   // STDP (null when plasticity never runs).  Synapse state is kept in
@@ -192,8 +202,10 @@ struct Lifca {
 // consecutive neurons j0..j0+63 (j0 % 64 == 0) in step t: one bitmap word,
 // the emission times of the neurons that fired, their last-spike ms
 // (setLastEmittedSpikeTime_ms, DPSNN_neuron_init.c:200-202) and the
-// per-(column, subPop) counts.  Every word of the slot is written by
-// exactly one wave, so the slot needs no clearing.  Whole wave, uniform.
+// per-(column, subPop) counts.  Every word of the slot and every neuron's
+// entry in the slot's Te history (its emission time, or kNotFired) is
+// written by exactly one wave, so the slot needs no clearing.  Whole wave,
+// uniform.
 // --------------------------------------------------------------------------
 __device__ inline uint64_t publish_spikes(const DevNet &P, uint32_t t1, uint32_t j, bool f, double te,
                                           uint32_t *popCnt, int count) {
@@ -203,8 +215,8 @@ __device__ inline uint64_t publish_spikes(const DevNet &P, uint32_t t1, uint32_t
     P.bits[(size_t)slot * P.NW + (j >> 6)] = mask;
     if (mask && count) atomicAdd(&P.counters[2], (unsigned long long)__popcll(mask));
   }
+  if (j < P.N) P.teHist[(size_t)slot * P.N + j] = f ? te : kNotFired;
   if (f) {
-    P.teHist[(size_t)slot * P.N + j] = te;
     if (P.lastSpike) P.lastSpike[j] = (int32_t)t1;
     if (popCnt) atomicAdd(&popCnt[(j / P.npc) * P.nsub + P.subPop[j]], 1u);
   }
@@ -378,78 +390,183 @@ __device__ inline uint64_t lane_mask_lt() {
   return lane ? (~0ull >> (64 - lane)) : 0ull;
 }
 
-// Phase 2: merge the sorted internal events of neuron j (keys ks[0..n))
-// with its external bath and run LIFCADynamic for each; returns the number
-// of external events and whether (and when) the neuron fired.
-__device__ inline uint32_t integrate_neuron(const DevNet &P, uint32_t t, const ExtView &X, uint32_t j,
-                                            const uint64_t *ks, uint32_t n, bool &fired, double &te) {
-  const double tF = (double)t, nextF = (double)(tF + 1.0);
-  const uint64_t tBits = dpsnn_asu64(tF);
-  const uint64_t beg = P.inOff[j];
-  const uint32_t sp = P.subPop[j];
-  const double *prm = P.sp + (size_t)sp * kParams;
-  Lifca L;
-  L.v = P.v[j]; L.c = P.c[j]; L.Tr = P.Tr[j]; L.Te = P.Te[j];
-  L.Tau = prm[kTau]; L.Theta = prm[kTheta]; L.H = prm[kH]; L.Tarp = prm[kTarp];
-  L.AlphaC = prm[kAlphaC]; L.TauC = prm[kTauC]; L.gC = prm[kGC];
-  L.fired = false;
-  uint32_t nExt = 0, ii = 0;
-  double kI = 0;
-  float wI = 0;
-  auto load_internal = [&](uint32_t i) {
-    const uint64_t key = ks[i];
-    kI = dpsnn_asf64(tBits + (key >> 12));
-    wI = ((float)P.inW[beg + (key & 0xFFFu)]) * P.w2f;
-  };
-  if (n) load_internal(0);
-  if (P.drive == DPSNN_DRIVE_REPLAY) {
-    uint32_t x = X.off[j];
-    const uint32_t xe = X.off[j + 1];
-    nExt = xe - x;
-    while (ii < n || x < xe) {
-      if (ii < n && (x >= xe || kI <= X.time[x])) {
-        L.event(kI, (double)wI);
-        if (++ii < n) load_internal(ii);
-      } else {
-        L.event(X.time[x], X.value[x]);
-        ++x;
-      }
-    }
-  } else {
-    const double inv = P.invNuExt[j];
-    const uint32_t cap = P.maxExt[j];
-    const double *lut = P.lut + (size_t)sp * kLutDepth;
-    double a = P.assigned[j];
-    PhiloxStream R{(uint32_t)P.philoxSeed, (uint32_t)(P.philoxSeed >> 32) ^ j, P.philoxCtr[j], 0.0, 0};
-    const bool onE = inv > 0;
-    for (;;) {
-      // the reference's do-while: the first due event is always drawn,
-      // the following ones while fewer than the per-ms cap
-      const bool hasE = onE && a < nextF && (nExt == 0 || nExt < cap);
-      if (ii < n && (!hasE || kI <= a)) {
-        L.event(kI, (double)wI);
-        if (++ii < n) load_internal(ii);
-      } else if (hasE) {
-        const double val = lut[(uint32_t)(R.next() * kLutDepth)];
-        L.event(a, val);
-        ++nExt;
-        const double u = 1.0 - R.next();
-        a += -logf((float)u) * inv;
-      } else {
-        break;
-      }
-    }
+// External bath of one neuron in step t, one event at a time in time order
+// (due(): an event is pending at time at(); take() returns its value and
+// moves on).
+struct ExtReplayCursor {
+  const double *time, *value;
+  uint32_t x, xe;
+  __device__ inline bool due() const { return x < xe; }
+  __device__ inline double at() const { return time[x]; }
+  __device__ inline double take() { return value[x++]; }
+  __device__ inline void finish() {}
+};
+
+// The reference's Poisson bath (DPSNN_neuron_sim.c:604-640, :844-851) from
+// the Philox stream: the first due event is always drawn (do-while), the
+// following ones while fewer than the per-ms cap; past the cap only the
+// lapses are drawn.
+struct ExtPhiloxCursor {
+  double a, inv, nextF;
+  const double *lut;
+  uint32_t cap, nExt;
+  bool onE;
+  PhiloxStream R;
+  __device__ inline bool due() const { return onE && a < nextF && (nExt == 0 || nExt < cap); }
+  __device__ inline double at() const { return a; }
+  __device__ inline double take() {
+    const double val = lut[(uint32_t)(R.next() * kLutDepth)];
+    ++nExt;
+    const double u = 1.0 - R.next();
+    a += -logf((float)u) * inv;
+    return val;
+  }
+  __device__ inline void finish() {
     if (onE && nExt >= cap && a < nextF) {
       do {
         const double u = 1.0 - R.next();
         a += -logf((float)u) * inv;
       } while (a < nextF);
     }
-    P.assigned[j] = a;
-    P.philoxCtr[j] = R.ctr;
   }
-  P.v[j] = L.v; P.c[j] = L.c; P.Tr[j] = L.Tr; P.Te[j] = L.Te;
-  fired = L.fired;
+};
+
+constexpr int kChunk = 8;  // internal events per lane per LDS-staged chunk
+// LDS of one wave in phase 2: two chunks of 64 lanes' ticks
+constexpr size_t kStageBytes = 2 * kChunk * kWave * sizeof(uint64_t);
+
+// Internal events of neuron q's chunk c, loaded cooperatively: instruction
+// r covers neurons 8r..8r+7, 8 consecutive ticks each (64-byte runs).
+__device__ inline void load_chunk(const uint64_t *rowsK, uint32_t cap, uint32_t n, uint32_t c,
+                                  uint64_t (&rk)[kChunk]) {
+  const uint32_t lane = threadIdx.x & 63, e = lane & 7;
+#pragma unroll
+  for (int r = 0; r < kChunk; ++r) {
+    const uint32_t q = (uint32_t)r * 8 + (lane >> 3);
+    const uint32_t nq = (uint32_t)__shfl((int)n, (int)q);
+    const uint32_t i = c * kChunk + e;
+    rk[r] = i < nq ? rowsK[(size_t)q * cap + i] : 0ull;
+  }
+}
+__device__ inline void store_chunk(uint64_t *stage, const uint64_t (&rk)[kChunk]) {
+  const uint32_t lane = threadIdx.x & 63, e = lane & 7;
+#pragma unroll
+  for (int r = 0; r < kChunk; ++r) stage[e * kWave + (uint32_t)r * 8 + (lane >> 3)] = rk[r];
+}
+__device__ inline uint4 load_weights(const int16_t *rowsW, uint32_t cap, uint32_t n, uint32_t c) {
+  const uint32_t lane = threadIdx.x & 63, i0 = c * kChunk;
+  if (i0 >= n) return make_uint4(0, 0, 0, 0);
+  return *(const uint4 *)(rowsW + (size_t)lane * cap + i0);
+}
+
+// Phase 2, one lane per neuron: lane q integrates neuron j0 + q (q < m) over
+// its n sorted internal events (rows rowsK/rowsW + q*cap, ticks relative to
+// bits(t) and int16 weights) merged with its external bath, LIFCADynamic
+// per event in FP64.  The ticks are staged through LDS (stage, kStageBytes)
+// a chunk ahead.  Returns the lane's external event count and whether (and
+// when) its neuron fired.  Whole wave, uniform.
+template <bool REPLAY>
+__device__ inline uint32_t integrate_wave(const DevNet &P, uint32_t t, const ExtView &X, uint32_t j0, uint32_t m,
+                                          const uint64_t *rowsK, const int16_t *rowsW, uint32_t cap, uint32_t n,
+                                          uint64_t *stage, bool &fired, double &te) {
+  const uint32_t lane = threadIdx.x & 63;
+  const bool on = lane < m;
+  const uint32_t j = j0 + lane;
+  if (!on) n = 0;
+  uint32_t nmax = n;
+  for (int o = 32; o > 0; o >>= 1) nmax = max(nmax, (uint32_t)__shfl_xor((int)nmax, o));
+  const double tF = (double)t, nextF = (double)(tF + 1.0);
+  const uint64_t tBits = dpsnn_asu64(tF);
+  Lifca L;
+  uint32_t sp = 0;
+  if (on) {
+    sp = P.subPop[j];
+    const double *prm = P.sp + (size_t)sp * kParams;
+    L.v = P.v[j]; L.c = P.c[j]; L.Tr = P.Tr[j]; L.Te = P.Te[j];
+    L.Tau = prm[kTau]; L.Theta = prm[kTheta]; L.H = prm[kH]; L.Tarp = prm[kTarp];
+    L.AlphaC = prm[kAlphaC]; L.TauC = prm[kTauC]; L.gC = prm[kGC];
+  }
+  L.fired = false;
+  ExtReplayCursor xr{};
+  ExtPhiloxCursor xp{};
+  if (on) {
+    if (REPLAY) {
+      xr = ExtReplayCursor{X.time, X.value, X.off[j], X.off[j + 1]};
+    } else {
+      xp.a = P.assigned[j];
+      xp.inv = P.invNuExt[j];
+      xp.nextF = nextF;
+      xp.lut = P.lut + (size_t)sp * kLutDepth;
+      xp.cap = P.maxExt[j];
+      xp.nExt = 0;
+      xp.onE = xp.inv > 0;
+      xp.R = PhiloxStream{(uint32_t)P.philoxSeed, (uint32_t)(P.philoxSeed >> 32) ^ j, P.philoxCtr[j], 0.0, 0};
+    }
+  }
+  // one merge loop with a single LIFCADynamic site: internal events from
+  // the staged chunk, the bath event instead when it is strictly earlier
+  // (internal first on equal times), and the rest of the bath once the
+  // lane's internal events are used up
+  const uint32_t chunks = (nmax + kChunk - 1) / kChunk;
+  const uint32_t iters = chunks ? chunks : 1u;
+  uint64_t rk[kChunk];
+  uint4 wCur = make_uint4(0, 0, 0, 0), wNext = wCur;
+  if (chunks) {
+    load_chunk(rowsK, cap, n, 0, rk);
+    wCur = load_weights(rowsW, cap, n, 0);
+    store_chunk(stage, rk);
+    wave_sync();
+  }
+  for (uint32_t c = 0; c < iters; ++c) {
+    const uint64_t *cur = stage + (c & 1) * (kChunk * kWave);
+    const bool more = c + 1 < chunks;
+    if (more) {
+      load_chunk(rowsK, cap, n, c + 1, rk);
+      wNext = load_weights(rowsW, cap, n, c + 1);
+    }
+    uint64_t wlo = (uint64_t)wCur.x | ((uint64_t)wCur.y << 32), whi = (uint64_t)wCur.z | ((uint64_t)wCur.w << 32);
+    const uint32_t left = n > c * kChunk ? min((uint32_t)kChunk, n - c * kChunk) : 0u;
+    const bool last = c * kChunk + left >= n;
+    uint32_t e = 0;
+    for (;;) {
+      const bool hasI = e < left;
+      const double kI = hasI ? dpsnn_asf64(tBits + cur[e * kWave + lane]) : 0.0;
+      const bool eDue = REPLAY ? xr.due() : xp.due();
+      double at, val;
+      if (eDue && (hasI ? (REPLAY ? xr.at() : xp.at()) < kI : last)) {
+        at = REPLAY ? xr.at() : xp.at();
+        val = REPLAY ? xr.take() : xp.take();
+      } else if (hasI) {
+        const int16_t w16 = (int16_t)(uint16_t)wlo;
+        wlo = (wlo >> 16) | (whi << 48);
+        whi >>= 16;
+        at = kI;
+        val = (double)(((float)w16) * P.w2f);
+        ++e;
+      } else {
+        break;
+      }
+      L.event(at, val);
+    }
+    if (more) {
+      store_chunk(stage + ((c + 1) & 1) * (kChunk * kWave), rk);
+      wCur = wNext;
+    }
+    wave_sync();
+  }
+  uint32_t nExt = 0;
+  if (on) {
+    if (REPLAY) {
+      nExt = xr.xe - X.off[j];
+    } else {
+      xp.finish();
+      nExt = xp.nExt;
+      P.assigned[j] = xp.a;
+      P.philoxCtr[j] = xp.R.ctr;
+    }
+    P.v[j] = L.v; P.c[j] = L.c; P.Tr[j] = L.Tr; P.Te[j] = L.Te;
+  }
+  fired = on && L.fired;
   te = L.Te;
   return nExt;
 }
@@ -465,14 +582,16 @@ __device__ inline uint32_t key_bucket(uint64_t tick, int shift) {
 }
 
 // Keys of in-list items base + k*64 + lane (k < K) of neuron j; kNoKey for
-// items that are absent or not live.  With `first`, also counts the live
-// keys per bucket (LDS) and applies LTD to their synapses.  All K chunks'
-// loads are independent: the in-list words, then the spike-history words,
-// then the emission times, so a wave keeps 3K loads in flight.
+// items that are absent or not live (the source did not fire in step
+// t - delay: its entry in that slot's Te history is kNotFired).  With
+// `first`, also counts the live keys per bucket (LDS), keeps their weights
+// in LDS by position (ldsW) and applies LTD to their synapses.  The K
+// chunks' loads are independent: the in-list words and weights, then the
+// emission times, so a wave keeps 2K loads in flight per round.
 template <int K, int B>
 __device__ inline uint32_t gather_keys(const DevNet &P, uint32_t t, uint32_t j, uint64_t beg, uint32_t len,
-                                       uint32_t base, uint64_t (&key)[K], uint32_t *bcnt, int shift,
-                                       uint64_t tBits, bool first, int plastic) {
+                                       uint32_t base, uint64_t (&key)[K], uint32_t *bcnt, int16_t *ldsW,
+                                       int shift, uint64_t tBits, bool first, int plastic) {
   const uint32_t lane = threadIdx.x & 63;
   const double tF = (double)t;
   uint32_t it[K];
@@ -481,29 +600,28 @@ __device__ inline uint32_t gather_keys(const DevNet &P, uint32_t t, uint32_t j, 
     const uint32_t idx = base + k * kWave + lane;
     it[k] = idx < len ? P.inPD[beg + idx] : kPad;
   }
+  double te[K];
+  int16_t w[K];
 #pragma unroll
   for (int k = 0; k < K; ++k) {
-    key[k] = kNoKey;
     const uint32_t d = it[k] >> 24, pre = it[k] & 0xFFFFFFu;
-    if (it[k] != kPad && d <= t) {
-      const uint32_t s = (t - d) % P.RS;
-      const uint64_t word = P.bits[(size_t)s * P.NW + (pre >> 6)];
-      if ((word >> (pre & 63)) & 1ull) key[k] = s;  // live: remember the slot for now
-    }
+    te[k] = kNotFired;
+    if (it[k] != kPad && d <= t) te[k] = P.teHist[(size_t)((t - d) % P.RS) * P.N + pre];
+    if (first) w[k] = it[k] != kPad ? P.inW[beg + base + k * kWave + lane] : (int16_t)0;
   }
   uint32_t n = 0;
 #pragma unroll
   for (int k = 0; k < K; ++k) {
-    const bool on = key[k] != kNoKey;
+    const bool on = te[k] >= 0.0;
+    key[k] = kNoKey;
     if (on) {
-      const uint32_t s = (uint32_t)key[k], pre = it[k] & 0xFFFFFFu;
       const uint32_t idx = base + k * kWave + lane;
-      const double te = P.teHist[(size_t)s * P.N + pre];
-      const double kk = tF + (te - trunc(te));  // modf, exact
+      const double kk = tF + (te[k] - trunc(te[k]));  // modf, exact
       const uint64_t tick = dpsnn_asu64(kk) - tBits;
       key[k] = (tick << 12) | idx;
       if (first) {
         atomicAdd(&bcnt[key_bucket<B>(tick, shift)], 1u);
+        ldsW[idx] = w[k];
         if (plastic) {
           // LTD, synapseSetLastActiveTime_antiCausalSTDP
           // (DPSNN_neuron_sim.c:207-277): last = t, td += anti[t - lastSpike]
@@ -526,14 +644,24 @@ __device__ inline void scatter_keys(const uint64_t (&key)[K], uint32_t *bcnt, ui
     if (key[k] != kNoKey) sorted[atomicAdd(&bcnt[key_bucket<B>(key[k] >> 12, shift)], 1u)] = key[k];
 }
 
+// LDS of one phase-1 wave: sort buffer (u64) and weights by position (i16)
+// for the longest in-list, then the bucket counters.
+__host__ __device__ inline size_t phase1_lds(uint32_t cap, int B) {
+  return (size_t)cap * sizeof(uint64_t) + (size_t)cap * sizeof(int16_t) + (size_t)(B + 1) * sizeof(uint32_t);
+}
+
 // Phase 1 for neuron j, whole wave: live keys of its in-list, bucket-sorted
-// in LDS (sorted[len], bcnt[B + 1]) and stored to out[0..n); returns n.
+// in LDS (lds: sorted[cap], ldsW[cap], bcnt[B + 1]), stored to outK[0..n) as
+// ticks relative to bits(t) with their weights in outW[0..n); returns n.
 template <int K, int B>
-__device__ inline uint32_t gather_sort_neuron(const DevNet &P, uint32_t t, uint32_t j, int plastic,
-                                              uint64_t *sorted, uint32_t *bcnt, uint64_t *out) {
+__device__ inline uint32_t gather_sort_neuron(const DevNet &P, uint32_t t, uint32_t j, int plastic, uint8_t *lds,
+                                              uint32_t cap, uint64_t *outK, int16_t *outW) {
   static_assert(B % kWave == 0, "bucket count must be a multiple of 64");
   constexpr int R = B / kWave;  // buckets per lane
   constexpr int LOG2B = B == 64 ? 6 : B == 128 ? 7 : B == 256 ? 8 : B == 512 ? 9 : 10;
+  uint64_t *sorted = (uint64_t *)lds;
+  int16_t *ldsW = (int16_t *)(lds + (size_t)cap * sizeof(uint64_t));
+  uint32_t *bcnt = (uint32_t *)(lds + (size_t)cap * (sizeof(uint64_t) + sizeof(int16_t)));
   const uint32_t lane = threadIdx.x & 63;
   const uint64_t tBits = dpsnn_asu64((double)t);
   const int e = t ? (31 - __clz((int)t)) : 0;
@@ -547,7 +675,7 @@ __device__ inline uint32_t gather_sort_neuron(const DevNet &P, uint32_t t, uint3
   uint64_t key[K];
   if (t > 0)
     for (uint32_t base = 0; base < len; base += kSuper)
-      n += gather_keys<K, B>(P, t, j, beg, len, base, key, bcnt, shift, tBits, true, plastic);
+      n += gather_keys<K, B>(P, t, j, beg, len, base, key, bcnt, ldsW, shift, tBits, true, plastic);
   __syncthreads();
   // exclusive scan of the bucket counters (R per lane)
   {
@@ -573,7 +701,7 @@ __device__ inline uint32_t gather_sort_neuron(const DevNet &P, uint32_t t, uint3
       scatter_keys<K, B>(key, bcnt, sorted, shift);
     } else {
       for (uint32_t base = 0; base < len; base += kSuper) {
-        gather_keys<K, B>(P, t, j, beg, len, base, key, bcnt, shift, tBits, false, 0);
+        gather_keys<K, B>(P, t, j, beg, len, base, key, bcnt, ldsW, shift, tBits, false, 0);
         scatter_keys<K, B>(key, bcnt, sorted, shift);
       }
     }
@@ -594,7 +722,11 @@ __device__ inline uint32_t gather_sort_neuron(const DevNet &P, uint32_t t, uint3
     }
   }
   __syncthreads();
-  for (uint32_t i = lane; i < n; i += kWave) out[i] = sorted[i];
+  for (uint32_t i = lane; i < n; i += kWave) {
+    const uint64_t k = sorted[i];
+    outK[i] = k >> 12;
+    outW[i] = ldsW[k & 0xFFFu];
+  }
   __syncthreads();
   return n;
 }
@@ -607,30 +739,42 @@ struct NextScan {
   int count;  // step t+1 is simulated: its scan counts the spikes
 };
 
+// LDS of one wave of the fused step: phase 1's sort, then phase 2's stage.
+__host__ __device__ inline size_t step_lds(uint32_t cap, int B) {
+  const size_t p1 = phase1_lds(cap, B);
+  return p1 > kStageBytes ? p1 : kStageBytes;
+}
+
 // One tile (neurons 64*tile .. 64*tile+63) of step t, one wave: phase 1 for
-// each neuron into the wave's scratch wkeys[64][cap] / wcnt[64], phase 2 one
-// lane per neuron, then the fired scan of step t+1.  Returns the tile's
-// delivered events (uniform) and adds external events to *exSum (per lane).
+// each neuron into the wave's scratch rows wkeys/wW[64][cap] and wcnt[64],
+// phase 2 one lane per neuron, then the fired scan of step t+1.  Returns the
+// tile's delivered events (uniform) and adds external events to *exSum (per
+// lane).
 template <int K, int B>
 __device__ inline unsigned long long step_tile(const DevNet &P, uint32_t t, const ExtView &X, int ltd,
-                                               const NextScan &S, uint32_t tile, uint32_t cap, uint64_t *sorted,
-                                               uint32_t *bcnt, uint64_t *wkeys, uint32_t *wcnt,
+                                               const NextScan &S, uint32_t tile, uint32_t cap, uint8_t *lds,
+                                               uint64_t *wkeys, int16_t *wW, uint32_t *wcnt,
                                                unsigned long long &exSum) {
   const uint32_t lane = threadIdx.x & 63;
   const uint32_t j0 = tile * kWave;
   const uint32_t m = min((uint32_t)kWave, P.N - j0);
   unsigned long long evSum = 0;
   for (uint32_t q = 0; q < m; ++q) {
-    const uint32_t n = gather_sort_neuron<K, B>(P, t, j0 + q, ltd, sorted, bcnt, wkeys + (size_t)q * cap);
+    const uint32_t n =
+        gather_sort_neuron<K, B>(P, t, j0 + q, ltd, lds, cap, wkeys + (size_t)q * cap, wW + (size_t)q * cap);
     if (lane == 0) wcnt[q] = n;
     evSum += n;
   }
   __threadfence_block();
   __syncthreads();
   // ---------------- phase 2 ----------------
+  const uint32_t n = lane < m ? wcnt[lane] : 0u;
   bool f = false;
   double te = 0.0;
-  if (lane < m) exSum += integrate_neuron(P, t, X, j0 + lane, wkeys + (size_t)lane * cap, wcnt[lane], f, te);
+  if (P.drive == DPSNN_DRIVE_REPLAY)
+    exSum += integrate_wave<true>(P, t, X, j0, m, wkeys, wW, cap, n, (uint64_t *)lds, f, te);
+  else
+    exSum += integrate_wave<false>(P, t, X, j0, m, wkeys, wW, cap, n, (uint64_t *)lds, f, te);
   // ---------------- fired scan of step t+1 ----------------
   const uint64_t mask = publish_spikes(P, t + 1, j0 + lane, f, te, S.popCnt, S.count);
   if (S.ltp) ltp_word(P, t + 1, mask, j0);
@@ -642,13 +786,12 @@ __device__ inline unsigned long long step_tile(const DevNet &P, uint32_t t, cons
 template <int K, int B>
 __global__ __launch_bounds__(kWave) void k_step(DevNet P, uint32_t t, ExtView X, int ltd, NextScan S) {
   extern __shared__ uint64_t lds[];
-  uint64_t *sorted = lds;                              // [cap]
-  uint32_t *bcnt = (uint32_t *)(lds + (size_t)P.cap);  // [B + 1]
   uint64_t *wkeys = P.keys + (size_t)blockIdx.x * kWave * P.cap;
+  int16_t *wW = P.keyW + (size_t)blockIdx.x * kWave * P.cap;
   uint32_t *wcnt = P.keyCount + (size_t)blockIdx.x * kWave;
   unsigned long long evSum = 0, exSum = 0;
   for (uint32_t tile = blockIdx.x; tile < P.NW; tile += gridDim.x)
-    evSum += step_tile<K, B>(P, t, X, ltd, S, tile, P.cap, sorted, bcnt, wkeys, wcnt, exSum);
+    evSum += step_tile<K, B>(P, t, X, ltd, S, tile, P.cap, (uint8_t *)lds, wkeys, wW, wcnt, exSum);
   for (int o = 32; o > 0; o >>= 1) exSum += __shfl_down(exSum, o);
   if (threadIdx.x == 0) {
     atomicAdd(&P.counters[0], evSum);
@@ -663,6 +806,7 @@ struct BatchItem {
   NextScan S;
   ExtView X;
   uint32_t tileBegin;  // first global tile of this network
+  uint32_t net;        // index into the batch's DevNet array
 };
 
 // Batched step: many independent networks (each at its own ms) in one
@@ -672,11 +816,10 @@ struct BatchItem {
 template <int K, int B>
 __global__ __launch_bounds__(kWave) void k_batch_step(const DevNet *nets, const BatchItem *items, uint32_t count,
                                                       uint32_t totalTiles, uint32_t cap, uint64_t *keys,
-                                                      uint32_t *keyCount) {
+                                                      int16_t *keyW, uint32_t *keyCount) {
   extern __shared__ uint64_t lds[];
-  uint64_t *sorted = lds;
-  uint32_t *bcnt = (uint32_t *)(lds + (size_t)cap);
   uint64_t *wkeys = keys + (size_t)blockIdx.x * kWave * cap;
+  int16_t *wW = keyW + (size_t)blockIdx.x * kWave * cap;
   uint32_t *wcnt = keyCount + (size_t)blockIdx.x * kWave;
   for (uint32_t g = blockIdx.x; g < totalTiles; g += gridDim.x) {
     uint32_t lo = 0, hi = count;  // last item with tileBegin <= g
@@ -684,11 +827,11 @@ __global__ __launch_bounds__(kWave) void k_batch_step(const DevNet *nets, const 
       const uint32_t mid = (lo + hi) / 2;
       if (items[mid].tileBegin <= g) lo = mid; else hi = mid;
     }
-    const DevNet &P = nets[lo];
     const BatchItem &it = items[lo];
+    const DevNet &P = nets[it.net];
     unsigned long long exSum = 0;
-    const unsigned long long ev =
-        step_tile<K, B>(P, it.t, it.X, it.ltd, it.S, g - it.tileBegin, cap, sorted, bcnt, wkeys, wcnt, exSum);
+    const unsigned long long ev = step_tile<K, B>(P, it.t, it.X, it.ltd, it.S, g - it.tileBegin, cap,
+                                                  (uint8_t *)lds, wkeys, wW, wcnt, exSum);
     for (int o = 32; o > 0; o >>= 1) exSum += __shfl_down(exSum, o);
     if (threadIdx.x == 0) {
       atomicAdd(&P.counters[0], ev);
@@ -697,18 +840,17 @@ __global__ __launch_bounds__(kWave) void k_batch_step(const DevNet *nets, const 
   }
 }
 
-// Split form: k_gather sorts every neuron's live events into its own slot
-// of a network-wide key buffer (one wave per neuron), then k_integrate runs
+// Split form: k_gather sorts every neuron's live events into its own rows
+// of network-wide buffers (one wave per neuron), then k_integrate runs
 // phase 2 one lane per neuron and the fired scan of step t+1.  Each kernel
 // gets its own register budget.
 template <int K, int B>
 __global__ __launch_bounds__(kWave) void k_gather(DevNet P, uint32_t t, int ltd) {
   extern __shared__ uint64_t lds[];
-  uint64_t *sorted = lds;
-  uint32_t *bcnt = (uint32_t *)(lds + (size_t)P.cap);
   unsigned long long evSum = 0;
   for (uint32_t j = blockIdx.x; j < P.N; j += gridDim.x) {
-    const uint32_t n = gather_sort_neuron<K, B>(P, t, j, ltd, sorted, bcnt, P.keys + (size_t)j * P.cap);
+    const uint32_t n = gather_sort_neuron<K, B>(P, t, j, ltd, (uint8_t *)lds, P.cap, P.keys + (size_t)j * P.cap,
+                                                P.keyW + (size_t)j * P.cap);
     if (threadIdx.x == 0) P.keyCount[j] = n;
     evSum += n;
   }
@@ -718,15 +860,30 @@ __global__ __launch_bounds__(kWave) void k_gather(DevNet P, uint32_t t, int ltd)
 // Blocks of kBlock threads: each wave holds 64 consecutive neurons starting
 // at a multiple of 64, as publish_spikes requires.
 __global__ __launch_bounds__(kBlock) void k_integrate(DevNet P, uint32_t t, ExtView X, NextScan S) {
-  const uint32_t j = blockIdx.x * kBlock + threadIdx.x;
-  unsigned long long ex = 0;
+  __shared__ uint64_t stage[kBlock / kWave][2 * kChunk * kWave];
+  const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const uint32_t j0 = blockIdx.x * kBlock + wave * kWave;
+  const uint32_t m = j0 < P.N ? min((uint32_t)kWave, P.N - j0) : 0u;
+  const uint32_t n = lane < m ? P.keyCount[j0 + lane] : 0u;
+  const uint64_t *rowsK = P.keys + (size_t)min(j0, P.N) * P.cap;
+  const int16_t *rowsW = P.keyW + (size_t)min(j0, P.N) * P.cap;
   bool f = false;
   double te = 0.0;
-  if (j < P.N) ex = integrate_neuron(P, t, X, j, P.keys + (size_t)j * P.cap, P.keyCount[j], f, te);
-  const uint64_t mask = publish_spikes(P, t + 1, j, f, te, S.popCnt, S.count);
-  if (S.ltp) ltp_word(P, t + 1, mask, j & ~63u);
+  unsigned long long ex;
+  if (P.drive == DPSNN_DRIVE_REPLAY)
+    ex = integrate_wave<true>(P, t, X, j0, m, rowsK, rowsW, P.cap, n, stage[wave], f, te);
+  else
+    ex = integrate_wave<false>(P, t, X, j0, m, rowsK, rowsW, P.cap, n, stage[wave], f, te);
+  const uint64_t mask = publish_spikes(P, t + 1, j0 + lane, f, te, S.popCnt, S.count);
+  if (S.ltp) ltp_word(P, t + 1, mask, j0);
   for (int o = 32; o > 0; o >>= 1) ex += __shfl_down(ex, o);
-  if ((threadIdx.x & 63) == 0 && ex) atomicAdd(&P.counters[1], ex);
+  if (lane == 0 && ex) atomicAdd(&P.counters[1], ex);
+}
+
+// Fills n doubles with v.
+__global__ void k_fill_f64(double *p, double v, uint64_t n) {
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
+    p[i] = v;
 }
 
 // Launch shapes: K = in-list chunks of 64 items held in registers per
@@ -734,7 +891,8 @@ __global__ __launch_bounds__(kBlock) void k_integrate(DevNet P, uint32_t t, ExtV
 // same result bit for bit; they differ only in speed.
 using StepFn = void (*)(DevNet, uint32_t, ExtView, int, NextScan);
 using GatherFn = void (*)(DevNet, uint32_t, int);
-using BatchFn = void (*)(const DevNet *, const BatchItem *, uint32_t, uint32_t, uint32_t, uint64_t *, uint32_t *);
+using BatchFn = void (*)(const DevNet *, const BatchItem *, uint32_t, uint32_t, uint32_t, uint64_t *, int16_t *,
+                         uint32_t *);
 struct KernelShape {
   int K, B;
   StepFn step;
@@ -756,9 +914,7 @@ static int find_shape(int K, int B) {
   return -1;
 }
 
-// LDS of one phase-1 wave: the sort buffer for the longest in-list and the
-// bucket counters.
-static size_t lds_bytes(uint32_t cap, int B) { return (size_t)cap * sizeof(uint64_t) + (B + 1) * sizeof(uint32_t); }
+static size_t lds_bytes(uint32_t cap, int B) { return step_lds(cap, B); }
 
 }  // namespace dpsnn
 
@@ -877,7 +1033,8 @@ static int build_device(dpsnn_net *n) {
   P.nsub = q.subPopNumber;
   P.npc = q.neuronsPerCM;
   P.ncols = q.globCFT;
-  P.cap = std::max<uint32_t>(c.max_in, 1);
+  // longest in-list, rounded to 8 (scratch rows of 16-byte weight chunks)
+  P.cap = (std::max<uint32_t>(c.max_in, 1) + 7u) & ~7u;
   P.drive = (uint32_t)n->opt.drive;
   P.w2f = q.factorWeightType_2_Float;
   P.philoxSeed = n->opt.philox_seed;
@@ -914,7 +1071,8 @@ static int build_device(dpsnn_net *n) {
   if ((rc = dev_alloc(n, &P.bits, size_t(P.RS) * P.NW))) return rc;
   HIP_TRY(hipMemsetAsync(P.bits, 0, sizeof(uint64_t) * size_t(P.RS) * P.NW, n->stream));
   if ((rc = dev_alloc(n, &P.teHist, size_t(P.RS) * N))) return rc;
-  HIP_TRY(hipMemsetAsync(P.teHist, 0, sizeof(double) * size_t(P.RS) * N, n->stream));
+  hipLaunchKernelGGL(k_fill_f64, dim3(1024), dim3(256), 0, n->stream, P.teHist, kNotFired, uint64_t(P.RS) * N);
+  HIP_TRY(hipGetLastError());
   // in-lists (CSR in delivery insertion order)
   if (c.max_in >= kMaxInList)
     return fail(DPSNN_ERR_CONFIG, "a neuron has %u in-list items; this build handles up to %u", c.max_in,
@@ -1001,10 +1159,9 @@ static int build_device(dpsnn_net *n) {
   n->stdpBlocks = std::min<uint32_t>((P.NW + 3) / 4, (uint32_t)cus * 8);
   n->split = tiles < (uint32_t)cus * 4;
   if (const char *e = std::getenv("DPSNN_SPLIT")) n->split = std::atoi(e) != 0;
-  const size_t keySlots = n->split ? size_t(N) : size_t(n->grid) * kWave;
   n->gatherGrid = std::min<uint32_t>(N, (uint32_t)cus * wavesPerCu * 4);
-  if ((rc = dev_alloc(n, &P.keys, keySlots * P.cap)) || (rc = dev_alloc(n, &P.keyCount, keySlots)))
-    return rc;
+  // the key scratch of stand-alone steps is allocated at the first
+  // dpsnn_step (a network stepped only by a batch uses the batch's)
   if ((rc = dev_alloc(n, &P.counters, 4))) return rc;
   HIP_TRY(hipMemsetAsync(P.counters, 0, 4 * sizeof(unsigned long long), n->stream));
   // raster / rates drain buffers
@@ -1211,33 +1368,56 @@ static bool plastic_at(const dpsnn_net *n, uint32_t t) {
   return n->stdp && t >= q.startPlasticity_ms && t <= q.stopPlasticity_ms;
 }
 
-// Step t: delivery + dynamics of t, and the fired scan of t+1 (which the
-// reference runs at the head of step t+1; nothing reads the scan's outputs
-// in between).  The scan of step 0 finds no spikes.
-static int one_step(dpsnn_net *n) {
+// Key scratch of stand-alone steps: [grid][64][cap] (fused) or [N][cap]
+// (split).
+static int ensure_scratch(dpsnn_net *n) {
+  DevNet &P = n->dev;
+  if (P.keys) return 0;
+  const size_t slots = n->split ? size_t(P.N) : size_t(n->grid) * kWave;
+  int rc;
+  if ((rc = dev_alloc(n, &P.keys, slots * P.cap)) || (rc = dev_alloc(n, &P.keyW, slots * P.cap)) ||
+      (rc = dev_alloc(n, &P.keyCount, slots)))
+    return rc;
+  return 0;
+}
+
+// What one step of a network needs around its kernel launch.
+struct StepPlan {
+  uint32_t t = 0;
+  int ltd = 0;
+  bool commitNow = false, ltpNext = false, recordNext = false;
+  uint32_t rec = 0;
+  NextScan S{nullptr, 0, 0};
+  ExtView X{nullptr, nullptr, nullptr};
+  ExtSlot *slot = nullptr;
+};
+
+// Host work before step t's kernel: raster/rates records, the replayed
+// bath of step t, and the flags of the step.
+static int pre_step(dpsnn_net *n, StepPlan &sp) {
   DevNet &P = n->dev;
   const uint32_t t = n->t, next = t + 1;
   const uint32_t T = n->par.pub.totalSimTime_ms;
   const size_t popPer = size_t(P.ncols) * P.nsub;
   int rc;
-  uint32_t rec;
+  sp = StepPlan{};
+  sp.t = t;
   if (n->collect && t == 0) {
-    if ((rc = open_record(n, 0, &rec)) || (rc = compact(n, 0, rec))) return rc;
+    uint32_t rec0;
+    if ((rc = open_record(n, 0, &rec0)) || (rc = compact(n, 0, rec0))) return rc;
   }
-  const int ltd = plastic_at(n, t);
-  const bool commitNow = ltd && t % 1000 == 999;
-  const bool ltpNext = next < T && plastic_at(n, next);
-  const bool recordNext = n->collect && next < T;
-  NextScan S{nullptr, ltpNext && !commitNow, next < T};
-  if (recordNext) {
-    if ((rc = open_record(n, next, &rec))) return rc;
-    S.popCnt = n->d_popCnt + popPer * rec;
+  sp.ltd = plastic_at(n, t);
+  sp.commitNow = sp.ltd && t % 1000 == 999;
+  sp.ltpNext = next < T && plastic_at(n, next);
+  sp.recordNext = n->collect && next < T;
+  sp.S = NextScan{nullptr, sp.ltpNext && !sp.commitNow, next < T};
+  if (sp.recordNext) {
+    if ((rc = open_record(n, next, &sp.rec))) return rc;
+    sp.S.popCnt = n->d_popCnt + popPer * sp.rec;
   }
-  ExtView X{nullptr, nullptr, nullptr};
-  ExtSlot *slot = nullptr;
   if (P.drive == DPSNN_DRIVE_REPLAY) {
     ExtSlot &s = n->ext[n->extNext];
-    slot = &s;
+    sp.slot = &s;
     n->extNext = (n->extNext + 1) % kExtRing;
     if (s.pending) HIP_TRY(hipEventSynchronize(s.done));
     n->replay.generate(t, n->extStep);
@@ -1253,32 +1433,49 @@ static int one_step(dpsnn_net *n) {
       HIP_TRY(hipMemcpyAsync(s.d_time, s.h_time, sizeof(double) * ne, hipMemcpyHostToDevice, n->stream));
       HIP_TRY(hipMemcpyAsync(s.d_value, s.h_value, sizeof(double) * ne, hipMemcpyHostToDevice, n->stream));
     }
-    X = ExtView{s.d_off, s.d_time, s.d_value};
+    sp.X = ExtView{s.d_off, s.d_time, s.d_value};
   }
-  if ((rc = prof_begin(n))) return rc;
-  const KernelShape &K = kShapes[n->shape];
-  if (n->split) {
-    hipLaunchKernelGGL(K.gather, dim3(n->gatherGrid), dim3(kWave), n->ldsBytes, n->stream, P, t, ltd);
-    const uint32_t blocks = (P.N + kBlock - 1) / kBlock;
-    hipLaunchKernelGGL(k_integrate, dim3(blocks), dim3(kBlock), 0, n->stream, P, t, X, S);
-  } else {
-    hipLaunchKernelGGL(K.step, dim3(n->grid), dim3(kWave), n->ldsBytes, n->stream, P, t, X, ltd, S);
+  return 0;
+}
+
+// Host work after step t's kernel: the per-second STDP commit (and the LTP
+// of the scan of t+1, which follows it), the raster of t+1.
+static int post_step(dpsnn_net *n, StepPlan &sp) {
+  int rc;
+  const uint32_t next = sp.t + 1;
+  if (sp.commitNow) {
+    if ((rc = commit(n, sp.t))) return rc;
+    if (sp.ltpNext) hipLaunchKernelGGL(k_ltp, dim3(n->stdpBlocks), dim3(kWaveLTP), 0, n->stream, n->dev, next);
   }
-  if ((rc = prof_end(n))) return rc;
-  if (commitNow) {
-    if ((rc = commit(n, t))) return rc;
-    // the scan of step t+1 (with its LTP) follows the commit
-    if (ltpNext) hipLaunchKernelGGL(k_ltp, dim3(n->stdpBlocks), dim3(kWaveLTP), 0, n->stream, P, next);
-  }
-  if (recordNext && (rc = compact(n, next, rec))) return rc;
-  if (slot) {
-    HIP_TRY(hipEventRecord(slot->done, n->stream));
-    slot->pending = true;
+  if (sp.recordNext && (rc = compact(n, next, sp.rec))) return rc;
+  if (sp.slot) {
+    HIP_TRY(hipEventRecord(sp.slot->done, n->stream));
+    sp.slot->pending = true;
   }
   HIP_TRY(hipGetLastError());
   n->t = next;
   if (n->opt.keep_spikes) return drain(n);
   return 0;
+}
+
+// Step t of one network: delivery + dynamics of t, and the fired scan of
+// t+1 (which the reference runs at the head of step t+1; nothing reads the
+// scan's outputs in between).  The scan of step 0 finds no spikes.
+static int one_step(dpsnn_net *n) {
+  StepPlan sp;
+  int rc;
+  if ((rc = pre_step(n, sp)) || (rc = prof_begin(n))) return rc;
+  const DevNet &P = n->dev;
+  const KernelShape &K = kShapes[n->shape];
+  if (n->split) {
+    hipLaunchKernelGGL(K.gather, dim3(n->gatherGrid), dim3(kWave), n->ldsBytes, n->stream, P, sp.t, sp.ltd);
+    const uint32_t blocks = (P.N + kBlock - 1) / kBlock;
+    hipLaunchKernelGGL(k_integrate, dim3(blocks), dim3(kBlock), 0, n->stream, P, sp.t, sp.X, sp.S);
+  } else {
+    hipLaunchKernelGGL(K.step, dim3(n->grid), dim3(kWave), n->ldsBytes, n->stream, P, sp.t, sp.X, sp.ltd, sp.S);
+  }
+  if ((rc = prof_end(n))) return rc;
+  return post_step(n, sp);
 }
 
 // ============================================================================
@@ -1342,6 +1539,7 @@ int dpsnn_create(const char *inidir, const dpsnn_options *opt, dpsnn_net **out) 
   if (n->writing) n->writer.init(n->par, n->opt.output_dir);
   HIP_TRY(hipSetDevice(n->opt.device));
   HIP_TRY(hipStreamCreateWithFlags(&n->stream, hipStreamNonBlocking));
+  n->ownStream = n->stream;
   int rc = build_device(n.get());
   if (rc) {
     free_all(n.get());
@@ -1360,7 +1558,10 @@ int dpsnn_create(const char *inidir, const dpsnn_options *opt, dpsnn_net **out) 
 
 int dpsnn_step(dpsnn_net *n, uint32_t nsteps) {
   if (!n) return fail(DPSNN_ERR_ARG, "null network");
+  if (n->batch) return fail(DPSNN_ERR_ARG, "network is a member of a batch: step it through dpsnn_batch_step");
   HIP_TRY(hipSetDevice(n->opt.device));
+  int rc0 = ensure_scratch(n);
+  if (rc0) return rc0;
   for (uint32_t k = 0; k < nsteps; ++k) {
     if (n->t >= n->par.pub.totalSimTime_ms) {
       int rc = drain(n);
@@ -1474,10 +1675,356 @@ int dpsnn_kernel_time(dpsnn_net *n, int kernel, double *avg_ms, uint64_t *launch
 
 void dpsnn_destroy(dpsnn_net *n) {
   if (!n) return;
+  if (n->batch) dpsnn_batch_remove(n->batch, n);
   drain(n);
   n->writer.close();
   free_all(n);
   delete n;
+}
+
+}  // extern "C"
+
+// ============================================================================
+// Batching: many independent networks advance in one persistent launch per
+// step (k_batch_step).  Members keep their own state, history, in-lists,
+// counters and output files; the batch owns the stream, the per-step item
+// ring and the key scratch.
+// ============================================================================
+struct dpsnn_batch {
+  int device = 0;
+  int cus = 1;
+  hipStream_t stream = nullptr;
+  std::vector<dpsnn_net *> nets;
+  bool dirty = true;
+  DevNet *d_nets = nullptr;
+  size_t netsCap = 0;
+  static constexpr int kRing = 4;
+  BatchItem *h_items[kRing] = {};
+  BatchItem *d_items[kRing] = {};
+  hipEvent_t done[kRing] = {};
+  bool pending[kRing] = {};
+  size_t itemsCap = 0;
+  int ringNext = 0;
+  uint64_t *keys = nullptr;
+  int16_t *keyW = nullptr;
+  uint32_t *keyCount = nullptr;
+  size_t scratchSlots = 0;
+  uint32_t scratchCap = 0;
+  int shape = -1;
+  uint32_t wavesPerCu = 0;
+  bool profile = false;
+  std::vector<std::pair<hipEvent_t, hipEvent_t>> ev;
+  size_t evUsed = 0;
+  std::vector<StepPlan> plans;
+  std::vector<dpsnn_net *> active;
+};
+
+namespace {
+
+uint32_t batch_cap(const dpsnn_batch *b) {
+  uint32_t cap = 1;
+  for (auto *n : b->nets) cap = std::max(cap, n->dev.cap);
+  return cap;
+}
+
+uint32_t lds_waves(size_t lds) {
+  return std::max<uint32_t>(1, std::min<uint32_t>(16, (uint32_t)((160u << 10) / lds)));
+}
+
+void batch_free_items(dpsnn_batch *b) {
+  for (int r = 0; r < dpsnn_batch::kRing; ++r) {
+    if (b->h_items[r]) hipHostFree(b->h_items[r]);
+    if (b->d_items[r]) hipFree(b->d_items[r]);
+    b->h_items[r] = nullptr;
+    b->d_items[r] = nullptr;
+  }
+  b->itemsCap = 0;
+}
+
+// Device copy of the members' DevNet structs, scratch for the longest
+// in-list at the largest grid any shape uses, item ring for every member.
+int batch_prepare(dpsnn_batch *b) {
+  if (!b->dirty) return 0;
+  HIP_TRY(hipStreamSynchronize(b->stream));
+  const size_t count = b->nets.size();
+  if (count > b->netsCap) {
+    if (b->d_nets) HIP_TRY(hipFree(b->d_nets));
+    b->d_nets = nullptr;
+    HIP_TRY(hipMalloc((void **)&b->d_nets, sizeof(DevNet) * count));
+    b->netsCap = count;
+  }
+  std::vector<DevNet> h(count);
+  for (size_t i = 0; i < count; ++i) h[i] = b->nets[i]->dev;
+  if (count) HIP_TRY(hipMemcpy(b->d_nets, h.data(), sizeof(DevNet) * count, hipMemcpyHostToDevice));
+  if (count > b->itemsCap) {
+    batch_free_items(b);
+    const size_t cap = count + 8;
+    for (int r = 0; r < dpsnn_batch::kRing; ++r) {
+      HIP_TRY(hipHostMalloc((void **)&b->h_items[r], sizeof(BatchItem) * cap, 0));
+      HIP_TRY(hipMalloc((void **)&b->d_items[r], sizeof(BatchItem) * cap));
+      b->pending[r] = false;
+    }
+    b->itemsCap = cap;
+  }
+  const uint32_t cap = batch_cap(b);
+  const size_t slots = (size_t)b->cus * lds_waves(lds_bytes(cap, 64)) * kWave;
+  if (cap > b->scratchCap || slots > b->scratchSlots) {
+    if (b->keys) HIP_TRY(hipFree(b->keys));
+    if (b->keyW) HIP_TRY(hipFree(b->keyW));
+    if (b->keyCount) HIP_TRY(hipFree(b->keyCount));
+    b->keys = nullptr;
+    b->keyW = nullptr;
+    b->keyCount = nullptr;
+    hipError_t e = hipMalloc((void **)&b->keys, sizeof(uint64_t) * slots * cap);
+    if (e != hipSuccess) return fail(DPSNN_ERR_MEMORY, "batch key scratch: %s", hipGetErrorString(e));
+    e = hipMalloc((void **)&b->keyW, sizeof(int16_t) * slots * cap);
+    if (e != hipSuccess) return fail(DPSNN_ERR_MEMORY, "batch weight scratch: %s", hipGetErrorString(e));
+    HIP_TRY(hipMalloc((void **)&b->keyCount, sizeof(uint32_t) * slots));
+    b->scratchCap = cap;
+    b->scratchSlots = slots;
+  }
+  if (b->shape < 0) {
+    const int K = cap <= 8 * 64 ? 8 : (cap <= 16 * 64 ? 16 : 24);
+    b->shape = find_shape(K, 256);
+  }
+  b->dirty = false;
+  return 0;
+}
+
+// One batched step of every member that has steps left; *any = false when
+// none has.
+int batch_one_step(dpsnn_batch *b, bool *any) {
+  b->active.clear();
+  for (auto *n : b->nets)
+    if (n->t < n->par.pub.totalSimTime_ms) b->active.push_back(n);
+  *any = !b->active.empty();
+  if (!*any) return 0;
+  int rc;
+  if ((rc = batch_prepare(b))) return rc;
+  const int r = b->ringNext;
+  b->ringNext = (r + 1) % dpsnn_batch::kRing;
+  if (b->pending[r]) HIP_TRY(hipEventSynchronize(b->done[r]));
+  b->plans.resize(b->active.size());
+  uint32_t tiles = 0;
+  for (size_t i = 0; i < b->active.size(); ++i) {
+    dpsnn_net *n = b->active[i];
+    StepPlan &sp = b->plans[i];
+    if ((rc = pre_step(n, sp))) return rc;
+    const uint32_t idx = (uint32_t)(std::find(b->nets.begin(), b->nets.end(), n) - b->nets.begin());
+    b->h_items[r][i] = BatchItem{sp.t, sp.ltd, sp.S, sp.X, tiles, idx};
+    tiles += n->dev.NW;
+  }
+  const uint32_t count = (uint32_t)b->active.size();
+  HIP_TRY(hipMemcpyAsync(b->d_items[r], b->h_items[r], sizeof(BatchItem) * count, hipMemcpyHostToDevice, b->stream));
+  const KernelShape &K = kShapes[b->shape];
+  const uint32_t cap = b->scratchCap;
+  const size_t lds = lds_bytes(cap, K.B);
+  uint32_t wpc = lds_waves(lds);
+  if (b->wavesPerCu) wpc = std::min(wpc, b->wavesPerCu);
+  const uint32_t grid = std::min<uint32_t>(tiles, (uint32_t)b->cus * wpc);
+  if (b->profile) {
+    if (b->evUsed == b->ev.size()) {
+      hipEvent_t x, y;
+      HIP_TRY(hipEventCreate(&x));
+      HIP_TRY(hipEventCreate(&y));
+      b->ev.emplace_back(x, y);
+    }
+    HIP_TRY(hipEventRecord(b->ev[b->evUsed].first, b->stream));
+  }
+  hipLaunchKernelGGL(K.batch, dim3(grid), dim3(kWave), lds, b->stream, b->d_nets, b->d_items[r], count, tiles, cap,
+                     b->keys, b->keyW, b->keyCount);
+  if (b->profile) HIP_TRY(hipEventRecord(b->ev[b->evUsed++].second, b->stream));
+  HIP_TRY(hipEventRecord(b->done[r], b->stream));
+  b->pending[r] = true;
+  for (size_t i = 0; i < b->active.size(); ++i)
+    if ((rc = post_step(b->active[i], b->plans[i]))) return rc;
+  return 0;
+}
+
+int batch_kernel_ms(dpsnn_batch *b, double *avg, uint64_t *launches) {
+  HIP_TRY(hipStreamSynchronize(b->stream));
+  double total = 0;
+  for (size_t k = 0; k < b->evUsed; ++k) {
+    float ms = 0;
+    HIP_TRY(hipEventElapsedTime(&ms, b->ev[k].first, b->ev[k].second));
+    total += ms;
+  }
+  if (avg) *avg = b->evUsed ? total / (double)b->evUsed : 0.0;
+  if (launches) *launches = b->evUsed;
+  b->evUsed = 0;
+  return 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+int dpsnn_batch_create(int device, dpsnn_batch **out) {
+  if (!out) return fail(DPSNN_ERR_ARG, "null argument");
+  *out = nullptr;
+  std::unique_ptr<dpsnn_batch> b(new dpsnn_batch());
+  b->device = device;
+  HIP_TRY(hipSetDevice(device));
+  HIP_TRY(hipDeviceGetAttribute(&b->cus, hipDeviceAttributeMultiprocessorCount, device));
+  b->cus = std::max(b->cus, 1);
+  HIP_TRY(hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking));
+  for (int r = 0; r < dpsnn_batch::kRing; ++r) HIP_TRY(hipEventCreateWithFlags(&b->done[r], hipEventDisableTiming));
+  *out = b.release();
+  return 0;
+}
+
+int dpsnn_batch_add(dpsnn_batch *b, dpsnn_net *n) {
+  if (!b || !n) return fail(DPSNN_ERR_ARG, "null argument");
+  if (n->batch) return fail(DPSNN_ERR_ARG, "network is already a member of a batch");
+  if (n->opt.device != b->device) return fail(DPSNN_ERR_ARG, "network and batch are on different devices");
+  HIP_TRY(hipSetDevice(b->device));
+  HIP_TRY(hipStreamSynchronize(n->stream));
+  n->ownStream = n->stream;
+  n->stream = b->stream;
+  n->batch = b;
+  b->nets.push_back(n);
+  b->dirty = true;
+  return 0;
+}
+
+int dpsnn_batch_remove(dpsnn_batch *b, dpsnn_net *n) {
+  if (!b || !n) return fail(DPSNN_ERR_ARG, "null argument");
+  auto it = std::find(b->nets.begin(), b->nets.end(), n);
+  if (it == b->nets.end() || n->batch != b) return fail(DPSNN_ERR_ARG, "network is not a member of this batch");
+  HIP_TRY(hipSetDevice(b->device));
+  HIP_TRY(hipStreamSynchronize(b->stream));
+  b->nets.erase(it);
+  n->stream = n->ownStream;
+  n->ownStream = nullptr;
+  n->batch = nullptr;
+  b->dirty = true;
+  return 0;
+}
+
+uint32_t dpsnn_batch_size(const dpsnn_batch *b) { return b ? (uint32_t)b->nets.size() : 0; }
+
+int dpsnn_batch_step(dpsnn_batch *b, uint32_t nsteps) {
+  if (!b) return fail(DPSNN_ERR_ARG, "null batch");
+  HIP_TRY(hipSetDevice(b->device));
+  for (uint32_t k = 0; k < nsteps; ++k) {
+    bool any = false;
+    int rc = batch_one_step(b, &any);
+    if (rc) return rc;
+    if (!any) {
+      for (auto *n : b->nets)
+        if ((rc = drain(n))) return rc;
+      return DPSNN_DONE;
+    }
+  }
+  return 0;
+}
+
+int dpsnn_batch_sync(dpsnn_batch *b) {
+  if (!b) return fail(DPSNN_ERR_ARG, "null batch");
+  HIP_TRY(hipSetDevice(b->device));
+  HIP_TRY(hipStreamSynchronize(b->stream));
+  int rc;
+  for (auto *n : b->nets)
+    if ((rc = drain(n))) return rc;
+  return 0;
+}
+
+int dpsnn_batch_get_shape(const dpsnn_batch *b, dpsnn_batch_shape *s) {
+  if (!b || !s) return fail(DPSNN_ERR_ARG, "null argument");
+  const int i = b->shape < 0 ? find_shape(24, 256) : b->shape;
+  s->chunks = kShapes[i].K;
+  s->buckets = kShapes[i].B;
+  s->waves_per_cu = (int)b->wavesPerCu;
+  s->ms_per_step = 0;
+  return 0;
+}
+
+int dpsnn_batch_set_shape(dpsnn_batch *b, const dpsnn_batch_shape *s) {
+  if (!b || !s) return fail(DPSNN_ERR_ARG, "null argument");
+  const int i = find_shape(s->chunks, s->buckets);
+  if (i < 0) return fail(DPSNN_ERR_ARG, "no kernel shape with %d register chunks and %d buckets", s->chunks, s->buckets);
+  if (s->waves_per_cu < 0 || s->waves_per_cu > 16) return fail(DPSNN_ERR_ARG, "waves_per_cu must be in 0..16");
+  b->shape = i;
+  b->wavesPerCu = (uint32_t)s->waves_per_cu;
+  return 0;
+}
+
+int dpsnn_batch_profile(dpsnn_batch *b, int enable) {
+  if (!b) return fail(DPSNN_ERR_ARG, "null batch");
+  HIP_TRY(hipStreamSynchronize(b->stream));
+  b->profile = enable != 0;
+  b->evUsed = 0;
+  return 0;
+}
+
+int dpsnn_batch_kernel_time(dpsnn_batch *b, double *avg_ms, uint64_t *launches) {
+  if (!b) return fail(DPSNN_ERR_ARG, "null batch");
+  return batch_kernel_ms(b, avg_ms, launches);
+}
+
+int dpsnn_batch_tune(dpsnn_batch *b, uint32_t trial_steps, dpsnn_batch_shape *chosen) {
+  if (!b) return fail(DPSNN_ERR_ARG, "null batch");
+  if (trial_steps == 0) trial_steps = 1;
+  HIP_TRY(hipSetDevice(b->device));
+  const bool wasProfiling = b->profile;
+  int rc;
+  if ((rc = batch_prepare(b))) return rc;
+  const uint32_t cap = batch_cap(b);
+  double best = 1e300;
+  int bestShape = b->shape;
+  uint32_t bestWaves = b->wavesPerCu;
+  for (int i = 0; i < kNumShapes; ++i) {
+    const uint32_t top = lds_waves(lds_bytes(cap, kShapes[i].B));
+    for (uint32_t w : {top, std::max<uint32_t>(1, top / 2), std::max<uint32_t>(1, top / 4)}) {
+      b->shape = i;
+      b->wavesPerCu = w;
+      HIP_TRY(hipStreamSynchronize(b->stream));
+      b->profile = true;
+      b->evUsed = 0;
+      bool any = true;
+      for (uint32_t k = 0; k < trial_steps && any; ++k)
+        if ((rc = batch_one_step(b, &any))) return rc;
+      double ms = 0;
+      uint64_t launches = 0;
+      if ((rc = batch_kernel_ms(b, &ms, &launches))) return rc;
+      if (launches && ms < best) {
+        best = ms;
+        bestShape = i;
+        bestWaves = w;
+      }
+      if (!any) break;
+    }
+  }
+  b->shape = bestShape;
+  b->wavesPerCu = bestWaves;
+  b->profile = wasProfiling;
+  if (chosen) {
+    chosen->chunks = kShapes[bestShape].K;
+    chosen->buckets = kShapes[bestShape].B;
+    chosen->waves_per_cu = (int)bestWaves;
+    chosen->ms_per_step = best < 1e300 ? best : 0.0;
+  }
+  return 0;
+}
+
+void dpsnn_batch_destroy(dpsnn_batch *b) {
+  if (!b) return;
+  hipSetDevice(b->device);
+  while (!b->nets.empty()) dpsnn_batch_remove(b, b->nets.back());
+  hipStreamSynchronize(b->stream);
+  batch_free_items(b);
+  if (b->d_nets) hipFree(b->d_nets);
+  if (b->keys) hipFree(b->keys);
+  if (b->keyW) hipFree(b->keyW);
+  if (b->keyCount) hipFree(b->keyCount);
+  for (int r = 0; r < dpsnn_batch::kRing; ++r)
+    if (b->done[r]) hipEventDestroy(b->done[r]);
+  for (auto &e : b->ev) {
+    hipEventDestroy(e.first);
+    hipEventDestroy(e.second);
+  }
+  hipStreamDestroy(b->stream);
+  delete b;
 }
 
 }  // extern "C"

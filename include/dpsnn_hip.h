@@ -148,6 +148,38 @@ int dpsnn_profile(dpsnn_net *net, int enable);
 int dpsnn_kernel_time(dpsnn_net *net, int kernel, double *avg_ms,
                       uint64_t *launches);
 
+/* ---- batching -----------------------------------------------------------
+ * Many independent networks of uneven size advance in ONE persistent kernel
+ * launch per step.  A network joins with dpsnn_batch_add (same device) and
+ * is then stepped only through its batch; each batched step advances every
+ * member that has steps left by one ms (members may be at different ms).
+ * A member's results are bit-identical to stepping it alone.  Files,
+ * counters, state and synapse read-back work per member as usual. */
+typedef struct dpsnn_batch dpsnn_batch;
+typedef struct {
+  int chunks;        /* in-list chunks of 64 items held in registers: 8/16/24 */
+  int buckets;       /* arrival-time buckets of the LDS sort: 64/256/1024 */
+  int waves_per_cu;  /* persistent waves per CU (0 = as many as LDS allows) */
+  double ms_per_step;/* dpsnn_batch_tune: measured kernel time of the shape */
+} dpsnn_batch_shape;
+int dpsnn_batch_create(int device, dpsnn_batch **out);
+int dpsnn_batch_add(dpsnn_batch *b, dpsnn_net *net);
+int dpsnn_batch_remove(dpsnn_batch *b, dpsnn_net *net);
+uint32_t dpsnn_batch_size(const dpsnn_batch *b);
+/* Up to nsteps batched steps; DPSNN_DONE once no member has steps left. */
+int dpsnn_batch_step(dpsnn_batch *b, uint32_t nsteps);
+int dpsnn_batch_sync(dpsnn_batch *b);
+/* Autotuner: runs `trial_steps` batched steps (real steps of the
+ * simulation) with each launch shape and keeps the fastest. */
+int dpsnn_batch_tune(dpsnn_batch *b, uint32_t trial_steps, dpsnn_batch_shape *chosen);
+int dpsnn_batch_get_shape(const dpsnn_batch *b, dpsnn_batch_shape *s);
+int dpsnn_batch_set_shape(dpsnn_batch *b, const dpsnn_batch_shape *s);
+/* Average device time (ms) of the batched kernel since the last call. */
+int dpsnn_batch_profile(dpsnn_batch *b, int enable);
+int dpsnn_batch_kernel_time(dpsnn_batch *b, double *avg_ms, uint64_t *launches);
+/* Members are removed (not destroyed). */
+void dpsnn_batch_destroy(dpsnn_batch *b);
+
 /* ---- host-only access (no device needed) ------------------------------
  * The connectome generator and delivery layout as the device receives
  * them, and the replayed external drive, for inspection and CPU checks. */

@@ -1,0 +1,116 @@
+"""Batching: independent networks of uneven size advancing in one launch per
+step give, member by member, the same results as each network stepped alone
+(bit-exact state, counters, raster/rates files, STDP synapses), also when
+members join at different ms, when the launch shape changes mid-run and
+after the autotuner's trial steps."""
+import os
+
+import numpy as np
+import pytest
+
+from helpers import sha256_file
+
+pytestmark = pytest.mark.gpu
+
+# (config, env overrides, drive): grids of 1, 2 and 4 columns, two seeds,
+# both drives, one plastic network whose run crosses the t = 999 commit
+MEMBERS = [
+    ("c0", dict(env_CFX=1, env_CFY=1, env_globalSeed=3, env_totalSimTime_ms=300), "philox"),
+    ("c0", dict(env_CFX=2, env_CFY=1, env_globalSeed=5, env_totalSimTime_ms=250), "replay"),
+    ("c0", dict(env_CFX=2, env_CFY=2, env_globalSeed=1, env_totalSimTime_ms=320), "replay"),
+    ("c2_small", dict(env_totalSimTime_ms=1040), "philox"),
+]
+
+
+def _make(pkg, spec, out):
+    name, over, drive = spec
+    cfg = pkg.config_dir(name)
+    env = pkg.read_env_file(os.path.join(cfg, "env.txt"))
+    env.update({k: str(v) for k, v in over.items()})
+    return pkg.Network(cfg, drive=pkg.DRIVE_PHILOX if drive == "philox" else pkg.DRIVE_REPLAY,
+                       philox_seed=17, output_dir=str(out), env=env)
+
+
+def _result(net, out, plastic):
+    r = {"stats": {k: v for k, v in net.stats().items() if k not in ("init_seconds",)},
+         "state": net.state(),
+         "files": {f: sha256_file(os.path.join(out, f)) for f in sorted(os.listdir(out))}}
+    if plastic:
+        r["syn"] = net.synapses()
+    return r
+
+
+def _same(a, b):
+    assert a["files"] == b["files"]
+    assert a["stats"]["spikes"] == b["stats"]["spikes"]
+    assert a["stats"]["synaptic_events"] == b["stats"]["synaptic_events"]
+    assert a["stats"]["external_events"] == b["stats"]["external_events"]
+    assert a["stats"]["steps"] == b["stats"]["steps"]
+    for k in ("v", "c", "Tr", "Te", "fired"):
+        assert np.array_equal(a["state"][k].view(np.uint8), b["state"][k].view(np.uint8)), k
+    if "syn" in a:
+        for k in ("weight", "tderiv", "last"):
+            assert np.array_equal(a["syn"][k].view(np.uint8), b["syn"][k].view(np.uint8)), k
+
+
+def test_batch_members_match_alone(pkg, tmp_path):
+    alone = []
+    for i, spec in enumerate(MEMBERS):
+        out = tmp_path / f"alone{i}"
+        out.mkdir()
+        with _make(pkg, spec, out) as net:
+            while not net.step(100):
+                pass
+            net.sync()
+            alone.append(_result(net, str(out), spec[0] == "c2_small"))
+        assert alone[-1]["stats"]["spikes"] > 0
+
+    outs = [tmp_path / f"batch{i}" for i in range(len(MEMBERS))]
+    for o in outs:
+        o.mkdir()
+    with pkg.Batch() as b:
+        nets = [_make(pkg, MEMBERS[i], outs[i]) for i in (0, 1)]
+        for n in nets:
+            b.add(n)
+        assert not b.step(40)
+        # the others join while the first two are at ms 40
+        for i in (2, 3):
+            nets.append(_make(pkg, MEMBERS[i], outs[i]))
+            b.add(nets[-1])
+        assert len(b) == 4
+        b.set_shape(16, 64)
+        b.step(60)
+        b.set_shape(8, 1024, 4)
+        b.step(30)
+        chosen = b.tune(1)
+        assert chosen["chunks"] in (8, 16, 24) and chosen["buckets"] in (64, 256, 1024)
+        with pytest.raises(pkg.DpsnnError):
+            nets[0].step(1)  # members are stepped through their batch
+        while not b.step(200):
+            pass
+        b.sync()
+        got = [_result(n, str(outs[i]), MEMBERS[i][0] == "c2_small") for i, n in enumerate(nets)]
+        for n in nets:
+            b.remove(n)
+            n.close()
+    for i, (a, g) in enumerate(zip(alone, got)):
+        _same(a, g)
+
+
+def test_removed_member_steps_alone_again(pkg, tmp_path):
+    spec = ("c0", dict(env_CFX=1, env_CFY=1, env_globalSeed=9, env_totalSimTime_ms=120), "philox")
+    (tmp_path / "a").mkdir()
+    (tmp_path / "b").mkdir()
+    with _make(pkg, spec, tmp_path / "a") as ref:
+        while not ref.step(50):
+            pass
+        ref.sync()
+        want = _result(ref, str(tmp_path / "a"), False)
+    with pkg.Batch() as b, _make(pkg, spec, tmp_path / "b") as net:
+        b.add(net)
+        b.step(70)
+        b.remove(net)
+        while not net.step(50):
+            pass
+        net.sync()
+        _same(want, _result(net, str(tmp_path / "b"), False))
