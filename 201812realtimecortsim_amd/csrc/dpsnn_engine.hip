@@ -73,6 +73,22 @@ static int fail(int code, const char *fmt, ...) {
 constexpr int kWave = 64;
 constexpr double kNotFired = -1.0;  // Te history entry of a neuron that did not fire (spike times are >= 0)
 
+// XCD-aware persistent walk over work items [0, total).  The dispatcher deals
+// workgroups to the 8 XCDs round-robin (workgroup b on XCD b % 8), and each
+// XCD has its own L2: the blocks of one XCD walk one contiguous eighth of
+// the items, so neighbouring neurons, which share most of their sources,
+// read the spike history through the same L2.  gridDim.x is a multiple of 8.
+constexpr uint32_t kXcds = 8;
+struct XcdWalk {
+  uint32_t first, end, stride;
+};
+__device__ inline XcdWalk xcd_walk(uint32_t total) {
+  const uint32_t x = blockIdx.x % kXcds, lb = blockIdx.x / kXcds, per = gridDim.x / kXcds;
+  const uint32_t begin = (uint32_t)((uint64_t)total * x / kXcds);
+  const uint32_t end = (uint32_t)((uint64_t)total * (x + 1) / kXcds);
+  return XcdWalk{begin + lb, end, per};
+}
+
 // LDS written by some lanes of a wave and read by others: order the wave's
 // accesses around this point.
 __device__ inline void wave_sync() {
@@ -96,14 +112,14 @@ struct DevNet {
   const uint32_t *maxExt;
   double *v, *c, *Tr, *Te, *assigned;
   uint64_t *philoxCtr;
-  uint64_t *bits;          // [RS][NW]
+  uint64_t *bits;          // [RS][NW], RS a power of two
   double *teHist;          // [RS][N] emission time, or kNotFired
   const uint64_t *inOff;   // [N+1] start of each neuron's in-list
   const uint32_t *inPD;    // pre | delay << 24, in delivery insertion order
   const int16_t *inW;
-  uint64_t *keys;          // sorted event ticks: [grid][64][cap] (fused) or [N][cap] (split)
+  uint64_t *keys;          // sorted event ticks: [grid or N/64][cap/8][64][8]
   int16_t *keyW;           // their weights, same layout
-  uint32_t *keyCount;      // [grid][64] or [N]
+  uint32_t *keyCount;      // [grid][64] or [N/64][64]
   unsigned long long *counters;  // [0] events, [1] external, [2] spikes
   // STDP (null when plasticity never runs).  Synapse state is kept in
   // target-major order: synapse k of neuron j's incoming list lives at
@@ -210,7 +226,7 @@ struct Lifca {
 __device__ inline uint64_t publish_spikes(const DevNet &P, uint32_t t1, uint32_t j, bool f, double te,
                                           uint32_t *popCnt, int count) {
   const uint64_t mask = __ballot(f);
-  const uint32_t slot = t1 % P.RS;
+  const uint32_t slot = t1 & (P.RS - 1);
   if ((threadIdx.x & 63) == 0 && (j >> 6) < P.NW) {
     P.bits[(size_t)slot * P.NW + (j >> 6)] = mask;
     if (mask && count) atomicAdd(&P.counters[2], (unsigned long long)__popcll(mask));
@@ -257,7 +273,7 @@ __device__ inline void ltp_word(const DevNet &P, uint32_t t, uint64_t mask, uint
 __global__ __launch_bounds__(kWaveLTP) void k_ltp(DevNet P, uint32_t t) {
   const uint32_t wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
   const uint32_t waves = (gridDim.x * blockDim.x) >> 6;
-  const uint32_t slot = t % P.RS;
+  const uint32_t slot = t & (P.RS - 1);
   for (uint32_t w = wave; w < P.NW; w += waves) ltp_word(P, t, P.bits[(size_t)slot * P.NW + w], w * 64);
 }
 
@@ -324,7 +340,7 @@ __global__ __launch_bounds__(kCompactThreads) void k_compact(DevNet P, uint32_t 
                                                              double *te, uint32_t *rasterCount,
                                                              uint32_t *stepCount) {
   __shared__ uint32_t part[kCompactThreads];
-  const uint32_t slot = t % P.RS;
+  const uint32_t slot = t & (P.RS - 1);
   const uint64_t *w = P.bits + (size_t)slot * P.NW;
   const uint32_t per = (P.NW + kCompactThreads - 1) / kCompactThreads;
   const uint32_t w0 = threadIdx.x * per, w1 = min(P.NW, w0 + per);
@@ -435,40 +451,36 @@ constexpr int kChunk = 8;  // internal events per lane per LDS-staged chunk
 // LDS of one wave in phase 2: two chunks of 64 lanes' ticks
 constexpr size_t kStageBytes = 2 * kChunk * kWave * sizeof(uint64_t);
 
-// Internal events of neuron q's chunk c, loaded cooperatively: instruction
-// r covers neurons 8r..8r+7, 8 consecutive ticks each (64-byte runs).
-__device__ inline void load_chunk(const uint64_t *rowsK, uint32_t cap, uint32_t n, uint32_t c,
-                                  uint64_t (&rk)[kChunk]) {
+// Chunk c (events 8c..8c+7 of every lane) from the tile's blocked scratch
+// [i / 8][q][i % 8]: the chunk's ticks are one contiguous 4 KB block, loaded
+// by the wave in 8 coalesced instructions (instruction r: neurons 8r..8r+7)
+// and transposed through LDS; each lane's 8 weights are one 16-byte load.
+// Entries past a neuron's count are never consumed.
+static_assert(kChunk == 8, "the scratch is blocked by 8 events");
+__device__ inline void load_chunk(const uint64_t *tileK, const int16_t *tileW, uint32_t c, uint64_t (&rk)[kChunk],
+                                  uint4 &rw) {
+  const uint32_t lane = threadIdx.x & 63;
+  const uint64_t *blk = tileK + (size_t)c * (kWave * 8);
+#pragma unroll
+  for (int r = 0; r < kChunk; ++r) rk[r] = blk[r * kWave + lane];
+  rw = *(const uint4 *)(tileW + (size_t)c * (kWave * 8) + lane * 8);
+}
+__device__ inline void store_chunk(uint64_t *stageK, const uint64_t (&rk)[kChunk]) {
   const uint32_t lane = threadIdx.x & 63, e = lane & 7;
 #pragma unroll
-  for (int r = 0; r < kChunk; ++r) {
-    const uint32_t q = (uint32_t)r * 8 + (lane >> 3);
-    const uint32_t nq = (uint32_t)__shfl((int)n, (int)q);
-    const uint32_t i = c * kChunk + e;
-    rk[r] = i < nq ? rowsK[(size_t)q * cap + i] : 0ull;
-  }
-}
-__device__ inline void store_chunk(uint64_t *stage, const uint64_t (&rk)[kChunk]) {
-  const uint32_t lane = threadIdx.x & 63, e = lane & 7;
-#pragma unroll
-  for (int r = 0; r < kChunk; ++r) stage[e * kWave + (uint32_t)r * 8 + (lane >> 3)] = rk[r];
-}
-__device__ inline uint4 load_weights(const int16_t *rowsW, uint32_t cap, uint32_t n, uint32_t c) {
-  const uint32_t lane = threadIdx.x & 63, i0 = c * kChunk;
-  if (i0 >= n) return make_uint4(0, 0, 0, 0);
-  return *(const uint4 *)(rowsW + (size_t)lane * cap + i0);
+  for (int r = 0; r < kChunk; ++r) stageK[e * kWave + (uint32_t)r * 8 + (lane >> 3)] = rk[r];
 }
 
 // Phase 2, one lane per neuron: lane q integrates neuron j0 + q (q < m) over
-// its n sorted internal events (rows rowsK/rowsW + q*cap, ticks relative to
-// bits(t) and int16 weights) merged with its external bath, LIFCADynamic
-// per event in FP64.  The ticks are staged through LDS (stage, kStageBytes)
-// a chunk ahead.  Returns the lane's external event count and whether (and
-// when) its neuron fired.  Whole wave, uniform.
+// its n sorted internal events (the tile's blocked scratch tileK/tileW:
+// ticks relative to bits(t) and int16 weights) merged with its external
+// bath, LIFCADynamic per event in FP64.  The events are staged through LDS
+// (stage, kStageBytes) a chunk ahead.  Returns the lane's external event
+// count and whether (and when) its neuron fired.  Whole wave, uniform.
 template <bool REPLAY>
 __device__ inline uint32_t integrate_wave(const DevNet &P, uint32_t t, const ExtView &X, uint32_t j0, uint32_t m,
-                                          const uint64_t *rowsK, const int16_t *rowsW, uint32_t cap, uint32_t n,
-                                          uint64_t *stage, bool &fired, double &te) {
+                                          const uint64_t *tileK, const int16_t *tileW, uint32_t n, uint8_t *stage,
+                                          bool &fired, double &te) {
   const uint32_t lane = threadIdx.x & 63;
   const bool on = lane < m;
   const uint32_t j = j0 + lane;
@@ -509,28 +521,25 @@ __device__ inline uint32_t integrate_wave(const DevNet &P, uint32_t t, const Ext
   // lane's internal events are used up
   const uint32_t chunks = (nmax + kChunk - 1) / kChunk;
   const uint32_t iters = chunks ? chunks : 1u;
+  uint64_t *stageK = (uint64_t *)stage;  // [2][kChunk][64]
   uint64_t rk[kChunk];
   uint4 wCur = make_uint4(0, 0, 0, 0), wNext = wCur;
   if (chunks) {
-    load_chunk(rowsK, cap, n, 0, rk);
-    wCur = load_weights(rowsW, cap, n, 0);
-    store_chunk(stage, rk);
+    load_chunk(tileK, tileW, 0, rk, wCur);
+    store_chunk(stageK, rk);
     wave_sync();
   }
   for (uint32_t c = 0; c < iters; ++c) {
-    const uint64_t *cur = stage + (c & 1) * (kChunk * kWave);
+    const uint64_t *curK = stageK + (c & 1) * (kChunk * kWave);
     const bool more = c + 1 < chunks;
-    if (more) {
-      load_chunk(rowsK, cap, n, c + 1, rk);
-      wNext = load_weights(rowsW, cap, n, c + 1);
-    }
+    if (more) load_chunk(tileK, tileW, c + 1, rk, wNext);
     uint64_t wlo = (uint64_t)wCur.x | ((uint64_t)wCur.y << 32), whi = (uint64_t)wCur.z | ((uint64_t)wCur.w << 32);
     const uint32_t left = n > c * kChunk ? min((uint32_t)kChunk, n - c * kChunk) : 0u;
     const bool last = c * kChunk + left >= n;
     uint32_t e = 0;
     for (;;) {
       const bool hasI = e < left;
-      const double kI = hasI ? dpsnn_asf64(tBits + cur[e * kWave + lane]) : 0.0;
+      const double kI = hasI ? dpsnn_asf64(tBits + curK[e * kWave + lane]) : 0.0;
       const bool eDue = REPLAY ? xr.due() : xp.due();
       double at, val;
       if (eDue && (hasI ? (REPLAY ? xr.at() : xp.at()) < kI : last)) {
@@ -549,7 +558,7 @@ __device__ inline uint32_t integrate_wave(const DevNet &P, uint32_t t, const Ext
       L.event(at, val);
     }
     if (more) {
-      store_chunk(stage + ((c + 1) & 1) * (kChunk * kWave), rk);
+      store_chunk(stageK + ((c + 1) & 1) * (kChunk * kWave), rk);
       wCur = wNext;
     }
     wave_sync();
@@ -606,7 +615,7 @@ __device__ inline uint32_t gather_keys(const DevNet &P, uint32_t t, uint32_t j, 
   for (int k = 0; k < K; ++k) {
     const uint32_t d = it[k] >> 24, pre = it[k] & 0xFFFFFFu;
     te[k] = kNotFired;
-    if (it[k] != kPad && d <= t) te[k] = P.teHist[(size_t)((t - d) % P.RS) * P.N + pre];
+    if (it[k] != kPad && d <= t) te[k] = P.teHist[((t - d) & (P.RS - 1)) * P.N + pre];
     if (first) w[k] = it[k] != kPad ? P.inW[beg + base + k * kWave + lane] : (int16_t)0;
   }
   uint32_t n = 0;
@@ -651,8 +660,10 @@ __host__ __device__ inline size_t phase1_lds(uint32_t cap, int B) {
 }
 
 // Phase 1 for neuron j, whole wave: live keys of its in-list, bucket-sorted
-// in LDS (lds: sorted[cap], ldsW[cap], bcnt[B + 1]), stored to outK[0..n) as
-// ticks relative to bits(t) with their weights in outW[0..n); returns n.
+// in LDS (lds: sorted[cap], ldsW[cap], bcnt[B + 1]), stored in arrival order
+// as ticks relative to bits(t) (outK) with their weights (outW); returns n.
+// The scratch of a tile of 64 neurons is blocked by 8 events: event i of
+// neuron q at [i / 8][q][i % 8], and outK/outW point at neuron q's [0][q][0].
 template <int K, int B>
 __device__ inline uint32_t gather_sort_neuron(const DevNet &P, uint32_t t, uint32_t j, int plastic, uint8_t *lds,
                                               uint32_t cap, uint64_t *outK, int16_t *outW) {
@@ -707,25 +718,24 @@ __device__ inline uint32_t gather_sort_neuron(const DevNet &P, uint32_t t, uint3
     }
   }
   __syncthreads();
-  // bcnt[b] now holds the end of bucket b; sort buckets by insertion
-  for (int r = 0; r < R; ++r) {
-    const uint32_t b = lane * R + r;
-    const uint32_t hi = bcnt[b], lo = b ? bcnt[b - 1] : 0;
-    for (uint32_t x = lo + 1; x < hi; ++x) {
-      const uint64_t kx = sorted[x];
-      uint32_t y = x;
-      while (y > lo && sorted[y - 1] > kx) {
-        sorted[y] = sorted[y - 1];
-        --y;
-      }
-      sorted[y] = kx;
-    }
-  }
-  __syncthreads();
+  // bcnt[b] now holds the end of bucket b.  Within a bucket, each key's
+  // place is its rank (keys are unique): count the bucket's smaller keys.
+  // Lanes take the keys in bucket order, so lanes of one bucket read the
+  // same words together; the reads are independent of each other.
   for (uint32_t i = lane; i < n; i += kWave) {
-    const uint64_t k = sorted[i];
-    outK[i] = k >> 12;
-    outW[i] = ldsW[k & 0xFFFu];
+    const uint64_t x = sorted[i];
+    const uint32_t b = key_bucket<B>(x >> 12, shift);
+    const uint32_t lo = b ? bcnt[b - 1] : 0u, hi = bcnt[b];
+    uint32_t r = 0, y = lo;
+    for (; y + 4 <= hi; y += 4) {  // four independent reads per round
+      const uint64_t a0 = sorted[y], a1 = sorted[y + 1], a2 = sorted[y + 2], a3 = sorted[y + 3];
+      r += (a0 < x ? 1u : 0u) + (a1 < x ? 1u : 0u) + (a2 < x ? 1u : 0u) + (a3 < x ? 1u : 0u);
+    }
+    for (; y < hi; ++y) r += sorted[y] < x ? 1u : 0u;
+    const uint32_t p = lo + r;
+    const size_t at = (size_t)(p >> 3) * (kWave * 8) + (p & 7);
+    outK[at] = x >> 12;
+    outW[at] = ldsW[x & 0xFFFu];
   }
   __syncthreads();
   return n;
@@ -746,7 +756,8 @@ __host__ __device__ inline size_t step_lds(uint32_t cap, int B) {
 }
 
 // One tile (neurons 64*tile .. 64*tile+63) of step t, one wave: phase 1 for
-// each neuron into the wave's scratch rows wkeys/wW[64][cap] and wcnt[64],
+// each neuron into the wave's blocked scratch wkeys/wW[cap/8][64][8] and
+// wcnt[64],
 // phase 2 one lane per neuron, then the fired scan of step t+1.  Returns the
 // tile's delivered events (uniform) and adds external events to *exSum (per
 // lane).
@@ -760,8 +771,7 @@ __device__ inline unsigned long long step_tile(const DevNet &P, uint32_t t, cons
   const uint32_t m = min((uint32_t)kWave, P.N - j0);
   unsigned long long evSum = 0;
   for (uint32_t q = 0; q < m; ++q) {
-    const uint32_t n =
-        gather_sort_neuron<K, B>(P, t, j0 + q, ltd, lds, cap, wkeys + (size_t)q * cap, wW + (size_t)q * cap);
+    const uint32_t n = gather_sort_neuron<K, B>(P, t, j0 + q, ltd, lds, cap, wkeys + q * 8, wW + q * 8);
     if (lane == 0) wcnt[q] = n;
     evSum += n;
   }
@@ -772,9 +782,9 @@ __device__ inline unsigned long long step_tile(const DevNet &P, uint32_t t, cons
   bool f = false;
   double te = 0.0;
   if (P.drive == DPSNN_DRIVE_REPLAY)
-    exSum += integrate_wave<true>(P, t, X, j0, m, wkeys, wW, cap, n, (uint64_t *)lds, f, te);
+    exSum += integrate_wave<true>(P, t, X, j0, m, wkeys, wW, n, lds, f, te);
   else
-    exSum += integrate_wave<false>(P, t, X, j0, m, wkeys, wW, cap, n, (uint64_t *)lds, f, te);
+    exSum += integrate_wave<false>(P, t, X, j0, m, wkeys, wW, n, lds, f, te);
   // ---------------- fired scan of step t+1 ----------------
   const uint64_t mask = publish_spikes(P, t + 1, j0 + lane, f, te, S.popCnt, S.count);
   if (S.ltp) ltp_word(P, t + 1, mask, j0);
@@ -790,7 +800,8 @@ __global__ __launch_bounds__(kWave) void k_step(DevNet P, uint32_t t, ExtView X,
   int16_t *wW = P.keyW + (size_t)blockIdx.x * kWave * P.cap;
   uint32_t *wcnt = P.keyCount + (size_t)blockIdx.x * kWave;
   unsigned long long evSum = 0, exSum = 0;
-  for (uint32_t tile = blockIdx.x; tile < P.NW; tile += gridDim.x)
+  const XcdWalk w = xcd_walk(P.NW);
+  for (uint32_t tile = w.first; tile < w.end; tile += w.stride)
     evSum += step_tile<K, B>(P, t, X, ltd, S, tile, P.cap, (uint8_t *)lds, wkeys, wW, wcnt, exSum);
   for (int o = 32; o > 0; o >>= 1) exSum += __shfl_down(exSum, o);
   if (threadIdx.x == 0) {
@@ -821,7 +832,8 @@ __global__ __launch_bounds__(kWave) void k_batch_step(const DevNet *nets, const 
   uint64_t *wkeys = keys + (size_t)blockIdx.x * kWave * cap;
   int16_t *wW = keyW + (size_t)blockIdx.x * kWave * cap;
   uint32_t *wcnt = keyCount + (size_t)blockIdx.x * kWave;
-  for (uint32_t g = blockIdx.x; g < totalTiles; g += gridDim.x) {
+  const XcdWalk w = xcd_walk(totalTiles);
+  for (uint32_t g = w.first; g < w.end; g += w.stride) {
     uint32_t lo = 0, hi = count;  // last item with tileBegin <= g
     while (hi - lo > 1) {
       const uint32_t mid = (lo + hi) / 2;
@@ -848,9 +860,10 @@ template <int K, int B>
 __global__ __launch_bounds__(kWave) void k_gather(DevNet P, uint32_t t, int ltd) {
   extern __shared__ uint64_t lds[];
   unsigned long long evSum = 0;
-  for (uint32_t j = blockIdx.x; j < P.N; j += gridDim.x) {
-    const uint32_t n = gather_sort_neuron<K, B>(P, t, j, ltd, (uint8_t *)lds, P.cap, P.keys + (size_t)j * P.cap,
-                                                P.keyW + (size_t)j * P.cap);
+  const XcdWalk w = xcd_walk(P.N);
+  for (uint32_t j = w.first; j < w.end; j += w.stride) {
+    const size_t at = (size_t)(j >> 6) * P.cap * kWave + (j & 63) * 8;  // the tile's blocked scratch
+    const uint32_t n = gather_sort_neuron<K, B>(P, t, j, ltd, (uint8_t *)lds, P.cap, P.keys + at, P.keyW + at);
     if (threadIdx.x == 0) P.keyCount[j] = n;
     evSum += n;
   }
@@ -860,20 +873,19 @@ __global__ __launch_bounds__(kWave) void k_gather(DevNet P, uint32_t t, int ltd)
 // Blocks of kBlock threads: each wave holds 64 consecutive neurons starting
 // at a multiple of 64, as publish_spikes requires.
 __global__ __launch_bounds__(kBlock) void k_integrate(DevNet P, uint32_t t, ExtView X, NextScan S) {
-  __shared__ uint64_t stage[kBlock / kWave][2 * kChunk * kWave];
+  __shared__ uint8_t stage[kBlock / kWave][kStageBytes];
   const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const uint32_t j0 = blockIdx.x * kBlock + wave * kWave;
   const uint32_t m = j0 < P.N ? min((uint32_t)kWave, P.N - j0) : 0u;
   const uint32_t n = lane < m ? P.keyCount[j0 + lane] : 0u;
-  const uint64_t *rowsK = P.keys + (size_t)min(j0, P.N) * P.cap;
-  const int16_t *rowsW = P.keyW + (size_t)min(j0, P.N) * P.cap;
+  const size_t at = (size_t)(min(j0, P.N) >> 6) * P.cap * kWave;  // the wave's tile
   bool f = false;
   double te = 0.0;
   unsigned long long ex;
   if (P.drive == DPSNN_DRIVE_REPLAY)
-    ex = integrate_wave<true>(P, t, X, j0, m, rowsK, rowsW, P.cap, n, stage[wave], f, te);
+    ex = integrate_wave<true>(P, t, X, j0, m, P.keys + at, P.keyW + at, n, stage[wave], f, te);
   else
-    ex = integrate_wave<false>(P, t, X, j0, m, rowsK, rowsW, P.cap, n, stage[wave], f, te);
+    ex = integrate_wave<false>(P, t, X, j0, m, P.keys + at, P.keyW + at, n, stage[wave], f, te);
   const uint64_t mask = publish_spikes(P, t + 1, j0 + lane, f, te, S.popCnt, S.count);
   if (S.ltp) ltp_word(P, t + 1, mask, j0);
   for (int o = 32; o > 0; o >>= 1) ex += __shfl_down(ex, o);
@@ -907,6 +919,8 @@ static const KernelShape kShapes[] = {
 };
 #undef DPSNN_SHAPE
 constexpr int kNumShapes = (int)(sizeof(kShapes) / sizeof(kShapes[0]));
+
+static uint32_t round_up(uint32_t x, uint32_t m) { return (x + m - 1) / m * m; }
 
 static int find_shape(int K, int B) {
   for (int i = 0; i < kNumShapes; ++i)
@@ -1029,7 +1043,11 @@ static int build_device(dpsnn_net *n) {
   DevNet &P = n->dev;
   P.N = c.N;
   P.NW = (c.N + 63) / 64;
-  P.RS = c.max_delay + 2;  // live slots t-maxDelay..t, plus one spare
+  // live slots t-maxDelay..t plus the one step t+1 writes, rounded to a
+  // power of two (slot = ms & (RS - 1))
+  P.RS = 1;
+  while (P.RS < c.max_delay + 2) P.RS <<= 1;
+  if (uint64_t(P.RS) * c.N >= (1ull << 32)) return fail(DPSNN_ERR_CONFIG, "spike history larger than 2^32 entries");
   P.nsub = q.subPopNumber;
   P.npc = q.neuronsPerCM;
   P.ncols = q.globCFT;
@@ -1155,11 +1173,11 @@ static int build_device(dpsnn_net *n) {
   if (n->shape < 0) return fail(DPSNN_ERR_ARG, "no kernel shape with %d register chunks and %d buckets", K, B);
   n->ldsBytes = lds_bytes(P.cap, B);
   const uint32_t wavesPerCu = std::max<uint32_t>(1, std::min<uint32_t>(16, (uint32_t)((160u << 10) / n->ldsBytes)));
-  n->grid = std::min<uint32_t>(tiles, (uint32_t)cus * wavesPerCu);
+  n->grid = round_up(std::min<uint32_t>(tiles, (uint32_t)cus * wavesPerCu), kXcds);
   n->stdpBlocks = std::min<uint32_t>((P.NW + 3) / 4, (uint32_t)cus * 8);
   n->split = tiles < (uint32_t)cus * 4;
   if (const char *e = std::getenv("DPSNN_SPLIT")) n->split = std::atoi(e) != 0;
-  n->gatherGrid = std::min<uint32_t>(N, (uint32_t)cus * wavesPerCu * 4);
+  n->gatherGrid = round_up(std::min<uint32_t>(N, (uint32_t)cus * wavesPerCu * 4), kXcds);
   // the key scratch of stand-alone steps is allocated at the first
   // dpsnn_step (a network stepped only by a batch uses the batch's)
   if ((rc = dev_alloc(n, &P.counters, 4))) return rc;
@@ -1368,12 +1386,12 @@ static bool plastic_at(const dpsnn_net *n, uint32_t t) {
   return n->stdp && t >= q.startPlasticity_ms && t <= q.stopPlasticity_ms;
 }
 
-// Key scratch of stand-alone steps: [grid][64][cap] (fused) or [N][cap]
-// (split).
+// Key scratch of stand-alone steps, blocked by 8 events per neuron:
+// [grid][cap/8][64][8] (fused) or [N/64][cap/8][64][8] (split).
 static int ensure_scratch(dpsnn_net *n) {
   DevNet &P = n->dev;
   if (P.keys) return 0;
-  const size_t slots = n->split ? size_t(P.N) : size_t(n->grid) * kWave;
+  const size_t slots = n->split ? size_t(P.NW) * kWave : size_t(n->grid) * kWave;
   int rc;
   if ((rc = dev_alloc(n, &P.keys, slots * P.cap)) || (rc = dev_alloc(n, &P.keyW, slots * P.cap)) ||
       (rc = dev_alloc(n, &P.keyCount, slots)))
@@ -1593,7 +1611,7 @@ int dpsnn_get_state(dpsnn_net *n, double *v, double *c, double *Tr, double *Te, 
   if (fired) {
     // neurons that fired in the last completed step: bitmap of slot t
     std::vector<uint64_t> w(n->dev.NW);
-    const size_t slot = n->t % n->dev.RS;
+    const size_t slot = n->t & (n->dev.RS - 1);
     HIP_TRY(hipMemcpy(w.data(), n->dev.bits + slot * n->dev.NW, 8 * w.size(), hipMemcpyDeviceToHost));
     for (size_t i = 0; i < N; ++i) fired[i] = (uint8_t)((w[i >> 6] >> (i & 63)) & 1u);
   }
@@ -1767,7 +1785,7 @@ int batch_prepare(dpsnn_batch *b) {
     b->itemsCap = cap;
   }
   const uint32_t cap = batch_cap(b);
-  const size_t slots = (size_t)b->cus * lds_waves(lds_bytes(cap, 64)) * kWave;
+  const size_t slots = (size_t)round_up((uint32_t)b->cus * lds_waves(lds_bytes(cap, 64)), kXcds) * kWave;
   if (cap > b->scratchCap || slots > b->scratchSlots) {
     if (b->keys) HIP_TRY(hipFree(b->keys));
     if (b->keyW) HIP_TRY(hipFree(b->keyW));
@@ -1821,7 +1839,7 @@ int batch_one_step(dpsnn_batch *b, bool *any) {
   const size_t lds = lds_bytes(cap, K.B);
   uint32_t wpc = lds_waves(lds);
   if (b->wavesPerCu) wpc = std::min(wpc, b->wavesPerCu);
-  const uint32_t grid = std::min<uint32_t>(tiles, (uint32_t)b->cus * wpc);
+  const uint32_t grid = round_up(std::min<uint32_t>(tiles, (uint32_t)b->cus * wpc), kXcds);
   if (b->profile) {
     if (b->evUsed == b->ev.size()) {
       hipEvent_t x, y;
