@@ -85,7 +85,7 @@ def _p(t):
 
 class BatchShape(ctypes.Structure):
     _fields_ = [("chunks", ctypes.c_int), ("buckets", ctypes.c_int), ("waves_per_cu", ctypes.c_int),
-                ("ms_per_step", ctypes.c_double)]
+                ("ms_per_step", ctypes.c_double), ("split", ctypes.c_int)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -391,8 +391,8 @@ class Batch:
         _check("dpsnn_batch_get_shape", self._lib.dpsnn_batch_get_shape(self._h, ctypes.byref(s)))
         return s.as_dict()
 
-    def set_shape(self, chunks, buckets, waves_per_cu=0):
-        s = BatchShape(chunks, buckets, waves_per_cu, 0.0)
+    def set_shape(self, chunks, buckets, waves_per_cu=0, split=1):
+        s = BatchShape(chunks, buckets, waves_per_cu, 0.0, split)
         _check("dpsnn_batch_set_shape", self._lib.dpsnn_batch_set_shape(self._h, ctypes.byref(s)))
 
     def profile(self, enable=True):

@@ -892,6 +892,71 @@ __global__ __launch_bounds__(kBlock) void k_integrate(DevNet P, uint32_t t, ExtV
   if (lane == 0 && ex) atomicAdd(&P.counters[1], ex);
 }
 
+// The item of global tile g: the last one with tileBegin <= g.
+__device__ inline uint32_t batch_item_of(const BatchItem *items, uint32_t count, uint32_t g) {
+  uint32_t lo = 0, hi = count;
+  while (hi - lo > 1) {
+    const uint32_t mid = (lo + hi) / 2;
+    if (items[mid].tileBegin <= g) lo = mid; else hi = mid;
+  }
+  return lo;
+}
+
+// Split form of a batched step over the global tiles [g0, g1) (one chunk of
+// the members' concatenated tiles): k_batch_gather runs phase 1 one wave per
+// neuron into the chunk's blocked scratch (tile g at (g - g0)), then
+// k_batch_integrate runs phase 2, one wave per tile, and the fired scan of
+// each member's next step.
+template <int K, int B>
+__global__ __launch_bounds__(kWave) void k_batch_gather(const DevNet *nets, const BatchItem *items, uint32_t count,
+                                                        uint32_t g0, uint32_t g1, uint32_t cap, uint64_t *keys,
+                                                        int16_t *keyW, uint32_t *keyCount) {
+  extern __shared__ uint64_t lds[];
+  const XcdWalk w = xcd_walk((g1 - g0) * kWave);
+  for (uint32_t u = w.first; u < w.end; u += w.stride) {
+    const uint32_t g = g0 + u / kWave, q = u % kWave;
+    const BatchItem &it = items[batch_item_of(items, count, g)];
+    const DevNet &P = nets[it.net];
+    const uint32_t j = (g - it.tileBegin) * kWave + q;
+    if (j >= P.N) continue;
+    const size_t at = (size_t)(u / kWave) * cap * kWave + q * 8;
+    const uint32_t n =
+        gather_sort_neuron<K, B>(P, it.t, j, it.ltd, (uint8_t *)lds, cap, keys + at, keyW + at);
+    if (threadIdx.x == 0) {
+      keyCount[u] = n;
+      atomicAdd(&P.counters[0], (unsigned long long)n);
+    }
+  }
+}
+
+__global__ __launch_bounds__(kBlock) void k_batch_integrate(const DevNet *nets, const BatchItem *items,
+                                                            uint32_t count, uint32_t g0, uint32_t g1, uint32_t cap,
+                                                            const uint64_t *keys, const int16_t *keyW,
+                                                            const uint32_t *keyCount) {
+  __shared__ uint8_t stage[kBlock / kWave][kStageBytes];
+  const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const uint32_t g = g0 + blockIdx.x * (kBlock / kWave) + wave;
+  if (g >= g1) return;  // whole wave
+  const BatchItem &it = items[batch_item_of(items, count, g)];
+  const DevNet &P = nets[it.net];
+  const uint32_t j0 = (g - it.tileBegin) * kWave;
+  const uint32_t m = min((uint32_t)kWave, P.N - j0);
+  const uint32_t u = (g - g0) * kWave;
+  const uint32_t n = lane < m ? keyCount[u + lane] : 0u;
+  const size_t at = (size_t)(g - g0) * cap * kWave;
+  bool f = false;
+  double te = 0.0;
+  unsigned long long ex;
+  if (P.drive == DPSNN_DRIVE_REPLAY)
+    ex = integrate_wave<true>(P, it.t, it.X, j0, m, keys + at, keyW + at, n, stage[wave], f, te);
+  else
+    ex = integrate_wave<false>(P, it.t, it.X, j0, m, keys + at, keyW + at, n, stage[wave], f, te);
+  const uint64_t mask = publish_spikes(P, it.t + 1, j0 + lane, f, te, it.S.popCnt, it.S.count);
+  if (it.S.ltp) ltp_word(P, it.t + 1, mask, j0);
+  for (int o = 32; o > 0; o >>= 1) ex += __shfl_down(ex, o);
+  if (lane == 0 && ex) atomicAdd(&P.counters[1], ex);
+}
+
 // Fills n doubles with v.
 __global__ void k_fill_f64(double *p, double v, uint64_t n) {
   for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
@@ -905,13 +970,16 @@ using StepFn = void (*)(DevNet, uint32_t, ExtView, int, NextScan);
 using GatherFn = void (*)(DevNet, uint32_t, int);
 using BatchFn = void (*)(const DevNet *, const BatchItem *, uint32_t, uint32_t, uint32_t, uint64_t *, int16_t *,
                          uint32_t *);
+using BatchGatherFn = void (*)(const DevNet *, const BatchItem *, uint32_t, uint32_t, uint32_t, uint32_t, uint64_t *,
+                               int16_t *, uint32_t *);
 struct KernelShape {
   int K, B;
   StepFn step;
   GatherFn gather;
   BatchFn batch;
+  BatchGatherFn batchGather;
 };
-#define DPSNN_SHAPE(k, b) {k, b, k_step<k, b>, k_gather<k, b>, k_batch_step<k, b>}
+#define DPSNN_SHAPE(k, b) {k, b, k_step<k, b>, k_gather<k, b>, k_batch_step<k, b>, k_batch_gather<k, b>}
 static const KernelShape kShapes[] = {
     DPSNN_SHAPE(8, 64),  DPSNN_SHAPE(8, 256),  DPSNN_SHAPE(8, 1024),
     DPSNN_SHAPE(16, 64), DPSNN_SHAPE(16, 256), DPSNN_SHAPE(16, 1024),
@@ -1159,9 +1227,9 @@ static int build_device(dpsnn_net *n) {
     P.decayMul = (1 - 1 / q.tauDerivativeDecay_s);
     HIP_TRY(hipStreamSynchronize(n->stream));
   }
-  // launch shapes.  The fused k_step gives each wave whole tiles of 64
-  // neurons; a network with fewer tiles than ~4 per CU runs the split form
-  // (one wave per neuron in phase 1) instead.
+  // launch shapes.  The split form (k_gather, one wave per neuron, then
+  // k_integrate) is the default; the fused k_step (each wave whole tiles of
+  // 64 neurons) is kept as an option (DPSNN_SPLIT=0).
   int cus = 0;
   HIP_TRY(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, n->opt.device));
   cus = std::max(cus, 1);
@@ -1175,7 +1243,7 @@ static int build_device(dpsnn_net *n) {
   const uint32_t wavesPerCu = std::max<uint32_t>(1, std::min<uint32_t>(16, (uint32_t)((160u << 10) / n->ldsBytes)));
   n->grid = round_up(std::min<uint32_t>(tiles, (uint32_t)cus * wavesPerCu), kXcds);
   n->stdpBlocks = std::min<uint32_t>((P.NW + 3) / 4, (uint32_t)cus * 8);
-  n->split = tiles < (uint32_t)cus * 4;
+  n->split = true;
   if (const char *e = std::getenv("DPSNN_SPLIT")) n->split = std::atoi(e) != 0;
   n->gatherGrid = round_up(std::min<uint32_t>(N, (uint32_t)cus * wavesPerCu * 4), kXcds);
   // the key scratch of stand-alone steps is allocated at the first
@@ -1728,14 +1796,27 @@ struct dpsnn_batch {
   uint32_t *keyCount = nullptr;
   size_t scratchSlots = 0;
   uint32_t scratchCap = 0;
+  uint32_t totalTiles = 0;   // of all members
+  uint32_t chunkTiles = 0;   // split form: tiles per chunk (scratch bound)
   int shape = -1;
   uint32_t wavesPerCu = 0;
+  bool split = true;         // k_batch_gather + k_batch_integrate per chunk; else k_batch_step
+  bool useGraph = true;      // replay each step's launches from a captured HIP graph
+  struct Graph {
+    hipGraphExec_t exec = nullptr;
+    uint32_t count = 0, tiles = 0, chunkTiles = 0, wpc = 0;
+    int shape = -1;
+    bool split = false;
+  } graphs[kRing];
   bool profile = false;
   std::vector<std::pair<hipEvent_t, hipEvent_t>> ev;
   size_t evUsed = 0;
   std::vector<StepPlan> plans;
   std::vector<dpsnn_net *> active;
 };
+
+// Scratch the split form may use for one chunk of tiles.
+constexpr size_t kBatchScratchBytes = size_t(4) << 30;
 
 namespace {
 
@@ -1764,6 +1845,11 @@ void batch_free_items(dpsnn_batch *b) {
 int batch_prepare(dpsnn_batch *b) {
   if (!b->dirty) return 0;
   HIP_TRY(hipStreamSynchronize(b->stream));
+  // captured graphs name the buffers below, which may move
+  for (auto &g : b->graphs) {
+    if (g.exec) HIP_TRY(hipGraphExecDestroy(g.exec));
+    g = dpsnn_batch::Graph{};
+  }
   const size_t count = b->nets.size();
   if (count > b->netsCap) {
     if (b->d_nets) HIP_TRY(hipFree(b->d_nets));
@@ -1785,7 +1871,12 @@ int batch_prepare(dpsnn_batch *b) {
     b->itemsCap = cap;
   }
   const uint32_t cap = batch_cap(b);
-  const size_t slots = (size_t)round_up((uint32_t)b->cus * lds_waves(lds_bytes(cap, 64)), kXcds) * kWave;
+  b->totalTiles = 0;
+  for (auto *n : b->nets) b->totalTiles += n->dev.NW;
+  const size_t perTile = (size_t)cap * kWave * (sizeof(uint64_t) + sizeof(int16_t));
+  b->chunkTiles = (uint32_t)std::max<size_t>(1, std::min<size_t>(b->totalTiles, kBatchScratchBytes / perTile));
+  const size_t fusedSlots = (size_t)round_up((uint32_t)b->cus * lds_waves(lds_bytes(cap, 64)), kXcds) * kWave;
+  const size_t slots = std::max(fusedSlots, (size_t)b->chunkTiles * kWave);
   if (cap > b->scratchCap || slots > b->scratchSlots) {
     if (b->keys) HIP_TRY(hipFree(b->keys));
     if (b->keyW) HIP_TRY(hipFree(b->keyW));
@@ -1806,6 +1897,82 @@ int batch_prepare(dpsnn_batch *b) {
     b->shape = find_shape(K, 256);
   }
   b->dirty = false;
+  return 0;
+}
+
+uint32_t batch_waves(const dpsnn_batch *b, size_t lds) {
+  uint32_t wpc = lds_waves(lds);
+  if (b->wavesPerCu) wpc = std::min(wpc, b->wavesPerCu);
+  return wpc;
+}
+
+// The device work of one batched step: the items of ring slot r, then either
+// the persistent fused kernel or, chunk by chunk, phase 1 and phase 2.
+int batch_launch(dpsnn_batch *b, int r, uint32_t count, uint32_t tiles) {
+  HIP_TRY(hipMemcpyAsync(b->d_items[r], b->h_items[r], sizeof(BatchItem) * count, hipMemcpyHostToDevice, b->stream));
+  const KernelShape &K = kShapes[b->shape];
+  const uint32_t cap = b->scratchCap;
+  if (!b->split) {
+    const size_t lds = lds_bytes(cap, K.B);
+    const uint32_t grid = round_up(std::min<uint32_t>(tiles, (uint32_t)b->cus * batch_waves(b, lds)), kXcds);
+    hipLaunchKernelGGL(K.batch, dim3(grid), dim3(kWave), lds, b->stream, b->d_nets, b->d_items[r], count, tiles, cap,
+                       b->keys, b->keyW, b->keyCount);
+  } else {
+    const size_t lds = phase1_lds(cap, K.B);
+    for (uint32_t g0 = 0; g0 < tiles; g0 += b->chunkTiles) {
+      const uint32_t g1 = std::min(tiles, g0 + b->chunkTiles);
+      const uint32_t grid =
+          round_up(std::min<uint32_t>((g1 - g0) * kWave, (uint32_t)b->cus * batch_waves(b, lds) * 4), kXcds);
+      hipLaunchKernelGGL(K.batchGather, dim3(grid), dim3(kWave), lds, b->stream, b->d_nets, b->d_items[r], count, g0,
+                         g1, cap, b->keys, b->keyW, b->keyCount);
+      const uint32_t blocks = (g1 - g0 + kBlock / kWave - 1) / (kBlock / kWave);
+      hipLaunchKernelGGL(k_batch_integrate, dim3(blocks), dim3(kBlock), 0, b->stream, b->d_nets, b->d_items[r], count,
+                         g0, g1, cap, b->keys, b->keyW, b->keyCount);
+    }
+  }
+  HIP_TRY(hipGetLastError());
+  return 0;
+}
+
+void batch_drop_graphs(dpsnn_batch *b) {
+  for (auto &g : b->graphs) {
+    if (g.exec) (void)hipGraphExecDestroy(g.exec);
+    g = dpsnn_batch::Graph{};
+  }
+}
+
+// The same work replayed from a graph captured once per ring slot and launch
+// configuration (the per-step arguments live in the slot's items).
+int batch_graph_launch(dpsnn_batch *b, int r, uint32_t count, uint32_t tiles) {
+  dpsnn_batch::Graph &g = b->graphs[r];
+  const uint32_t wpc = b->wavesPerCu;
+  if (!g.exec || g.count != count || g.tiles != tiles || g.chunkTiles != b->chunkTiles || g.wpc != wpc ||
+      g.shape != b->shape || g.split != b->split) {
+    if (g.exec) HIP_TRY(hipGraphExecDestroy(g.exec));
+    g = dpsnn_batch::Graph{};
+    hipGraph_t graph = nullptr;
+    HIP_TRY(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+    const int rc = batch_launch(b, r, count, tiles);
+    const hipError_t e = hipStreamEndCapture(b->stream, &graph);
+    if (rc) {
+      if (graph) (void)hipGraphDestroy(graph);
+      return rc;
+    }
+    if (e != hipSuccess) return fail(DPSNN_ERR_DEVICE, "graph capture failed: %s", hipGetErrorString(e));
+    const hipError_t ei = hipGraphInstantiate(&g.exec, graph, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(graph);
+    if (ei != hipSuccess) {
+      g.exec = nullptr;
+      return fail(DPSNN_ERR_DEVICE, "graph instantiation failed: %s", hipGetErrorString(ei));
+    }
+    g.count = count;
+    g.tiles = tiles;
+    g.chunkTiles = b->chunkTiles;
+    g.wpc = wpc;
+    g.shape = b->shape;
+    g.split = b->split;
+  }
+  HIP_TRY(hipGraphLaunch(g.exec, b->stream));
   return 0;
 }
 
@@ -1833,13 +2000,6 @@ int batch_one_step(dpsnn_batch *b, bool *any) {
     tiles += n->dev.NW;
   }
   const uint32_t count = (uint32_t)b->active.size();
-  HIP_TRY(hipMemcpyAsync(b->d_items[r], b->h_items[r], sizeof(BatchItem) * count, hipMemcpyHostToDevice, b->stream));
-  const KernelShape &K = kShapes[b->shape];
-  const uint32_t cap = b->scratchCap;
-  const size_t lds = lds_bytes(cap, K.B);
-  uint32_t wpc = lds_waves(lds);
-  if (b->wavesPerCu) wpc = std::min(wpc, b->wavesPerCu);
-  const uint32_t grid = round_up(std::min<uint32_t>(tiles, (uint32_t)b->cus * wpc), kXcds);
   if (b->profile) {
     if (b->evUsed == b->ev.size()) {
       hipEvent_t x, y;
@@ -1849,8 +2009,11 @@ int batch_one_step(dpsnn_batch *b, bool *any) {
     }
     HIP_TRY(hipEventRecord(b->ev[b->evUsed].first, b->stream));
   }
-  hipLaunchKernelGGL(K.batch, dim3(grid), dim3(kWave), lds, b->stream, b->d_nets, b->d_items[r], count, tiles, cap,
-                     b->keys, b->keyW, b->keyCount);
+  if (b->useGraph) {
+    if ((rc = batch_graph_launch(b, r, count, tiles))) return rc;
+  } else {
+    if ((rc = batch_launch(b, r, count, tiles))) return rc;
+  }
   if (b->profile) HIP_TRY(hipEventRecord(b->ev[b->evUsed++].second, b->stream));
   HIP_TRY(hipEventRecord(b->done[r], b->stream));
   b->pending[r] = true;
@@ -1887,6 +2050,8 @@ int dpsnn_batch_create(int device, dpsnn_batch **out) {
   b->cus = std::max(b->cus, 1);
   HIP_TRY(hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking));
   for (int r = 0; r < dpsnn_batch::kRing; ++r) HIP_TRY(hipEventCreateWithFlags(&b->done[r], hipEventDisableTiming));
+  if (const char *e = std::getenv("DPSNN_GRAPH")) b->useGraph = std::atoi(e) != 0;
+  if (const char *e = std::getenv("DPSNN_SPLIT")) b->split = std::atoi(e) != 0;
   *out = b.release();
   return 0;
 }
@@ -1954,6 +2119,7 @@ int dpsnn_batch_get_shape(const dpsnn_batch *b, dpsnn_batch_shape *s) {
   s->buckets = kShapes[i].B;
   s->waves_per_cu = (int)b->wavesPerCu;
   s->ms_per_step = 0;
+  s->split = b->split ? 1 : 0;
   return 0;
 }
 
@@ -1964,6 +2130,7 @@ int dpsnn_batch_set_shape(dpsnn_batch *b, const dpsnn_batch_shape *s) {
   if (s->waves_per_cu < 0 || s->waves_per_cu > 16) return fail(DPSNN_ERR_ARG, "waves_per_cu must be in 0..16");
   b->shape = i;
   b->wavesPerCu = (uint32_t)s->waves_per_cu;
+  b->split = s->split != 0;
   return 0;
 }
 
@@ -1991,36 +2158,42 @@ int dpsnn_batch_tune(dpsnn_batch *b, uint32_t trial_steps, dpsnn_batch_shape *ch
   double best = 1e300;
   int bestShape = b->shape;
   uint32_t bestWaves = b->wavesPerCu;
-  for (int i = 0; i < kNumShapes; ++i) {
-    const uint32_t top = lds_waves(lds_bytes(cap, kShapes[i].B));
-    for (uint32_t w : {top, std::max<uint32_t>(1, top / 2), std::max<uint32_t>(1, top / 4)}) {
-      b->shape = i;
-      b->wavesPerCu = w;
-      HIP_TRY(hipStreamSynchronize(b->stream));
-      b->profile = true;
-      b->evUsed = 0;
-      bool any = true;
-      for (uint32_t k = 0; k < trial_steps && any; ++k)
-        if ((rc = batch_one_step(b, &any))) return rc;
-      double ms = 0;
-      uint64_t launches = 0;
-      if ((rc = batch_kernel_ms(b, &ms, &launches))) return rc;
-      if (launches && ms < best) {
-        best = ms;
-        bestShape = i;
-        bestWaves = w;
+  bool bestSplit = b->split, any = true;
+  // every candidate computes the same result; the trial steps are real steps
+  for (int split = 1; split >= 0 && any; --split)
+    for (int i = 0; i < kNumShapes && any; ++i) {
+      const uint32_t top = lds_waves(split ? phase1_lds(cap, kShapes[i].B) : lds_bytes(cap, kShapes[i].B));
+      for (uint32_t w : {0u, std::max<uint32_t>(1, top / 2)}) {
+        b->shape = i;
+        b->wavesPerCu = w;
+        b->split = split != 0;
+        HIP_TRY(hipStreamSynchronize(b->stream));
+        b->profile = true;
+        b->evUsed = 0;
+        for (uint32_t k = 0; k < trial_steps && any; ++k)
+          if ((rc = batch_one_step(b, &any))) return rc;
+        double ms = 0;
+        uint64_t launches = 0;
+        if ((rc = batch_kernel_ms(b, &ms, &launches))) return rc;
+        if (launches && ms < best) {
+          best = ms;
+          bestShape = i;
+          bestWaves = w;
+          bestSplit = b->split;
+        }
+        if (!any) break;
       }
-      if (!any) break;
     }
-  }
   b->shape = bestShape;
   b->wavesPerCu = bestWaves;
+  b->split = bestSplit;
   b->profile = wasProfiling;
   if (chosen) {
     chosen->chunks = kShapes[bestShape].K;
     chosen->buckets = kShapes[bestShape].B;
     chosen->waves_per_cu = (int)bestWaves;
     chosen->ms_per_step = best < 1e300 ? best : 0.0;
+    chosen->split = bestSplit ? 1 : 0;
   }
   return 0;
 }
@@ -2029,9 +2202,10 @@ void dpsnn_batch_destroy(dpsnn_batch *b) {
   if (!b) return;
   hipSetDevice(b->device);
   while (!b->nets.empty()) dpsnn_batch_remove(b, b->nets.back());
-  hipStreamSynchronize(b->stream);
+  (void)hipStreamSynchronize(b->stream);
+  batch_drop_graphs(b);
   batch_free_items(b);
-  if (b->d_nets) hipFree(b->d_nets);
+  if (b->d_nets) (void)hipFree(b->d_nets);
   if (b->keys) hipFree(b->keys);
   if (b->keyW) hipFree(b->keyW);
   if (b->keyCount) hipFree(b->keyCount);

@@ -159,8 +159,11 @@ typedef struct dpsnn_batch dpsnn_batch;
 typedef struct {
   int chunks;        /* in-list chunks of 64 items held in registers: 8/16/24 */
   int buckets;       /* arrival-time buckets of the LDS sort: 64/256/1024 */
-  int waves_per_cu;  /* persistent waves per CU (0 = as many as LDS allows) */
-  double ms_per_step;/* dpsnn_batch_tune: measured kernel time of the shape */
+  int waves_per_cu;  /* phase-1 waves per CU (0 = as many as LDS allows) */
+  double ms_per_step;/* dpsnn_batch_tune: measured device time of the shape */
+  int split;         /* 1: per chunk of tiles, a phase-1 launch (one wave per
+                        neuron) then a phase-2 launch; 0: one persistent
+                        fused launch per step */
 } dpsnn_batch_shape;
 int dpsnn_batch_create(int device, dpsnn_batch **out);
 int dpsnn_batch_add(dpsnn_batch *b, dpsnn_net *net);
@@ -170,7 +173,9 @@ uint32_t dpsnn_batch_size(const dpsnn_batch *b);
 int dpsnn_batch_step(dpsnn_batch *b, uint32_t nsteps);
 int dpsnn_batch_sync(dpsnn_batch *b);
 /* Autotuner: runs `trial_steps` batched steps (real steps of the
- * simulation) with each launch shape and keeps the fastest. */
+ * simulation) with each launch shape and keeps the fastest.  Each step's
+ * launches are replayed from a captured HIP graph (DPSNN_GRAPH=0: plain
+ * launches). */
 int dpsnn_batch_tune(dpsnn_batch *b, uint32_t trial_steps, dpsnn_batch_shape *chosen);
 int dpsnn_batch_get_shape(const dpsnn_batch *b, dpsnn_batch_shape *s);
 int dpsnn_batch_set_shape(dpsnn_batch *b, const dpsnn_batch_shape *s);

@@ -80,7 +80,9 @@ def test_batch_members_match_alone(pkg, tmp_path):
         assert len(b) == 4
         b.set_shape(16, 64)
         b.step(60)
-        b.set_shape(8, 1024, 4)
+        b.set_shape(8, 1024, 4, split=0)  # the persistent fused launch
+        b.step(30)
+        b.set_shape(24, 256, 0, split=1)
         b.step(30)
         chosen = b.tune(1)
         assert chosen["chunks"] in (8, 16, 24) and chosen["buckets"] in (64, 256, 1024)
@@ -114,3 +116,29 @@ def test_removed_member_steps_alone_again(pkg, tmp_path):
             pass
         net.sync()
         _same(want, _result(net, str(tmp_path / "b"), False))
+
+
+def test_batch_plain_launches_match_graph_replays(pkg, tmp_path, monkeypatch):
+    """The same mix stepped with every launch issued directly and with each
+    step replayed from captured HIP graphs (several replays per graph)."""
+    specs = [("c0", dict(env_CFX=1, env_CFY=1, env_globalSeed=4, env_totalSimTime_ms=150), "philox"),
+             ("c0", dict(env_CFX=2, env_CFY=1, env_globalSeed=6, env_totalSimTime_ms=150), "replay")]
+    res = {}
+    for graph in ("0", "1"):
+        monkeypatch.setenv("DPSNN_GRAPH", graph)
+        outs = [tmp_path / f"g{graph}_{i}" for i in range(len(specs))]
+        for o in outs:
+            o.mkdir()
+        with pkg.Batch() as b:
+            nets = [_make(pkg, sp, o) for sp, o in zip(specs, outs)]
+            for n in nets:
+                b.add(n)
+            while not b.step(25):
+                pass
+            b.sync()
+            res[graph] = [_result(n, str(o), False) for n, o in zip(nets, outs)]
+            for n in nets:
+                b.remove(n)
+                n.close()
+    for a, g in zip(res["0"], res["1"]):
+        _same(a, g)
