@@ -43,7 +43,7 @@ class Options(ctypes.Structure):
     _fields_ = [("device", ctypes.c_int), ("drive", ctypes.c_int),
                 ("philox_seed", ctypes.c_uint64), ("output_dir", ctypes.c_char_p),
                 ("drain_every_ms", ctypes.c_uint32), ("keep_spikes", ctypes.c_int),
-                ("env", ctypes.c_char_p)]
+                ("env", ctypes.c_char_p), ("drop_host_synapses", ctypes.c_int)]
 
 
 _PARAM_U32 = ["globH", "locN", "globN", "D", "totalSimTime_ms", "globCFX", "globCFY", "globCFT",
@@ -202,13 +202,14 @@ class Network:
     """One local network on one GPU (a dpsnn_net)."""
 
     def __init__(self, inidir, device=0, drive=DRIVE_REPLAY, philox_seed=1, output_dir=None,
-                 drain_every_ms=0, keep_spikes=False, env=None):
+                 drain_every_ms=0, keep_spikes=False, env=None, drop_host_synapses=False):
         """env: {"env_name": value} settings that take precedence over the
         process environment for this network only."""
         self._lib = library()
         self._out = output_dir.encode() if output_dir else None
         env_text = ";".join(f"{k}={v}" for k, v in env.items()).encode() if env else None
-        opt = Options(device, drive, philox_seed, self._out, drain_every_ms, int(keep_spikes), env_text)
+        opt = Options(device, drive, philox_seed, self._out, drain_every_ms, int(keep_spikes), env_text,
+                      int(drop_host_synapses))
         h = ctypes.c_void_p()
         _check("dpsnn_create", self._lib.dpsnn_create(inidir.encode(), ctypes.byref(opt), ctypes.byref(h)))
         self._h = h

@@ -1622,6 +1622,8 @@ int dpsnn_create(const char *inidir, const dpsnn_options *opt, dpsnn_net **out) 
   }
   if (n->opt.drive == DPSNN_DRIVE_REPLAY) n->replay.init(n->con, &n->rng);
   n->writing = n->opt.output_dir && n->opt.output_dir[0];
+  if (n->opt.drop_host_synapses && n->stdp && n->writing && !q.statSynPeriodicProbe_writeDisable)
+    return fail(DPSNN_ERR_CONFIG, "synPeriodicProbe files need the host synapse list (drop_host_synapses is set)");
   if (n->writing) n->writer.init(n->par, n->opt.output_dir);
   HIP_TRY(hipSetDevice(n->opt.device));
   HIP_TRY(hipStreamCreateWithFlags(&n->stream, hipStreamNonBlocking));
@@ -1632,11 +1634,20 @@ int dpsnn_create(const char *inidir, const dpsnn_options *opt, dpsnn_net **out) 
     return rc;
   }
   // the device holds the in-lists; the host keeps the backward list for
-  // dumps and drops the delivery staging
+  // dumps (unless asked not to) and drops the delivery staging
   std::vector<uint32_t>().swap(n->con.in_pre_d);
   std::vector<int16_t>().swap(n->con.in_w);
   std::vector<uint32_t>().swap(n->con.in_syn);
   std::vector<uint16_t>().swap(n->inSynLocal);
+  if (n->opt.drop_host_synapses) {
+    std::vector<uint32_t>().swap(n->con.pre);
+    std::vector<uint32_t>().swap(n->con.post);
+    std::vector<uint8_t>().swap(n->con.delay);
+    std::vector<uint8_t>().swap(n->con.kind);
+    std::vector<int16_t>().swap(n->con.weight);
+    std::vector<uint64_t>().swap(n->con.inc_off);
+    std::vector<uint32_t>().swap(n->con.inc_syn);
+  }
   n->initSeconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
   *out = n.release();
   return 0;
@@ -1703,6 +1714,7 @@ uint64_t dpsnn_num_synapses(const dpsnn_net *n) { return n ? n->con.S : 0; }
 int dpsnn_get_synapses(dpsnn_net *n, uint32_t *pre, uint32_t *post, uint8_t *delay, int16_t *weight,
                        float *tderiv, int32_t *last_act) {
   if (!n) return fail(DPSNN_ERR_ARG, "null network");
+  if (n->opt.drop_host_synapses) return fail(DPSNN_ERR_ARG, "the host synapse list was dropped (drop_host_synapses)");
   const Connectome &c = n->con;
   if (pre) std::memcpy(pre, c.pre.data(), 4 * c.S);
   if (post) std::memcpy(post, c.post.data(), 4 * c.S);
@@ -2328,6 +2340,7 @@ void dpsnn_host_destroy(dpsnn_host_net *h) { delete h; }
 
 int dpsnn_write_synapse_dump(dpsnn_net *n, const char *path) {
   if (!n || !path) return fail(DPSNN_ERR_ARG, "null argument");
+  if (n->opt.drop_host_synapses) return fail(DPSNN_ERR_ARG, "the host synapse list was dropped (drop_host_synapses)");
   if (!n->stdp) return write_dump(n->con, n->par.pub.factorWeightType_2_Float, path);
   // plastic network: the device's current weights and derivatives
   const Connectome &c = n->con;

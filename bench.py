@@ -1,6 +1,7 @@
 """Benchmark of the MI355X engine on DPSNN's per-millisecond LIFCA step.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--config c1]
+    python bench.py --mode batch [--steps K] [--warmup W]   (configs[3])
 
 One "step" = one simulated millisecond of the whole local network on one GPU
 (the reference's completeTimeStep_ms).  Default workload: configs[1], a
@@ -61,12 +62,107 @@ def cpu_baseline(pkg, sample_ms):
                       f"{sample_ms} ms simulated after init, {el:.1f} s wall"}
 
 
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def bench_batch(pkg, args):
+    """64 independent networks (grid side 2..12, own seeds, bath
+    2-6 Hz per input, STDP on half) advancing in one batched step; then the
+    same networks stepped one at a time for the same number of steps."""
+    import tempfile
+    W = importlib.import_module("201812realtimecortsim_amd.workloads")
+    mix = W.batch_mix(count=args.batch_count)
+    tmp = tempfile.mkdtemp(prefix="dpsnn_batch_")
+    nets = []
+    t0 = time.perf_counter()
+    for i, spec in enumerate(mix):
+        d = os.path.join(tmp, f"n{i:02d}")
+        env = W.write_instance(d, spec)
+        nets.append(pkg.Network(d, drive=pkg.DRIVE_PHILOX, philox_seed=7000 + i, env=env, drop_host_synapses=True))
+        if i % 8 == 7 or i == len(mix) - 1:
+            log(f"built {i + 1}/{len(mix)} networks, {time.perf_counter() - t0:.0f} s")
+    init_s = time.perf_counter() - t0
+    neurons = sum(n.n for n in nets)
+    synapses = sum(n.stats()["synapses"] for n in nets)
+    dev_bytes = sum(n.stats()["device_bytes"] for n in nets)
+    b = pkg.Batch()
+    for n in nets:
+        b.add(n)
+    b.step(args.warmup)
+    b.sync()
+    shape = b.tune(1) if not args.no_tune else b.shape()
+    log(f"tuned shape {shape}")
+    b.step(5)
+    b.sync()
+
+    def totals():
+        st = [n.stats() for n in nets]
+        return sum(x["synaptic_events"] for x in st), sum(x["spikes"] for x in st)
+
+    e0, s0 = totals()
+    b.profile(True)
+    t0 = time.perf_counter()
+    b.step(args.steps)
+    b.sync()
+    wall = time.perf_counter() - t0
+    kms, launches = b.kernel_time()
+    e1, s1 = totals()
+    batched = (e1 - e0) / wall
+    log(f"batched: {1000 * wall / args.steps:.2f} ms per step, {batched:.3e} events/s")
+    for n in list(nets):
+        b.remove(n)
+    b.close()
+    pe, pw = 0, 0.0
+    for n in nets:
+        a = n.stats()["synaptic_events"]
+        n.step(1)  # stand-alone scratch
+        n.sync()
+        a = n.stats()["synaptic_events"]
+        t0 = time.perf_counter()
+        n.step(args.steps)
+        n.sync()
+        pw += time.perf_counter() - t0
+        pe += n.stats()["synaptic_events"] - a
+    plain = pe / pw
+    log(f"plain (one network at a time): {plain:.3e} events/s")
+    for n in nets:
+        n.close()
+    per_launch = (25.0 * (e1 - e0) + 72.0 * neurons * args.steps) / args.steps
+    achieved = per_launch / (kms * 1e-3) / 1e9 if kms > 0 else 0.0
+    return {
+        "metric": "synaptic events/s + real-time factor on 1 GPU; batched/served vs plain",
+        "value": batched,
+        "unit": "synaptic events/s",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": 1000.0 * wall / args.steps,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "data": "synthetic (reference generator, seeded)",
+        "config": {"workload": f"configs[3] batching: {len(mix)} independent LIFCA networks, grid side 2-12 "
+                               f"columns x 1250 neurons, own seeds, bath 2-6 Hz/input, STDP on "
+                               f"{sum(x['stdp'] for x in mix)}; one batched step = 1 ms of every network",
+                   "networks": len(mix), "neurons": neurons, "synapses": synapses, "device_bytes": dev_bytes,
+                   "drive": "philox", "init_seconds": init_s, "tuned_shape": shape,
+                   "mean_rate_hz": (s1 - s0) / args.steps * 1000.0 / neurons},
+        "batched_vs_plain": batched / plain,
+        "plain_events_per_s": plain,
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "kernel": "batched step (graph of k_batch_gather + k_batch_integrate per chunk)",
+                     "kernel_ms": kms, "launches": launches, "algorithmic_bytes_per_launch": per_launch},
+        "cpu_baseline": None,
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--config", default="c1")
+    ap.add_argument("--mode", default="plain", choices=["plain", "batch"])
+    ap.add_argument("--batch-count", type=int, default=64)
+    ap.add_argument("--no-tune", action="store_true")
     ap.add_argument("--drive", default="philox", choices=["philox", "replay"])
     ap.add_argument("--cpu-sample-ms", type=int, default=40)
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -77,6 +173,9 @@ def main():
               file=sys.stderr)
         return 2
     pkg = importlib.import_module("201812realtimecortsim_amd")
+    if args.mode == "batch":
+        print(json.dumps(bench_batch(pkg, args)), flush=True)
+        return 0
     cfg = pkg.config_dir(args.config)
     total = args.warmup + args.steps
     pkg.apply_env(cfg, env_totalSimTime_ms=max(total, 1), env_spikes_writeDisable=1)

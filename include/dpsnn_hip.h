@@ -64,6 +64,11 @@ typedef struct {
                                 process environment (several networks with
                                 their own settings in one process); NULL =
                                 the environment only */
+  int drop_host_synapses;    /* 1 = free the host copy of the synapse list
+                                once the device holds it (large batches and
+                                served jobs): dpsnn_get_synapses, synapse
+                                dumps and synPeriodicProbe files are then
+                                unavailable */
 } dpsnn_options;
 
 /* Parameters as getParameters returns them (the subset this path reads;
