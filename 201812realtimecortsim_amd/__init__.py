@@ -27,6 +27,8 @@ CONFIGS = os.path.join(HERE, "configs")
 DRIVE_REPLAY = 0
 DRIVE_PHILOX = 1
 DONE = 1
+# dpsnn_status codes
+ERR_CONFIG, ERR_DEVICE, ERR_MEMORY, ERR_ARG, ERR_OVERFLOW, ERR_IO = -1, -2, -3, -4, -5, -6
 
 _lib = None
 
@@ -43,7 +45,8 @@ class Options(ctypes.Structure):
     _fields_ = [("device", ctypes.c_int), ("drive", ctypes.c_int),
                 ("philox_seed", ctypes.c_uint64), ("output_dir", ctypes.c_char_p),
                 ("drain_every_ms", ctypes.c_uint32), ("keep_spikes", ctypes.c_int),
-                ("env", ctypes.c_char_p), ("drop_host_synapses", ctypes.c_int)]
+                ("env", ctypes.c_char_p), ("drop_host_synapses", ctypes.c_int),
+                ("host_threads", ctypes.c_uint32)]
 
 
 _PARAM_U32 = ["globH", "locN", "globN", "D", "totalSimTime_ms", "globCFX", "globCFY", "globCFT",
@@ -83,6 +86,39 @@ def _p(t):
     return ctypes.POINTER(t)
 
 
+class ServerOptions(ctypes.Structure):
+    _fields_ = [("device", ctypes.c_int), ("hbm_budget_bytes", ctypes.c_uint64),
+                ("build_threads", ctypes.c_uint32), ("steps_per_round", ctypes.c_uint32),
+                ("keep_finished", ctypes.c_int)]
+
+
+class JobInfo(ctypes.Structure):
+    _fields_ = [("id", ctypes.c_uint64), ("state", ctypes.c_int), ("status", ctypes.c_int),
+                ("submitted_s", ctypes.c_double), ("built_s", ctypes.c_double),
+                ("admitted_s", ctypes.c_double), ("finished_s", ctypes.c_double),
+                ("device_bytes", ctypes.c_uint64), ("stats", Stats), ("message", ctypes.c_char * 160)]
+
+    def as_dict(self):
+        d = {k: getattr(self, k) for k, _ in self._fields_ if k not in ("stats", "message")}
+        d["state"] = JOB_STATES[self.state]
+        d["stats"] = self.stats.as_dict()
+        d["message"] = self.message.decode(errors="replace")
+        return d
+
+
+class ServerStats(ctypes.Structure):
+    _fields_ = [("submitted", ctypes.c_uint64), ("finished", ctypes.c_uint64), ("failed", ctypes.c_uint64),
+                ("running", ctypes.c_uint64), ("waiting", ctypes.c_uint64), ("building", ctypes.c_uint64),
+                ("synaptic_events", ctypes.c_uint64), ("device_bytes", ctypes.c_uint64),
+                ("elapsed_s", ctypes.c_double)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+JOB_STATES = ("queued", "building", "waiting", "running", "done", "failed")
+
+
 class BatchShape(ctypes.Structure):
     _fields_ = [("chunks", ctypes.c_int), ("buckets", ctypes.c_int), ("waves_per_cu", ctypes.c_int),
                 ("ms_per_step", ctypes.c_double), ("split", ctypes.c_int)]
@@ -94,6 +130,9 @@ class BatchShape(ctypes.Structure):
 _SIGS = {
     "dpsnn_get_parameters": (ctypes.c_int, [ctypes.c_char_p, _p(Parameters)]),
     "dpsnn_create": (ctypes.c_int, [ctypes.c_char_p, _p(Options), _p(ctypes.c_void_p)]),
+    "dpsnn_prepare": (ctypes.c_int, [ctypes.c_char_p, _p(Options), _p(ctypes.c_void_p)]),
+    "dpsnn_device_bytes_needed": (ctypes.c_uint64, [ctypes.c_void_p]),
+    "dpsnn_upload": (ctypes.c_int, [ctypes.c_void_p]),
     "dpsnn_step": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint32]),
     "dpsnn_sync": (ctypes.c_int, [ctypes.c_void_p]),
     "dpsnn_now_ms": (ctypes.c_uint32, [ctypes.c_void_p]),
@@ -134,6 +173,15 @@ _SIGS = {
     "dpsnn_batch_profile": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
     "dpsnn_batch_kernel_time": (ctypes.c_int, [ctypes.c_void_p, _p(ctypes.c_double), _p(ctypes.c_uint64)]),
     "dpsnn_batch_destroy": (None, [ctypes.c_void_p]),
+    "dpsnn_device_memory": (ctypes.c_int, [ctypes.c_int, _p(ctypes.c_uint64), _p(ctypes.c_uint64)]),
+    "dpsnn_server_create": (ctypes.c_int, [_p(ServerOptions), _p(ctypes.c_void_p)]),
+    "dpsnn_server_submit": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_char_p, _p(Options), _p(ctypes.c_uint64)]),
+    "dpsnn_server_job": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint64, _p(JobInfo)]),
+    "dpsnn_server_wait": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint64, ctypes.c_double, _p(JobInfo)]),
+    "dpsnn_server_wait_all": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_double]),
+    "dpsnn_server_take": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint64, _p(ctypes.c_void_p)]),
+    "dpsnn_server_get_stats": (ctypes.c_int, [ctypes.c_void_p, _p(ServerStats)]),
+    "dpsnn_server_destroy": (None, [ctypes.c_void_p]),
     "dpsnn_last_error": (ctypes.c_char_p, []),
 }
 
@@ -214,6 +262,15 @@ class Network:
         _check("dpsnn_create", self._lib.dpsnn_create(inidir.encode(), ctypes.byref(opt), ctypes.byref(h)))
         self._h = h
         self.n = self.stats()["neurons"]
+
+    @classmethod
+    def _wrap(cls, handle):
+        net = cls.__new__(cls)
+        net._lib = library()
+        net._out = None
+        net._h = handle
+        net.n = net.stats()["neurons"]
+        return net
 
     def close(self):
         if getattr(self, "_h", None):
@@ -411,6 +468,68 @@ class Batch:
             self._lib.dpsnn_batch_destroy(self._h)
             self._h = None
             self.members = []
+
+    def __del__(self):
+        self.close()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+
+class Server:
+    """Request loop for simulation jobs with continuous batching (a
+    dpsnn_server): submit() queues a network (ini directory + options);
+    host threads build it, the scheduler admits it into the running batch
+    when its device footprint fits the HBM budget."""
+
+    def __init__(self, device=0, hbm_budget_bytes=0, build_threads=0, steps_per_round=0, keep_finished=False):
+        self._lib = library()
+        o = ServerOptions(device, hbm_budget_bytes, build_threads, steps_per_round, int(keep_finished))
+        h = ctypes.c_void_p()
+        _check("dpsnn_server_create", self._lib.dpsnn_server_create(ctypes.byref(o), ctypes.byref(h)))
+        self._h = h
+
+    def submit(self, inidir, drive=DRIVE_PHILOX, philox_seed=1, output_dir=None, env=None,
+               drop_host_synapses=False, keep_spikes=False):
+        env_text = ";".join(f"{k}={v}" for k, v in env.items()).encode() if env else None
+        out = output_dir.encode() if output_dir else None
+        opt = Options(0, drive, philox_seed, out, 0, int(keep_spikes), env_text, int(drop_host_synapses), 0)
+        jid = ctypes.c_uint64()
+        _check("dpsnn_server_submit", self._lib.dpsnn_server_submit(self._h, inidir.encode(), ctypes.byref(opt),
+                                                                    ctypes.byref(jid)))
+        return jid.value
+
+    def job(self, jid):
+        info = JobInfo()
+        _check("dpsnn_server_job", self._lib.dpsnn_server_job(self._h, jid, ctypes.byref(info)))
+        return info.as_dict()
+
+    def wait(self, jid, timeout_s=-1.0):
+        info = JobInfo()
+        _check("dpsnn_server_wait", self._lib.dpsnn_server_wait(self._h, jid, timeout_s, ctypes.byref(info)))
+        return info.as_dict()
+
+    def wait_all(self, timeout_s=-1.0):
+        return _check("dpsnn_server_wait_all", self._lib.dpsnn_server_wait_all(self._h, timeout_s)) == DONE
+
+    def take(self, jid):
+        """The finished job's network (keep_finished servers) as a Network."""
+        h = ctypes.c_void_p()
+        _check("dpsnn_server_take", self._lib.dpsnn_server_take(self._h, jid, ctypes.byref(h)))
+        return Network._wrap(h)
+
+    def stats(self):
+        st = ServerStats()
+        _check("dpsnn_server_get_stats", self._lib.dpsnn_server_get_stats(self._h, ctypes.byref(st)))
+        return st.as_dict()
+
+    def close(self):
+        if getattr(self, "_h", None):
+            self._lib.dpsnn_server_destroy(self._h)
+            self._h = None
 
     def __del__(self):
         self.close()

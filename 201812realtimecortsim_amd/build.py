@@ -27,7 +27,7 @@ HIP_FLAGS = [
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     "-Wall", "-Wno-unused-function",
 ]
-HOST_FLAGS = ["-O2", "-fPIC", "-std=c++17", "-ffp-contract=off", "-Wall"]
+HOST_FLAGS = ["-O2", "-fPIC", "-std=c++17", "-ffp-contract=off", "-Wall", "-pthread"]
 
 
 def _hipcc():
@@ -53,7 +53,7 @@ def build(verbose=False):
     host_obj = os.path.join(objdir, "dpsnn_host.o")
     _run(["g++", *HOST_FLAGS, *inc, "-c", os.path.join(CSRC, "dpsnn_host.cpp"), "-o", host_obj])
     objs = [host_obj]
-    for src in ("dpsnn_engine.hip", "dpsnn_process.cpp"):
+    for src in ("dpsnn_engine.hip", "dpsnn_process.cpp", "dpsnn_server.cpp"):
         path = os.path.join(CSRC, src)
         if not os.path.exists(path):
             continue
@@ -64,7 +64,7 @@ def build(verbose=False):
             _run(["g++", *HOST_FLAGS, *inc, "-c", path, "-o", obj])
         objs.append(obj)
     tmp = LIB + ".tmp"
-    _run([hipcc, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp, *objs])
+    _run([hipcc, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-pthread", "-o", tmp, *objs])
     os.replace(tmp, LIB)
     main_src = os.path.join(CSRC, "dpsnn_main.c")
     if os.path.exists(main_src):

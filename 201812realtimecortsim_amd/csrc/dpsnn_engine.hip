@@ -1052,6 +1052,8 @@ struct dpsnn_net {
   uint32_t grid = 0;
   size_t ldsBytes = 0;
   bool stdp = false;         // plasticity runs at some step of [0, total)
+  bool uploaded = false;     // build_device done (dpsnn_upload)
+  std::string outDir;        // owned copy of opt.output_dir
   uint32_t stdpBlocks = 0;
   int shape = 0;             // kShapes index
   struct dpsnn_batch *batch = nullptr;  // stepped by this batch (its stream)
@@ -1580,7 +1582,7 @@ int dpsnn_get_parameters(const char *inidir, dpsnn_parameters *out) {
   return 0;
 }
 
-int dpsnn_create(const char *inidir, const dpsnn_options *opt, dpsnn_net **out) {
+int dpsnn_prepare(const char *inidir, const dpsnn_options *opt, dpsnn_net **out) {
   if (!inidir || !out) return fail(DPSNN_ERR_ARG, "null argument");
   *out = nullptr;
   auto t0 = std::chrono::steady_clock::now();
@@ -1593,6 +1595,10 @@ int dpsnn_create(const char *inidir, const dpsnn_options *opt, dpsnn_net **out) 
   if (!parse_env_overrides(n->opt.env, over, err) || !read_params(inidir, n->par, err, &over))
     return fail(DPSNN_ERR_CONFIG, "%s", err.c_str());
   n->opt.env = nullptr;  // not kept past the call
+  if (n->opt.output_dir) {
+    n->outDir = n->opt.output_dir;
+    n->opt.output_dir = n->outDir.c_str();
+  }
   const dpsnn_parameters &q = n->par.pub;
   n->stdp = q.startPlasticity_ms <= q.stopPlasticity_ms && q.startPlasticity_ms < q.totalSimTime_ms;
   if (n->stdp)
@@ -1604,8 +1610,9 @@ int dpsnn_create(const char *inidir, const dpsnn_options *opt, dpsnn_net **out) 
   // localNet init: seed with env_globalSeed (DPSNN_localNet_init.c:119),
   // generate, then continue the stream from SetRandomSeed(loc_h=0) (:283)
   n->rng.seed((int32_t)q.globalSeed);
-  if (!generate_connectome(n->par, n->rng, n->con, err)) return fail(DPSNN_ERR_CONFIG, "%s", err.c_str());
-  build_delivery(n->par, n->con, /*with_incoming=*/n->stdp);
+  if (!generate_connectome(n->par, n->rng, n->con, err, n->opt.host_threads))
+    return fail(DPSNN_ERR_CONFIG, "%s", err.c_str());
+  build_delivery(n->par, n->con, /*with_incoming=*/n->stdp, n->opt.host_threads);
   n->rng.seed(0);
   if (n->stdp) {
     // each in-list item names its synapse by its rank in the target's
@@ -1620,19 +1627,63 @@ int dpsnn_create(const char *inidir, const dpsnn_options *opt, dpsnn_net **out) 
     n->inSynLocal.resize(c.in_off[c.N]);
     for (uint64_t it = 0; it < c.in_off[c.N]; ++it) n->inSynLocal[it] = (uint16_t)rank[c.in_syn[it]];
   }
+  if (n->con.max_in >= kMaxInList)
+    return fail(DPSNN_ERR_CONFIG, "a neuron has %u in-list items; this build handles up to %u", n->con.max_in,
+                kMaxInList - 1);
   if (n->opt.drive == DPSNN_DRIVE_REPLAY) n->replay.init(n->con, &n->rng);
   n->writing = n->opt.output_dir && n->opt.output_dir[0];
   if (n->opt.drop_host_synapses && n->stdp && n->writing && !q.statSynPeriodicProbe_writeDisable)
     return fail(DPSNN_ERR_CONFIG, "synPeriodicProbe files need the host synapse list (drop_host_synapses is set)");
-  if (n->writing) n->writer.init(n->par, n->opt.output_dir);
+  if (n->writing && !n->writer.init(n->par, n->opt.output_dir))
+    return fail(DPSNN_ERR_IO, "cannot write output files in %s", n->opt.output_dir);
+  n->initSeconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+  *out = n.release();
+  return 0;
+}
+
+uint64_t dpsnn_device_bytes_needed(const dpsnn_net *n) {
+  if (!n) return 0;
+  if (n->uploaded) return n->devBytes;
+  const Connectome &c = n->con;
+  const dpsnn_parameters &q = n->par.pub;
+  const uint64_t N = c.N, NW = (N + 63) / 64, items = c.in_off.empty() ? 0 : c.in_off[N];
+  uint64_t RS = 1;
+  while (RS < c.max_delay + 2) RS <<= 1;
+  auto r = [](uint64_t b) { return (std::max<uint64_t>(b, 1) + 255) / 256 * 256; };
+  uint64_t b = r(8 * q.subPopNumber * kParams) + r(8 * c.bathLut.size()) + r(8 * N) + r(N) + r(4 * N);
+  b += 5 * r(8 * N) + r(8 * N);                                // state, Philox counters
+  b += r(8 * RS * NW) + r(8 * RS * N);                         // spike history
+  b += r(8 * (N + 1)) + r(4 * items) + r(2 * items);           // in-lists
+  if (n->stdp) {
+    const uint64_t S = c.S;
+    b += r(8 * (N + 1)) + r(2 * S) + r(S) + r(2 * items) + 2 * r(4 * S) + r(4 * N);
+    if (!q.statSynPeriodicProbe_writeDisable) b += r(4 * S);
+    b += 2 * r(4 * kStdpSpan) + 2 * r(4 * q.subPopNumber * q.subPopNumber);
+  }
+  const uint64_t drainEvery = n->opt.drain_every_ms ? n->opt.drain_every_ms : 100;
+  b += r(8 * 4) + r(4 * (uint64_t)q.globCFT * q.subPopNumber * drainEvery);
+  const bool raster = (n->writing && !q.spikes_writeDisable) || n->opt.keep_spikes;
+  if (raster) {
+    const uint64_t cap = N * std::min<uint64_t>(drainEvery, 1000) + 1;
+    b += r(4 * cap) + r(8 * cap) + r(4) + r(4 * drainEvery);
+  }
+  if (n->opt.drive == DPSNN_DRIVE_REPLAY) b += kExtRing * r(4 * (N + 1));
+  return b;
+}
+
+int dpsnn_upload(dpsnn_net *n) {
+  if (!n) return fail(DPSNN_ERR_ARG, "null network");
+  if (n->uploaded) return 0;
+  auto t0 = std::chrono::steady_clock::now();
   HIP_TRY(hipSetDevice(n->opt.device));
   HIP_TRY(hipStreamCreateWithFlags(&n->stream, hipStreamNonBlocking));
   n->ownStream = n->stream;
-  int rc = build_device(n.get());
+  int rc = build_device(n);
   if (rc) {
-    free_all(n.get());
+    free_all(n);
     return rc;
   }
+  n->uploaded = true;
   // the device holds the in-lists; the host keeps the backward list for
   // dumps (unless asked not to) and drops the delivery staging
   std::vector<uint32_t>().swap(n->con.in_pre_d);
@@ -1648,14 +1699,29 @@ int dpsnn_create(const char *inidir, const dpsnn_options *opt, dpsnn_net **out) 
     std::vector<uint64_t>().swap(n->con.inc_off);
     std::vector<uint32_t>().swap(n->con.inc_syn);
   }
-  n->initSeconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
-  *out = n.release();
+  n->initSeconds += std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+  return 0;
+}
+
+int dpsnn_create(const char *inidir, const dpsnn_options *opt, dpsnn_net **out) {
+  if (!out) return fail(DPSNN_ERR_ARG, "null argument");
+  dpsnn_net *n = nullptr;
+  int rc = dpsnn_prepare(inidir, opt, &n);
+  if (rc) return rc;
+  if ((rc = dpsnn_upload(n))) {
+    std::string keep = g_err;
+    dpsnn_destroy(n);
+    g_err = keep;
+    return rc;
+  }
+  *out = n;
   return 0;
 }
 
 int dpsnn_step(dpsnn_net *n, uint32_t nsteps) {
   if (!n) return fail(DPSNN_ERR_ARG, "null network");
   if (n->batch) return fail(DPSNN_ERR_ARG, "network is a member of a batch: step it through dpsnn_batch_step");
+  if (!n->uploaded) return fail(DPSNN_ERR_ARG, "network not uploaded (dpsnn_upload)");
   HIP_TRY(hipSetDevice(n->opt.device));
   int rc0 = ensure_scratch(n);
   if (rc0) return rc0;
@@ -2072,6 +2138,7 @@ int dpsnn_batch_add(dpsnn_batch *b, dpsnn_net *n) {
   if (!b || !n) return fail(DPSNN_ERR_ARG, "null argument");
   if (n->batch) return fail(DPSNN_ERR_ARG, "network is already a member of a batch");
   if (n->opt.device != b->device) return fail(DPSNN_ERR_ARG, "network and batch are on different devices");
+  if (!n->uploaded) return fail(DPSNN_ERR_ARG, "network not uploaded (dpsnn_upload)");
   HIP_TRY(hipSetDevice(b->device));
   HIP_TRY(hipStreamSynchronize(n->stream));
   n->ownStream = n->stream;
@@ -2358,6 +2425,15 @@ int dpsnn_write_synapse_dump(dpsnn_net *n, const char *path) {
     std::fwrite(line.data(), 1, (size_t)len, f);
   }
   std::fclose(f);
+  return 0;
+}
+
+int dpsnn_device_memory(int device, uint64_t *free_bytes, uint64_t *total_bytes) {
+  HIP_TRY(hipSetDevice(device));
+  size_t f = 0, t = 0;
+  HIP_TRY(hipMemGetInfo(&f, &t));
+  if (free_bytes) *free_bytes = f;
+  if (total_bytes) *total_bytes = t;
   return 0;
 }
 

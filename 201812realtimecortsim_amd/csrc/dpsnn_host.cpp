@@ -169,7 +169,8 @@ struct EnvReader {
 // float (the readers' float initValue[]).  Returns the token count.
 int ini_tokens(char *line, std::vector<float> &vals) {
   int n = 0;
-  for (char *tok = std::strtok(line, " \t\n"); tok; tok = std::strtok(nullptr, " \t\n")) {
+  char *save = nullptr;  // strtok_r: networks may be read on several threads at once
+  for (char *tok = strtok_r(line, " \t\n", &save); tok; tok = strtok_r(nullptr, " \t\n", &save)) {
     if (tok[0] == '#') break;
     if ((size_t)n < vals.size()) vals[n] = (float)std::atof(tok);
     ++n;
@@ -272,7 +273,8 @@ bool read_bath(const std::string &dir, Params &p, std::string &err) {
   uint32_t y = 0;
   while (std::fgets(line.data(), (int)line.size(), f)) {
     uint32_t x = 0, n = 0;
-    for (char *tok = std::strtok(line.data(), " \t\n"); tok; tok = std::strtok(nullptr, " \t\n")) {
+    char *save = nullptr;
+    for (char *tok = strtok_r(line.data(), " \t\n", &save); tok; tok = strtok_r(nullptr, " \t\n", &save)) {
       if (tok[0] == '#') break;
       if (x < kMaxStencil && y < kMaxStencil) p.bath[size_t(x) * kMaxStencil + y] = (uint32_t)std::atoi(tok);
       ++x;

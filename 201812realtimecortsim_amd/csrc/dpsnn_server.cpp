@@ -1,0 +1,329 @@
+// dpsnn_server.cpp -- request loop for simulation jobs on one GPU
+// (continuous batching over the engine's C ABI).
+//
+// Builder threads turn submitted jobs into host-side networks
+// (dpsnn_prepare: parameters, generator, delivery layout).  One scheduler
+// thread makes every device call: it admits built jobs first-fit in arrival
+// order while their exact device footprint (dpsnn_device_bytes_needed) fits
+// the HBM budget, uploads them and adds them to its dpsnn_batch, steps the
+// batch a few ms per round (all running jobs in the same launches, each at
+// its own ms), and retires the jobs that reached their
+// env_totalSimTime_ms: they leave the batch, their files are drained, their
+// memory returns to the budget.  Jobs arriving while others run join the
+// batch at the next round.
+#include <algorithm>
+#include <atomic>
+#include <chrono>
+#include <condition_variable>
+#include <cstdio>
+#include <cstring>
+#include <deque>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <string>
+#include <thread>
+#include <vector>
+
+#include "dpsnn_hip.h"
+
+namespace {
+
+using Clock = std::chrono::steady_clock;
+
+// Device memory of the batch itself (key scratch of the longest in-list at
+// the largest chunk, items, graphs), kept out of the jobs' budget.
+constexpr uint64_t kBatchReserve = uint64_t(9) << 29;  // 4.5 GiB
+
+struct Job {
+  std::string inidir, outdir, env;
+  dpsnn_options opt{};
+  dpsnn_job_info info{};
+  dpsnn_net *net = nullptr;
+  std::string message;
+};
+
+}  // namespace
+
+struct dpsnn_server {
+  dpsnn_server_options o{};
+  Clock::time_point t0;
+  std::mutex mu;
+  std::condition_variable cvBuild, cvSched, cvDone;
+  std::deque<Job *> buildQ, waiting;
+  std::vector<Job *> running;
+  std::map<uint64_t, std::unique_ptr<Job>> jobs;
+  std::vector<std::thread> builders;
+  std::thread sched;
+  bool stop = false;
+  uint64_t nextId = 1, used = 0, budget = 0;
+  uint64_t eventsDone = 0, eventsRunning = 0;
+  uint32_t hostThreads = 0;
+  dpsnn_batch *batch = nullptr;
+
+  double now() const { return std::chrono::duration<double>(Clock::now() - t0).count(); }
+};
+
+namespace {
+
+void finish_locked(dpsnn_server *s, Job *j, int state, int status, const std::string &msg) {
+  j->info.state = state;
+  j->info.status = status;
+  j->info.finished_s = s->now();
+  if (!msg.empty()) {
+    j->message = msg;
+    std::snprintf(j->info.message, sizeof j->info.message, "%s", msg.c_str());
+  }
+  s->cvDone.notify_all();
+}
+
+void builder_loop(dpsnn_server *s) {
+  for (;;) {
+    Job *j = nullptr;
+    {
+      std::unique_lock<std::mutex> lk(s->mu);
+      s->cvBuild.wait(lk, [&] { return s->stop || !s->buildQ.empty(); });
+      if (s->stop) return;
+      j = s->buildQ.front();
+      s->buildQ.pop_front();
+      j->info.state = DPSNN_JOB_BUILDING;
+    }
+    dpsnn_options opt = j->opt;
+    opt.env = j->env.empty() ? nullptr : j->env.c_str();
+    opt.output_dir = j->outdir.empty() ? nullptr : j->outdir.c_str();
+    if (!opt.host_threads) opt.host_threads = s->hostThreads;
+    dpsnn_net *n = nullptr;
+    const int rc = dpsnn_prepare(j->inidir.c_str(), &opt, &n);
+    const std::string err = rc ? dpsnn_last_error() : "";
+    const uint64_t need = rc ? 0 : dpsnn_device_bytes_needed(n);
+    std::unique_lock<std::mutex> lk(s->mu);
+    if (rc) {
+      finish_locked(s, j, DPSNN_JOB_FAILED, rc, err);
+    } else if (need > s->budget) {
+      lk.unlock();
+      dpsnn_destroy(n);
+      lk.lock();
+      finish_locked(s, j, DPSNN_JOB_FAILED, DPSNN_ERR_MEMORY, "job needs more device memory than the budget");
+    } else {
+      j->net = n;
+      j->info.device_bytes = need;
+      j->info.built_s = s->now();
+      j->info.state = DPSNN_JOB_WAITING;
+      s->waiting.push_back(j);
+      s->cvSched.notify_one();
+    }
+  }
+}
+
+void scheduler_loop(dpsnn_server *s) {
+  const uint32_t round = s->o.steps_per_round ? s->o.steps_per_round : 10;
+  for (;;) {
+    std::vector<Job *> admit;
+    {
+      std::unique_lock<std::mutex> lk(s->mu);
+      s->cvSched.wait(lk, [&] { return s->stop || !s->waiting.empty() || !s->running.empty(); });
+      if (s->stop) return;
+      // first fit in arrival order
+      for (auto it = s->waiting.begin(); it != s->waiting.end();) {
+        Job *j = *it;
+        if (s->used + j->info.device_bytes <= s->budget) {
+          s->used += j->info.device_bytes;
+          admit.push_back(j);
+          it = s->waiting.erase(it);
+        } else {
+          ++it;
+        }
+      }
+    }
+    for (Job *j : admit) {
+      int rc = dpsnn_upload(j->net);
+      if (!rc) rc = dpsnn_batch_add(s->batch, j->net);
+      if (rc) {
+        const std::string err = dpsnn_last_error();
+        dpsnn_destroy(j->net);
+        std::lock_guard<std::mutex> lk(s->mu);
+        j->net = nullptr;
+        s->used -= j->info.device_bytes;
+        finish_locked(s, j, DPSNN_JOB_FAILED, rc, err);
+        continue;
+      }
+      std::lock_guard<std::mutex> lk(s->mu);
+      j->info.admitted_s = s->now();
+      j->info.state = DPSNN_JOB_RUNNING;
+      s->running.push_back(j);
+    }
+    if (s->running.empty()) continue;
+    const int rc = dpsnn_batch_step(s->batch, round);
+    const std::string err = rc < 0 ? dpsnn_last_error() : "";
+    std::vector<Job *> done;
+    uint64_t events = 0;
+    for (Job *j : std::vector<Job *>(s->running)) {
+      if (rc >= 0 && dpsnn_now_ms(j->net) < dpsnn_total_ms(j->net)) {
+        dpsnn_stats st{};
+        if (dpsnn_get_stats(j->net, &st) == 0) events += st.synaptic_events;
+        continue;
+      }
+      dpsnn_batch_remove(s->batch, j->net);
+      int frc = rc < 0 ? rc : dpsnn_sync(j->net);
+      if (!frc) frc = dpsnn_get_stats(j->net, &j->info.stats);
+      const std::string ferr = frc ? (rc < 0 ? err : std::string(dpsnn_last_error())) : "";
+      if (frc || !s->o.keep_finished) {
+        dpsnn_destroy(j->net);
+        j->net = nullptr;
+      }
+      std::lock_guard<std::mutex> lk(s->mu);
+      if (!j->net) s->used -= j->info.device_bytes;
+      s->running.erase(std::find(s->running.begin(), s->running.end(), j));
+      if (frc) {
+        finish_locked(s, j, DPSNN_JOB_FAILED, frc, ferr);
+      } else {
+        s->eventsDone += j->info.stats.synaptic_events;
+        finish_locked(s, j, DPSNN_JOB_DONE, 0, "");
+      }
+    }
+    std::lock_guard<std::mutex> lk(s->mu);
+    s->eventsRunning = events;
+  }
+}
+
+bool job_over(const Job *j) { return j->info.state == DPSNN_JOB_DONE || j->info.state == DPSNN_JOB_FAILED; }
+
+}  // namespace
+
+extern "C" {
+
+int dpsnn_server_create(const dpsnn_server_options *opt, dpsnn_server **out) {
+  if (!out) return DPSNN_ERR_ARG;
+  *out = nullptr;
+  std::unique_ptr<dpsnn_server> s(new dpsnn_server());
+  if (opt) s->o = *opt;
+  s->t0 = Clock::now();
+  uint64_t freeB = 0, totalB = 0;
+  int rc = dpsnn_device_memory(s->o.device, &freeB, &totalB);
+  if (rc) return rc;
+  uint64_t budget = s->o.hbm_budget_bytes ? s->o.hbm_budget_bytes : freeB / 10 * 9;
+  budget = std::min(budget, freeB);
+  if (budget <= kBatchReserve) return DPSNN_ERR_MEMORY;
+  s->budget = budget - kBatchReserve;
+  if ((rc = dpsnn_batch_create(s->o.device, &s->batch))) return rc;
+  const uint32_t nb = s->o.build_threads ? s->o.build_threads : 2;
+  const uint32_t hw = std::max(1u, std::thread::hardware_concurrency());
+  s->hostThreads = std::max(1u, hw / nb);
+  dpsnn_server *p = s.get();
+  for (uint32_t i = 0; i < nb; ++i) s->builders.emplace_back(builder_loop, p);
+  s->sched = std::thread(scheduler_loop, p);
+  *out = s.release();
+  return 0;
+}
+
+int dpsnn_server_submit(dpsnn_server *s, const char *inidir, const dpsnn_options *opt, uint64_t *id) {
+  if (!s || !inidir) return DPSNN_ERR_ARG;
+  std::unique_ptr<Job> j(new Job());
+  j->inidir = inidir;
+  if (opt) {
+    j->opt = *opt;
+    if (opt->env) j->env = opt->env;
+    if (opt->output_dir) j->outdir = opt->output_dir;
+  }
+  j->opt.env = nullptr;
+  j->opt.output_dir = nullptr;
+  std::lock_guard<std::mutex> lk(s->mu);
+  j->info.id = s->nextId++;
+  j->info.state = DPSNN_JOB_QUEUED;
+  j->info.submitted_s = s->now();
+  if (id) *id = j->info.id;
+  s->buildQ.push_back(j.get());
+  s->jobs[j->info.id] = std::move(j);
+  s->cvBuild.notify_one();
+  return 0;
+}
+
+int dpsnn_server_job(dpsnn_server *s, uint64_t id, dpsnn_job_info *info) {
+  if (!s || !info) return DPSNN_ERR_ARG;
+  std::lock_guard<std::mutex> lk(s->mu);
+  auto it = s->jobs.find(id);
+  if (it == s->jobs.end()) return DPSNN_ERR_ARG;
+  *info = it->second->info;
+  return 0;
+}
+
+int dpsnn_server_wait(dpsnn_server *s, uint64_t id, double timeout_s, dpsnn_job_info *info) {
+  if (!s) return DPSNN_ERR_ARG;
+  std::unique_lock<std::mutex> lk(s->mu);
+  auto it = s->jobs.find(id);
+  if (it == s->jobs.end()) return DPSNN_ERR_ARG;
+  Job *j = it->second.get();
+  auto pred = [&] { return job_over(j); };
+  if (timeout_s < 0)
+    s->cvDone.wait(lk, pred);
+  else
+    s->cvDone.wait_for(lk, std::chrono::duration<double>(timeout_s), pred);
+  if (info) *info = j->info;
+  return job_over(j) ? DPSNN_DONE : 0;
+}
+
+int dpsnn_server_wait_all(dpsnn_server *s, double timeout_s) {
+  if (!s) return DPSNN_ERR_ARG;
+  std::unique_lock<std::mutex> lk(s->mu);
+  auto pred = [&] {
+    for (auto &kv : s->jobs)
+      if (!job_over(kv.second.get())) return false;
+    return true;
+  };
+  if (timeout_s < 0)
+    s->cvDone.wait(lk, pred);
+  else
+    s->cvDone.wait_for(lk, std::chrono::duration<double>(timeout_s), pred);
+  return pred() ? DPSNN_DONE : 0;
+}
+
+int dpsnn_server_take(dpsnn_server *s, uint64_t id, dpsnn_net **net) {
+  if (!s || !net) return DPSNN_ERR_ARG;
+  std::lock_guard<std::mutex> lk(s->mu);
+  auto it = s->jobs.find(id);
+  if (it == s->jobs.end()) return DPSNN_ERR_ARG;
+  Job *j = it->second.get();
+  if (j->info.state != DPSNN_JOB_DONE || !j->net) return DPSNN_ERR_ARG;
+  *net = j->net;
+  j->net = nullptr;
+  s->used -= j->info.device_bytes;
+  s->cvSched.notify_one();
+  return 0;
+}
+
+int dpsnn_server_get_stats(dpsnn_server *s, dpsnn_server_stats *st) {
+  if (!s || !st) return DPSNN_ERR_ARG;
+  std::lock_guard<std::mutex> lk(s->mu);
+  *st = dpsnn_server_stats{};
+  for (auto &kv : s->jobs) {
+    const int state = kv.second->info.state;
+    st->submitted++;
+    if (state == DPSNN_JOB_DONE) st->finished++;
+    if (state == DPSNN_JOB_FAILED) st->failed++;
+    if (state == DPSNN_JOB_RUNNING) st->running++;
+    if (state == DPSNN_JOB_WAITING) st->waiting++;
+    if (state == DPSNN_JOB_BUILDING || state == DPSNN_JOB_QUEUED) st->building++;
+  }
+  st->synaptic_events = s->eventsDone + s->eventsRunning;
+  st->device_bytes = s->used + kBatchReserve;
+  st->elapsed_s = s->now();
+  return 0;
+}
+
+void dpsnn_server_destroy(dpsnn_server *s) {
+  if (!s) return;
+  {
+    std::lock_guard<std::mutex> lk(s->mu);
+    s->stop = true;
+  }
+  s->cvBuild.notify_all();
+  s->cvSched.notify_all();
+  for (auto &t : s->builders) t.join();
+  if (s->sched.joinable()) s->sched.join();
+  dpsnn_batch_destroy(s->batch);
+  for (auto &kv : s->jobs)
+    if (kv.second->net) dpsnn_destroy(kv.second->net);
+  delete s;
+}
+
+}  // extern "C"

@@ -69,6 +69,8 @@ typedef struct {
                                 served jobs): dpsnn_get_synapses, synapse
                                 dumps and synPeriodicProbe files are then
                                 unavailable */
+  uint32_t host_threads;     /* host threads of the connectome generator and
+                                delivery layout (0 = all hardware threads) */
 } dpsnn_options;
 
 /* Parameters as getParameters returns them (the subset this path reads;
@@ -114,6 +116,14 @@ int dpsnn_get_parameters(const char *inidir, dpsnn_parameters *out);
  * per-neuron in-lists to the device.  Replaces localNetClass::init
  * (DPSNN_localNet_init.c:43-307) at world size 1. */
 int dpsnn_create(const char *inidir, const dpsnn_options *opt, dpsnn_net **out);
+/* dpsnn_create in two stages, so that host work (parameters, generator,
+ * layout: no device call) can overlap device work: dpsnn_prepare builds the
+ * network on the host, dpsnn_device_bytes_needed tells what its upload will
+ * allocate, dpsnn_upload allocates and uploads.  A prepared network can be
+ * destroyed without ever being uploaded. */
+int dpsnn_prepare(const char *inidir, const dpsnn_options *opt, dpsnn_net **out);
+uint64_t dpsnn_device_bytes_needed(const dpsnn_net *net);
+int dpsnn_upload(dpsnn_net *net);
 
 /* Advances `nsteps` milliseconds (each = one completeTimeStep_ms).  Returns
  * DPSNN_DONE once totalSimTime_ms has been reached (the reference's fire()
@@ -190,6 +200,65 @@ int dpsnn_batch_kernel_time(dpsnn_batch *b, double *avg_ms, uint64_t *launches);
 /* Members are removed (not destroyed). */
 void dpsnn_batch_destroy(dpsnn_batch *b);
 
+/* ---- serving -----------------------------------------------------------
+ * A request loop for simulation jobs.  Jobs are submitted at any time; host
+ * threads build their networks (dpsnn_prepare); a scheduler thread admits a
+ * built job as soon as its device footprint fits the HBM budget (first fit
+ * in arrival order), uploads it and adds it to the running batch, so it
+ * advances in the same batched launches as the jobs already on the GPU
+ * (continuous batching); finished jobs leave the batch and free their
+ * memory.  State stays on the device from admission to completion.  A job's
+ * results (files, counters, state) are those of the network run alone. */
+typedef struct dpsnn_server dpsnn_server;
+typedef struct {
+  int device;
+  uint64_t hbm_budget_bytes;  /* device bytes admitted jobs may hold, batch
+                                 scratch included (0 = 90% of free memory) */
+  uint32_t build_threads;     /* concurrent host builds (0 = 2) */
+  uint32_t steps_per_round;   /* batched steps between admissions (0 = 10) */
+  int keep_finished;          /* 1 = a finished job keeps its network (and
+                                 its memory) until dpsnn_server_take */
+} dpsnn_server_options;
+typedef enum {
+  DPSNN_JOB_QUEUED = 0,       /* waiting for a host builder */
+  DPSNN_JOB_BUILDING = 1,     /* network being built on the host */
+  DPSNN_JOB_WAITING = 2,      /* built, waiting for device memory */
+  DPSNN_JOB_RUNNING = 3,      /* on the device, in the batch */
+  DPSNN_JOB_DONE = 4,
+  DPSNN_JOB_FAILED = 5
+} dpsnn_job_state;
+typedef struct {
+  uint64_t id;
+  int state;                  /* dpsnn_job_state */
+  int status;                 /* 0, or the failing call's dpsnn_status */
+  double submitted_s, built_s, admitted_s, finished_s;  /* server clock */
+  uint64_t device_bytes;
+  dpsnn_stats stats;          /* final counters (DONE) */
+  char message[160];          /* FAILED: the error */
+} dpsnn_job_info;
+typedef struct {
+  uint64_t submitted, finished, failed, running, waiting, building;
+  uint64_t synaptic_events;   /* delivered by all jobs so far */
+  uint64_t device_bytes;      /* held by admitted jobs + batch scratch */
+  double elapsed_s;           /* server clock */
+} dpsnn_server_stats;
+int dpsnn_server_create(const dpsnn_server_options *opt, dpsnn_server **out);
+/* Queues a job: the network of `inidir` with options `opt` (copied; its
+ * env, output_dir and drive are the job's own). */
+int dpsnn_server_submit(dpsnn_server *s, const char *inidir, const dpsnn_options *opt, uint64_t *id);
+int dpsnn_server_job(dpsnn_server *s, uint64_t id, dpsnn_job_info *info);
+/* Waits until job `id` is DONE or FAILED (timeout_s < 0: no limit);
+ * returns DPSNN_DONE when it is, 0 on timeout. */
+int dpsnn_server_wait(dpsnn_server *s, uint64_t id, double timeout_s, dpsnn_job_info *info);
+/* Waits until every submitted job is DONE or FAILED. */
+int dpsnn_server_wait_all(dpsnn_server *s, double timeout_s);
+/* keep_finished: hands a DONE job's network to the caller, who destroys it
+ * with dpsnn_destroy (its memory leaves the budget then). */
+int dpsnn_server_take(dpsnn_server *s, uint64_t id, dpsnn_net **net);
+int dpsnn_server_get_stats(dpsnn_server *s, dpsnn_server_stats *st);
+/* Stops the loop; jobs not finished are dropped. */
+void dpsnn_server_destroy(dpsnn_server *s);
+
 /* ---- host-only access (no device needed) ------------------------------
  * The connectome generator and delivery layout as the device receives
  * them, and the replayed external drive, for inspection and CPU checks. */
@@ -215,6 +284,9 @@ int dpsnn_host_write_synapse_dump(const dpsnn_host_net *h, const char *path);
 int dpsnn_host_external_step(dpsnn_host_net *h, uint32_t *off, double *time,
                              double *value, uint64_t cap, uint64_t *count);
 void dpsnn_host_destroy(dpsnn_host_net *h);
+
+/* Free and total memory of a device. */
+int dpsnn_device_memory(int device, uint64_t *free_bytes, uint64_t *total_bytes);
 
 /* Device exp of x[0..n) into y (diagnostic: the device's glibc-exact exp
  * checked against the host libm). */
