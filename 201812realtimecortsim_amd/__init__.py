@@ -89,7 +89,7 @@ def _p(t):
 class ServerOptions(ctypes.Structure):
     _fields_ = [("device", ctypes.c_int), ("hbm_budget_bytes", ctypes.c_uint64),
                 ("build_threads", ctypes.c_uint32), ("steps_per_round", ctypes.c_uint32),
-                ("keep_finished", ctypes.c_int)]
+                ("keep_finished", ctypes.c_int), ("max_running", ctypes.c_uint32)]
 
 
 class JobInfo(ctypes.Structure):
@@ -485,9 +485,10 @@ class Server:
     host threads build it, the scheduler admits it into the running batch
     when its device footprint fits the HBM budget."""
 
-    def __init__(self, device=0, hbm_budget_bytes=0, build_threads=0, steps_per_round=0, keep_finished=False):
+    def __init__(self, device=0, hbm_budget_bytes=0, build_threads=0, steps_per_round=0, keep_finished=False,
+                 max_running=0):
         self._lib = library()
-        o = ServerOptions(device, hbm_budget_bytes, build_threads, steps_per_round, int(keep_finished))
+        o = ServerOptions(device, hbm_budget_bytes, build_threads, steps_per_round, int(keep_finished), max_running)
         h = ctypes.c_void_p()
         _check("dpsnn_server_create", self._lib.dpsnn_server_create(ctypes.byref(o), ctypes.byref(h)))
         self._h = h

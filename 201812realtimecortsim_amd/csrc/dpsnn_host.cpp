@@ -341,6 +341,15 @@ bool parse_env_overrides(const char *text, EnvMap &out, std::string &err) {
   return true;
 }
 
+unsigned default_threads() {
+  unsigned n = std::max(1u, std::thread::hardware_concurrency());
+  if (const char *e = std::getenv("OMP_NUM_THREADS")) {  // the process's CPU share, where set
+    const int k = std::atoi(e);
+    if (k > 0) n = std::min(n, (unsigned)k);
+  }
+  return n;
+}
+
 bool read_params(const std::string &dir, Params &p, std::string &err, const EnvMap *over) {
   dpsnn_parameters &q = p.pub;
   q = dpsnn_parameters{};
@@ -619,7 +628,7 @@ bool generate_connectome(const Params &p, Ran3 &rng, Connectome &c, std::string 
     err = "reseeding period (tabNeuron) differs from the column size";
     return false;
   }
-  if (threads == 0) threads = std::max(1u, std::thread::hardware_concurrency());
+  if (threads == 0) threads = default_threads();
   // column 0 continues the stream seeded with env_globalSeed: SetRandomSeed(0)
   // only advances it (DPSNN_LIFCAcorticalModuleGen.cpp:96-107, randdev.c:477-489)
   std::vector<uint64_t> gen_off(N + 1, 0);
@@ -771,7 +780,7 @@ bool generate_connectome(const Params &p, Ran3 &rng, Connectome &c, std::string 
 void build_delivery(const Params &p, Connectome &c, bool with_incoming, unsigned threads) {
   const uint32_t N = c.N, D = std::min<uint32_t>(p.pub.D, kDelaySlots);
   const uint64_t S = c.S;
-  if (threads == 0) threads = std::max(1u, std::thread::hardware_concurrency());
+  if (threads == 0) threads = default_threads();
   threads = std::max(1u, std::min<unsigned>(threads, (N + 1023) / 1024));
   // bucket boundaries of the backward list
   std::vector<uint64_t> bstart(kDelaySlots + 1, 0);

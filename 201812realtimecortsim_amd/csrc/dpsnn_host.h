@@ -98,7 +98,11 @@ struct Connectome {
   std::vector<uint32_t> inc_syn;
 };
 
-// threads = 0: all hardware threads.
+// Host threads used when a caller asks for 0: the hardware threads, capped
+// by OMP_NUM_THREADS where it is set.
+unsigned default_threads();
+
+// threads = 0: default_threads().
 bool generate_connectome(const Params &p, Ran3 &rng, Connectome &c, std::string &err,
                          unsigned threads = 0);
 // with_incoming: also build the per-target incoming synapse lists (STDP).

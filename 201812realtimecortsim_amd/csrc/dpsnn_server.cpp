@@ -26,6 +26,7 @@
 #include <vector>
 
 #include "dpsnn_hip.h"
+#include "dpsnn_host.h"
 
 namespace {
 
@@ -126,7 +127,8 @@ void scheduler_loop(dpsnn_server *s) {
       // first fit in arrival order
       for (auto it = s->waiting.begin(); it != s->waiting.end();) {
         Job *j = *it;
-        if (s->used + j->info.device_bytes <= s->budget) {
+        const bool room = !s->o.max_running || s->running.size() + admit.size() < s->o.max_running;
+        if (room && s->used + j->info.device_bytes <= s->budget) {
           s->used += j->info.device_bytes;
           admit.push_back(j);
           it = s->waiting.erase(it);
@@ -207,8 +209,7 @@ int dpsnn_server_create(const dpsnn_server_options *opt, dpsnn_server **out) {
   s->budget = budget - kBatchReserve;
   if ((rc = dpsnn_batch_create(s->o.device, &s->batch))) return rc;
   const uint32_t nb = s->o.build_threads ? s->o.build_threads : 2;
-  const uint32_t hw = std::max(1u, std::thread::hardware_concurrency());
-  s->hostThreads = std::max(1u, hw / nb);
+  s->hostThreads = std::max(1u, dpsnn::default_threads() / nb);
   dpsnn_server *p = s.get();
   for (uint32_t i = 0; i < nb; ++i) s->builders.emplace_back(builder_loop, p);
   s->sched = std::thread(scheduler_loop, p);

@@ -2,6 +2,7 @@
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--config c1]
     python bench.py --mode batch [--steps K] [--warmup W]   (configs[3])
+    python bench.py --mode serve [--serve-window S]          (configs[4])
 
 One "step" = one simulated millisecond of the whole local network on one GPU
 (the reference's completeTimeStep_ms).  Default workload: configs[1], a
@@ -25,6 +26,8 @@ import json
 import os
 import sys
 import time
+
+import numpy as np
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
@@ -154,14 +157,97 @@ def bench_batch(pkg, args):
     }
 
 
+def run_server(pkg, jobs, dirs, envs, args, max_running):
+    """Replays the job arrivals against a server for the window; returns
+    what it finished and delivered by the window's end."""
+    srv = pkg.Server(hbm_budget_bytes=int(args.serve_budget_gb * 1e9), build_threads=args.build_threads,
+                     steps_per_round=10, max_running=max_running)
+    t0 = time.perf_counter()
+    ids, k, last = [], 0, t0
+    while True:
+        now = time.perf_counter() - t0
+        while k < len(jobs) and jobs[k]["arrival_s"] <= now:
+            ids.append(srv.submit(dirs[k], drive=pkg.DRIVE_PHILOX, philox_seed=9000 + k, env=envs[k],
+                                  drop_host_synapses=True))
+            k += 1
+        if now >= args.serve_window:
+            break
+        if time.perf_counter() - last >= 10.0:
+            st = srv.stats()
+            log(f"{'plain' if max_running == 1 else 'served'} t={now:5.0f} s: submitted {st['submitted']} "
+                f"building {st['building']} waiting {st['waiting']} running {st['running']} "
+                f"finished {st['finished']} events {st['synaptic_events']:.3e}")
+            last = time.perf_counter()
+        time.sleep(0.005)
+    st = srv.stats()
+    infos = [srv.job(j) for j in ids]
+    srv.close()
+    done = [i for i in infos if i["state"] == "done"]
+    lat = np.array([i["finished_s"] - i["submitted_s"] for i in done]) if done else np.zeros(0)
+    pct = (lambda q: float(np.percentile(lat, q)) if lat.size else None)
+    return {"submitted": len(ids), "finished": len(done), "failed": sum(i["state"] == "failed" for i in infos),
+            "running_at_end": st["running"], "waiting_at_end": st["waiting"], "building_at_end": st["building"],
+            "events": st["synaptic_events"], "elapsed_s": st["elapsed_s"],
+            "events_per_s": st["synaptic_events"] / st["elapsed_s"],
+            "latency_p50_s": pct(50), "latency_p99_s": pct(99),
+            "finished_sim_ms": sum(i["stats"]["steps"] for i in done)}
+
+
+def bench_serve(pkg, args):
+    """Poisson job arrivals (20 jobs/s for 120 s), each job a
+    LIFCA network of 1-16 x 1-16 columns, 0.5-5 s simulated, STDP with
+    probability 0.3, admitted under a 200 GB HBM budget with continuous
+    batching; the same arrival stream served one job at a time is the plain
+    comparison."""
+    import tempfile
+    W = importlib.import_module("201812realtimecortsim_amd.workloads")
+    jobs = W.serve_jobs(rate_per_s=args.serve_rate, window_s=args.serve_window)
+    tmp = tempfile.mkdtemp(prefix="dpsnn_serve_")
+    dirs, envs = [], []
+    for i, spec in enumerate(jobs):
+        d = os.path.join(tmp, f"j{i:05d}")
+        envs.append(W.write_instance(d, spec))
+        dirs.append(d)
+    # offered work at the saturated rate of these networks (~400 Hz, 1125
+    # synapses per neuron)
+    offered = sum(j["side"] ** 2 * 1250 * 0.40 * 1125 * j["duration_ms"] for j in jobs) / args.serve_window
+    log(f"{len(jobs)} jobs over {args.serve_window:.0f} s; offered ~{offered:.3e} synaptic events/s")
+    served = run_server(pkg, jobs, dirs, envs, args, 0)
+    log(f"served: {served}")
+    plain = run_server(pkg, jobs, dirs, envs, args, 1)
+    log(f"plain: {plain}")
+    return {
+        "metric": "synaptic events/s + real-time factor on 1 GPU; batched/served vs plain",
+        "value": served["events_per_s"],
+        "unit": "synaptic events/s",
+        "n_gpus": 1, "steps": None, "warmup": 0,
+        "ms_per_step": None,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "data": "synthetic (reference generator, seeded)",
+        "config": {"workload": f"configs[4] serving: Poisson arrivals {args.serve_rate:g} jobs/s for "
+                               f"{args.serve_window:g} s, grid side 1-16 x 1250 neurons, 0.5-5 s simulated, "
+                               f"STDP p=0.3, {args.serve_budget_gb:g} GB HBM admission budget, continuous batching",
+                   "jobs_arrived": len(jobs), "offered_events_per_s": offered, "drive": "philox"},
+        "served": served,
+        "plain": plain,
+        "served_vs_plain": served["events_per_s"] / plain["events_per_s"] if plain["events_per_s"] else None,
+        "roofline": None,
+        "cpu_baseline": None,
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--config", default="c1")
-    ap.add_argument("--mode", default="plain", choices=["plain", "batch"])
+    ap.add_argument("--mode", default="plain", choices=["plain", "batch", "serve"])
     ap.add_argument("--batch-count", type=int, default=64)
+    ap.add_argument("--serve-rate", type=float, default=20.0)
+    ap.add_argument("--serve-window", type=float, default=120.0)
+    ap.add_argument("--serve-budget-gb", type=float, default=200.0)
+    ap.add_argument("--build-threads", type=int, default=4)
     ap.add_argument("--no-tune", action="store_true")
     ap.add_argument("--drive", default="philox", choices=["philox", "replay"])
     ap.add_argument("--cpu-sample-ms", type=int, default=40)
@@ -175,6 +261,9 @@ def main():
     pkg = importlib.import_module("201812realtimecortsim_amd")
     if args.mode == "batch":
         print(json.dumps(bench_batch(pkg, args)), flush=True)
+        return 0
+    if args.mode == "serve":
+        print(json.dumps(bench_serve(pkg, args)), flush=True)
         return 0
     cfg = pkg.config_dir(args.config)
     total = args.warmup + args.steps

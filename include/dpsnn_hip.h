@@ -70,7 +70,8 @@ typedef struct {
                                 dumps and synPeriodicProbe files are then
                                 unavailable */
   uint32_t host_threads;     /* host threads of the connectome generator and
-                                delivery layout (0 = all hardware threads) */
+                                delivery layout (0 = the hardware threads,
+                                capped by OMP_NUM_THREADS where set) */
 } dpsnn_options;
 
 /* Parameters as getParameters returns them (the subset this path reads;
@@ -218,6 +219,8 @@ typedef struct {
   uint32_t steps_per_round;   /* batched steps between admissions (0 = 10) */
   int keep_finished;          /* 1 = a finished job keeps its network (and
                                  its memory) until dpsnn_server_take */
+  uint32_t max_running;       /* jobs on the device at once (0 = as many as
+                                 the budget holds; 1 = one job at a time) */
 } dpsnn_server_options;
 typedef enum {
   DPSNN_JOB_QUEUED = 0,       /* waiting for a host builder */
