@@ -3,19 +3,22 @@
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--config c1]
     python bench.py --mode batch [--steps K] [--warmup W]   (configs[3])
     python bench.py --mode serve [--serve-window S]          (configs[4])
+    python bench.py --mode copy                              (counter calibration)
 
 One "step" = one simulated millisecond of the whole local network on one GPU
 (the reference's completeTimeStep_ms).  Default workload: configs[1], a
-24x24 grid of 1250-neuron columns (720,000 neurons, ~795 M synapses),
+24x24 grid of 1250-neuron columns (720,000 neurons, ~810 M synapses),
 on-device Philox external drive, no raster files; synthetic network from
 the reference generator (no external data).  Prints one JSON line:
   value         synaptic events delivered per wall second over the K timed
                 steps (inputs resident in HBM; host I/O not in the region)
   ms_per_step   wall ms per simulated ms = real-time factor x 1000
-  roofline      the integration kernel (k_integrate) against HBM bandwidth:
-                algorithmic bytes = 25 B per synaptic event + 72 B per neuron
-                (+16 B per replayed external event), divided by its average
-                launch time from HIP events on the network's stream
+  roofline      one step's device work (phase-1 and phase-2 kernels) against
+                HBM bandwidth: algorithmic bytes = 25 B per synaptic event +
+                72 B per neuron (+16 B per replayed external event), divided
+                by its average time from HIP events on the network's stream;
+                traffic = HBM bytes per step from the committed counter
+                summary (profiles/step_<config>_traffic.json)
   cpu_baseline  the oracle (scalar C restatement of DPSNN) on the host, on a
                 bounded sample (configs[0]'s network, a few tens of ms)
 """
@@ -236,13 +239,39 @@ def bench_serve(pkg, args):
     }
 
 
+def bench_copy(args):
+    """A device-to-device copy of known size (256 MiB, five times): the
+    calibration kernel for the FETCH_SIZE / WRITE_SIZE counters."""
+    import torch
+    n = 256 * 1024 * 1024
+    x = torch.empty(n, dtype=torch.uint8, device="cuda")
+    y = torch.empty_like(x)
+    x.fill_(1)
+    torch.cuda.synchronize()
+    for _ in range(5):
+        y.copy_(x)
+    torch.cuda.synchronize()
+    return {"copy_bytes": n, "copies": 5}
+
+
+def traffic_from_profile(config):
+    """HBM bytes per step of the measured workload from the committed counter
+    summary (two rocprofv3 --pmc passes, FETCH_SIZE then WRITE_SIZE, scaled
+    by the same counters read for a plain copy), or None."""
+    path = os.path.join(ROOT, "profiles", f"step_{config}_traffic.json")
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        return json.load(f).get("hbm_bytes_per_step")
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=200)
-    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=100)
     ap.add_argument("--config", default="c1")
-    ap.add_argument("--mode", default="plain", choices=["plain", "batch", "serve"])
+    ap.add_argument("--mode", default="plain", choices=["plain", "batch", "serve", "copy"])
     ap.add_argument("--batch-count", type=int, default=64)
     ap.add_argument("--serve-rate", type=float, default=20.0)
     ap.add_argument("--serve-window", type=float, default=120.0)
@@ -250,7 +279,7 @@ def main():
     ap.add_argument("--build-threads", type=int, default=4)
     ap.add_argument("--no-tune", action="store_true")
     ap.add_argument("--drive", default="philox", choices=["philox", "replay"])
-    ap.add_argument("--cpu-sample-ms", type=int, default=40)
+    ap.add_argument("--cpu-sample-ms", type=int, default=200)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -264,6 +293,9 @@ def main():
         return 0
     if args.mode == "serve":
         print(json.dumps(bench_serve(pkg, args)), flush=True)
+        return 0
+    if args.mode == "copy":
+        print(json.dumps(bench_copy(args)), flush=True)
         return 0
     cfg = pkg.config_dir(args.config)
     total = args.warmup + args.steps
@@ -316,8 +348,9 @@ def main():
                    "mean_rate_hz": spikes / args.steps * 1000.0 / neurons,
                    "init_seconds": init_s},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-                     "kernel": "k_integrate", "kernel_ms": kms, "launches": launches,
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic_from_profile(args.config),
+                     "kernel": "one step's device work (k_gather + k_integrate, or k_step)",
+                     "kernel_ms": kms, "launches": launches,
                      "algorithmic_bytes_per_launch": per_launch},
         "cpu_baseline": None,
     }
