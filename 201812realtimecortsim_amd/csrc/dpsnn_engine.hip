@@ -89,6 +89,38 @@ __device__ inline XcdWalk xcd_walk(uint32_t total) {
   return XcdWalk{begin + lb, end, per};
 }
 
+// XCD-ordered work queue over items [0, total): XCD x (= blockIdx.x % 8, as
+// the dispatcher deals workgroups) owns the x-th contiguous eighth, and its
+// waves take the items in order from the XCD's counter, so the items in
+// flight on one XCD stay a narrow window (neighbouring neurons, shared
+// sources, one L2).  An XCD whose eighth is exhausted helps the next ones.
+// A wave takes kGrab consecutive items per counter update; next_item
+// returns the first of them and their end (*last), or `total` when none is
+// left.  Whole wave; ctr holds kXcds counters that are zero at the launch.
+// Each counter sits on its own 128-byte line (ctr[x * kCtrStride]); an
+// exhausted counter is recognised by a plain read before any atomic.
+constexpr uint32_t kGrab = 4;
+constexpr uint32_t kCtrStride = 32;
+__device__ inline uint32_t next_item(uint32_t *ctr, uint32_t total, uint32_t &x, uint32_t home, uint32_t *last) {
+  for (;;) {
+    const uint32_t begin = (uint32_t)((uint64_t)total * x / kXcds);
+    const uint32_t end = (uint32_t)((uint64_t)total * (x + 1) / kXcds);
+    uint32_t *c = ctr + x * kCtrStride;
+    uint32_t k = end - begin;
+    if ((threadIdx.x & 63) == 0) {
+      k = __hip_atomic_load(c, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      if (begin + k < end) k = atomicAdd(c, kGrab);
+    }
+    k = (uint32_t)__shfl((int)k, 0);
+    if (begin + k < end) {
+      *last = min(end, begin + k + kGrab);
+      return begin + k;
+    }
+    x = (x + 1) % kXcds;
+    if (x == home) return total;
+  }
+}
+
 // LDS written by some lanes of a wave and read by others: order the wave's
 // accesses around this point.
 __device__ inline void wave_sync() {
@@ -120,6 +152,7 @@ struct DevNet {
   uint64_t *keys;          // sorted event ticks: [grid or N/64][cap/8][64][8]
   int16_t *keyW;           // their weights, same layout
   uint32_t *keyCount;      // [grid][64] or [N/64][64]
+  uint32_t *workCtr;       // [kXcds][kCtrStride] phase-1 work counters (zero between steps)
   unsigned long long *counters;  // [0] events, [1] external, [2] spikes
   // STDP (null when plasticity never runs).  Synapse state is kept in
   // target-major order: synapse k of neuron j's incoming list lives at
@@ -448,22 +481,23 @@ struct ExtPhiloxCursor {
 };
 
 constexpr int kChunk = 8;  // internal events per lane per LDS-staged chunk
+constexpr int kRun = 16;   // events per neuron run in the blocked scratch
 // LDS of one wave in phase 2: two chunks of 64 lanes' ticks
 constexpr size_t kStageBytes = 2 * kChunk * kWave * sizeof(uint64_t);
 
 // Chunk c (events 8c..8c+7 of every lane) from the tile's blocked scratch
-// [i / 8][q][i % 8]: the chunk's ticks are one contiguous 4 KB block, loaded
-// by the wave in 8 coalesced instructions (instruction r: neurons 8r..8r+7)
-// and transposed through LDS; each lane's 8 weights are one 16-byte load.
-// Entries past a neuron's count are never consumed.
-static_assert(kChunk == 8, "the scratch is blocked by 8 events");
+// [i / kRun][q][i % kRun]: each neuron's 8 ticks are one 64-byte piece of
+// its run, loaded by the wave in 8 coalesced instructions (instruction r:
+// neurons 8r..8r+7) and transposed through LDS; each lane's 8 weights are
+// one 16-byte load.  Entries past a neuron's count are never consumed.
+static_assert(kChunk == 8 && kRun % kChunk == 0, "chunks are whole pieces of a run");
 __device__ inline void load_chunk(const uint64_t *tileK, const int16_t *tileW, uint32_t c, uint64_t (&rk)[kChunk],
                                   uint4 &rw) {
   const uint32_t lane = threadIdx.x & 63;
-  const uint64_t *blk = tileK + (size_t)c * (kWave * 8);
+  const size_t blk = (size_t)(c * kChunk / kRun) * (kWave * kRun) + (c * kChunk) % kRun;
 #pragma unroll
-  for (int r = 0; r < kChunk; ++r) rk[r] = blk[r * kWave + lane];
-  rw = *(const uint4 *)(tileW + (size_t)c * (kWave * 8) + lane * 8);
+  for (int r = 0; r < kChunk; ++r) rk[r] = tileK[blk + (size_t)(r * 8 + (lane >> 3)) * kRun + (lane & 7)];
+  rw = *(const uint4 *)(tileW + blk + (size_t)lane * kRun);
 }
 __device__ inline void store_chunk(uint64_t *stageK, const uint64_t (&rk)[kChunk]) {
   const uint32_t lane = threadIdx.x & 63, e = lane & 7;
@@ -662,8 +696,9 @@ __host__ __device__ inline size_t phase1_lds(uint32_t cap, int B) {
 // Phase 1 for neuron j, whole wave: live keys of its in-list, bucket-sorted
 // in LDS (lds: sorted[cap], ldsW[cap], bcnt[B + 1]), stored in arrival order
 // as ticks relative to bits(t) (outK) with their weights (outW); returns n.
-// The scratch of a tile of 64 neurons is blocked by 8 events: event i of
-// neuron q at [i / 8][q][i % 8], and outK/outW point at neuron q's [0][q][0].
+// The scratch of a tile of 64 neurons is blocked by runs of kRun events:
+// event i of neuron q at [i / kRun][q][i % kRun] (a neuron's keys fill whole
+// 128-byte lines), and outK/outW point at neuron q's [0][q][0].
 template <int K, int B>
 __device__ inline uint32_t gather_sort_neuron(const DevNet &P, uint32_t t, uint32_t j, int plastic, uint8_t *lds,
                                               uint32_t cap, uint64_t *outK, int16_t *outW) {
@@ -733,7 +768,7 @@ __device__ inline uint32_t gather_sort_neuron(const DevNet &P, uint32_t t, uint3
     }
     for (; y < hi; ++y) r += sorted[y] < x ? 1u : 0u;
     const uint32_t p = lo + r;
-    const size_t at = (size_t)(p >> 3) * (kWave * 8) + (p & 7);
+    const size_t at = (size_t)(p / kRun) * (kWave * kRun) + (p % kRun);
     outK[at] = x >> 12;
     outW[at] = ldsW[x & 0xFFFu];
   }
@@ -771,7 +806,7 @@ __device__ inline unsigned long long step_tile(const DevNet &P, uint32_t t, cons
   const uint32_t m = min((uint32_t)kWave, P.N - j0);
   unsigned long long evSum = 0;
   for (uint32_t q = 0; q < m; ++q) {
-    const uint32_t n = gather_sort_neuron<K, B>(P, t, j0 + q, ltd, lds, cap, wkeys + q * 8, wW + q * 8);
+    const uint32_t n = gather_sort_neuron<K, B>(P, t, j0 + q, ltd, lds, cap, wkeys + q * kRun, wW + q * kRun);
     if (lane == 0) wcnt[q] = n;
     evSum += n;
   }
@@ -860,13 +895,15 @@ template <int K, int B>
 __global__ __launch_bounds__(kWave) void k_gather(DevNet P, uint32_t t, int ltd) {
   extern __shared__ uint64_t lds[];
   unsigned long long evSum = 0;
-  const XcdWalk w = xcd_walk(P.N);
-  for (uint32_t j = w.first; j < w.end; j += w.stride) {
-    const size_t at = (size_t)(j >> 6) * P.cap * kWave + (j & 63) * 8;  // the tile's blocked scratch
-    const uint32_t n = gather_sort_neuron<K, B>(P, t, j, ltd, (uint8_t *)lds, P.cap, P.keys + at, P.keyW + at);
-    if (threadIdx.x == 0) P.keyCount[j] = n;
-    evSum += n;
-  }
+  const uint32_t home = blockIdx.x % kXcds;
+  uint32_t x = home;
+  for (uint32_t j0, j1; (j0 = next_item(P.workCtr, P.N, x, home, &j1)) < P.N;)
+    for (uint32_t j = j0; j < j1; ++j) {
+      const size_t at = (size_t)(j >> 6) * P.cap * kWave + (j & 63) * kRun;  // the tile's blocked scratch
+      const uint32_t n = gather_sort_neuron<K, B>(P, t, j, ltd, (uint8_t *)lds, P.cap, P.keys + at, P.keyW + at);
+      if (threadIdx.x == 0) P.keyCount[j] = n;
+      evSum += n;
+    }
   if (threadIdx.x == 0) atomicAdd(&P.counters[0], evSum);
 }
 
@@ -875,6 +912,7 @@ __global__ __launch_bounds__(kWave) void k_gather(DevNet P, uint32_t t, int ltd)
 __global__ __launch_bounds__(kBlock) void k_integrate(DevNet P, uint32_t t, ExtView X, NextScan S) {
   __shared__ uint8_t stage[kBlock / kWave][kStageBytes];
   const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  if (blockIdx.x == 0 && threadIdx.x < kXcds) P.workCtr[threadIdx.x * kCtrStride] = 0;  // for the next k_gather
   const uint32_t j0 = blockIdx.x * kBlock + wave * kWave;
   const uint32_t m = j0 < P.N ? min((uint32_t)kWave, P.N - j0) : 0u;
   const uint32_t n = lane < m ? P.keyCount[j0 + lane] : 0u;
@@ -910,16 +948,18 @@ __device__ inline uint32_t batch_item_of(const BatchItem *items, uint32_t count,
 template <int K, int B>
 __global__ __launch_bounds__(kWave) void k_batch_gather(const DevNet *nets, const BatchItem *items, uint32_t count,
                                                         uint32_t g0, uint32_t g1, uint32_t cap, uint64_t *keys,
-                                                        int16_t *keyW, uint32_t *keyCount) {
+                                                        int16_t *keyW, uint32_t *keyCount, uint32_t *workCtr) {
   extern __shared__ uint64_t lds[];
-  const XcdWalk w = xcd_walk((g1 - g0) * kWave);
-  for (uint32_t u = w.first; u < w.end; u += w.stride) {
+  const uint32_t total = (g1 - g0) * kWave, home = blockIdx.x % kXcds;
+  uint32_t x = home;
+  for (uint32_t u0, u1; (u0 = next_item(workCtr, total, x, home, &u1)) < total;)
+  for (uint32_t u = u0; u < u1; ++u) {
     const uint32_t g = g0 + u / kWave, q = u % kWave;
     const BatchItem &it = items[batch_item_of(items, count, g)];
     const DevNet &P = nets[it.net];
     const uint32_t j = (g - it.tileBegin) * kWave + q;
     if (j >= P.N) continue;
-    const size_t at = (size_t)(u / kWave) * cap * kWave + q * 8;
+    const size_t at = (size_t)(u / kWave) * cap * kWave + q * kRun;
     const uint32_t n =
         gather_sort_neuron<K, B>(P, it.t, j, it.ltd, (uint8_t *)lds, cap, keys + at, keyW + at);
     if (threadIdx.x == 0) {
@@ -932,9 +972,10 @@ __global__ __launch_bounds__(kWave) void k_batch_gather(const DevNet *nets, cons
 __global__ __launch_bounds__(kBlock) void k_batch_integrate(const DevNet *nets, const BatchItem *items,
                                                             uint32_t count, uint32_t g0, uint32_t g1, uint32_t cap,
                                                             const uint64_t *keys, const int16_t *keyW,
-                                                            const uint32_t *keyCount) {
+                                                            const uint32_t *keyCount, uint32_t *workCtr) {
   __shared__ uint8_t stage[kBlock / kWave][kStageBytes];
   const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  if (blockIdx.x == 0 && threadIdx.x < kXcds) workCtr[threadIdx.x * kCtrStride] = 0;  // for the next chunk's gather
   const uint32_t g = g0 + blockIdx.x * (kBlock / kWave) + wave;
   if (g >= g1) return;  // whole wave
   const BatchItem &it = items[batch_item_of(items, count, g)];
@@ -971,7 +1012,7 @@ using GatherFn = void (*)(DevNet, uint32_t, int);
 using BatchFn = void (*)(const DevNet *, const BatchItem *, uint32_t, uint32_t, uint32_t, uint64_t *, int16_t *,
                          uint32_t *);
 using BatchGatherFn = void (*)(const DevNet *, const BatchItem *, uint32_t, uint32_t, uint32_t, uint32_t, uint64_t *,
-                               int16_t *, uint32_t *);
+                               int16_t *, uint32_t *, uint32_t *);
 struct KernelShape {
   int K, B;
   StepFn step;
@@ -989,6 +1030,15 @@ static const KernelShape kShapes[] = {
 constexpr int kNumShapes = (int)(sizeof(kShapes) / sizeof(kShapes[0]));
 
 static uint32_t round_up(uint32_t x, uint32_t m) { return (x + m - 1) / m * m; }
+
+// Phase-1 waves (one-wave workgroups) resident per CU for a kernel and its
+// LDS, from the runtime's occupancy calculator.
+static uint32_t resident_waves(const void *kernel, size_t lds) {
+  int blocks = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, kernel, kWave, lds) != hipSuccess || blocks < 1)
+    blocks = 1;
+  return (uint32_t)blocks;
+}
 
 static int find_shape(int K, int B) {
   for (int i = 0; i < kNumShapes; ++i)
@@ -1121,8 +1171,8 @@ static int build_device(dpsnn_net *n) {
   P.nsub = q.subPopNumber;
   P.npc = q.neuronsPerCM;
   P.ncols = q.globCFT;
-  // longest in-list, rounded to 8 (scratch rows of 16-byte weight chunks)
-  P.cap = (std::max<uint32_t>(c.max_in, 1) + 7u) & ~7u;
+  // longest in-list, rounded to a whole number of scratch runs
+  P.cap = (std::max<uint32_t>(c.max_in, 1) + (kRun - 1)) / kRun * kRun;
   P.drive = (uint32_t)n->opt.drive;
   P.w2f = q.factorWeightType_2_Float;
   P.philoxSeed = n->opt.philox_seed;
@@ -1247,9 +1297,14 @@ static int build_device(dpsnn_net *n) {
   n->stdpBlocks = std::min<uint32_t>((P.NW + 3) / 4, (uint32_t)cus * 8);
   n->split = true;
   if (const char *e = std::getenv("DPSNN_SPLIT")) n->split = std::atoi(e) != 0;
-  n->gatherGrid = round_up(std::min<uint32_t>(N, (uint32_t)cus * wavesPerCu * 4), kXcds);
+  n->gatherGrid = round_up(
+      std::min<uint32_t>(N, (uint32_t)cus * resident_waves((const void *)kShapes[n->shape].gather,
+                                                               phase1_lds(P.cap, kShapes[n->shape].B))),
+      kXcds);
   // the key scratch of stand-alone steps is allocated at the first
   // dpsnn_step (a network stepped only by a batch uses the batch's)
+  if ((rc = dev_alloc(n, &P.workCtr, kXcds * kCtrStride))) return rc;
+  HIP_TRY(hipMemsetAsync(P.workCtr, 0, sizeof(uint32_t) * kXcds * kCtrStride, n->stream));
   if ((rc = dev_alloc(n, &P.counters, 4))) return rc;
   HIP_TRY(hipMemsetAsync(P.counters, 0, 4 * sizeof(unsigned long long), n->stream));
   // raster / rates drain buffers
@@ -1661,7 +1716,7 @@ uint64_t dpsnn_device_bytes_needed(const dpsnn_net *n) {
     b += 2 * r(4 * kStdpSpan) + 2 * r(4 * q.subPopNumber * q.subPopNumber);
   }
   const uint64_t drainEvery = n->opt.drain_every_ms ? n->opt.drain_every_ms : 100;
-  b += r(8 * 4) + r(4 * (uint64_t)q.globCFT * q.subPopNumber * drainEvery);
+  b += r(8 * 4) + r(4 * kXcds * kCtrStride) + r(4 * (uint64_t)q.globCFT * q.subPopNumber * drainEvery);
   const bool raster = (n->writing && !q.spikes_writeDisable) || n->opt.keep_spikes;
   if (raster) {
     const uint64_t cap = N * std::min<uint64_t>(drainEvery, 1000) + 1;
@@ -1872,6 +1927,7 @@ struct dpsnn_batch {
   uint64_t *keys = nullptr;
   int16_t *keyW = nullptr;
   uint32_t *keyCount = nullptr;
+  uint32_t *workCtr = nullptr;  // [kXcds], zero between launches
   size_t scratchSlots = 0;
   uint32_t scratchCap = 0;
   uint32_t totalTiles = 0;   // of all members
@@ -1999,13 +2055,15 @@ int batch_launch(dpsnn_batch *b, int r, uint32_t count, uint32_t tiles) {
     const size_t lds = phase1_lds(cap, K.B);
     for (uint32_t g0 = 0; g0 < tiles; g0 += b->chunkTiles) {
       const uint32_t g1 = std::min(tiles, g0 + b->chunkTiles);
-      const uint32_t grid =
-          round_up(std::min<uint32_t>((g1 - g0) * kWave, (uint32_t)b->cus * batch_waves(b, lds) * 4), kXcds);
+      const uint32_t grid = round_up(
+          std::min<uint32_t>((g1 - g0) * kWave,
+                             (uint32_t)b->cus * std::min(batch_waves(b, lds), resident_waves((const void *)K.batchGather, lds))),
+          kXcds);
       hipLaunchKernelGGL(K.batchGather, dim3(grid), dim3(kWave), lds, b->stream, b->d_nets, b->d_items[r], count, g0,
-                         g1, cap, b->keys, b->keyW, b->keyCount);
+                         g1, cap, b->keys, b->keyW, b->keyCount, b->workCtr);
       const uint32_t blocks = (g1 - g0 + kBlock / kWave - 1) / (kBlock / kWave);
       hipLaunchKernelGGL(k_batch_integrate, dim3(blocks), dim3(kBlock), 0, b->stream, b->d_nets, b->d_items[r], count,
-                         g0, g1, cap, b->keys, b->keyW, b->keyCount);
+                         g0, g1, cap, b->keys, b->keyW, b->keyCount, b->workCtr);
     }
   }
   HIP_TRY(hipGetLastError());
@@ -2128,6 +2186,8 @@ int dpsnn_batch_create(int device, dpsnn_batch **out) {
   b->cus = std::max(b->cus, 1);
   HIP_TRY(hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking));
   for (int r = 0; r < dpsnn_batch::kRing; ++r) HIP_TRY(hipEventCreateWithFlags(&b->done[r], hipEventDisableTiming));
+  HIP_TRY(hipMalloc((void **)&b->workCtr, sizeof(uint32_t) * kXcds * kCtrStride));
+  HIP_TRY(hipMemset(b->workCtr, 0, sizeof(uint32_t) * kXcds * kCtrStride));
   if (const char *e = std::getenv("DPSNN_GRAPH")) b->useGraph = std::atoi(e) != 0;
   if (const char *e = std::getenv("DPSNN_SPLIT")) b->split = std::atoi(e) != 0;
   *out = b.release();
@@ -2287,7 +2347,8 @@ void dpsnn_batch_destroy(dpsnn_batch *b) {
   if (b->d_nets) (void)hipFree(b->d_nets);
   if (b->keys) hipFree(b->keys);
   if (b->keyW) hipFree(b->keyW);
-  if (b->keyCount) hipFree(b->keyCount);
+  if (b->keyCount) (void)hipFree(b->keyCount);
+  if (b->workCtr) (void)hipFree(b->workCtr);
   for (int r = 0; r < dpsnn_batch::kRing; ++r)
     if (b->done[r]) hipEventDestroy(b->done[r]);
   for (auto &e : b->ev) {
