@@ -281,14 +281,29 @@ __device__ inline uint64_t publish_spikes(const DevNet &P, uint32_t t1, uint32_t
 // --------------------------------------------------------------------------
 __device__ inline void ltp_neuron(const DevNet &P, uint32_t t, uint32_t j) {
   const uint32_t lane = threadIdx.x & 63;
-  for (uint64_t s = P.incOff[j] + lane; s < P.incOff[j + 1]; s += 64) {
-    const int32_t last = P.lastT[s];
-    float f = 0.0f;
-    if (last != 0) {
-      const int dt = (int)t - 1 - last;
-      if (dt < kStdpSpan) f = P.causal[dt];
+  const uint64_t end = P.incOff[j + 1];
+  constexpr int U = 4;  // independent synapses per lane per round: loads first, then stores
+  for (uint64_t s0 = P.incOff[j] + lane; s0 < end; s0 += U * kWave) {
+    int32_t last[U];
+    float td[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const uint64_t s = s0 + (uint64_t)u * kWave;
+      last[u] = s < end ? P.lastT[s] : 0;
+      td[u] = s < end ? P.tdT[s] : 0.0f;
     }
-    P.tdT[s] = P.tdT[s] + f;
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const uint64_t s = s0 + (uint64_t)u * kWave;
+      if (s < end) {
+        float f = 0.0f;
+        if (last[u] != 0) {
+          const int dt = (int)t - 1 - last[u];
+          if (dt < kStdpSpan) f = P.causal[dt];
+        }
+        P.tdT[s] = td[u] + f;
+      }
+    }
   }
 }
 
@@ -301,13 +316,16 @@ __device__ inline void ltp_word(const DevNet &P, uint32_t t, uint64_t mask, uint
   }
 }
 
-// Stand-alone LTP of the scan at step t (after a commit, which must come
-// first).
+// Stand-alone LTP of the scan at step t: one wave per neuron that fired
+// (so the sweeps of different neurons overlap), the bitmap of slot t telling
+// which.  Runs after the step's phase 2 (and after a commit at ms % 1000 ==
+// 999, which must come first).
 __global__ __launch_bounds__(kWaveLTP) void k_ltp(DevNet P, uint32_t t) {
   const uint32_t wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
   const uint32_t waves = (gridDim.x * blockDim.x) >> 6;
-  const uint32_t slot = t & (P.RS - 1);
-  for (uint32_t w = wave; w < P.NW; w += waves) ltp_word(P, t, P.bits[(size_t)slot * P.NW + w], w * 64);
+  const uint64_t *bits = P.bits + (size_t)(t & (P.RS - 1)) * P.NW;
+  for (uint32_t j = wave; j < P.N; j += waves)
+    if ((bits[j >> 6] >> (j & 63)) & 1ull) ltp_neuron(P, t, j);
 }
 
 // longTermPlasticity (DPSNN_localNet_sim.c:312-414), once per simulated
@@ -665,17 +683,36 @@ __device__ inline uint32_t gather_keys(const DevNet &P, uint32_t t, uint32_t j, 
       if (first) {
         atomicAdd(&bcnt[key_bucket<B>(tick, shift)], 1u);
         ldsW[idx] = w[k];
-        if (plastic) {
-          // LTD, synapseSetLastActiveTime_antiCausalSTDP
-          // (DPSNN_neuron_sim.c:207-277): last = t, td += anti[t - lastSpike]
-          const uint64_t sy = P.incOff[j] + P.inSyn[beg + idx];
-          P.lastT[sy] = (int32_t)t;
-          const int dt = (int)t - P.lastSpike[j];
-          P.tdT[sy] = P.tdT[sy] + (dt < kStdpSpan ? P.anti[dt] : 0.0f);
-        }
       }
     }
     n += (uint32_t)__popcll(__ballot(on));
+  }
+  if (first && plastic) {
+    // LTD, synapseSetLastActiveTime_antiCausalSTDP (DPSNN_neuron_sim.c:
+    // 207-277): every synapse delivered in step t gets last = t and
+    // td += anti[t - lastSpike], one value per target.  A synapse is live at
+    // most once per step (refractory >= 1 ms), so the updates are
+    // independent: batches of 8 issue their index loads, then their value
+    // loads, then their stores.
+    const int dt = (int)t - P.lastSpike[j];
+    const float a = dt < kStdpSpan ? P.anti[dt] : 0.0f;
+    const uint64_t sbase = P.incOff[j];
+#pragma unroll
+    for (int k0 = 0; k0 < K; k0 += 8) {
+      uint32_t sy[8];
+      float td[8];
+#pragma unroll
+      for (int u = 0; u < 8 && k0 + u < K; ++u)
+        sy[u] = key[k0 + u] != kNoKey ? P.inSyn[beg + base + (k0 + u) * kWave + lane] : 0u;
+#pragma unroll
+      for (int u = 0; u < 8 && k0 + u < K; ++u) td[u] = key[k0 + u] != kNoKey ? P.tdT[sbase + sy[u]] : 0.0f;
+#pragma unroll
+      for (int u = 0; u < 8 && k0 + u < K; ++u)
+        if (key[k0 + u] != kNoKey) {
+          P.lastT[sbase + sy[u]] = (int32_t)t;
+          P.tdT[sbase + sy[u]] = td[u] + a;
+        }
+    }
   }
   return n;
 }
@@ -924,8 +961,7 @@ __global__ __launch_bounds__(kBlock) void k_integrate(DevNet P, uint32_t t, ExtV
     ex = integrate_wave<true>(P, t, X, j0, m, P.keys + at, P.keyW + at, n, stage[wave], f, te);
   else
     ex = integrate_wave<false>(P, t, X, j0, m, P.keys + at, P.keyW + at, n, stage[wave], f, te);
-  const uint64_t mask = publish_spikes(P, t + 1, j0 + lane, f, te, S.popCnt, S.count);
-  if (S.ltp) ltp_word(P, t + 1, mask, j0);
+  publish_spikes(P, t + 1, j0 + lane, f, te, S.popCnt, S.count);  // LTP follows in k_ltp
   for (int o = 32; o > 0; o >>= 1) ex += __shfl_down(ex, o);
   if (lane == 0 && ex) atomicAdd(&P.counters[1], ex);
 }
@@ -992,10 +1028,27 @@ __global__ __launch_bounds__(kBlock) void k_batch_integrate(const DevNet *nets, 
     ex = integrate_wave<true>(P, it.t, it.X, j0, m, keys + at, keyW + at, n, stage[wave], f, te);
   else
     ex = integrate_wave<false>(P, it.t, it.X, j0, m, keys + at, keyW + at, n, stage[wave], f, te);
-  const uint64_t mask = publish_spikes(P, it.t + 1, j0 + lane, f, te, it.S.popCnt, it.S.count);
-  if (it.S.ltp) ltp_word(P, it.t + 1, mask, j0);
+  publish_spikes(P, it.t + 1, j0 + lane, f, te, it.S.popCnt, it.S.count);  // LTP follows in k_batch_ltp
   for (int o = 32; o > 0; o >>= 1) ex += __shfl_down(ex, o);
   if (lane == 0 && ex) atomicAdd(&P.counters[1], ex);
+}
+
+// LTP of the members' next scans over global tiles [g0, g1), one wave per
+// neuron that fired (item S.ltp: the member's next scan runs LTP).
+__global__ __launch_bounds__(kWaveLTP) void k_batch_ltp(const DevNet *nets, const BatchItem *items, uint32_t count,
+                                                        uint32_t g0, uint32_t g1) {
+  const uint32_t wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  const uint32_t waves = (gridDim.x * blockDim.x) >> 6;
+  for (uint32_t u = wave; u < (g1 - g0) * kWave; u += waves) {
+    const uint32_t g = g0 + u / kWave;
+    const BatchItem &it = items[batch_item_of(items, count, g)];
+    if (!it.S.ltp) continue;
+    const DevNet &P = nets[it.net];
+    const uint32_t j = (g - it.tileBegin) * kWave + u % kWave;
+    if (j >= P.N) continue;
+    const uint64_t *bits = P.bits + (size_t)((it.t + 1) & (P.RS - 1)) * P.NW;
+    if ((bits[j >> 6] >> (j & 63)) & 1ull) ltp_neuron(P, it.t + 1, j);
+  }
 }
 
 // Fills n doubles with v.
@@ -1294,7 +1347,7 @@ static int build_device(dpsnn_net *n) {
   n->ldsBytes = lds_bytes(P.cap, B);
   const uint32_t wavesPerCu = std::max<uint32_t>(1, std::min<uint32_t>(16, (uint32_t)((160u << 10) / n->ldsBytes)));
   n->grid = round_up(std::min<uint32_t>(tiles, (uint32_t)cus * wavesPerCu), kXcds);
-  n->stdpBlocks = std::min<uint32_t>((P.NW + 3) / 4, (uint32_t)cus * 8);
+  n->stdpBlocks = std::min<uint32_t>((N + 3) / 4, (uint32_t)cus * 16);
   n->split = true;
   if (const char *e = std::getenv("DPSNN_SPLIT")) n->split = std::atoi(e) != 0;
   n->gatherGrid = round_up(
@@ -1614,6 +1667,7 @@ static int one_step(dpsnn_net *n) {
     hipLaunchKernelGGL(K.gather, dim3(n->gatherGrid), dim3(kWave), n->ldsBytes, n->stream, P, sp.t, sp.ltd);
     const uint32_t blocks = (P.N + kBlock - 1) / kBlock;
     hipLaunchKernelGGL(k_integrate, dim3(blocks), dim3(kBlock), 0, n->stream, P, sp.t, sp.X, sp.S);
+    if (sp.S.ltp) hipLaunchKernelGGL(k_ltp, dim3(n->stdpBlocks), dim3(kWaveLTP), 0, n->stream, P, sp.t + 1);
   } else {
     hipLaunchKernelGGL(K.step, dim3(n->grid), dim3(kWave), n->ldsBytes, n->stream, P, sp.t, sp.X, sp.ltd, sp.S);
   }
@@ -1931,6 +1985,7 @@ struct dpsnn_batch {
   size_t scratchSlots = 0;
   uint32_t scratchCap = 0;
   uint32_t totalTiles = 0;   // of all members
+  bool anyPlastic = false;   // some member has STDP (the split form then runs k_batch_ltp)
   uint32_t chunkTiles = 0;   // split form: tiles per chunk (scratch bound)
   int shape = -1;
   uint32_t wavesPerCu = 0;
@@ -2006,7 +2061,11 @@ int batch_prepare(dpsnn_batch *b) {
   }
   const uint32_t cap = batch_cap(b);
   b->totalTiles = 0;
-  for (auto *n : b->nets) b->totalTiles += n->dev.NW;
+  b->anyPlastic = false;
+  for (auto *n : b->nets) {
+    b->totalTiles += n->dev.NW;
+    b->anyPlastic = b->anyPlastic || n->stdp;
+  }
   const size_t perTile = (size_t)cap * kWave * (sizeof(uint64_t) + sizeof(int16_t));
   b->chunkTiles = (uint32_t)std::max<size_t>(1, std::min<size_t>(b->totalTiles, kBatchScratchBytes / perTile));
   const size_t fusedSlots = (size_t)round_up((uint32_t)b->cus * lds_waves(lds_bytes(cap, 64)), kXcds) * kWave;
@@ -2064,6 +2123,9 @@ int batch_launch(dpsnn_batch *b, int r, uint32_t count, uint32_t tiles) {
       const uint32_t blocks = (g1 - g0 + kBlock / kWave - 1) / (kBlock / kWave);
       hipLaunchKernelGGL(k_batch_integrate, dim3(blocks), dim3(kBlock), 0, b->stream, b->d_nets, b->d_items[r], count,
                          g0, g1, cap, b->keys, b->keyW, b->keyCount, b->workCtr);
+      if (b->anyPlastic)
+        hipLaunchKernelGGL(k_batch_ltp, dim3(std::min<uint32_t>((g1 - g0) * (kWave * kWave / kWaveLTP), b->cus * 16u)),
+                           dim3(kWaveLTP), 0, b->stream, b->d_nets, b->d_items[r], count, g0, g1);
     }
   }
   HIP_TRY(hipGetLastError());
