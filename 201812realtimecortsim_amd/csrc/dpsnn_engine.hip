@@ -151,6 +151,8 @@ struct DevNet {
   const int16_t *inW;
   uint64_t *keys;          // sorted event ticks: [grid or N/64][cap/8][64][8]
   int16_t *keyW;           // their weights, same layout
+  uint16_t *keyPos;        // STDP: their in-list positions, same layout
+  float *keyA;             // STDP: LTD increment of the step, per neuron slot (keyCount's indexing)
   uint32_t *keyCount;      // [grid][64] or [N/64][64]
   uint32_t *workCtr;       // [kXcds][kCtrStride] phase-1 work counters (zero between steps)
   unsigned long long *counters;  // [0] events, [1] external, [2] spikes
@@ -687,33 +689,6 @@ __device__ inline uint32_t gather_keys(const DevNet &P, uint32_t t, uint32_t j, 
     }
     n += (uint32_t)__popcll(__ballot(on));
   }
-  if (first && plastic) {
-    // LTD, synapseSetLastActiveTime_antiCausalSTDP (DPSNN_neuron_sim.c:
-    // 207-277): every synapse delivered in step t gets last = t and
-    // td += anti[t - lastSpike], one value per target.  A synapse is live at
-    // most once per step (refractory >= 1 ms), so the updates are
-    // independent: batches of 8 issue their index loads, then their value
-    // loads, then their stores.
-    const int dt = (int)t - P.lastSpike[j];
-    const float a = dt < kStdpSpan ? P.anti[dt] : 0.0f;
-    const uint64_t sbase = P.incOff[j];
-#pragma unroll
-    for (int k0 = 0; k0 < K; k0 += 8) {
-      uint32_t sy[8];
-      float td[8];
-#pragma unroll
-      for (int u = 0; u < 8 && k0 + u < K; ++u)
-        sy[u] = key[k0 + u] != kNoKey ? P.inSyn[beg + base + (k0 + u) * kWave + lane] : 0u;
-#pragma unroll
-      for (int u = 0; u < 8 && k0 + u < K; ++u) td[u] = key[k0 + u] != kNoKey ? P.tdT[sbase + sy[u]] : 0.0f;
-#pragma unroll
-      for (int u = 0; u < 8 && k0 + u < K; ++u)
-        if (key[k0 + u] != kNoKey) {
-          P.lastT[sbase + sy[u]] = (int32_t)t;
-          P.tdT[sbase + sy[u]] = td[u] + a;
-        }
-    }
-  }
   return n;
 }
 
@@ -732,13 +707,14 @@ __host__ __device__ inline size_t phase1_lds(uint32_t cap, int B) {
 
 // Phase 1 for neuron j, whole wave: live keys of its in-list, bucket-sorted
 // in LDS (lds: sorted[cap], ldsW[cap], bcnt[B + 1]), stored in arrival order
-// as ticks relative to bits(t) (outK) with their weights (outW); returns n.
+// as ticks relative to bits(t) (outK) with their weights (outW) and, with
+// plastic, their in-list positions (outP, for LTD); returns n.
 // The scratch of a tile of 64 neurons is blocked by runs of kRun events:
 // event i of neuron q at [i / kRun][q][i % kRun] (a neuron's keys fill whole
 // 128-byte lines), and outK/outW point at neuron q's [0][q][0].
 template <int K, int B>
 __device__ inline uint32_t gather_sort_neuron(const DevNet &P, uint32_t t, uint32_t j, int plastic, uint8_t *lds,
-                                              uint32_t cap, uint64_t *outK, int16_t *outW) {
+                                              uint32_t cap, uint64_t *outK, int16_t *outW, uint16_t *outP) {
   static_assert(B % kWave == 0, "bucket count must be a multiple of 64");
   constexpr int R = B / kWave;  // buckets per lane
   constexpr int LOG2B = B == 64 ? 6 : B == 128 ? 7 : B == 256 ? 8 : B == 512 ? 9 : 10;
@@ -808,9 +784,49 @@ __device__ inline uint32_t gather_sort_neuron(const DevNet &P, uint32_t t, uint3
     const size_t at = (size_t)(p / kRun) * (kWave * kRun) + (p % kRun);
     outK[at] = x >> 12;
     outW[at] = ldsW[x & 0xFFFu];
+    if (plastic) outP[at] = (uint16_t)(x & 0xFFFu);
   }
   __syncthreads();
   return n;
+}
+
+// LTD, synapseSetLastActiveTime_antiCausalSTDP (DPSNN_neuron_sim.c:
+// 207-277): every synapse delivered to neuron j in step t gets last = t and
+// td += anti[t - lastSpike_j], one increment per target, taken before step
+// t's dynamics move lastSpike.
+__device__ inline float ltd_increment(const DevNet &P, uint32_t t, uint32_t j) {
+  const int dt = (int)t - P.lastSpike[j];
+  return dt < kStdpSpan ? P.anti[dt] : 0.0f;
+}
+
+// Applies it to the n synapses whose in-list positions phase 1 left in the
+// blocked scratch (pos = neuron j's [0][q][0]).  A synapse is delivered at
+// most once per step (refractory >= 1 ms), so the updates are independent:
+// rounds of 4 per lane issue the position loads, the synapse-index loads,
+// the value loads, then the stores.  Whole wave.
+__device__ inline void ltd_neuron(const DevNet &P, uint32_t t, uint32_t j, float a, uint32_t n, const uint16_t *pos) {
+  const uint32_t lane = threadIdx.x & 63;
+  const uint64_t beg = P.inOff[j], sbase = P.incOff[j];
+  constexpr int U = 4;
+  for (uint32_t i0 = lane; i0 < n; i0 += U * kWave) {
+    uint32_t p[U], sy[U];
+    float td[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const uint32_t i = i0 + (uint32_t)u * kWave;
+      p[u] = i < n ? pos[(size_t)(i / kRun) * (kWave * kRun) + (i % kRun)] : 0u;
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u) sy[u] = i0 + (uint32_t)u * kWave < n ? P.inSyn[beg + p[u]] : 0u;
+#pragma unroll
+    for (int u = 0; u < U; ++u) td[u] = i0 + (uint32_t)u * kWave < n ? P.tdT[sbase + sy[u]] : 0.0f;
+#pragma unroll
+    for (int u = 0; u < U; ++u)
+      if (i0 + (uint32_t)u * kWave < n) {
+        P.lastT[sbase + sy[u]] = (int32_t)t;
+        P.tdT[sbase + sy[u]] = td[u] + a;
+      }
+  }
 }
 
 // What the tail of step t does for the scan of step t+1: where its counts
@@ -836,14 +852,16 @@ __host__ __device__ inline size_t step_lds(uint32_t cap, int B) {
 template <int K, int B>
 __device__ inline unsigned long long step_tile(const DevNet &P, uint32_t t, const ExtView &X, int ltd,
                                                const NextScan &S, uint32_t tile, uint32_t cap, uint8_t *lds,
-                                               uint64_t *wkeys, int16_t *wW, uint32_t *wcnt,
+                                               uint64_t *wkeys, int16_t *wW, uint16_t *wP, uint32_t *wcnt,
                                                unsigned long long &exSum) {
   const uint32_t lane = threadIdx.x & 63;
   const uint32_t j0 = tile * kWave;
   const uint32_t m = min((uint32_t)kWave, P.N - j0);
   unsigned long long evSum = 0;
   for (uint32_t q = 0; q < m; ++q) {
-    const uint32_t n = gather_sort_neuron<K, B>(P, t, j0 + q, ltd, lds, cap, wkeys + q * kRun, wW + q * kRun);
+    const uint32_t n =
+        gather_sort_neuron<K, B>(P, t, j0 + q, ltd, lds, cap, wkeys + q * kRun, wW + q * kRun, wP + q * kRun);
+    if (ltd) ltd_neuron(P, t, j0 + q, ltd_increment(P, t, j0 + q), n, wP + q * kRun);
     if (lane == 0) wcnt[q] = n;
     evSum += n;
   }
@@ -870,11 +888,12 @@ __global__ __launch_bounds__(kWave) void k_step(DevNet P, uint32_t t, ExtView X,
   extern __shared__ uint64_t lds[];
   uint64_t *wkeys = P.keys + (size_t)blockIdx.x * kWave * P.cap;
   int16_t *wW = P.keyW + (size_t)blockIdx.x * kWave * P.cap;
+  uint16_t *wP = P.keyPos ? P.keyPos + (size_t)blockIdx.x * kWave * P.cap : nullptr;
   uint32_t *wcnt = P.keyCount + (size_t)blockIdx.x * kWave;
   unsigned long long evSum = 0, exSum = 0;
   const XcdWalk w = xcd_walk(P.NW);
   for (uint32_t tile = w.first; tile < w.end; tile += w.stride)
-    evSum += step_tile<K, B>(P, t, X, ltd, S, tile, P.cap, (uint8_t *)lds, wkeys, wW, wcnt, exSum);
+    evSum += step_tile<K, B>(P, t, X, ltd, S, tile, P.cap, (uint8_t *)lds, wkeys, wW, wP, wcnt, exSum);
   for (int o = 32; o > 0; o >>= 1) exSum += __shfl_down(exSum, o);
   if (threadIdx.x == 0) {
     atomicAdd(&P.counters[0], evSum);
@@ -899,10 +918,11 @@ struct BatchItem {
 template <int K, int B>
 __global__ __launch_bounds__(kWave) void k_batch_step(const DevNet *nets, const BatchItem *items, uint32_t count,
                                                       uint32_t totalTiles, uint32_t cap, uint64_t *keys,
-                                                      int16_t *keyW, uint32_t *keyCount) {
+                                                      int16_t *keyW, uint16_t *keyPos, uint32_t *keyCount) {
   extern __shared__ uint64_t lds[];
   uint64_t *wkeys = keys + (size_t)blockIdx.x * kWave * cap;
   int16_t *wW = keyW + (size_t)blockIdx.x * kWave * cap;
+  uint16_t *wP = keyPos + (size_t)blockIdx.x * kWave * cap;
   uint32_t *wcnt = keyCount + (size_t)blockIdx.x * kWave;
   const XcdWalk w = xcd_walk(totalTiles);
   for (uint32_t g = w.first; g < w.end; g += w.stride) {
@@ -915,7 +935,7 @@ __global__ __launch_bounds__(kWave) void k_batch_step(const DevNet *nets, const 
     const DevNet &P = nets[it.net];
     unsigned long long exSum = 0;
     const unsigned long long ev = step_tile<K, B>(P, it.t, it.X, it.ltd, it.S, g - it.tileBegin, cap,
-                                                  (uint8_t *)lds, wkeys, wW, wcnt, exSum);
+                                                  (uint8_t *)lds, wkeys, wW, wP, wcnt, exSum);
     for (int o = 32; o > 0; o >>= 1) exSum += __shfl_down(exSum, o);
     if (threadIdx.x == 0) {
       atomicAdd(&P.counters[0], ev);
@@ -937,8 +957,12 @@ __global__ __launch_bounds__(kWave) void k_gather(DevNet P, uint32_t t, int ltd)
   for (uint32_t j0, j1; (j0 = next_item(P.workCtr, P.N, x, home, &j1)) < P.N;)
     for (uint32_t j = j0; j < j1; ++j) {
       const size_t at = (size_t)(j >> 6) * P.cap * kWave + (j & 63) * kRun;  // the tile's blocked scratch
-      const uint32_t n = gather_sort_neuron<K, B>(P, t, j, ltd, (uint8_t *)lds, P.cap, P.keys + at, P.keyW + at);
-      if (threadIdx.x == 0) P.keyCount[j] = n;
+      const uint32_t n = gather_sort_neuron<K, B>(P, t, j, ltd, (uint8_t *)lds, P.cap, P.keys + at, P.keyW + at,
+                                                  ltd ? P.keyPos + at : nullptr);
+      if (threadIdx.x == 0) {
+        P.keyCount[j] = n;
+        if (ltd) P.keyA[j] = ltd_increment(P, t, j);
+      }
       evSum += n;
     }
   if (threadIdx.x == 0) atomicAdd(&P.counters[0], evSum);
@@ -984,7 +1008,8 @@ __device__ inline uint32_t batch_item_of(const BatchItem *items, uint32_t count,
 template <int K, int B>
 __global__ __launch_bounds__(kWave) void k_batch_gather(const DevNet *nets, const BatchItem *items, uint32_t count,
                                                         uint32_t g0, uint32_t g1, uint32_t cap, uint64_t *keys,
-                                                        int16_t *keyW, uint32_t *keyCount, uint32_t *workCtr) {
+                                                        int16_t *keyW, uint16_t *keyPos, float *keyA,
+                                                        uint32_t *keyCount, uint32_t *workCtr) {
   extern __shared__ uint64_t lds[];
   const uint32_t total = (g1 - g0) * kWave, home = blockIdx.x % kXcds;
   uint32_t x = home;
@@ -996,10 +1021,11 @@ __global__ __launch_bounds__(kWave) void k_batch_gather(const DevNet *nets, cons
     const uint32_t j = (g - it.tileBegin) * kWave + q;
     if (j >= P.N) continue;
     const size_t at = (size_t)(u / kWave) * cap * kWave + q * kRun;
-    const uint32_t n =
-        gather_sort_neuron<K, B>(P, it.t, j, it.ltd, (uint8_t *)lds, cap, keys + at, keyW + at);
+    const uint32_t n = gather_sort_neuron<K, B>(P, it.t, j, it.ltd, (uint8_t *)lds, cap, keys + at, keyW + at,
+                                                it.ltd ? keyPos + at : nullptr);
     if (threadIdx.x == 0) {
       keyCount[u] = n;
+      if (it.ltd) keyA[u] = ltd_increment(P, it.t, j);
       atomicAdd(&P.counters[0], (unsigned long long)n);
     }
   }
@@ -1051,6 +1077,34 @@ __global__ __launch_bounds__(kWaveLTP) void k_batch_ltp(const DevNet *nets, cons
   }
 }
 
+// LTD of step t, after phase 1 (and beside phase 2, on another stream: it
+// touches only the synapses' last / td and phase 1's positions): one wave
+// per neuron, four waves per block.
+__global__ __launch_bounds__(kBlock) void k_ltd(DevNet P, uint32_t t) {
+  const uint32_t wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  const uint32_t waves = (gridDim.x * blockDim.x) >> 6;
+  for (uint32_t j = wave; j < P.N; j += waves) {
+    const size_t at = (size_t)(j >> 6) * P.cap * kWave + (j & 63) * kRun;
+    ltd_neuron(P, t, j, P.keyA[j], P.keyCount[j], P.keyPos + at);
+  }
+}
+
+__global__ __launch_bounds__(kBlock) void k_batch_ltd(const DevNet *nets, const BatchItem *items, uint32_t count,
+                                                      uint32_t g0, uint32_t g1, uint32_t cap, const uint16_t *keyPos,
+                                                      const float *keyA, const uint32_t *keyCount) {
+  const uint32_t wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  const uint32_t waves = (gridDim.x * blockDim.x) >> 6;
+  for (uint32_t u = wave; u < (g1 - g0) * kWave; u += waves) {
+    const uint32_t g = g0 + u / kWave, q = u % kWave;
+    const BatchItem &it = items[batch_item_of(items, count, g)];
+    if (!it.ltd) continue;
+    const DevNet &P = nets[it.net];
+    const uint32_t j = (g - it.tileBegin) * kWave + q;
+    if (j >= P.N) continue;
+    ltd_neuron(P, it.t, j, keyA[u], keyCount[u], keyPos + (size_t)(u / kWave) * cap * kWave + q * kRun);
+  }
+}
+
 // Fills n doubles with v.
 __global__ void k_fill_f64(double *p, double v, uint64_t n) {
   for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
@@ -1063,9 +1117,9 @@ __global__ void k_fill_f64(double *p, double v, uint64_t n) {
 using StepFn = void (*)(DevNet, uint32_t, ExtView, int, NextScan);
 using GatherFn = void (*)(DevNet, uint32_t, int);
 using BatchFn = void (*)(const DevNet *, const BatchItem *, uint32_t, uint32_t, uint32_t, uint64_t *, int16_t *,
-                         uint32_t *);
+                         uint16_t *, uint32_t *);
 using BatchGatherFn = void (*)(const DevNet *, const BatchItem *, uint32_t, uint32_t, uint32_t, uint32_t, uint64_t *,
-                               int16_t *, uint32_t *, uint32_t *);
+                               int16_t *, uint16_t *, float *, uint32_t *, uint32_t *);
 struct KernelShape {
   int K, B;
   StepFn step;
@@ -1156,6 +1210,8 @@ struct dpsnn_net {
   size_t ldsBytes = 0;
   bool stdp = false;         // plasticity runs at some step of [0, total)
   bool uploaded = false;     // build_device done (dpsnn_upload)
+  hipStream_t side = nullptr;            // STDP: LTD beside phase 2
+  hipEvent_t evPhase1 = nullptr, evLtd = nullptr;
   std::string outDir;        // owned copy of opt.output_dir
   uint32_t stdpBlocks = 0;
   int shape = 0;             // kShapes index
@@ -1206,6 +1262,12 @@ static void free_all(dpsnn_net *n) {
     hipEventDestroy(e.second);
   }
   n->profEv.clear();
+  if (n->side) (void)hipStreamSynchronize(n->side);
+  if (n->evPhase1) (void)hipEventDestroy(n->evPhase1);
+  if (n->evLtd) (void)hipEventDestroy(n->evLtd);
+  if (n->side) (void)hipStreamDestroy(n->side);
+  n->side = nullptr;
+  n->evPhase1 = n->evLtd = nullptr;
   if (n->stream) hipStreamDestroy(n->stream);
   n->stream = nullptr;
 }
@@ -1331,6 +1393,9 @@ static int build_device(dpsnn_net *n) {
     P.decayReset = q.tauDerivativeDecay_s == 1.0 ? 1 : 0;
     P.decayMul = (1 - 1 / q.tauDerivativeDecay_s);
     HIP_TRY(hipStreamSynchronize(n->stream));
+    HIP_TRY(hipStreamCreateWithFlags(&n->side, hipStreamNonBlocking));
+    HIP_TRY(hipEventCreateWithFlags(&n->evPhase1, hipEventDisableTiming));
+    HIP_TRY(hipEventCreateWithFlags(&n->evLtd, hipEventDisableTiming));
   }
   // launch shapes.  The split form (k_gather, one wave per neuron, then
   // k_integrate) is the default; the fused k_step (each wave whole tiles of
@@ -1574,6 +1639,7 @@ static int ensure_scratch(dpsnn_net *n) {
   if ((rc = dev_alloc(n, &P.keys, slots * P.cap)) || (rc = dev_alloc(n, &P.keyW, slots * P.cap)) ||
       (rc = dev_alloc(n, &P.keyCount, slots)))
     return rc;
+  if (n->stdp && ((rc = dev_alloc(n, &P.keyPos, slots * P.cap)) || (rc = dev_alloc(n, &P.keyA, slots)))) return rc;
   return 0;
 }
 
@@ -1665,8 +1731,15 @@ static int one_step(dpsnn_net *n) {
   const KernelShape &K = kShapes[n->shape];
   if (n->split) {
     hipLaunchKernelGGL(K.gather, dim3(n->gatherGrid), dim3(kWave), n->ldsBytes, n->stream, P, sp.t, sp.ltd);
+    if (sp.ltd) {  // LTD on the side stream, beside phase 2
+      HIP_TRY(hipEventRecord(n->evPhase1, n->stream));
+      HIP_TRY(hipStreamWaitEvent(n->side, n->evPhase1, 0));
+      hipLaunchKernelGGL(k_ltd, dim3(n->stdpBlocks), dim3(kBlock), 0, n->side, P, sp.t);
+      HIP_TRY(hipEventRecord(n->evLtd, n->side));
+    }
     const uint32_t blocks = (P.N + kBlock - 1) / kBlock;
     hipLaunchKernelGGL(k_integrate, dim3(blocks), dim3(kBlock), 0, n->stream, P, sp.t, sp.X, sp.S);
+    if (sp.ltd) HIP_TRY(hipStreamWaitEvent(n->stream, n->evLtd, 0));
     if (sp.S.ltp) hipLaunchKernelGGL(k_ltp, dim3(n->stdpBlocks), dim3(kWaveLTP), 0, n->stream, P, sp.t + 1);
   } else {
     hipLaunchKernelGGL(K.step, dim3(n->grid), dim3(kWave), n->ldsBytes, n->stream, P, sp.t, sp.X, sp.ltd, sp.S);
@@ -1980,8 +2053,12 @@ struct dpsnn_batch {
   int ringNext = 0;
   uint64_t *keys = nullptr;
   int16_t *keyW = nullptr;
+  uint16_t *keyPos = nullptr;
+  float *keyA = nullptr;
   uint32_t *keyCount = nullptr;
   uint32_t *workCtr = nullptr;  // [kXcds], zero between launches
+  hipStream_t side = nullptr;   // LTD beside phase 2
+  hipEvent_t evPhase1 = nullptr, evLtd = nullptr;
   size_t scratchSlots = 0;
   uint32_t scratchCap = 0;
   uint32_t totalTiles = 0;   // of all members
@@ -2066,21 +2143,28 @@ int batch_prepare(dpsnn_batch *b) {
     b->totalTiles += n->dev.NW;
     b->anyPlastic = b->anyPlastic || n->stdp;
   }
-  const size_t perTile = (size_t)cap * kWave * (sizeof(uint64_t) + sizeof(int16_t));
+  const size_t perTile = (size_t)cap * kWave * (sizeof(uint64_t) + 2 * sizeof(int16_t));
   b->chunkTiles = (uint32_t)std::max<size_t>(1, std::min<size_t>(b->totalTiles, kBatchScratchBytes / perTile));
   const size_t fusedSlots = (size_t)round_up((uint32_t)b->cus * lds_waves(lds_bytes(cap, 64)), kXcds) * kWave;
   const size_t slots = std::max(fusedSlots, (size_t)b->chunkTiles * kWave);
   if (cap > b->scratchCap || slots > b->scratchSlots) {
     if (b->keys) HIP_TRY(hipFree(b->keys));
     if (b->keyW) HIP_TRY(hipFree(b->keyW));
+    if (b->keyPos) HIP_TRY(hipFree(b->keyPos));
+    if (b->keyA) HIP_TRY(hipFree(b->keyA));
     if (b->keyCount) HIP_TRY(hipFree(b->keyCount));
     b->keys = nullptr;
     b->keyW = nullptr;
+    b->keyPos = nullptr;
+    b->keyA = nullptr;
     b->keyCount = nullptr;
     hipError_t e = hipMalloc((void **)&b->keys, sizeof(uint64_t) * slots * cap);
     if (e != hipSuccess) return fail(DPSNN_ERR_MEMORY, "batch key scratch: %s", hipGetErrorString(e));
     e = hipMalloc((void **)&b->keyW, sizeof(int16_t) * slots * cap);
     if (e != hipSuccess) return fail(DPSNN_ERR_MEMORY, "batch weight scratch: %s", hipGetErrorString(e));
+    e = hipMalloc((void **)&b->keyPos, sizeof(uint16_t) * slots * cap);
+    if (e != hipSuccess) return fail(DPSNN_ERR_MEMORY, "batch position scratch: %s", hipGetErrorString(e));
+    HIP_TRY(hipMalloc((void **)&b->keyA, sizeof(float) * slots));
     HIP_TRY(hipMalloc((void **)&b->keyCount, sizeof(uint32_t) * slots));
     b->scratchCap = cap;
     b->scratchSlots = slots;
@@ -2109,7 +2193,7 @@ int batch_launch(dpsnn_batch *b, int r, uint32_t count, uint32_t tiles) {
     const size_t lds = lds_bytes(cap, K.B);
     const uint32_t grid = round_up(std::min<uint32_t>(tiles, (uint32_t)b->cus * batch_waves(b, lds)), kXcds);
     hipLaunchKernelGGL(K.batch, dim3(grid), dim3(kWave), lds, b->stream, b->d_nets, b->d_items[r], count, tiles, cap,
-                       b->keys, b->keyW, b->keyCount);
+                       b->keys, b->keyW, b->keyPos, b->keyCount);
   } else {
     const size_t lds = phase1_lds(cap, K.B);
     for (uint32_t g0 = 0; g0 < tiles; g0 += b->chunkTiles) {
@@ -2119,10 +2203,19 @@ int batch_launch(dpsnn_batch *b, int r, uint32_t count, uint32_t tiles) {
                              (uint32_t)b->cus * std::min(batch_waves(b, lds), resident_waves((const void *)K.batchGather, lds))),
           kXcds);
       hipLaunchKernelGGL(K.batchGather, dim3(grid), dim3(kWave), lds, b->stream, b->d_nets, b->d_items[r], count, g0,
-                         g1, cap, b->keys, b->keyW, b->keyCount, b->workCtr);
+                         g1, cap, b->keys, b->keyW, b->keyPos, b->keyA, b->keyCount, b->workCtr);
+      if (b->anyPlastic) {  // LTD on the side stream, beside phase 2
+        HIP_TRY(hipEventRecord(b->evPhase1, b->stream));
+        HIP_TRY(hipStreamWaitEvent(b->side, b->evPhase1, 0));
+        hipLaunchKernelGGL(k_batch_ltd, dim3(std::min<uint32_t>((g1 - g0) * (kWave / (kBlock / kWave)), b->cus * 16u)),
+                           dim3(kBlock), 0, b->side, b->d_nets, b->d_items[r], count, g0, g1, cap, b->keyPos, b->keyA,
+                           b->keyCount);
+        HIP_TRY(hipEventRecord(b->evLtd, b->side));
+      }
       const uint32_t blocks = (g1 - g0 + kBlock / kWave - 1) / (kBlock / kWave);
       hipLaunchKernelGGL(k_batch_integrate, dim3(blocks), dim3(kBlock), 0, b->stream, b->d_nets, b->d_items[r], count,
                          g0, g1, cap, b->keys, b->keyW, b->keyCount, b->workCtr);
+      if (b->anyPlastic) HIP_TRY(hipStreamWaitEvent(b->stream, b->evLtd, 0));
       if (b->anyPlastic)
         hipLaunchKernelGGL(k_batch_ltp, dim3(std::min<uint32_t>((g1 - g0) * (kWave * kWave / kWaveLTP), b->cus * 16u)),
                            dim3(kWaveLTP), 0, b->stream, b->d_nets, b->d_items[r], count, g0, g1);
@@ -2248,6 +2341,9 @@ int dpsnn_batch_create(int device, dpsnn_batch **out) {
   b->cus = std::max(b->cus, 1);
   HIP_TRY(hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking));
   for (int r = 0; r < dpsnn_batch::kRing; ++r) HIP_TRY(hipEventCreateWithFlags(&b->done[r], hipEventDisableTiming));
+  HIP_TRY(hipStreamCreateWithFlags(&b->side, hipStreamNonBlocking));
+  HIP_TRY(hipEventCreateWithFlags(&b->evPhase1, hipEventDisableTiming));
+  HIP_TRY(hipEventCreateWithFlags(&b->evLtd, hipEventDisableTiming));
   HIP_TRY(hipMalloc((void **)&b->workCtr, sizeof(uint32_t) * kXcds * kCtrStride));
   HIP_TRY(hipMemset(b->workCtr, 0, sizeof(uint32_t) * kXcds * kCtrStride));
   if (const char *e = std::getenv("DPSNN_GRAPH")) b->useGraph = std::atoi(e) != 0;
@@ -2410,7 +2506,13 @@ void dpsnn_batch_destroy(dpsnn_batch *b) {
   if (b->keys) hipFree(b->keys);
   if (b->keyW) hipFree(b->keyW);
   if (b->keyCount) (void)hipFree(b->keyCount);
+  if (b->keyPos) (void)hipFree(b->keyPos);
+  if (b->keyA) (void)hipFree(b->keyA);
   if (b->workCtr) (void)hipFree(b->workCtr);
+  (void)hipStreamSynchronize(b->side);
+  (void)hipEventDestroy(b->evPhase1);
+  (void)hipEventDestroy(b->evLtd);
+  (void)hipStreamDestroy(b->side);
   for (int r = 0; r < dpsnn_batch::kRing; ++r)
     if (b->done[r]) hipEventDestroy(b->done[r]);
   for (auto &e : b->ev) {
