@@ -646,31 +646,49 @@ __device__ inline uint32_t key_bucket(uint64_t tick, int shift) {
 
 // Keys of in-list items base + k*64 + lane (k < K) of neuron j; kNoKey for
 // items that are absent or not live (the source did not fire in step
-// t - delay: its entry in that slot's Te history is kNotFired).  With
-// `first`, also counts the live keys per bucket (LDS), keeps their weights
-// in LDS by position (ldsW) and applies LTD to their synapses.  The K
-// chunks' loads are independent: the in-list words and weights, then the
-// emission times, so a wave keeps 2K loads in flight per round.
+// t - delay: its bit in that slot's fired bitmap is clear).  With `first`,
+// also counts the live keys per bucket (LDS) and keeps their weights in LDS
+// by position (ldsW).  Three rounds of K independent loads: the in-list
+// words; the bitmap words of their sources (a target's sources are a few
+// neighbouring columns, so these mostly hit the CU's L1); then, for live
+// items only, the emission times and weights.  A source's bitmap bit is set
+// exactly when its Te history entry is an emission time (publish_spikes
+// writes both), so the emission-time loads go to L2 only for delivered
+// events.
 template <int K, int B>
 __device__ inline uint32_t gather_keys(const DevNet &P, uint32_t t, uint32_t j, uint64_t beg, uint32_t len,
                                        uint32_t base, uint64_t (&key)[K], uint32_t *bcnt, int16_t *ldsW,
                                        int shift, uint64_t tBits, bool first, int plastic) {
+  static_assert(K <= 32, "live items are kept in one 32-bit mask");
   const uint32_t lane = threadIdx.x & 63;
   const double tF = (double)t;
+  const uint32_t *bw = (const uint32_t *)P.bits;  // word s * 2NW + (pre >> 5) holds pre's bit of slot s
   uint32_t it[K];
 #pragma unroll
   for (int k = 0; k < K; ++k) {
     const uint32_t idx = base + k * kWave + lane;
     it[k] = idx < len ? P.inPD[beg + idx] : kPad;
   }
+  uint32_t wb[K];
+#pragma unroll
+  for (int k = 0; k < K; ++k) {
+    const uint32_t d = it[k] >> 24, pre = it[k] & 0xFFFFFFu;
+    wb[k] = 0u;
+    if (it[k] != kPad && d <= t) wb[k] = bw[((t - d) & (P.RS - 1)) * (2u * P.NW) + (pre >> 5)];
+  }
+  uint32_t live = 0u;
+#pragma unroll
+  for (int k = 0; k < K; ++k) live |= ((wb[k] >> (it[k] & 31u)) & 1u) << k;
   double te[K];
   int16_t w[K];
 #pragma unroll
   for (int k = 0; k < K; ++k) {
     const uint32_t d = it[k] >> 24, pre = it[k] & 0xFFFFFFu;
     te[k] = kNotFired;
-    if (it[k] != kPad && d <= t) te[k] = P.teHist[((t - d) & (P.RS - 1)) * P.N + pre];
-    if (first) w[k] = it[k] != kPad ? P.inW[beg + base + k * kWave + lane] : (int16_t)0;
+    if ((live >> k) & 1u) {
+      te[k] = P.teHist[((t - d) & (P.RS - 1)) * P.N + pre];
+      if (first) w[k] = P.inW[beg + base + k * kWave + lane];
+    }
   }
   uint32_t n = 0;
 #pragma unroll
