@@ -111,7 +111,7 @@ __device__ inline uint32_t next_item(uint32_t *ctr, uint32_t total, uint32_t &x,
       k = __hip_atomic_load(c, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
       if (begin + k < end) k = atomicAdd(c, kGrab);
     }
-    k = (uint32_t)__shfl((int)k, 0);
+    k = __builtin_amdgcn_readfirstlane(k);  // lane 0's value, known uniform (scalar registers)
     if (begin + k < end) {
       *last = min(end, begin + k + kGrab);
       return begin + k;
@@ -135,6 +135,7 @@ enum { kTau, kTheta, kH, kTarp, kAlphaC, kTauC, kGC, kParams = 8 };
 
 struct DevNet {
   uint32_t N, NW, RS, nsub, npc, ncols, cap, drive;
+  uint32_t denseMax;       // phase 1 takes its fast path for up to this many live items (<= kDense * 64)
   float w2f;
   uint64_t philoxSeed;
   const uint8_t *subPop;
@@ -644,6 +645,65 @@ __device__ inline uint32_t key_bucket(uint64_t tick, int shift) {
   return b < (uint64_t)B ? (uint32_t)b : (uint32_t)(B - 1);
 }
 
+// *(base + byteOff): a 32-bit byte offset from a uniform base, so that a
+// load needs one address register (the spike history is under 4 GiB:
+// checked at build).
+template <class T>
+__device__ inline T load_at(const T *base, uint32_t byteOff) {
+  return *(const T *)((const char *)base + byteOff);
+}
+
+// Byte offsets of source `pre`'s entries of slot s: its emission time in the
+// Te history ([RS][N] doubles) and the 32-bit word of the fired bitmap
+// ([RS][NW] u64, read as [RS][2NW] u32) that holds its bit (bit pre & 31).
+// N < 2^24 (checked at build), so the products are 24-bit multiplies.
+__device__ inline uint32_t te_offset(const DevNet &P, uint32_t s, uint32_t pre) {
+  return (__umul24(s, P.N) + pre) * 8u;
+}
+__device__ inline uint32_t bit_offset(const DevNet &P, uint32_t s, uint32_t pre) {
+  return (__umul24(s, 2u * P.NW) + (pre >> 5)) * 4u;
+}
+
+constexpr int kDense = 16;  // fast path: up to kDense * 64 live keys per neuron, kDense per lane in registers
+constexpr int kSlowK = 8;   // register batches (of 64 items) of the general path
+
+// Fast path, rounds one and two: the live items of neuron j's in-list
+// (beg, len), compacted into rec[0, n) in in-list order as
+// item | position << 32; returns n (uniform).  K batches of 64 in-list words
+// are loaded at once, then the bitmap words of their sources (a target's
+// sources are a few neighbouring columns, so these mostly hit the CU's L1).
+// The in-list array is padded by one wave, so the words past len that the
+// last batch loads exist; they are never live.
+template <int K>
+__device__ inline uint32_t compact_live(const DevNet &P, uint32_t t, uint64_t beg, uint32_t len, uint64_t *rec) {
+  const uint32_t lane = threadIdx.x & 63;
+  const uint32_t *bw = (const uint32_t *)P.bits;
+  uint32_t n = 0;
+  for (uint32_t base = 0; base < len; base += (uint32_t)K * kWave) {
+    uint32_t it[K], wb[K];
+#pragma unroll
+    for (int k = 0; k < K; ++k)
+      if (base + k * kWave < len) it[k] = P.inPD[beg + base + k * kWave + lane];
+#pragma unroll
+    for (int k = 0; k < K; ++k)
+      if (base + k * kWave < len)
+        wb[k] = load_at(bw, bit_offset(P, (t - (it[k] >> 24)) & (P.RS - 1), it[k] & 0xFFFFFFu));
+#pragma unroll
+    for (int k = 0; k < K; ++k)
+      if (base + k * kWave < len) {
+        const uint32_t idx = base + k * kWave + lane;
+        const bool live = idx < len && (it[k] >> 24) <= t && ((wb[k] >> (it[k] & 31u)) & 1u);
+        const uint64_t m = __ballot(live);
+        if (live) {
+          const uint32_t below = __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
+          rec[n + below] = (uint64_t)it[k] | ((uint64_t)idx << 32);
+        }
+        n += (uint32_t)__popcll(m);
+      }
+  }
+  return n;
+}
+
 // Keys of in-list items base + k*64 + lane (k < K) of neuron j; kNoKey for
 // items that are absent or not live (the source did not fire in step
 // t - delay: its bit in that slot's fired bitmap is clear).  With `first`,
@@ -662,7 +722,7 @@ __device__ inline uint32_t gather_keys(const DevNet &P, uint32_t t, uint32_t j, 
   static_assert(K <= 32, "live items are kept in one 32-bit mask");
   const uint32_t lane = threadIdx.x & 63;
   const double tF = (double)t;
-  const uint32_t *bw = (const uint32_t *)P.bits;  // word s * 2NW + (pre >> 5) holds pre's bit of slot s
+  const uint32_t *bw = (const uint32_t *)P.bits;
   uint32_t it[K];
 #pragma unroll
   for (int k = 0; k < K; ++k) {
@@ -674,7 +734,7 @@ __device__ inline uint32_t gather_keys(const DevNet &P, uint32_t t, uint32_t j, 
   for (int k = 0; k < K; ++k) {
     const uint32_t d = it[k] >> 24, pre = it[k] & 0xFFFFFFu;
     wb[k] = 0u;
-    if (it[k] != kPad && d <= t) wb[k] = bw[((t - d) & (P.RS - 1)) * (2u * P.NW) + (pre >> 5)];
+    if (it[k] != kPad && d <= t) wb[k] = load_at(bw, bit_offset(P, (t - d) & (P.RS - 1), pre));
   }
   uint32_t live = 0u;
 #pragma unroll
@@ -686,7 +746,7 @@ __device__ inline uint32_t gather_keys(const DevNet &P, uint32_t t, uint32_t j, 
     const uint32_t d = it[k] >> 24, pre = it[k] & 0xFFFFFFu;
     te[k] = kNotFired;
     if ((live >> k) & 1u) {
-      te[k] = P.teHist[((t - d) & (P.RS - 1)) * P.N + pre];
+      te[k] = load_at(P.teHist, te_offset(P, (t - d) & (P.RS - 1), pre));
       if (first) w[k] = P.inW[beg + base + k * kWave + lane];
     }
   }
@@ -730,6 +790,11 @@ __host__ __device__ inline size_t phase1_lds(uint32_t cap, int B) {
 // The scratch of a tile of 64 neurons is blocked by runs of kRun events:
 // event i of neuron q at [i / kRun][q][i % kRun] (a neuron's keys fill whole
 // 128-byte lines), and outK/outW point at neuron q's [0][q][0].
+// Fast path (n <= kDense * 64): the live items are compacted into sorted[]
+// (compact_live), then each lane takes live items p = lane, lane + 64, ...:
+// emission time and weight loads for delivered events only, keys in
+// registers, bucket counts, scatter.  Otherwise the general path gathers
+// the in-list in register batches of kSlowK * 64 items, twice.
 template <int K, int B>
 __device__ inline uint32_t gather_sort_neuron(const DevNet &P, uint32_t t, uint32_t j, int plastic, uint8_t *lds,
                                               uint32_t cap, uint64_t *outK, int16_t *outW, uint16_t *outP) {
@@ -741,18 +806,53 @@ __device__ inline uint32_t gather_sort_neuron(const DevNet &P, uint32_t t, uint3
   uint32_t *bcnt = (uint32_t *)(lds + (size_t)cap * (sizeof(uint64_t) + sizeof(int16_t)));
   const uint32_t lane = threadIdx.x & 63;
   const uint64_t tBits = dpsnn_asu64((double)t);
+  const double tF = (double)t;
   const int e = t ? (31 - __clz((int)t)) : 0;
   const int shift = 52 - LOG2B - e;
-  constexpr uint32_t kSuper = (uint32_t)K * kWave;
+  constexpr uint32_t kSuper = (uint32_t)kSlowK * kWave;
   const uint64_t beg = P.inOff[j];
   const uint32_t len = (uint32_t)(P.inOff[j + 1] - beg);
   for (uint32_t b = lane; b <= (uint32_t)B; b += kWave) bcnt[b] = 0;
-  __syncthreads();
   uint32_t n = 0;
-  uint64_t key[K];
-  if (t > 0)
+  if (t > 0) n = compact_live<K>(P, t, beg, len, sorted);
+  __syncthreads();
+  const bool fast = n <= P.denseMax;
+  uint64_t key[kDense];
+  if (fast) {
+    uint64_t rc[kDense];
+#pragma unroll
+    for (int r = 0; r < kDense; ++r)
+      if ((uint32_t)r * kWave < n) {
+        const uint32_t p = r * kWave + lane;
+        rc[r] = sorted[p < n ? p : 0u];
+      }
+    double te[kDense];
+    int16_t w[kDense];
+#pragma unroll
+    for (int r = 0; r < kDense; ++r)
+      if ((uint32_t)r * kWave < n) {
+        const uint32_t it = (uint32_t)rc[r], idx = (uint32_t)(rc[r] >> 32);
+        te[r] = load_at(P.teHist, te_offset(P, (t - (it >> 24)) & (P.RS - 1), it & 0xFFFFFFu));
+        w[r] = P.inW[beg + idx];
+      }
+#pragma unroll
+    for (int r = 0; r < kDense; ++r)
+      if ((uint32_t)r * kWave < n) {
+        const uint32_t idx = (uint32_t)(rc[r] >> 32);
+        const double kk = tF + (te[r] - trunc(te[r]));  // modf, exact
+        const uint64_t tick = dpsnn_asu64(kk) - tBits;
+        key[r] = (tick << 12) | idx;
+        if (r * kWave + lane < n) {
+          atomicAdd(&bcnt[key_bucket<B>(tick, shift)], 1u);
+          ldsW[idx] = w[r];
+        }
+      }
+  } else {
+    n = 0;
+    uint64_t k8[kSlowK];
     for (uint32_t base = 0; base < len; base += kSuper)
-      n += gather_keys<K, B>(P, t, j, beg, len, base, key, bcnt, ldsW, shift, tBits, true, plastic);
+      n += gather_keys<kSlowK, B>(P, t, j, beg, len, base, k8, bcnt, ldsW, shift, tBits, true, plastic);
+  }
   __syncthreads();
   // exclusive scan of the bucket counters (R per lane)
   {
@@ -771,16 +871,18 @@ __device__ inline uint32_t gather_sort_neuron(const DevNet &P, uint32_t t, uint3
     if (lane == kWave - 1) bcnt[B] = run;
   }
   __syncthreads();
-  // scatter into buckets (order inside a bucket is fixed below); a list
-  // longer than one register batch is gathered again (L2 hits)
-  if (t > 0) {
-    if (len <= kSuper) {
-      scatter_keys<K, B>(key, bcnt, sorted, shift);
-    } else {
-      for (uint32_t base = 0; base < len; base += kSuper) {
-        gather_keys<K, B>(P, t, j, beg, len, base, key, bcnt, ldsW, shift, tBits, false, 0);
-        scatter_keys<K, B>(key, bcnt, sorted, shift);
-      }
+  // scatter into buckets (order inside a bucket is fixed below); the general
+  // path gathers its batches again (L2 hits)
+  if (fast) {
+#pragma unroll
+    for (int r = 0; r < kDense; ++r)
+      if ((uint32_t)r * kWave < n && r * kWave + lane < n)
+        sorted[atomicAdd(&bcnt[key_bucket<B>(key[r] >> 12, shift)], 1u)] = key[r];
+  } else {
+    uint64_t k8[kSlowK];
+    for (uint32_t base = 0; base < len; base += kSuper) {
+      gather_keys<kSlowK, B>(P, t, j, beg, len, base, k8, bcnt, ldsW, shift, tBits, false, 0);
+      scatter_keys<kSlowK, B>(k8, bcnt, sorted, shift);
     }
   }
   __syncthreads();
@@ -1300,12 +1402,20 @@ static int build_device(dpsnn_net *n) {
   // power of two (slot = ms & (RS - 1))
   P.RS = 1;
   while (P.RS < c.max_delay + 2) P.RS <<= 1;
-  if (uint64_t(P.RS) * c.N >= (1ull << 32)) return fail(DPSNN_ERR_CONFIG, "spike history larger than 2^32 entries");
+  // phase 1 addresses the spike history with 32-bit byte offsets and 24-bit
+  // neuron indices
+  if (c.N >= (1u << 24)) return fail(DPSNN_ERR_CONFIG, "%u neurons; this build handles fewer than 2^24", c.N);
+  if (uint64_t(P.RS) * c.N * sizeof(double) > (1ull << 32))
+    return fail(DPSNN_ERR_CONFIG, "spike history (%u slots x %u neurons) larger than 4 GiB", P.RS, c.N);
   P.nsub = q.subPopNumber;
   P.npc = q.neuronsPerCM;
   P.ncols = q.globCFT;
   // longest in-list, rounded to a whole number of scratch runs
   P.cap = (std::max<uint32_t>(c.max_in, 1) + (kRun - 1)) / kRun * kRun;
+  // DPSNN_PHASE1_DENSE_MAX lowers the fast path's limit (0: every neuron on
+  // the general path), so that tests can compare the two paths
+  P.denseMax = kDense * kWave;
+  if (const char *e = getenv("DPSNN_PHASE1_DENSE_MAX")) P.denseMax = std::min<uint32_t>(P.denseMax, (uint32_t)atoi(e));
   P.drive = (uint32_t)n->opt.drive;
   P.w2f = q.factorWeightType_2_Float;
   P.philoxSeed = n->opt.philox_seed;
@@ -1355,7 +1465,9 @@ static int build_device(dpsnn_net *n) {
     int16_t *d_w;
     const uint64_t items = c.in_off[N];
     if ((rc = dev_alloc(n, &d_off, N + 1)) || (rc = upload(n, d_off, c.in_off.data(), N + 1))) return rc;
-    if ((rc = dev_alloc(n, &d_pd, items)) || (rc = upload(n, d_pd, c.in_pre_d.data(), items))) return rc;
+    // one wave of padding: phase 1 loads whole waves of in-list words
+    if ((rc = dev_alloc(n, &d_pd, items + kWave)) || (rc = upload(n, d_pd, c.in_pre_d.data(), items))) return rc;
+    HIP_TRY(hipMemsetAsync(d_pd + items, 0, sizeof(uint32_t) * kWave, n->stream));
     if ((rc = dev_alloc(n, &d_w, items)) || (rc = upload(n, d_w, c.in_w.data(), items))) return rc;
     P.inOff = d_off; P.inPD = d_pd; P.inW = d_w;
     HIP_TRY(hipStreamSynchronize(n->stream));  // host staging is dropped after create
@@ -1853,7 +1965,7 @@ uint64_t dpsnn_device_bytes_needed(const dpsnn_net *n) {
   uint64_t b = r(8 * q.subPopNumber * kParams) + r(8 * c.bathLut.size()) + r(8 * N) + r(N) + r(4 * N);
   b += 5 * r(8 * N) + r(8 * N);                                // state, Philox counters
   b += r(8 * RS * NW) + r(8 * RS * N);                         // spike history
-  b += r(8 * (N + 1)) + r(4 * items) + r(2 * items);           // in-lists
+  b += r(8 * (N + 1)) + r(4 * (items + kWave)) + r(2 * items);  // in-lists
   if (n->stdp) {
     const uint64_t S = c.S;
     b += r(8 * (N + 1)) + r(2 * S) + r(S) + r(2 * items) + 2 * r(4 * S) + r(4 * N);

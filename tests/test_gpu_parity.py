@@ -158,6 +158,36 @@ def test_stdp_c2small_identical_to_reference(pkg, tmp_path):
                 g[f"synPeriodicProbe_{probe}"]["sha256"]), probe
 
 
+@pytest.mark.parametrize("dense_max", [0, 400])
+def test_phase1_general_path_identical_to_reference(pkg, tmp_path, monkeypatch, dense_max):
+    """Phase 1's general path (register batches, gathered twice), for every
+    neuron (0) or for those with more than 400 delivered events in a step
+    (400, both paths in one step): configs[0]'s first second and configs[2]'s
+    STDP run at configs[0]'s size are still the reference's, byte for byte."""
+    monkeypatch.setenv("DPSNN_PHASE1_DENSE_MAX", str(dense_max))
+    g = golden("c0_raster.json")
+    st = _run_c0(pkg, tmp_path, 1000)
+    assert sha256_file(tmp_path / "spikes-sec0-C4-H1-h000.dat") == g["sha256"]
+    assert st["spikes"] == g["lines"]
+    gs = golden("c2small_golden.json")
+    out = tmp_path / "stdp"
+    out.mkdir()
+    pkg.apply_env(pkg.config_dir("c2_small"), env_statSynPeriodicProbe_writeDisable=0,
+                  env_startSynPeriodicProbe_s=0, env_stopSynPeriodicProbe_s=0)
+    try:
+        with pkg.Network(pkg.config_dir("c2_small"), output_dir=str(out)) as net:
+            while not net.step(100):
+                pass
+            net.sync()
+    finally:
+        pkg.apply_env(pkg.config_dir("c0"))
+    for sec in (0, 1):
+        assert sha256_file(out / f"spikes-sec{sec}-C4-H1-h000.dat") == gs[f"spikes-sec{sec}"]["sha256"], sec
+    for probe in ("A", "B"):
+        assert sha256_file(out / f"synPeriodicProbe_{probe}-sec0-C4-H1-h000.dat") == \
+            gs[f"synPeriodicProbe_{probe}"]["sha256"], probe
+
+
 def test_stdp_rejects_sub_ms_refractory(pkg, tmp_path):
     cfg = _tiny_config(tmp_path, pkg)
     col = (tmp_path / "tiny" / "column.txt")
