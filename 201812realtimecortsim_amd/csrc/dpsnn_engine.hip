@@ -477,6 +477,7 @@ struct ExtReplayCursor {
 // following ones while fewer than the per-ms cap; past the cap only the
 // lapses are drawn.
 struct ExtPhiloxCursor {
+  static constexpr double kNone = __builtin_huge_val();
   double a, inv, nextF;
   const double *lut;
   uint32_t cap, nExt;
@@ -501,10 +502,53 @@ struct ExtPhiloxCursor {
   }
 };
 
+constexpr int kQueue = 4;  // Philox bath events per lane drawn ahead of the merge
+
+// The Philox bath with the step's first events drawn ahead (fill, all lanes
+// of the wave together, before the merge): up to kQueue (time, value) pairs
+// per lane in LDS (qt/qv: the lane's column of [kQueue][64]); events past
+// the queue are drawn from the cursor as they come.  The Philox stream is
+// consumed in the same order either way, so the events are the same.
+struct ExtQueuedCursor {
+  ExtPhiloxCursor xp;
+  double *qt, *qv;
+  uint32_t qi, qn;
+  double head;  // time of the next due event, or kNone
+  __device__ inline void fill(bool on) {
+    qn = 0;
+    for (;;) {
+      const bool d = on && qn < (uint32_t)kQueue && xp.due();
+      if (!__ballot(d)) break;
+      if (d) {
+        qt[qn * kWave] = xp.at();
+        qv[qn * kWave] = xp.take();
+        ++qn;
+      }
+    }
+    qi = 0;
+    head = qn ? qt[0] : (on && xp.due() ? xp.at() : ExtPhiloxCursor::kNone);
+  }
+  __device__ inline bool due() const { return head != ExtPhiloxCursor::kNone; }
+  __device__ inline double at() const { return head; }
+  __device__ inline double take() {
+    double v;
+    if (qi < qn) {
+      v = qv[qi * kWave];
+      ++qi;
+    } else {
+      v = xp.take();
+    }
+    head = qi < qn ? qt[qi * kWave] : (xp.due() ? xp.at() : ExtPhiloxCursor::kNone);
+    return v;
+  }
+  __device__ inline void finish() { xp.finish(); }
+};
+
 constexpr int kChunk = 8;  // internal events per lane per LDS-staged chunk
 constexpr int kRun = 16;   // events per neuron run in the blocked scratch
-// LDS of one wave in phase 2: two chunks of 64 lanes' ticks
-constexpr size_t kStageBytes = 2 * kChunk * kWave * sizeof(uint64_t);
+// LDS of one wave in phase 2: two chunks of 64 lanes' ticks, then the bath
+// queue (times, values: [kQueue][64] doubles each)
+constexpr size_t kStageBytes = 2 * kChunk * kWave * sizeof(uint64_t) + 2 * kQueue * kWave * sizeof(double);
 
 // Chunk c (events 8c..8c+7 of every lane) from the tile's blocked scratch
 // [i / kRun][q][i % kRun]: each neuron's 8 ticks are one 64-byte piece of
@@ -555,11 +599,12 @@ __device__ inline uint32_t integrate_wave(const DevNet &P, uint32_t t, const Ext
   }
   L.fired = false;
   ExtReplayCursor xr{};
-  ExtPhiloxCursor xp{};
+  ExtQueuedCursor xq{};
   if (on) {
     if (REPLAY) {
       xr = ExtReplayCursor{X.time, X.value, X.off[j], X.off[j + 1]};
     } else {
+      ExtPhiloxCursor &xp = xq.xp;
       xp.a = P.assigned[j];
       xp.inv = P.invNuExt[j];
       xp.nextF = nextF;
@@ -569,6 +614,12 @@ __device__ inline uint32_t integrate_wave(const DevNet &P, uint32_t t, const Ext
       xp.onE = xp.inv > 0;
       xp.R = PhiloxStream{(uint32_t)P.philoxSeed, (uint32_t)(P.philoxSeed >> 32) ^ j, P.philoxCtr[j], 0.0, 0};
     }
+  }
+  if (!REPLAY) {
+    double *q = (double *)(stage + 2 * kChunk * kWave * sizeof(uint64_t));
+    xq.qt = q + lane;
+    xq.qv = q + kQueue * kWave + lane;
+    xq.fill(on);
   }
   // one merge loop with a single LIFCADynamic site: internal events from
   // the staged chunk, the bath event instead when it is strictly earlier
@@ -595,11 +646,11 @@ __device__ inline uint32_t integrate_wave(const DevNet &P, uint32_t t, const Ext
     for (;;) {
       const bool hasI = e < left;
       const double kI = hasI ? dpsnn_asf64(tBits + curK[e * kWave + lane]) : 0.0;
-      const bool eDue = REPLAY ? xr.due() : xp.due();
+      const bool eDue = REPLAY ? xr.due() : xq.due();
       double at, val;
-      if (eDue && (hasI ? (REPLAY ? xr.at() : xp.at()) < kI : last)) {
-        at = REPLAY ? xr.at() : xp.at();
-        val = REPLAY ? xr.take() : xp.take();
+      if (eDue && (hasI ? (REPLAY ? xr.at() : xq.at()) < kI : last)) {
+        at = REPLAY ? xr.at() : xq.at();
+        val = REPLAY ? xr.take() : xq.take();
       } else if (hasI) {
         const int16_t w16 = (int16_t)(uint16_t)wlo;
         wlo = (wlo >> 16) | (whi << 48);
@@ -623,10 +674,10 @@ __device__ inline uint32_t integrate_wave(const DevNet &P, uint32_t t, const Ext
     if (REPLAY) {
       nExt = xr.xe - X.off[j];
     } else {
-      xp.finish();
-      nExt = xp.nExt;
-      P.assigned[j] = xp.a;
-      P.philoxCtr[j] = xp.R.ctr;
+      xq.finish();
+      nExt = xq.xp.nExt;
+      P.assigned[j] = xq.xp.a;
+      P.philoxCtr[j] = xq.xp.R.ctr;
     }
     P.v[j] = L.v; P.c[j] = L.c; P.Tr[j] = L.Tr; P.Te[j] = L.Te;
   }

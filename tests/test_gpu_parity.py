@@ -199,3 +199,18 @@ def test_stdp_rejects_sub_ms_refractory(pkg, tmp_path):
             pkg.Network(cfg)
     finally:
         pkg.apply_env(pkg.config_dir("c0"))
+
+
+def test_philox_drive_regression(pkg, tmp_path):
+    """The on-device Philox bath reproduces its committed vector (configs[0],
+    seed 20181201, 500 ms): raster, rates, counters and final state."""
+    import importlib.util
+    spec = importlib.util.spec_from_file_location("make_philox_golden",
+                                                  os.path.join(GOLDEN, "make_philox_golden.py"))
+    mod = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(mod)
+    try:
+        got = mod.run(pkg, str(tmp_path))
+    finally:
+        pkg.apply_env(pkg.config_dir("c0"))
+    assert got == golden("c0_philox.json")
