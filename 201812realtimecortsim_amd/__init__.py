@@ -161,6 +161,8 @@ _SIGS = {
     "dpsnn_host_destroy": (None, [ctypes.c_void_p]),
     "dpsnn_device_exp": (ctypes.c_int, [ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p,
                                         ctypes.c_uint64]),
+    "dpsnn_device_div": (ctypes.c_int, [ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
+                                        ctypes.c_uint64]),
     "dpsnn_batch_create": (ctypes.c_int, [ctypes.c_int, _p(ctypes.c_void_p)]),
     "dpsnn_batch_add": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p]),
     "dpsnn_batch_remove": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p]),
@@ -405,6 +407,15 @@ def device_exp(x, device=0):
     y = np.empty_like(x)
     _check("dpsnn_device_exp", library().dpsnn_device_exp(device, _ptr(x), _ptr(y), x.size))
     return y
+
+
+def device_div(a, b, device=0):
+    """a / b elementwise by the engine's device division by reciprocal."""
+    a = np.ascontiguousarray(a, np.float64)
+    b = np.ascontiguousarray(np.broadcast_to(b, a.shape), np.float64)
+    q = np.empty_like(a)
+    _check("dpsnn_device_div", library().dpsnn_device_div(device, _ptr(a), _ptr(b), _ptr(q), a.size))
+    return q
 
 
 class Batch:

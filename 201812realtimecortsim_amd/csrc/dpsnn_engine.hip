@@ -131,7 +131,7 @@ __device__ inline void wave_sync() {
 constexpr int kWaveLTP = 256;    // STDP kernels: 4 waves per block
 
 // Per-sub-population LIFCA parameters, in this order.
-enum { kTau, kTheta, kH, kTarp, kAlphaC, kTauC, kGC, kParams = 8 };
+enum { kTau, kTheta, kH, kTarp, kAlphaC, kTauC, kGC, kInvTau, kInvTauC, kParams };
 
 struct DevNet {
   uint32_t N, NW, RS, nsub, npc, ncols, cap, drive;
@@ -212,12 +212,26 @@ struct PhiloxStream {
   }
 };
 
+// a / b, correctly rounded (as IEEE division), for a normal b with
+// y = RN(1/b) and a / b within the normal range: q0 = RN(a y) is within two
+// ulps of a / b; one correction q1 = RN(q0 + (a - b q0) y) makes it faithful;
+// the second, q2 = RN(q1 + r1 y) with the exact residual r1 = a - b q1, is
+// RN(a / b) (Markstein's theorem: q faithful, y = RN(1/b)).  Five FMA-unit
+// operations instead of the general division's scaled Newton sequence.
+__device__ inline double div_rn(double a, double b, double y) {
+  const double q0 = a * y;
+  const double q1 = fma(fma(-q0, b, a), y, q0);
+  return fma(fma(-q1, b, a), y, q1);
+}
+
 // --------------------------------------------------------------------------
 // LIFCADynamic, DPSNN_neuron_sim.c:664-824 (second-order, non-Perseo path).
+// The divisions by Tau and TauC use their reciprocals (div_rn); the results
+// are the IEEE quotients.
 // --------------------------------------------------------------------------
 struct Lifca {
   double v, c, Tr, Te;
-  double Tau, Theta, H, Tarp, AlphaC, TauC, gC;
+  double Tau, Theta, H, Tarp, AlphaC, TauC, gC, invTau, invTauC;
   bool fired;
   __device__ inline void event(double t, double I) {
     double deltaT = t - Tr;
@@ -225,12 +239,12 @@ struct Lifca {
     if (TFLES > Tarp) {
       const double TrTe = Tr - Te;
       if (TrTe < Tarp) {
-        c *= dpsnn_exp(-(deltaT - TFLES + Tarp) / TauC);
+        c *= dpsnn_exp(div_rn(-(deltaT - TFLES + Tarp), TauC, invTauC));
         deltaT = TFLES - Tarp;
       }
       const double dtHalf = deltaT / 2;
-      const double rmHalf = dtHalf / Tau;
-      const double rc = deltaT / TauC;
+      const double rmHalf = div_rn(dtHalf, Tau, invTau);
+      const double rc = div_rn(deltaT, TauC, invTauC);
       v = v - v * rmHalf - gC * c * dtHalf;
       v = v - v * rmHalf - gC * c * dtHalf;
       c = c * (1 - rc + rc * rc / 2);
@@ -242,7 +256,7 @@ struct Lifca {
         fired = true;
       }
     } else {
-      const double rc = deltaT / TauC;
+      const double rc = div_rn(deltaT, TauC, invTauC);
       c = c * (1 - rc + rc * rc / 2);
     }
     Tr = t;
@@ -596,6 +610,7 @@ __device__ inline uint32_t integrate_wave(const DevNet &P, uint32_t t, const Ext
     L.v = P.v[j]; L.c = P.c[j]; L.Tr = P.Tr[j]; L.Te = P.Te[j];
     L.Tau = prm[kTau]; L.Theta = prm[kTheta]; L.H = prm[kH]; L.Tarp = prm[kTarp];
     L.AlphaC = prm[kAlphaC]; L.TauC = prm[kTauC]; L.gC = prm[kGC];
+    L.invTau = prm[kInvTau]; L.invTauC = prm[kInvTauC];
   }
   L.fired = false;
   ExtReplayCursor xr{};
@@ -1479,6 +1494,12 @@ static int build_device(dpsnn_net *n) {
     double *o = &sp[size_t(s) * kParams];
     o[kTau] = x.Tau; o[kTheta] = x.Theta; o[kH] = x.H; o[kTarp] = x.Tarp;
     o[kAlphaC] = x.AlphaC; o[kTauC] = x.TauC; o[kGC] = x.gC;
+    // div_rn needs normal divisors with normal reciprocals
+    for (double d : {x.Tau, x.TauC})
+      if (!(std::fabs(d) >= 0x1p-1021 && std::fabs(d) <= 0x1p1021))
+        return fail(DPSNN_ERR_CONFIG, "sub-population %u: time constant %g outside the supported range", s, d);
+    o[kInvTau] = 1.0 / x.Tau;
+    o[kInvTauC] = 1.0 / x.TauC;
   }
   double *d_sp, *d_lut, *d_inv;
   uint8_t *d_sub;
@@ -2737,6 +2758,10 @@ __global__ void k_exp(const double *x, double *y, uint64_t n) {
   const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) y[i] = dpsnn_exp(x[i]);
 }
+__global__ void k_div(const double *a, const double *b, const double *y, double *q, uint64_t n) {
+  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) q[i] = div_rn(a[i], b[i], y[i]);
+}
 }  // namespace
 
 extern "C" {
@@ -2856,6 +2881,25 @@ int dpsnn_device_exp(int device, const double *x, double *y, uint64_t n) {
   HIP_TRY(hipMemcpy(y, dy, 8 * n, hipMemcpyDeviceToHost));
   HIP_TRY(hipFree(dx));
   HIP_TRY(hipFree(dy));
+  return 0;
+}
+
+int dpsnn_device_div(int device, const double *a, const double *b, double *q, uint64_t n) {
+  if (!a || !b || !q) return fail(DPSNN_ERR_ARG, "null argument");
+  HIP_TRY(hipSetDevice(device));
+  std::vector<double> y(n);
+  for (uint64_t i = 0; i < n; ++i) y[i] = 1.0 / b[i];  // RN(1/b), as the engine's set-up computes it
+  double *d = nullptr;
+  const size_t bytes = 8 * std::max<uint64_t>(n, 1);
+  HIP_TRY(hipMalloc((void **)&d, 4 * bytes));
+  HIP_TRY(hipMemcpy(d, a, 8 * n, hipMemcpyHostToDevice));
+  HIP_TRY(hipMemcpy(d + n, b, 8 * n, hipMemcpyHostToDevice));
+  HIP_TRY(hipMemcpy(d + 2 * n, y.data(), 8 * n, hipMemcpyHostToDevice));
+  const uint64_t blocks = (n + 255) / 256;
+  if (n) hipLaunchKernelGGL(k_div, dim3((uint32_t)blocks), dim3(256), 0, 0, d, d + n, d + 2 * n, d + 3 * n, n);
+  HIP_TRY(hipGetLastError());
+  HIP_TRY(hipMemcpy(q, d + 3 * n, 8 * n, hipMemcpyDeviceToHost));
+  HIP_TRY(hipFree(d));
   return 0;
 }
 

@@ -295,6 +295,11 @@ int dpsnn_device_memory(int device, uint64_t *free_bytes, uint64_t *total_bytes)
  * checked against the host libm). */
 int dpsnn_device_exp(int device, const double *x, double *y, uint64_t n);
 
+/* a[i] / b[i] into q[0..n) by the device's division by reciprocal, the one
+ * the engine uses for its time constants (diagnostic: checked against IEEE
+ * division on the host).  b normal, quotients in the normal range. */
+int dpsnn_device_div(int device, const double *a, const double *b, double *q, uint64_t n);
+
 const char *dpsnn_last_error(void);
 
 #ifdef __cplusplus

@@ -37,6 +37,39 @@ def test_device_exp_matches_host_libm(pkg):
     assert np.array_equal(y.view(np.uint64), ref.view(np.uint64))
 
 
+def test_device_division_matches_ieee(pkg):
+    """The dynamics divide by the time constants through their reciprocals
+    (two FMA corrections); the quotients are the IEEE ones.  Random
+    magnitudes and signs, event-like intervals, and quotients next to a
+    rounding midpoint (built in extended precision), for the time constants
+    of the workloads, awkward significands and random divisors."""
+    rng = np.random.default_rng(11)
+    divisors = [20.0, 10.0, 500.0, 2.0, 3.0, 7.0, 0.1, 0.3, 1e-3, 123.456, 1.5,
+                float(np.nextafter(2.0, 0.0)), float(np.nextafter(1.0, 2.0))]
+    a_all, b_all = [], []
+    for b in divisors:
+        n = 400000
+        mag = 10.0 ** rng.uniform(-12, 6, n)
+        a_all.append(mag * rng.choice([-1.0, 1.0], n))
+        dt = rng.random(n) * 2.0
+        a_all.append(np.concatenate([dt[: n // 2], dt[n // 2:] / 2]))
+        # quotients next to a midpoint: a ~ b * (q + ulp(q) / 2)
+        q = 10.0 ** rng.uniform(-8, 3, n)
+        half = np.spacing(q).astype(np.longdouble) / 2
+        jitter = rng.choice([-1, 0, 1], n).astype(np.longdouble) * np.spacing(q).astype(np.longdouble) * 2.0 ** -20
+        a_all.append((np.longdouble(b) * (q.astype(np.longdouble) + half + jitter)).astype(np.float64))
+        b_all.append(np.full(3 * n, b))
+    n = 2000000
+    a_all.append(10.0 ** rng.uniform(-10, 6, n) * rng.choice([-1.0, 1.0], n))
+    b_all.append(10.0 ** rng.uniform(-3, 4, n))
+    a = np.concatenate(a_all)
+    b = np.concatenate(b_all)
+    q = pkg.device_div(a, b)
+    ref = a / b
+    bad = np.flatnonzero(q.view(np.uint64) != ref.view(np.uint64))
+    assert bad.size == 0, (bad.size, a[bad[:3]], b[bad[:3]])
+
+
 def _run_c0(pkg, out, ms, **env):
     cfg = pkg.config_dir("c0")
     pkg.apply_env(cfg, env_totalSimTime_ms=ms, **env)
