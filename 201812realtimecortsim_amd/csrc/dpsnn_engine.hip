@@ -153,7 +153,6 @@ struct DevNet {
   uint64_t *keys;          // sorted event ticks: [grid or N/64][cap/8][64][8]
   int16_t *keyW;           // their weights, same layout
   uint16_t *keyPos;        // STDP: their in-list positions, same layout
-  float *keyA;             // STDP: LTD increment of the step, per neuron slot (keyCount's indexing)
   uint32_t *keyCount;      // [grid][64] or [N/64][64]
   uint32_t *workCtr;       // [kXcds][kCtrStride] phase-1 work counters (zero between steps)
   unsigned long long *counters;  // [0] events, [1] external, [2] spikes
@@ -730,26 +729,70 @@ __device__ inline uint32_t bit_offset(const DevNet &P, uint32_t s, uint32_t pre)
   return (__umul24(s, 2u * P.NW) + (pre >> 5)) * 4u;
 }
 
+// LTD, synapseSetLastActiveTime_antiCausalSTDP (DPSNN_neuron_sim.c:
+// 207-277): every synapse delivered to neuron j in step t gets last = t and
+// td += anti[t - lastSpike_j], one increment per target, taken before step
+// t's dynamics move lastSpike.
+__device__ inline float ltd_increment(const DevNet &P, uint32_t t, uint32_t j) {
+  const int dt = (int)t - P.lastSpike[j];
+  return dt < kStdpSpan ? P.anti[dt] : 0.0f;
+}
+
+// Applies it to the n synapses whose in-list positions phase 1 left in the
+// blocked scratch (pos = neuron j's [0][q][0]).  A synapse is delivered at
+// most once per step (refractory >= 1 ms), so the updates are independent:
+// rounds of 4 per lane issue the position loads, the synapse-index loads,
+// the value loads, then the stores.  Whole wave.
+__device__ inline void ltd_neuron(const DevNet &P, uint32_t t, uint32_t j, float a, uint32_t n, const uint16_t *pos) {
+  const uint32_t lane = threadIdx.x & 63;
+  const uint64_t beg = P.inOff[j], sbase = P.incOff[j];
+  constexpr int U = 4;
+  for (uint32_t i0 = lane; i0 < n; i0 += U * kWave) {
+    uint32_t p[U], sy[U];
+    float td[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const uint32_t i = i0 + (uint32_t)u * kWave;
+      p[u] = i < n ? pos[(size_t)(i / kRun) * (kWave * kRun) + (i % kRun)] : 0u;
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u) sy[u] = i0 + (uint32_t)u * kWave < n ? P.inSyn[beg + p[u]] : 0u;
+#pragma unroll
+    for (int u = 0; u < U; ++u) td[u] = i0 + (uint32_t)u * kWave < n ? P.tdT[sbase + sy[u]] : 0.0f;
+#pragma unroll
+    for (int u = 0; u < U; ++u)
+      if (i0 + (uint32_t)u * kWave < n) {
+        P.lastT[sbase + sy[u]] = (int32_t)t;
+        P.tdT[sbase + sy[u]] = td[u] + a;
+      }
+  }
+}
+
 constexpr int kDense = 16;  // fast path: up to kDense * 64 live keys per neuron, kDense per lane in registers
 constexpr int kSlowK = 8;   // register batches (of 64 items) of the general path
 
 // Fast path, rounds one and two: the live items of neuron j's in-list
 // (beg, len), compacted into rec[0, n) in in-list order as
-// item | position << 32; returns n (uniform).  K batches of 64 in-list words
-// are loaded at once, then the bitmap words of their sources (a target's
-// sources are a few neighbouring columns, so these mostly hit the CU's L1).
-// The in-list array is padded by one wave, so the words past len that the
-// last batch loads exist; they are never live.
+// item | position << 32 | synapse << 44 (with plastic, the item's synapse
+// in j's incoming list, for LTD); returns n (uniform).  K batches of 64
+// in-list words are loaded at once, then the bitmap words of their sources
+// (a target's sources are a few neighbouring columns, so these mostly hit
+// the CU's L1).  The in-list arrays are padded by one wave, so the words
+// past len that the last batch loads exist; they are never live.
 template <int K>
-__device__ inline uint32_t compact_live(const DevNet &P, uint32_t t, uint64_t beg, uint32_t len, uint64_t *rec) {
+__device__ inline uint32_t compact_live(const DevNet &P, uint32_t t, uint64_t beg, uint32_t len, uint64_t *rec,
+                                        int plastic) {
   const uint32_t lane = threadIdx.x & 63;
   const uint32_t *bw = (const uint32_t *)P.bits;
   uint32_t n = 0;
   for (uint32_t base = 0; base < len; base += (uint32_t)K * kWave) {
-    uint32_t it[K], wb[K];
+    uint32_t it[K], wb[K], sy[K];
 #pragma unroll
     for (int k = 0; k < K; ++k)
-      if (base + k * kWave < len) it[k] = P.inPD[beg + base + k * kWave + lane];
+      if (base + k * kWave < len) {
+        it[k] = P.inPD[beg + base + k * kWave + lane];
+        sy[k] = plastic ? P.inSyn[beg + base + k * kWave + lane] : 0u;
+      }
 #pragma unroll
     for (int k = 0; k < K; ++k)
       if (base + k * kWave < len)
@@ -762,7 +805,7 @@ __device__ inline uint32_t compact_live(const DevNet &P, uint32_t t, uint64_t be
         const uint64_t m = __ballot(live);
         if (live) {
           const uint32_t below = __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
-          rec[n + below] = (uint64_t)it[k] | ((uint64_t)idx << 32);
+          rec[n + below] = (uint64_t)it[k] | ((uint64_t)idx << 32) | ((uint64_t)sy[k] << 44);
         }
         n += (uint32_t)__popcll(m);
       }
@@ -856,11 +899,14 @@ __host__ __device__ inline size_t phase1_lds(uint32_t cap, int B) {
 // The scratch of a tile of 64 neurons is blocked by runs of kRun events:
 // event i of neuron q at [i / kRun][q][i % kRun] (a neuron's keys fill whole
 // 128-byte lines), and outK/outW point at neuron q's [0][q][0].
-// Fast path (n <= kDense * 64): the live items are compacted into sorted[]
+// Fast path (n <= denseMax): the live items are compacted into sorted[]
 // (compact_live), then each lane takes live items p = lane, lane + 64, ...:
 // emission time and weight loads for delivered events only, keys in
 // registers, bucket counts, scatter.  Otherwise the general path gathers
 // the in-list in register batches of kSlowK * 64 items, twice.
+// With plastic, the step's LTD of neuron j's delivered synapses is applied
+// here too: inline on the fast path, from the stored positions (ltd_neuron)
+// on the general path.
 template <int K, int B>
 __device__ inline uint32_t gather_sort_neuron(const DevNet &P, uint32_t t, uint32_t j, int plastic, uint8_t *lds,
                                               uint32_t cap, uint64_t *outK, int16_t *outW, uint16_t *outP) {
@@ -880,7 +926,7 @@ __device__ inline uint32_t gather_sort_neuron(const DevNet &P, uint32_t t, uint3
   const uint32_t len = (uint32_t)(P.inOff[j + 1] - beg);
   for (uint32_t b = lane; b <= (uint32_t)B; b += kWave) bcnt[b] = 0;
   uint32_t n = 0;
-  if (t > 0) n = compact_live<K>(P, t, beg, len, sorted);
+  if (t > 0) n = compact_live<K>(P, t, beg, len, sorted, plastic);
   __syncthreads();
   const bool fast = n <= P.denseMax;
   uint64_t key[kDense];
@@ -894,17 +940,35 @@ __device__ inline uint32_t gather_sort_neuron(const DevNet &P, uint32_t t, uint3
       }
     double te[kDense];
     int16_t w[kDense];
+    float td[kDense];
 #pragma unroll
     for (int r = 0; r < kDense; ++r)
       if ((uint32_t)r * kWave < n) {
-        const uint32_t it = (uint32_t)rc[r], idx = (uint32_t)(rc[r] >> 32);
+        const uint32_t it = (uint32_t)rc[r], idx = (uint32_t)(rc[r] >> 32) & 0xFFFu;
         te[r] = load_at(P.teHist, te_offset(P, (t - (it >> 24)) & (P.RS - 1), it & 0xFFFFFFu));
         w[r] = P.inW[beg + idx];
       }
+    if (plastic) {
+      // LTD (synapseSetLastActiveTime_antiCausalSTDP): each delivered
+      // synapse gets last = t and td += a; a synapse is delivered at most
+      // once per step (refractory >= 1 ms), so the updates are independent
+      const uint64_t sbase = P.incOff[j];
+      const float a = ltd_increment(P, t, j);
+#pragma unroll
+      for (int r = 0; r < kDense; ++r)
+        if ((uint32_t)r * kWave < n) td[r] = P.tdT[sbase + (uint32_t)(rc[r] >> 44)];
+#pragma unroll
+      for (int r = 0; r < kDense; ++r)
+        if ((uint32_t)r * kWave < n && r * kWave + lane < n) {
+          const uint64_t s = sbase + (uint32_t)(rc[r] >> 44);
+          P.lastT[s] = (int32_t)t;
+          P.tdT[s] = td[r] + a;
+        }
+    }
 #pragma unroll
     for (int r = 0; r < kDense; ++r)
       if ((uint32_t)r * kWave < n) {
-        const uint32_t idx = (uint32_t)(rc[r] >> 32);
+        const uint32_t idx = (uint32_t)(rc[r] >> 32) & 0xFFFu;
         const double kk = tF + (te[r] - trunc(te[r]));  // modf, exact
         const uint64_t tick = dpsnn_asu64(kk) - tBits;
         key[r] = (tick << 12) | idx;
@@ -970,49 +1034,11 @@ __device__ inline uint32_t gather_sort_neuron(const DevNet &P, uint32_t t, uint3
     const size_t at = (size_t)(p / kRun) * (kWave * kRun) + (p % kRun);
     outK[at] = x >> 12;
     outW[at] = ldsW[x & 0xFFFu];
-    if (plastic) outP[at] = (uint16_t)(x & 0xFFFu);
+    if (plastic && !fast) outP[at] = (uint16_t)(x & 0xFFFu);
   }
   __syncthreads();
+  if (plastic && !fast) ltd_neuron(P, t, j, ltd_increment(P, t, j), n, outP);
   return n;
-}
-
-// LTD, synapseSetLastActiveTime_antiCausalSTDP (DPSNN_neuron_sim.c:
-// 207-277): every synapse delivered to neuron j in step t gets last = t and
-// td += anti[t - lastSpike_j], one increment per target, taken before step
-// t's dynamics move lastSpike.
-__device__ inline float ltd_increment(const DevNet &P, uint32_t t, uint32_t j) {
-  const int dt = (int)t - P.lastSpike[j];
-  return dt < kStdpSpan ? P.anti[dt] : 0.0f;
-}
-
-// Applies it to the n synapses whose in-list positions phase 1 left in the
-// blocked scratch (pos = neuron j's [0][q][0]).  A synapse is delivered at
-// most once per step (refractory >= 1 ms), so the updates are independent:
-// rounds of 4 per lane issue the position loads, the synapse-index loads,
-// the value loads, then the stores.  Whole wave.
-__device__ inline void ltd_neuron(const DevNet &P, uint32_t t, uint32_t j, float a, uint32_t n, const uint16_t *pos) {
-  const uint32_t lane = threadIdx.x & 63;
-  const uint64_t beg = P.inOff[j], sbase = P.incOff[j];
-  constexpr int U = 4;
-  for (uint32_t i0 = lane; i0 < n; i0 += U * kWave) {
-    uint32_t p[U], sy[U];
-    float td[U];
-#pragma unroll
-    for (int u = 0; u < U; ++u) {
-      const uint32_t i = i0 + (uint32_t)u * kWave;
-      p[u] = i < n ? pos[(size_t)(i / kRun) * (kWave * kRun) + (i % kRun)] : 0u;
-    }
-#pragma unroll
-    for (int u = 0; u < U; ++u) sy[u] = i0 + (uint32_t)u * kWave < n ? P.inSyn[beg + p[u]] : 0u;
-#pragma unroll
-    for (int u = 0; u < U; ++u) td[u] = i0 + (uint32_t)u * kWave < n ? P.tdT[sbase + sy[u]] : 0.0f;
-#pragma unroll
-    for (int u = 0; u < U; ++u)
-      if (i0 + (uint32_t)u * kWave < n) {
-        P.lastT[sbase + sy[u]] = (int32_t)t;
-        P.tdT[sbase + sy[u]] = td[u] + a;
-      }
-  }
 }
 
 // What the tail of step t does for the scan of step t+1: where its counts
@@ -1047,7 +1073,6 @@ __device__ inline unsigned long long step_tile(const DevNet &P, uint32_t t, cons
   for (uint32_t q = 0; q < m; ++q) {
     const uint32_t n =
         gather_sort_neuron<K, B>(P, t, j0 + q, ltd, lds, cap, wkeys + q * kRun, wW + q * kRun, wP + q * kRun);
-    if (ltd) ltd_neuron(P, t, j0 + q, ltd_increment(P, t, j0 + q), n, wP + q * kRun);
     if (lane == 0) wcnt[q] = n;
     evSum += n;
   }
@@ -1145,10 +1170,7 @@ __global__ __launch_bounds__(kWave) void k_gather(DevNet P, uint32_t t, int ltd)
       const size_t at = (size_t)(j >> 6) * P.cap * kWave + (j & 63) * kRun;  // the tile's blocked scratch
       const uint32_t n = gather_sort_neuron<K, B>(P, t, j, ltd, (uint8_t *)lds, P.cap, P.keys + at, P.keyW + at,
                                                   ltd ? P.keyPos + at : nullptr);
-      if (threadIdx.x == 0) {
-        P.keyCount[j] = n;
-        if (ltd) P.keyA[j] = ltd_increment(P, t, j);
-      }
+      if (threadIdx.x == 0) P.keyCount[j] = n;
       evSum += n;
     }
   if (threadIdx.x == 0) atomicAdd(&P.counters[0], evSum);
@@ -1194,8 +1216,8 @@ __device__ inline uint32_t batch_item_of(const BatchItem *items, uint32_t count,
 template <int K, int B>
 __global__ __launch_bounds__(kWave) void k_batch_gather(const DevNet *nets, const BatchItem *items, uint32_t count,
                                                         uint32_t g0, uint32_t g1, uint32_t cap, uint64_t *keys,
-                                                        int16_t *keyW, uint16_t *keyPos, float *keyA,
-                                                        uint32_t *keyCount, uint32_t *workCtr) {
+                                                        int16_t *keyW, uint16_t *keyPos, uint32_t *keyCount,
+                                                        uint32_t *workCtr) {
   extern __shared__ uint64_t lds[];
   const uint32_t total = (g1 - g0) * kWave, home = blockIdx.x % kXcds;
   uint32_t x = home;
@@ -1211,7 +1233,6 @@ __global__ __launch_bounds__(kWave) void k_batch_gather(const DevNet *nets, cons
                                                 it.ltd ? keyPos + at : nullptr);
     if (threadIdx.x == 0) {
       keyCount[u] = n;
-      if (it.ltd) keyA[u] = ltd_increment(P, it.t, j);
       atomicAdd(&P.counters[0], (unsigned long long)n);
     }
   }
@@ -1263,34 +1284,6 @@ __global__ __launch_bounds__(kWaveLTP) void k_batch_ltp(const DevNet *nets, cons
   }
 }
 
-// LTD of step t, after phase 1 (and beside phase 2, on another stream: it
-// touches only the synapses' last / td and phase 1's positions): one wave
-// per neuron, four waves per block.
-__global__ __launch_bounds__(kBlock) void k_ltd(DevNet P, uint32_t t) {
-  const uint32_t wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
-  const uint32_t waves = (gridDim.x * blockDim.x) >> 6;
-  for (uint32_t j = wave; j < P.N; j += waves) {
-    const size_t at = (size_t)(j >> 6) * P.cap * kWave + (j & 63) * kRun;
-    ltd_neuron(P, t, j, P.keyA[j], P.keyCount[j], P.keyPos + at);
-  }
-}
-
-__global__ __launch_bounds__(kBlock) void k_batch_ltd(const DevNet *nets, const BatchItem *items, uint32_t count,
-                                                      uint32_t g0, uint32_t g1, uint32_t cap, const uint16_t *keyPos,
-                                                      const float *keyA, const uint32_t *keyCount) {
-  const uint32_t wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
-  const uint32_t waves = (gridDim.x * blockDim.x) >> 6;
-  for (uint32_t u = wave; u < (g1 - g0) * kWave; u += waves) {
-    const uint32_t g = g0 + u / kWave, q = u % kWave;
-    const BatchItem &it = items[batch_item_of(items, count, g)];
-    if (!it.ltd) continue;
-    const DevNet &P = nets[it.net];
-    const uint32_t j = (g - it.tileBegin) * kWave + q;
-    if (j >= P.N) continue;
-    ltd_neuron(P, it.t, j, keyA[u], keyCount[u], keyPos + (size_t)(u / kWave) * cap * kWave + q * kRun);
-  }
-}
-
 // Fills n doubles with v.
 __global__ void k_fill_f64(double *p, double v, uint64_t n) {
   for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
@@ -1305,7 +1298,7 @@ using GatherFn = void (*)(DevNet, uint32_t, int);
 using BatchFn = void (*)(const DevNet *, const BatchItem *, uint32_t, uint32_t, uint32_t, uint64_t *, int16_t *,
                          uint16_t *, uint32_t *);
 using BatchGatherFn = void (*)(const DevNet *, const BatchItem *, uint32_t, uint32_t, uint32_t, uint32_t, uint64_t *,
-                               int16_t *, uint16_t *, float *, uint32_t *, uint32_t *);
+                               int16_t *, uint16_t *, uint32_t *, uint32_t *);
 struct KernelShape {
   int K, B;
   StepFn step;
@@ -1396,8 +1389,6 @@ struct dpsnn_net {
   size_t ldsBytes = 0;
   bool stdp = false;         // plasticity runs at some step of [0, total)
   bool uploaded = false;     // build_device done (dpsnn_upload)
-  hipStream_t side = nullptr;            // STDP: LTD beside phase 2
-  hipEvent_t evPhase1 = nullptr, evLtd = nullptr;
   std::string outDir;        // owned copy of opt.output_dir
   uint32_t stdpBlocks = 0;
   int shape = 0;             // kShapes index
@@ -1448,12 +1439,6 @@ static void free_all(dpsnn_net *n) {
     hipEventDestroy(e.second);
   }
   n->profEv.clear();
-  if (n->side) (void)hipStreamSynchronize(n->side);
-  if (n->evPhase1) (void)hipEventDestroy(n->evPhase1);
-  if (n->evLtd) (void)hipEventDestroy(n->evLtd);
-  if (n->side) (void)hipStreamDestroy(n->side);
-  n->side = nullptr;
-  n->evPhase1 = n->evLtd = nullptr;
   if (n->stream) hipStreamDestroy(n->stream);
   n->stream = nullptr;
 }
@@ -1560,9 +1545,10 @@ static int build_device(dpsnn_net *n) {
     if ((rc = dev_alloc(n, &d_incOff, N + 1)) || (rc = upload(n, d_incOff, c.inc_off.data(), N + 1))) return rc;
     if ((rc = dev_alloc(n, &P.wT, S)) || (rc = upload(n, P.wT, wT.data(), S))) return rc;
     if ((rc = dev_alloc(n, &d_kind, S)) || (rc = upload(n, d_kind, kindT.data(), S))) return rc;
-    if ((rc = dev_alloc(n, &d_inSyn, c.in_off[N])) ||
+    if ((rc = dev_alloc(n, &d_inSyn, c.in_off[N] + kWave)) ||  // padded like the in-list words
         (rc = upload(n, d_inSyn, n->inSynLocal.data(), c.in_off[N])))
       return rc;
+    HIP_TRY(hipMemsetAsync(d_inSyn + c.in_off[N], 0, sizeof(uint16_t) * kWave, n->stream));
     if ((rc = dev_alloc(n, &P.tdT, S)) || (rc = dev_alloc(n, &P.lastT, S))) return rc;
     HIP_TRY(hipMemsetAsync(P.tdT, 0, 4 * S, n->stream));
     HIP_TRY(hipMemsetAsync(P.lastT, 0, 4 * S, n->stream));
@@ -1595,9 +1581,6 @@ static int build_device(dpsnn_net *n) {
     P.decayReset = q.tauDerivativeDecay_s == 1.0 ? 1 : 0;
     P.decayMul = (1 - 1 / q.tauDerivativeDecay_s);
     HIP_TRY(hipStreamSynchronize(n->stream));
-    HIP_TRY(hipStreamCreateWithFlags(&n->side, hipStreamNonBlocking));
-    HIP_TRY(hipEventCreateWithFlags(&n->evPhase1, hipEventDisableTiming));
-    HIP_TRY(hipEventCreateWithFlags(&n->evLtd, hipEventDisableTiming));
   }
   // launch shapes.  The split form (k_gather, one wave per neuron, then
   // k_integrate) is the default; the fused k_step (each wave whole tiles of
@@ -1841,7 +1824,7 @@ static int ensure_scratch(dpsnn_net *n) {
   if ((rc = dev_alloc(n, &P.keys, slots * P.cap)) || (rc = dev_alloc(n, &P.keyW, slots * P.cap)) ||
       (rc = dev_alloc(n, &P.keyCount, slots)))
     return rc;
-  if (n->stdp && ((rc = dev_alloc(n, &P.keyPos, slots * P.cap)) || (rc = dev_alloc(n, &P.keyA, slots)))) return rc;
+  if (n->stdp && (rc = dev_alloc(n, &P.keyPos, slots * P.cap))) return rc;
   return 0;
 }
 
@@ -1933,15 +1916,8 @@ static int one_step(dpsnn_net *n) {
   const KernelShape &K = kShapes[n->shape];
   if (n->split) {
     hipLaunchKernelGGL(K.gather, dim3(n->gatherGrid), dim3(kWave), n->ldsBytes, n->stream, P, sp.t, sp.ltd);
-    if (sp.ltd) {  // LTD on the side stream, beside phase 2
-      HIP_TRY(hipEventRecord(n->evPhase1, n->stream));
-      HIP_TRY(hipStreamWaitEvent(n->side, n->evPhase1, 0));
-      hipLaunchKernelGGL(k_ltd, dim3(n->stdpBlocks), dim3(kBlock), 0, n->side, P, sp.t);
-      HIP_TRY(hipEventRecord(n->evLtd, n->side));
-    }
     const uint32_t blocks = (P.N + kBlock - 1) / kBlock;
     hipLaunchKernelGGL(k_integrate, dim3(blocks), dim3(kBlock), 0, n->stream, P, sp.t, sp.X, sp.S);
-    if (sp.ltd) HIP_TRY(hipStreamWaitEvent(n->stream, n->evLtd, 0));
     if (sp.S.ltp) hipLaunchKernelGGL(k_ltp, dim3(n->stdpBlocks), dim3(kWaveLTP), 0, n->stream, P, sp.t + 1);
   } else {
     hipLaunchKernelGGL(K.step, dim3(n->grid), dim3(kWave), n->ldsBytes, n->stream, P, sp.t, sp.X, sp.ltd, sp.S);
@@ -2256,11 +2232,8 @@ struct dpsnn_batch {
   uint64_t *keys = nullptr;
   int16_t *keyW = nullptr;
   uint16_t *keyPos = nullptr;
-  float *keyA = nullptr;
   uint32_t *keyCount = nullptr;
   uint32_t *workCtr = nullptr;  // [kXcds], zero between launches
-  hipStream_t side = nullptr;   // LTD beside phase 2
-  hipEvent_t evPhase1 = nullptr, evLtd = nullptr;
   size_t scratchSlots = 0;
   uint32_t scratchCap = 0;
   uint32_t totalTiles = 0;   // of all members
@@ -2353,12 +2326,10 @@ int batch_prepare(dpsnn_batch *b) {
     if (b->keys) HIP_TRY(hipFree(b->keys));
     if (b->keyW) HIP_TRY(hipFree(b->keyW));
     if (b->keyPos) HIP_TRY(hipFree(b->keyPos));
-    if (b->keyA) HIP_TRY(hipFree(b->keyA));
     if (b->keyCount) HIP_TRY(hipFree(b->keyCount));
     b->keys = nullptr;
     b->keyW = nullptr;
     b->keyPos = nullptr;
-    b->keyA = nullptr;
     b->keyCount = nullptr;
     hipError_t e = hipMalloc((void **)&b->keys, sizeof(uint64_t) * slots * cap);
     if (e != hipSuccess) return fail(DPSNN_ERR_MEMORY, "batch key scratch: %s", hipGetErrorString(e));
@@ -2366,7 +2337,6 @@ int batch_prepare(dpsnn_batch *b) {
     if (e != hipSuccess) return fail(DPSNN_ERR_MEMORY, "batch weight scratch: %s", hipGetErrorString(e));
     e = hipMalloc((void **)&b->keyPos, sizeof(uint16_t) * slots * cap);
     if (e != hipSuccess) return fail(DPSNN_ERR_MEMORY, "batch position scratch: %s", hipGetErrorString(e));
-    HIP_TRY(hipMalloc((void **)&b->keyA, sizeof(float) * slots));
     HIP_TRY(hipMalloc((void **)&b->keyCount, sizeof(uint32_t) * slots));
     b->scratchCap = cap;
     b->scratchSlots = slots;
@@ -2405,19 +2375,10 @@ int batch_launch(dpsnn_batch *b, int r, uint32_t count, uint32_t tiles) {
                              (uint32_t)b->cus * std::min(batch_waves(b, lds), resident_waves((const void *)K.batchGather, lds))),
           kXcds);
       hipLaunchKernelGGL(K.batchGather, dim3(grid), dim3(kWave), lds, b->stream, b->d_nets, b->d_items[r], count, g0,
-                         g1, cap, b->keys, b->keyW, b->keyPos, b->keyA, b->keyCount, b->workCtr);
-      if (b->anyPlastic) {  // LTD on the side stream, beside phase 2
-        HIP_TRY(hipEventRecord(b->evPhase1, b->stream));
-        HIP_TRY(hipStreamWaitEvent(b->side, b->evPhase1, 0));
-        hipLaunchKernelGGL(k_batch_ltd, dim3(std::min<uint32_t>((g1 - g0) * (kWave / (kBlock / kWave)), b->cus * 16u)),
-                           dim3(kBlock), 0, b->side, b->d_nets, b->d_items[r], count, g0, g1, cap, b->keyPos, b->keyA,
-                           b->keyCount);
-        HIP_TRY(hipEventRecord(b->evLtd, b->side));
-      }
+                         g1, cap, b->keys, b->keyW, b->keyPos, b->keyCount, b->workCtr);
       const uint32_t blocks = (g1 - g0 + kBlock / kWave - 1) / (kBlock / kWave);
       hipLaunchKernelGGL(k_batch_integrate, dim3(blocks), dim3(kBlock), 0, b->stream, b->d_nets, b->d_items[r], count,
                          g0, g1, cap, b->keys, b->keyW, b->keyCount, b->workCtr);
-      if (b->anyPlastic) HIP_TRY(hipStreamWaitEvent(b->stream, b->evLtd, 0));
       if (b->anyPlastic)
         hipLaunchKernelGGL(k_batch_ltp, dim3(std::min<uint32_t>((g1 - g0) * (kWave * kWave / kWaveLTP), b->cus * 16u)),
                            dim3(kWaveLTP), 0, b->stream, b->d_nets, b->d_items[r], count, g0, g1);
@@ -2543,9 +2504,6 @@ int dpsnn_batch_create(int device, dpsnn_batch **out) {
   b->cus = std::max(b->cus, 1);
   HIP_TRY(hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking));
   for (int r = 0; r < dpsnn_batch::kRing; ++r) HIP_TRY(hipEventCreateWithFlags(&b->done[r], hipEventDisableTiming));
-  HIP_TRY(hipStreamCreateWithFlags(&b->side, hipStreamNonBlocking));
-  HIP_TRY(hipEventCreateWithFlags(&b->evPhase1, hipEventDisableTiming));
-  HIP_TRY(hipEventCreateWithFlags(&b->evLtd, hipEventDisableTiming));
   HIP_TRY(hipMalloc((void **)&b->workCtr, sizeof(uint32_t) * kXcds * kCtrStride));
   HIP_TRY(hipMemset(b->workCtr, 0, sizeof(uint32_t) * kXcds * kCtrStride));
   if (const char *e = std::getenv("DPSNN_GRAPH")) b->useGraph = std::atoi(e) != 0;
@@ -2709,12 +2667,7 @@ void dpsnn_batch_destroy(dpsnn_batch *b) {
   if (b->keyW) hipFree(b->keyW);
   if (b->keyCount) (void)hipFree(b->keyCount);
   if (b->keyPos) (void)hipFree(b->keyPos);
-  if (b->keyA) (void)hipFree(b->keyA);
   if (b->workCtr) (void)hipFree(b->workCtr);
-  (void)hipStreamSynchronize(b->side);
-  (void)hipEventDestroy(b->evPhase1);
-  (void)hipEventDestroy(b->evLtd);
-  (void)hipStreamDestroy(b->side);
   for (int r = 0; r < dpsnn_batch::kRing; ++r)
     if (b->done[r]) hipEventDestroy(b->done[r]);
   for (auto &e : b->ev) {
