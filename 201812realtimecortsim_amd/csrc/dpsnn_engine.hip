@@ -272,7 +272,7 @@ struct Lifca {
 // written by exactly one wave, so the slot needs no clearing.  Whole wave,
 // uniform.
 // --------------------------------------------------------------------------
-__device__ inline uint64_t publish_spikes(const DevNet &P, uint32_t t1, uint32_t j, bool f, double te,
+__device__ __attribute__((always_inline)) inline uint64_t publish_spikes(const DevNet &P, uint32_t t1, uint32_t j, bool f, double te,
                                           uint32_t *popCnt, int count) {
   const uint64_t mask = __ballot(f);
   const uint32_t slot = t1 & (P.RS - 1);
@@ -295,7 +295,7 @@ __device__ inline uint64_t publish_spikes(const DevNet &P, uint32_t t1, uint32_t
 // timeDerivative += causal[t - 1 - last] (0 past the 128 ms table).  One
 // wave per firing neuron, its incoming synapses contiguous.
 // --------------------------------------------------------------------------
-__device__ inline void ltp_neuron(const DevNet &P, uint32_t t, uint32_t j) {
+__device__ __attribute__((always_inline)) inline void ltp_neuron(const DevNet &P, uint32_t t, uint32_t j) {
   const uint32_t lane = threadIdx.x & 63;
   const uint64_t end = P.incOff[j + 1];
   constexpr int U = 4;  // independent synapses per lane per round: loads first, then stores
@@ -590,7 +590,7 @@ __device__ inline void store_chunk(uint64_t *stageK, const uint64_t (&rk)[kChunk
 // (stage, kStageBytes) a chunk ahead.  Returns the lane's external event
 // count and whether (and when) its neuron fired.  Whole wave, uniform.
 template <bool REPLAY>
-__device__ inline uint32_t integrate_wave(const DevNet &P, uint32_t t, const ExtView &X, uint32_t j0, uint32_t m,
+__device__ __attribute__((always_inline)) inline uint32_t integrate_wave(const DevNet &P, uint32_t t, const ExtView &X, uint32_t j0, uint32_t m,
                                           const uint64_t *tileK, const int16_t *tileW, uint32_t n, uint8_t *stage,
                                           bool &fired, double &te) {
   const uint32_t lane = threadIdx.x & 63;
@@ -743,7 +743,7 @@ __device__ inline float ltd_increment(const DevNet &P, uint32_t t, uint32_t j) {
 // most once per step (refractory >= 1 ms), so the updates are independent:
 // rounds of 4 per lane issue the position loads, the synapse-index loads,
 // the value loads, then the stores.  Whole wave.
-__device__ inline void ltd_neuron(const DevNet &P, uint32_t t, uint32_t j, float a, uint32_t n, const uint16_t *pos) {
+__device__ __attribute__((always_inline)) inline void ltd_neuron(const DevNet &P, uint32_t t, uint32_t j, float a, uint32_t n, const uint16_t *pos) {
   const uint32_t lane = threadIdx.x & 63;
   const uint64_t beg = P.inOff[j], sbase = P.incOff[j];
   constexpr int U = 4;
@@ -780,7 +780,7 @@ constexpr int kSlowK = 8;   // register batches (of 64 items) of the general pat
 // the CU's L1).  The in-list arrays are padded by one wave, so the words
 // past len that the last batch loads exist; they are never live.
 template <int K>
-__device__ inline uint32_t compact_live(const DevNet &P, uint32_t t, uint64_t beg, uint32_t len, uint64_t *rec,
+__device__ __attribute__((always_inline)) inline uint32_t compact_live(const DevNet &P, uint32_t t, uint64_t beg, uint32_t len, uint64_t *rec,
                                         int plastic) {
   const uint32_t lane = threadIdx.x & 63;
   const uint32_t *bw = (const uint32_t *)P.bits;
@@ -825,7 +825,7 @@ __device__ inline uint32_t compact_live(const DevNet &P, uint32_t t, uint64_t be
 // writes both), so the emission-time loads go to L2 only for delivered
 // events.
 template <int K, int B>
-__device__ inline uint32_t gather_keys(const DevNet &P, uint32_t t, uint32_t j, uint64_t beg, uint32_t len,
+__device__ __attribute__((always_inline)) inline uint32_t gather_keys(const DevNet &P, uint32_t t, uint32_t j, uint64_t beg, uint32_t len,
                                        uint32_t base, uint64_t (&key)[K], uint32_t *bcnt, int16_t *ldsW,
                                        int shift, uint64_t tBits, bool first, int plastic) {
   static_assert(K <= 32, "live items are kept in one 32-bit mask");
@@ -880,7 +880,7 @@ __device__ inline uint32_t gather_keys(const DevNet &P, uint32_t t, uint32_t j, 
 }
 
 template <int K, int B>
-__device__ inline void scatter_keys(const uint64_t (&key)[K], uint32_t *bcnt, uint64_t *sorted, int shift) {
+__device__ __attribute__((always_inline)) inline void scatter_keys(const uint64_t (&key)[K], uint32_t *bcnt, uint64_t *sorted, int shift) {
 #pragma unroll
   for (int k = 0; k < K; ++k)
     if (key[k] != kNoKey) sorted[atomicAdd(&bcnt[key_bucket<B>(key[k] >> 12, shift)], 1u)] = key[k];
@@ -908,7 +908,7 @@ __host__ __device__ inline size_t phase1_lds(uint32_t cap, int B) {
 // here too: inline on the fast path, from the stored positions (ltd_neuron)
 // on the general path.
 template <int K, int B>
-__device__ inline uint32_t gather_sort_neuron(const DevNet &P, uint32_t t, uint32_t j, int plastic, uint8_t *lds,
+__device__ __attribute__((always_inline)) inline uint32_t gather_sort_neuron(const DevNet &P, uint32_t t, uint32_t j, int plastic, uint8_t *lds,
                                               uint32_t cap, uint64_t *outK, int16_t *outW, uint16_t *outP) {
   static_assert(B % kWave == 0, "bucket count must be a multiple of 64");
   constexpr int R = B / kWave;  // buckets per lane
@@ -1062,7 +1062,7 @@ __host__ __device__ inline size_t step_lds(uint32_t cap, int B) {
 // tile's delivered events (uniform) and adds external events to *exSum (per
 // lane).
 template <int K, int B>
-__device__ inline unsigned long long step_tile(const DevNet &P, uint32_t t, const ExtView &X, int ltd,
+__device__ __attribute__((always_inline)) inline unsigned long long step_tile(const DevNet &P, uint32_t t, const ExtView &X, int ltd,
                                                const NextScan &S, uint32_t tile, uint32_t cap, uint8_t *lds,
                                                uint64_t *wkeys, int16_t *wW, uint16_t *wP, uint32_t *wcnt,
                                                unsigned long long &exSum) {
