@@ -558,6 +558,11 @@ struct ExtQueuedCursor {
 };
 
 constexpr int kChunk = 8;  // internal events per lane per LDS-staged chunk
+// From step kPackT on, a scratch entry carries its weight in the low 16 bits
+// (tick << 16 | weight): for t >= 32 every tick is at most 2^47 (t + frac
+// stays within [t, t + 1] and 2^(52 - floor(log2 t)) <= 2^47).  Earlier
+// steps keep the weight in its own array.
+constexpr uint32_t kPackT = 32;
 constexpr int kRun = 16;   // events per neuron run in the blocked scratch
 // LDS of one wave in phase 2: two chunks of 64 lanes' ticks, then the bath
 // queue (times, values: [kQueue][64] doubles each)
@@ -570,12 +575,12 @@ constexpr size_t kStageBytes = 2 * kChunk * kWave * sizeof(uint64_t) + 2 * kQueu
 // one 16-byte load.  Entries past a neuron's count are never consumed.
 static_assert(kChunk == 8 && kRun % kChunk == 0, "chunks are whole pieces of a run");
 __device__ inline void load_chunk(const uint64_t *tileK, const int16_t *tileW, uint32_t c, uint64_t (&rk)[kChunk],
-                                  uint4 &rw) {
+                                  uint4 &rw, bool packed) {
   const uint32_t lane = threadIdx.x & 63;
   const size_t blk = (size_t)(c * kChunk / kRun) * (kWave * kRun) + (c * kChunk) % kRun;
 #pragma unroll
   for (int r = 0; r < kChunk; ++r) rk[r] = tileK[blk + (size_t)(r * 8 + (lane >> 3)) * kRun + (lane & 7)];
-  rw = *(const uint4 *)(tileW + blk + (size_t)lane * kRun);
+  if (!packed) rw = *(const uint4 *)(tileW + blk + (size_t)lane * kRun);
 }
 __device__ inline void store_chunk(uint64_t *stageK, const uint64_t (&rk)[kChunk]) {
   const uint32_t lane = threadIdx.x & 63, e = lane & 7;
@@ -644,29 +649,31 @@ __device__ __attribute__((always_inline)) inline uint32_t integrate_wave(const D
   uint64_t *stageK = (uint64_t *)stage;  // [2][kChunk][64]
   uint64_t rk[kChunk];
   uint4 wCur = make_uint4(0, 0, 0, 0), wNext = wCur;
+  const bool packed = t >= kPackT;
   if (chunks) {
-    load_chunk(tileK, tileW, 0, rk, wCur);
+    load_chunk(tileK, tileW, 0, rk, wCur, packed);
     store_chunk(stageK, rk);
     wave_sync();
   }
   for (uint32_t c = 0; c < iters; ++c) {
     const uint64_t *curK = stageK + (c & 1) * (kChunk * kWave);
     const bool more = c + 1 < chunks;
-    if (more) load_chunk(tileK, tileW, c + 1, rk, wNext);
+    if (more) load_chunk(tileK, tileW, c + 1, rk, wNext, packed);
     uint64_t wlo = (uint64_t)wCur.x | ((uint64_t)wCur.y << 32), whi = (uint64_t)wCur.z | ((uint64_t)wCur.w << 32);
     const uint32_t left = n > c * kChunk ? min((uint32_t)kChunk, n - c * kChunk) : 0u;
     const bool last = c * kChunk + left >= n;
     uint32_t e = 0;
     for (;;) {
       const bool hasI = e < left;
-      const double kI = hasI ? dpsnn_asf64(tBits + curK[e * kWave + lane]) : 0.0;
+      const uint64_t kv = hasI ? curK[e * kWave + lane] : 0ull;
+      const double kI = dpsnn_asf64(tBits + (packed ? kv >> 16 : kv));
       const bool eDue = REPLAY ? xr.due() : xq.due();
       double at, val;
       if (eDue && (hasI ? (REPLAY ? xr.at() : xq.at()) < kI : last)) {
         at = REPLAY ? xr.at() : xq.at();
         val = REPLAY ? xr.take() : xq.take();
       } else if (hasI) {
-        const int16_t w16 = (int16_t)(uint16_t)wlo;
+        const int16_t w16 = (int16_t)(uint16_t)(packed ? kv : wlo);
         wlo = (wlo >> 16) | (whi << 48);
         whi >>= 16;
         at = kI;
@@ -1032,8 +1039,13 @@ __device__ __attribute__((always_inline)) inline uint32_t gather_sort_neuron(con
     for (; y < hi; ++y) r += sorted[y] < x ? 1u : 0u;
     const uint32_t p = lo + r;
     const size_t at = (size_t)(p / kRun) * (kWave * kRun) + (p % kRun);
-    outK[at] = x >> 12;
-    outW[at] = ldsW[x & 0xFFFu];
+    const int16_t wv = ldsW[x & 0xFFFu];
+    if (t >= kPackT) {
+      outK[at] = ((x >> 12) << 16) | (uint16_t)wv;
+    } else {
+      outK[at] = x >> 12;
+      outW[at] = wv;
+    }
     if (plastic && !fast) outP[at] = (uint16_t)(x & 0xFFFu);
   }
   __syncthreads();
