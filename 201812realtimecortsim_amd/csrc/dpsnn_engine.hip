@@ -663,9 +663,11 @@ __device__ __attribute__((always_inline)) inline uint32_t integrate_wave(const D
     const uint32_t left = n > c * kChunk ? min((uint32_t)kChunk, n - c * kChunk) : 0u;
     const bool last = c * kChunk + left >= n;
     uint32_t e = 0;
+    // the next internal event's entry is read as soon as the previous one is
+    // taken, so that the LDS read overlaps the event's dynamics
+    uint64_t kv = left ? curK[lane] : 0ull;
     for (;;) {
       const bool hasI = e < left;
-      const uint64_t kv = hasI ? curK[e * kWave + lane] : 0ull;
       const double kI = dpsnn_asf64(tBits + (packed ? kv >> 16 : kv));
       const bool eDue = REPLAY ? xr.due() : xq.due();
       double at, val;
@@ -679,6 +681,7 @@ __device__ __attribute__((always_inline)) inline uint32_t integrate_wave(const D
         at = kI;
         val = (double)(((float)w16) * P.w2f);
         ++e;
+        if (e < left) kv = curK[e * kWave + lane];
       } else {
         break;
       }
