@@ -2,15 +2,16 @@
 // (continuous batching over the engine's C ABI).
 //
 // Builder threads turn submitted jobs into host-side networks
-// (dpsnn_prepare: parameters, generator, delivery layout).  One scheduler
-// thread makes every device call: it admits built jobs first-fit in arrival
-// order while their exact device footprint (dpsnn_device_bytes_needed) fits
-// the HBM budget, uploads them and adds them to its dpsnn_batch, steps the
-// batch a few ms per round (all running jobs in the same launches, each at
-// its own ms), and retires the jobs that reached their
-// env_totalSimTime_ms: they leave the batch, their files are drained, their
-// memory returns to the budget.  Jobs arriving while others run join the
-// batch at the next round.
+// (dpsnn_prepare: parameters, generator, delivery layout).  The scheduler
+// thread admits built jobs first-fit in arrival order while their exact
+// device footprint (dpsnn_device_bytes_needed) fits the HBM budget, and an
+// uploader thread copies each admitted job to the device (dpsnn_upload, on
+// the job's own stream) while the batch keeps stepping.  The scheduler adds
+// uploaded jobs to its dpsnn_batch, steps the batch a few ms per round (all
+// running jobs in the same launches, each at its own ms), and retires the
+// jobs that reached their env_totalSimTime_ms: they leave the batch, their
+// files are drained, their memory returns to the budget.  Jobs arriving
+// while others run join the batch at the next round.
 #include <algorithm>
 #include <atomic>
 #include <chrono>
@@ -50,15 +51,17 @@ struct dpsnn_server {
   dpsnn_server_options o{};
   Clock::time_point t0;
   std::mutex mu;
-  std::condition_variable cvBuild, cvSched, cvDone;
-  std::deque<Job *> buildQ, waiting;
+  std::condition_variable cvBuild, cvSched, cvDone, cvUpload;
+  std::deque<Job *> buildQ, waiting, uploadQ, ready;  // ready: uploaded, not yet in the batch
   std::vector<Job *> running;
   std::map<uint64_t, std::unique_ptr<Job>> jobs;
   std::vector<std::thread> builders;
-  std::thread sched;
+  std::thread sched, uploader;
   bool stop = false;
   uint64_t nextId = 1, used = 0, budget = 0;
   uint64_t eventsDone = 0, eventsRunning = 0;
+  uint32_t uploading = 0;                 // admitted, in uploadQ or being uploaded
+  double stepSeconds = 0, uploadSeconds = 0;
   uint32_t hostThreads = 0;
   dpsnn_batch *batch = nullptr;
 
@@ -116,30 +119,66 @@ void builder_loop(dpsnn_server *s) {
   }
 }
 
+// Uploads admitted jobs one at a time, beside the scheduler's steps.
+void uploader_loop(dpsnn_server *s) {
+  for (;;) {
+    Job *j = nullptr;
+    {
+      std::unique_lock<std::mutex> lk(s->mu);
+      s->cvUpload.wait(lk, [&] { return s->stop || !s->uploadQ.empty(); });
+      if (s->stop) return;
+      j = s->uploadQ.front();
+      s->uploadQ.pop_front();
+    }
+    const Clock::time_point t0 = Clock::now();
+    const int rc = dpsnn_upload(j->net);
+    const std::string err = rc ? dpsnn_last_error() : "";
+    if (rc) dpsnn_destroy(j->net);
+    std::lock_guard<std::mutex> lk(s->mu);
+    s->uploadSeconds += std::chrono::duration<double>(Clock::now() - t0).count();
+    s->uploading--;
+    if (rc) {
+      j->net = nullptr;
+      s->used -= j->info.device_bytes;
+      finish_locked(s, j, DPSNN_JOB_FAILED, rc, err);
+    } else {
+      s->ready.push_back(j);
+    }
+    s->cvSched.notify_one();
+  }
+}
+
 void scheduler_loop(dpsnn_server *s) {
   const uint32_t round = s->o.steps_per_round ? s->o.steps_per_round : 10;
   for (;;) {
     std::vector<Job *> admit;
     {
       std::unique_lock<std::mutex> lk(s->mu);
-      s->cvSched.wait(lk, [&] { return s->stop || !s->waiting.empty() || !s->running.empty(); });
+      // with nothing running and uploads pending, wait for them rather than spin
+      s->cvSched.wait(lk, [&] {
+        return s->stop || !s->ready.empty() || !s->running.empty() || (!s->waiting.empty() && !s->uploading);
+      });
       if (s->stop) return;
-      // first fit in arrival order
+      // first fit in arrival order; admitted jobs go to the uploader
       for (auto it = s->waiting.begin(); it != s->waiting.end();) {
         Job *j = *it;
-        const bool room = !s->o.max_running || s->running.size() + admit.size() < s->o.max_running;
+        const bool room =
+            !s->o.max_running || s->running.size() + s->uploading + s->ready.size() < s->o.max_running;
         if (room && s->used + j->info.device_bytes <= s->budget) {
           s->used += j->info.device_bytes;
-          admit.push_back(j);
+          s->uploadQ.push_back(j);
+          s->uploading++;
           it = s->waiting.erase(it);
+          s->cvUpload.notify_one();
         } else {
           ++it;
         }
       }
+      admit.assign(s->ready.begin(), s->ready.end());
+      s->ready.clear();
     }
     for (Job *j : admit) {
-      int rc = dpsnn_upload(j->net);
-      if (!rc) rc = dpsnn_batch_add(s->batch, j->net);
+      const int rc = dpsnn_batch_add(s->batch, j->net);
       if (rc) {
         const std::string err = dpsnn_last_error();
         dpsnn_destroy(j->net);
@@ -155,7 +194,9 @@ void scheduler_loop(dpsnn_server *s) {
       s->running.push_back(j);
     }
     if (s->running.empty()) continue;
+    const Clock::time_point t0 = Clock::now();
     const int rc = dpsnn_batch_step(s->batch, round);
+    const double dt = std::chrono::duration<double>(Clock::now() - t0).count();
     const std::string err = rc < 0 ? dpsnn_last_error() : "";
     std::vector<Job *> done;
     uint64_t events = 0;
@@ -185,6 +226,7 @@ void scheduler_loop(dpsnn_server *s) {
     }
     std::lock_guard<std::mutex> lk(s->mu);
     s->eventsRunning = events;
+    s->stepSeconds += dt;
   }
 }
 
@@ -213,6 +255,7 @@ int dpsnn_server_create(const dpsnn_server_options *opt, dpsnn_server **out) {
   dpsnn_server *p = s.get();
   for (uint32_t i = 0; i < nb; ++i) s->builders.emplace_back(builder_loop, p);
   s->sched = std::thread(scheduler_loop, p);
+  s->uploader = std::thread(uploader_loop, p);
   *out = s.release();
   return 0;
 }
@@ -308,6 +351,8 @@ int dpsnn_server_get_stats(dpsnn_server *s, dpsnn_server_stats *st) {
   st->synaptic_events = s->eventsDone + s->eventsRunning;
   st->device_bytes = s->used + kBatchReserve;
   st->elapsed_s = s->now();
+  st->step_s = s->stepSeconds;
+  st->upload_s = s->uploadSeconds;
   return 0;
 }
 
@@ -319,8 +364,10 @@ void dpsnn_server_destroy(dpsnn_server *s) {
   }
   s->cvBuild.notify_all();
   s->cvSched.notify_all();
+  s->cvUpload.notify_all();
   for (auto &t : s->builders) t.join();
   if (s->sched.joinable()) s->sched.join();
+  if (s->uploader.joinable()) s->uploader.join();
   dpsnn_batch_destroy(s->batch);
   for (auto &kv : s->jobs)
     if (kv.second->net) dpsnn_destroy(kv.second->net);

@@ -193,7 +193,8 @@ def run_server(pkg, jobs, dirs, envs, args, max_running):
             "events": st["synaptic_events"], "elapsed_s": st["elapsed_s"],
             "events_per_s": st["synaptic_events"] / st["elapsed_s"],
             "latency_p50_s": pct(50), "latency_p99_s": pct(99),
-            "finished_sim_ms": sum(i["stats"]["steps"] for i in done)}
+            "finished_sim_ms": sum(i["stats"]["steps"] for i in done),
+            "step_s": st["step_s"], "upload_s": st["upload_s"]}
 
 
 def bench_serve(pkg, args):

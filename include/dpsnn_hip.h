@@ -244,6 +244,8 @@ typedef struct {
   uint64_t synaptic_events;   /* delivered by all jobs so far */
   uint64_t device_bytes;      /* held by admitted jobs + batch scratch */
   double elapsed_s;           /* server clock */
+  double step_s;              /* scheduler time in batched steps */
+  double upload_s;            /* uploader time copying admitted jobs (beside the steps) */
 } dpsnn_server_stats;
 int dpsnn_server_create(const dpsnn_server_options *opt, dpsnn_server **out);
 /* Queues a job: the network of `inidir` with options `opt` (copied; its
