@@ -94,26 +94,37 @@ __device__ inline XcdWalk xcd_walk(uint32_t total) {
 // waves take the items in order from the XCD's counter, so the items in
 // flight on one XCD stay a narrow window (neighbouring neurons, shared
 // sources, one L2).  An XCD whose eighth is exhausted helps the next ones.
-// A wave takes kGrab consecutive items per counter update; next_item
-// returns the first of them and their end (*last), or `total` when none is
-// left.  Whole wave; ctr holds kXcds counters that are zero at the launch.
-// Each counter sits on its own 128-byte line (ctr[x * kCtrStride]); an
-// exhausted counter is recognised by a plain read before any atomic.
-constexpr uint32_t kGrab = 4;
+// A wave takes a run of consecutive items per counter update (guided: a
+// share of what is left of the range for each of the XCD's waves, between
+// kGrabMin and kGrabMax, so that counter updates are few while much is left
+// and the tail stays balanced); next_item returns the first item and the
+// run's end (*last), or `total` when none is left.  Whole wave; ctr holds
+// kXcds counters that are zero at the launch.  Each counter sits on its own
+// 128-byte line (ctr[x * kCtrStride]); an exhausted counter is recognised by
+// a plain read before any atomic.  The counters live in device-coherent
+// memory, where each update costs a round trip: with runs of 4, configs[1]'s
+// phase 1 was 9% slower than with the guided runs, with runs of 1 twice as
+// slow.
+constexpr uint32_t kGrabMin = 4, kGrabMax = 16;
 constexpr uint32_t kCtrStride = 32;
 __device__ inline uint32_t next_item(uint32_t *ctr, uint32_t total, uint32_t &x, uint32_t home, uint32_t *last) {
+  const uint32_t wavesPerXcd = max(1u, gridDim.x * (blockDim.x / 64u) / kXcds);
   for (;;) {
     const uint32_t begin = (uint32_t)((uint64_t)total * x / kXcds);
     const uint32_t end = (uint32_t)((uint64_t)total * (x + 1) / kXcds);
     uint32_t *c = ctr + x * kCtrStride;
-    uint32_t k = end - begin;
+    uint32_t k = end - begin, g = kGrabMin;
     if ((threadIdx.x & 63) == 0) {
       k = __hip_atomic_load(c, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      if (begin + k < end) k = atomicAdd(c, kGrab);
+      if (begin + k < end) {
+        g = min(kGrabMax, max(kGrabMin, (end - begin - k) / (2u * wavesPerXcd)));
+        k = atomicAdd(c, g);
+      }
     }
-    k = __builtin_amdgcn_readfirstlane(k);  // lane 0's value, known uniform (scalar registers)
+    k = __builtin_amdgcn_readfirstlane(k);  // lane 0's values, known uniform (scalar registers)
+    g = __builtin_amdgcn_readfirstlane(g);
     if (begin + k < end) {
-      *last = min(end, begin + k + kGrab);
+      *last = min(end, begin + k + g);
       return begin + k;
     }
     x = (x + 1) % kXcds;
