@@ -292,9 +292,20 @@ __device__ __attribute__((always_inline)) inline uint64_t publish_spikes(const D
     if (mask && count) atomicAdd(&P.counters[2], (unsigned long long)__popcll(mask));
   }
   if (j < P.N) P.teHist[(size_t)slot * P.N + j] = f ? te : kNotFired;
-  if (f) {
-    if (P.lastSpike) P.lastSpike[j] = (int32_t)t1;
-    if (popCnt) atomicAdd(&popCnt[(j / P.npc) * P.nsub + P.subPop[j]], 1u);
+  if (f && P.lastSpike) P.lastSpike[j] = (int32_t)t1;
+  if (popCnt && mask) {
+    // one counter update per distinct (column, sub-population) among the
+    // wave's fired lanes (64 consecutive neurons span at most a few)
+    const uint32_t lane = threadIdx.x & 63;
+    const uint32_t key = f ? (j / P.npc) * P.nsub + P.subPop[j] : 0u;
+    uint64_t pending = mask;
+    while (pending) {
+      const uint32_t lead = (uint32_t)__ffsll((long long)pending) - 1;
+      const uint32_t k = (uint32_t)__shfl((int)key, (int)lead);
+      const uint64_t same = __ballot(f && key == k) & pending;
+      if (lane == lead) atomicAdd(&popCnt[k], (uint32_t)__popcll(same));
+      pending &= ~same;
+    }
   }
   return mask;
 }
