@@ -1258,6 +1258,10 @@ __global__ __launch_bounds__(kWave) void k_batch_gather(const DevNet *nets, cons
   extern __shared__ uint64_t lds[];
   const uint32_t total = (g1 - g0) * kWave, home = blockIdx.x % kXcds;
   uint32_t x = home;
+  // delivered events are summed per member and added to its counter when
+  // the wave moves to another member (one update per run, not per neuron)
+  unsigned long long *evCtr = nullptr;
+  unsigned long long evSum = 0;
   for (uint32_t u0, u1; (u0 = next_item(workCtr, total, x, home, &u1)) < total;)
   for (uint32_t u = u0; u < u1; ++u) {
     const uint32_t g = g0 + u / kWave, q = u % kWave;
@@ -1268,11 +1272,15 @@ __global__ __launch_bounds__(kWave) void k_batch_gather(const DevNet *nets, cons
     const size_t at = (size_t)(u / kWave) * cap * kWave + q * kRun;
     const uint32_t n = gather_sort_neuron<K, B>(P, it.t, j, it.ltd, (uint8_t *)lds, cap, keys + at, keyW + at,
                                                 it.ltd ? keyPos + at : nullptr);
-    if (threadIdx.x == 0) {
-      keyCount[u] = n;
-      atomicAdd(&P.counters[0], (unsigned long long)n);
+    if (P.counters != evCtr) {
+      if (threadIdx.x == 0 && evSum) atomicAdd(evCtr, evSum);
+      evCtr = P.counters;
+      evSum = 0;
     }
+    evSum += n;
+    if (threadIdx.x == 0) keyCount[u] = n;
   }
+  if (threadIdx.x == 0 && evSum) atomicAdd(evCtr, evSum);
 }
 
 __global__ __launch_bounds__(kBlock) void k_batch_integrate(const DevNet *nets, const BatchItem *items,
