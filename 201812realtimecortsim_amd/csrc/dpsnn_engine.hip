@@ -166,7 +166,7 @@ struct DevNet {
   uint16_t *keyPos;        // STDP: their in-list positions, same layout
   uint32_t *keyCount;      // [grid][64] or [N/64][64]
   uint32_t *workCtr;       // [kXcds][kCtrStride] phase-1 work counters (zero between steps)
-  unsigned long long *counters;  // [0] events, [1] external, [2] spikes
+  unsigned long long *counters;  // [kCounterSlots][kCounterStride]: [0] events, [1] external, [2] spikes
   // STDP (null when plasticity never runs).  Synapse state is kept in
   // target-major order: synapse k of neuron j's incoming list lives at
   // incOff[j] + k; in-list items name it by k (inSyn).
@@ -182,6 +182,14 @@ struct DevNet {
   float f2w, drift, decayMul;
   int decayReset;
 };
+
+// The step's counters are spread over kCounterSlots cache lines (a block
+// adds to the line of blockIdx % kCounterSlots; the host sums them), so
+// that the per-wave updates of one step do not all queue on one line.
+constexpr uint32_t kCounterSlots = 16, kCounterStride = 16;
+__device__ inline unsigned long long *counter_slot(unsigned long long *counters) {
+  return counters + (blockIdx.x % kCounterSlots) * kCounterStride;
+}
 
 // --------------------------------------------------------------------------
 // Philox4x32-10 (Salmon et al., SC'11): counter-based stream for the
@@ -289,7 +297,7 @@ __device__ __attribute__((always_inline)) inline uint64_t publish_spikes(const D
   const uint32_t slot = t1 & (P.RS - 1);
   if ((threadIdx.x & 63) == 0 && (j >> 6) < P.NW) {
     P.bits[(size_t)slot * P.NW + (j >> 6)] = mask;
-    if (mask && count) atomicAdd(&P.counters[2], (unsigned long long)__popcll(mask));
+    if (mask && count) atomicAdd(counter_slot(P.counters) + 2, (unsigned long long)__popcll(mask));
   }
   if (j < P.N) P.teHist[(size_t)slot * P.N + j] = f ? te : kNotFired;
   if (f && P.lastSpike) P.lastSpike[j] = (int32_t)t1;
@@ -1144,8 +1152,8 @@ __global__ __launch_bounds__(kWave) void k_step(DevNet P, uint32_t t, ExtView X,
     evSum += step_tile<K, B>(P, t, X, ltd, S, tile, P.cap, (uint8_t *)lds, wkeys, wW, wP, wcnt, exSum);
   for (int o = 32; o > 0; o >>= 1) exSum += __shfl_down(exSum, o);
   if (threadIdx.x == 0) {
-    atomicAdd(&P.counters[0], evSum);
-    atomicAdd(&P.counters[1], exSum);
+    atomicAdd(counter_slot(P.counters) + 0, evSum);
+    atomicAdd(counter_slot(P.counters) + 1, exSum);
   }
 }
 
@@ -1186,8 +1194,8 @@ __global__ __launch_bounds__(kWave) void k_batch_step(const DevNet *nets, const 
                                                   (uint8_t *)lds, wkeys, wW, wP, wcnt, exSum);
     for (int o = 32; o > 0; o >>= 1) exSum += __shfl_down(exSum, o);
     if (threadIdx.x == 0) {
-      atomicAdd(&P.counters[0], ev);
-      if (exSum) atomicAdd(&P.counters[1], exSum);
+      atomicAdd(counter_slot(P.counters) + 0, ev);
+      if (exSum) atomicAdd(counter_slot(P.counters) + 1, exSum);
     }
   }
 }
@@ -1210,7 +1218,7 @@ __global__ __launch_bounds__(kWave) void k_gather(DevNet P, uint32_t t, int ltd)
       if (threadIdx.x == 0) P.keyCount[j] = n;
       evSum += n;
     }
-  if (threadIdx.x == 0) atomicAdd(&P.counters[0], evSum);
+  if (threadIdx.x == 0) atomicAdd(counter_slot(P.counters) + 0, evSum);
 }
 
 // Blocks of kBlock threads: each wave holds 64 consecutive neurons starting
@@ -1232,7 +1240,7 @@ __global__ __launch_bounds__(kBlock) void k_integrate(DevNet P, uint32_t t, ExtV
     ex = integrate_wave<false>(P, t, X, j0, m, P.keys + at, P.keyW + at, n, stage[wave], f, te);
   publish_spikes(P, t + 1, j0 + lane, f, te, S.popCnt, S.count);  // LTP follows in k_ltp
   for (int o = 32; o > 0; o >>= 1) ex += __shfl_down(ex, o);
-  if (lane == 0 && ex) atomicAdd(&P.counters[1], ex);
+  if (lane == 0 && ex) atomicAdd(counter_slot(P.counters) + 1, ex);
 }
 
 // The item of global tile g: the last one with tileBegin <= g.
@@ -1273,14 +1281,14 @@ __global__ __launch_bounds__(kWave) void k_batch_gather(const DevNet *nets, cons
     const uint32_t n = gather_sort_neuron<K, B>(P, it.t, j, it.ltd, (uint8_t *)lds, cap, keys + at, keyW + at,
                                                 it.ltd ? keyPos + at : nullptr);
     if (P.counters != evCtr) {
-      if (threadIdx.x == 0 && evSum) atomicAdd(evCtr, evSum);
+      if (threadIdx.x == 0 && evSum) atomicAdd(counter_slot(evCtr), evSum);
       evCtr = P.counters;
       evSum = 0;
     }
     evSum += n;
     if (threadIdx.x == 0) keyCount[u] = n;
   }
-  if (threadIdx.x == 0 && evSum) atomicAdd(evCtr, evSum);
+  if (threadIdx.x == 0 && evSum) atomicAdd(counter_slot(evCtr), evSum);
 }
 
 __global__ __launch_bounds__(kBlock) void k_batch_integrate(const DevNet *nets, const BatchItem *items,
@@ -1308,7 +1316,7 @@ __global__ __launch_bounds__(kBlock) void k_batch_integrate(const DevNet *nets, 
     ex = integrate_wave<false>(P, it.t, it.X, j0, m, keys + at, keyW + at, n, stage[wave], f, te);
   publish_spikes(P, it.t + 1, j0 + lane, f, te, it.S.popCnt, it.S.count);  // LTP follows in k_batch_ltp
   for (int o = 32; o > 0; o >>= 1) ex += __shfl_down(ex, o);
-  if (lane == 0 && ex) atomicAdd(&P.counters[1], ex);
+  if (lane == 0 && ex) atomicAdd(counter_slot(P.counters) + 1, ex);
 }
 
 // LTP of the members' next scans over global tiles [g0, g1), one wave per
@@ -1653,8 +1661,8 @@ static int build_device(dpsnn_net *n) {
   // dpsnn_step (a network stepped only by a batch uses the batch's)
   if ((rc = dev_alloc(n, &P.workCtr, kXcds * kCtrStride))) return rc;
   HIP_TRY(hipMemsetAsync(P.workCtr, 0, sizeof(uint32_t) * kXcds * kCtrStride, n->stream));
-  if ((rc = dev_alloc(n, &P.counters, 4))) return rc;
-  HIP_TRY(hipMemsetAsync(P.counters, 0, 4 * sizeof(unsigned long long), n->stream));
+  if ((rc = dev_alloc(n, &P.counters, kCounterSlots * kCounterStride))) return rc;
+  HIP_TRY(hipMemsetAsync(P.counters, 0, sizeof(unsigned long long) * kCounterSlots * kCounterStride, n->stream));
   // raster / rates drain buffers
   n->drainEvery = n->opt.drain_every_ms ? n->opt.drain_every_ms : 100;
   const size_t popPer = size_t(P.ncols) * P.nsub;
@@ -2066,7 +2074,7 @@ uint64_t dpsnn_device_bytes_needed(const dpsnn_net *n) {
     b += 2 * r(4 * kStdpSpan) + 2 * r(4 * q.subPopNumber * q.subPopNumber);
   }
   const uint64_t drainEvery = n->opt.drain_every_ms ? n->opt.drain_every_ms : 100;
-  b += r(8 * 4) + r(4 * kXcds * kCtrStride) + r(4 * (uint64_t)q.globCFT * q.subPopNumber * drainEvery);
+  b += r(8 * kCounterSlots * kCounterStride) + r(4 * kXcds * kCtrStride) + r(4 * (uint64_t)q.globCFT * q.subPopNumber * drainEvery);
   const bool raster = (n->writing && !q.spikes_writeDisable) || n->opt.keep_spikes;
   if (raster) {
     const uint64_t cap = N * std::min<uint64_t>(drainEvery, 1000) + 1;
@@ -2200,9 +2208,11 @@ int dpsnn_get_synapses(dpsnn_net *n, uint32_t *pre, uint32_t *post, uint8_t *del
 int dpsnn_write_synapse_dump(dpsnn_net *n, const char *path);
 int dpsnn_get_stats(dpsnn_net *n, dpsnn_stats *out) {
   if (!n || !out) return fail(DPSNN_ERR_ARG, "null argument");
-  unsigned long long cnt[4] = {0, 0, 0, 0};
+  unsigned long long slots[kCounterSlots * kCounterStride], cnt[4] = {0, 0, 0, 0};
   HIP_TRY(hipStreamSynchronize(n->stream));
-  HIP_TRY(hipMemcpy(cnt, n->dev.counters, sizeof cnt, hipMemcpyDeviceToHost));
+  HIP_TRY(hipMemcpy(slots, n->dev.counters, sizeof slots, hipMemcpyDeviceToHost));
+  for (uint32_t k = 0; k < kCounterSlots; ++k)
+    for (int i = 0; i < 4; ++i) cnt[i] += slots[k * kCounterStride + i];
   out->steps = n->t;
   out->synaptic_events = cnt[0];
   out->external_events = cnt[1];
