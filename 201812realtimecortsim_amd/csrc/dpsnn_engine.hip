@@ -159,11 +159,12 @@ struct DevNet {
   uint64_t *bits;          // [RS][NW], RS a power of two
   double *teHist;          // [RS][N] emission time, or kNotFired
   const uint64_t *inOff;   // [N+1] start of each neuron's in-list
-  const uint32_t *inPD;    // pre | delay << 24, in delivery insertion order
-  const int16_t *inW;
-  uint64_t *keys;          // sorted event ticks: [grid or N/64][cap/8][64][8]
-  int16_t *keyW;           // their weights, same layout
-  uint16_t *keyPos;        // STDP: their in-list positions, same layout
+  const uint32_t *inPD;    // pre | delay << 24, in delivery insertion order (padded by one wave)
+  const int16_t *inW;      // (padded by one wave)
+  uint64_t *keys;          // sorted event ticks, per tile of 64 neurons [cap/kRun][64][kRun];
+                           // from step kPackT on tick << 16 | weight
+  int16_t *keyW;           // their weights before step kPackT, same layout
+  uint16_t *keyPos;        // STDP, general path: their in-list positions, same layout
   uint32_t *keyCount;      // [grid][64] or [N/64][64]
   uint32_t *workCtr;       // [kXcds][kCtrStride] phase-1 work counters (zero between steps)
   unsigned long long *counters;  // [kCounterSlots][kCounterStride]: [0] events, [1] external, [2] spikes
@@ -483,11 +484,12 @@ __global__ __launch_bounds__(kCompactThreads) void k_compact(DevNet P, uint32_t 
 // is unique and sorts exactly as (k, insertion order).  The key also
 // recovers k (bits(t) + tick) and the synapse (position).
 //
-// Phase 1, per neuron of the wave's tile of 64, all 64 lanes: stream the in-list
-// (coalesced), keep live items (source fired in step t - delay: one bit of
-// the spike history, then its Te), compact the keys into LDS in insertion
-// order, bucket them by the top 8 bits of frac (LDS counters), sort each
-// bucket by insertion, and store the sorted keys to the wave's scratch.
+// Phase 1, one neuron per wave, all 64 lanes: stream the in-list
+// (coalesced), keep the live items (source fired in step t - delay: one bit
+// of the spike history), compact them into LDS in in-list order, load their
+// emission times and form the keys, bucket the keys by the top bits of the
+// tick (LDS counters), rank each key within its bucket, and store the
+// sorted ticks with their weights to the scratch.
 // Phase 2, one lane per neuron of the tile: merge the sorted internal
 // events with the external bath (internal first on equal time) and run
 // LIFCADynamic for each, in FP64.
