@@ -505,15 +505,20 @@ __device__ inline uint64_t lane_mask_lt() {
   return lane ? (~0ull >> (64 - lane)) : 0ull;
 }
 
-// External bath of one neuron in step t, one event at a time in time order
-// (due(): an event is pending at time at(); take() returns its value and
-// moves on).
+// External bath of one neuron in step t, one event at a time in time order:
+// head is the next event's time (+inf when none is pending); take() returns
+// its value and moves on.
+constexpr double kNoEvent = __builtin_huge_val();
 struct ExtReplayCursor {
   const double *time, *value;
   uint32_t x, xe;
-  __device__ inline bool due() const { return x < xe; }
-  __device__ inline double at() const { return time[x]; }
-  __device__ inline double take() { return value[x++]; }
+  double head;
+  __device__ inline void start() { head = x < xe ? time[x] : kNoEvent; }
+  __device__ inline double take() {
+    const double v = value[x++];
+    head = x < xe ? time[x] : kNoEvent;
+    return v;
+  }
   __device__ inline void finish() {}
 };
 
@@ -522,7 +527,6 @@ struct ExtReplayCursor {
 // following ones while fewer than the per-ms cap; past the cap only the
 // lapses are drawn.
 struct ExtPhiloxCursor {
-  static constexpr double kNone = __builtin_huge_val();
   double a, inv, nextF;
   const double *lut;
   uint32_t cap, nExt;
@@ -558,7 +562,7 @@ struct ExtQueuedCursor {
   ExtPhiloxCursor xp;
   double *qt, *qv;
   uint32_t qi, qn;
-  double head;  // time of the next due event, or kNone
+  double head;  // time of the next due event, or kNoEvent
   __device__ inline void fill(bool on) {
     qn = 0;
     for (;;) {
@@ -571,10 +575,8 @@ struct ExtQueuedCursor {
       }
     }
     qi = 0;
-    head = qn ? qt[0] : (on && xp.due() ? xp.at() : ExtPhiloxCursor::kNone);
+    head = qn ? qt[0] : (on && xp.due() ? xp.at() : kNoEvent);
   }
-  __device__ inline bool due() const { return head != ExtPhiloxCursor::kNone; }
-  __device__ inline double at() const { return head; }
   __device__ inline double take() {
     double v;
     if (qi < qn) {
@@ -583,7 +585,7 @@ struct ExtQueuedCursor {
     } else {
       v = xp.take();
     }
-    head = qi < qn ? qt[qi * kWave] : (xp.due() ? xp.at() : ExtPhiloxCursor::kNone);
+    head = qi < qn ? qt[qi * kWave] : (xp.due() ? xp.at() : kNoEvent);
     return v;
   }
   __device__ inline void finish() { xp.finish(); }
@@ -650,10 +652,12 @@ __device__ __attribute__((always_inline)) inline uint32_t integrate_wave(const D
   }
   L.fired = false;
   ExtReplayCursor xr{};
+  xr.head = kNoEvent;
   ExtQueuedCursor xq{};
   if (on) {
     if (REPLAY) {
-      xr = ExtReplayCursor{X.time, X.value, X.off[j], X.off[j + 1]};
+      xr = ExtReplayCursor{X.time, X.value, X.off[j], X.off[j + 1], kNoEvent};
+      xr.start();
     } else {
       ExtPhiloxCursor &xp = xq.xp;
       xp.a = P.assigned[j];
@@ -701,10 +705,14 @@ __device__ __attribute__((always_inline)) inline uint32_t integrate_wave(const D
     for (;;) {
       const bool hasI = e < left;
       const double kI = dpsnn_asf64(tBits + (packed ? kv >> 16 : kv));
-      const bool eDue = REPLAY ? xr.due() : xq.due();
+      // the bath event goes first when strictly earlier than the internal
+      // one; with no internal event left in this chunk, only on the last
+      // chunk (otherwise the next chunk is staged first)
+      const double lim = hasI ? kI : (last ? kNoEvent : -kNoEvent);
+      const double hx = REPLAY ? xr.head : xq.head;
       double at, val;
-      if (eDue && (hasI ? (REPLAY ? xr.at() : xq.at()) < kI : last)) {
-        at = REPLAY ? xr.at() : xq.at();
+      if (hx < lim) {
+        at = hx;
         val = REPLAY ? xr.take() : xq.take();
       } else if (hasI) {
         const int16_t w16 = (int16_t)(uint16_t)(packed ? kv : wlo);
