@@ -252,32 +252,32 @@ struct Lifca {
   double v, c, Tr, Te;
   double Tau, Theta, H, Tarp, AlphaC, TauC, gC, invTau, invTauC;
   bool fired;
+  // Both of the reference's branches (out of / in the refractory period)
+  // decay c by the same expression; the out-of-refractory one also moves v.
+  // The lanes of a wave take different branches, so v's update is computed
+  // for every lane and kept where the neuron is out of its refractory
+  // period: the same operations in the same order, without divergent
+  // control flow.  Only the refractory exit (exp) stays a branch; it is rare.
   __device__ inline void event(double t, double I) {
     double deltaT = t - Tr;
     const double TFLES = t - Te;
-    if (TFLES > Tarp) {
-      const double TrTe = Tr - Te;
-      if (TrTe < Tarp) {
-        c *= dpsnn_exp(div_rn(-(deltaT - TFLES + Tarp), TauC, invTauC));
-        deltaT = TFLES - Tarp;
-      }
-      const double dtHalf = deltaT / 2;
-      const double rmHalf = div_rn(dtHalf, Tau, invTau);
-      const double rc = div_rn(deltaT, TauC, invTauC);
-      v = v - v * rmHalf - gC * c * dtHalf;
-      v = v - v * rmHalf - gC * c * dtHalf;
-      c = c * (1 - rc + rc * rc / 2);
-      v += I;
-      if (v >= Theta) {
-        v = H;
-        c += AlphaC;
-        Te = t;
-        fired = true;
-      }
-    } else {
-      const double rc = div_rn(deltaT, TauC, invTauC);
-      c = c * (1 - rc + rc * rc / 2);
+    const bool active = TFLES > Tarp;
+    if (active && Tr - Te < Tarp) {
+      c *= dpsnn_exp(div_rn(-(deltaT - TFLES + Tarp), TauC, invTauC));
+      deltaT = TFLES - Tarp;
     }
+    const double dtHalf = deltaT / 2;
+    const double rmHalf = div_rn(dtHalf, Tau, invTau);
+    const double rc = div_rn(deltaT, TauC, invTauC);
+    double w = v - v * rmHalf - gC * c * dtHalf;
+    w = w - w * rmHalf - gC * c * dtHalf;
+    w += I;
+    c = c * (1 - rc + rc * rc / 2);
+    const bool fire = active && w >= Theta;
+    v = fire ? H : (active ? w : v);
+    c = fire ? c + AlphaC : c;
+    Te = fire ? t : Te;
+    fired = fired || fire;
     Tr = t;
   }
 };
