@@ -598,9 +598,10 @@ constexpr int kChunk = 8;  // internal events per lane per LDS-staged chunk
 // steps keep the weight in its own array.
 constexpr uint32_t kPackT = 32;
 constexpr int kRun = 16;   // events per neuron run in the blocked scratch
-// LDS of one wave in phase 2: two chunks of 64 lanes' ticks, then the bath
+// LDS of one wave in phase 2: one chunk of 64 lanes' ticks (the wave has
+// read all of a chunk before it stores the next one over it), then the bath
 // queue (times, values: [kQueue][64] doubles each)
-constexpr size_t kStageBytes = 2 * kChunk * kWave * sizeof(uint64_t) + 2 * kQueue * kWave * sizeof(double);
+constexpr size_t kStageBytes = kChunk * kWave * sizeof(uint64_t) + 2 * kQueue * kWave * sizeof(double);
 
 // Chunk c (events 8c..8c+7 of every lane) from the tile's blocked scratch
 // [i / kRun][q][i % kRun]: each neuron's 8 ticks are one 64-byte piece of
@@ -628,7 +629,10 @@ __device__ inline void store_chunk(uint64_t *stageK, const uint64_t (&rk)[kChunk
 // bath, LIFCADynamic per event in FP64.  The events are staged through LDS
 // (stage, kStageBytes) a chunk ahead.  Returns the lane's external event
 // count and whether (and when) its neuron fired.  Whole wave, uniform.
-template <bool REPLAY>
+// PK 1: every entry carries its weight (t >= kPackT), PK 2: t decides at
+// run time.  The packed-only form needs 16 fewer VGPRs (128: 4 waves/SIMD
+// instead of 3).
+template <bool REPLAY, int PK = 2>
 __device__ __attribute__((always_inline)) inline uint32_t integrate_wave(const DevNet &P, uint32_t t, const ExtView &X, uint32_t j0, uint32_t m,
                                           const uint64_t *tileK, const int16_t *tileW, uint32_t n, uint8_t *stage,
                                           bool &fired, double &te) {
@@ -671,7 +675,7 @@ __device__ __attribute__((always_inline)) inline uint32_t integrate_wave(const D
     }
   }
   if (!REPLAY) {
-    double *q = (double *)(stage + 2 * kChunk * kWave * sizeof(uint64_t));
+    double *q = (double *)(stage + kChunk * kWave * sizeof(uint64_t));
     xq.qt = q + lane;
     xq.qv = q + kQueue * kWave + lane;
     xq.fill(on);
@@ -682,17 +686,17 @@ __device__ __attribute__((always_inline)) inline uint32_t integrate_wave(const D
   // lane's internal events are used up
   const uint32_t chunks = (nmax + kChunk - 1) / kChunk;
   const uint32_t iters = chunks ? chunks : 1u;
-  uint64_t *stageK = (uint64_t *)stage;  // [2][kChunk][64]
+  uint64_t *stageK = (uint64_t *)stage;  // [kChunk][64]
   uint64_t rk[kChunk];
   uint4 wCur = make_uint4(0, 0, 0, 0), wNext = wCur;
-  const bool packed = t >= kPackT;
+  const bool packed = PK == 2 ? t >= kPackT : PK == 1;
   if (chunks) {
     load_chunk(tileK, tileW, 0, rk, wCur, packed);
     store_chunk(stageK, rk);
     wave_sync();
   }
   for (uint32_t c = 0; c < iters; ++c) {
-    const uint64_t *curK = stageK + (c & 1) * (kChunk * kWave);
+    const uint64_t *curK = stageK;
     const bool more = c + 1 < chunks;
     if (more) load_chunk(tileK, tileW, c + 1, rk, wNext, packed);
     uint64_t wlo = (uint64_t)wCur.x | ((uint64_t)wCur.y << 32), whi = (uint64_t)wCur.z | ((uint64_t)wCur.w << 32);
@@ -728,7 +732,7 @@ __device__ __attribute__((always_inline)) inline uint32_t integrate_wave(const D
       L.event(at, val);
     }
     if (more) {
-      store_chunk(stageK + ((c + 1) & 1) * (kChunk * kWave), rk);
+      store_chunk(stageK, rk);
       wCur = wNext;
     }
     wave_sync();
@@ -1233,6 +1237,7 @@ __global__ __launch_bounds__(kWave) void k_gather(DevNet P, uint32_t t, int ltd)
 
 // Blocks of kBlock threads: each wave holds 64 consecutive neurons starting
 // at a multiple of 64, as publish_spikes requires.
+template <int PK>
 __global__ __launch_bounds__(kBlock) void k_integrate(DevNet P, uint32_t t, ExtView X, NextScan S) {
   __shared__ uint8_t stage[kBlock / kWave][kStageBytes];
   const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
@@ -1245,9 +1250,9 @@ __global__ __launch_bounds__(kBlock) void k_integrate(DevNet P, uint32_t t, ExtV
   double te = 0.0;
   unsigned long long ex;
   if (P.drive == DPSNN_DRIVE_REPLAY)
-    ex = integrate_wave<true>(P, t, X, j0, m, P.keys + at, P.keyW + at, n, stage[wave], f, te);
+    ex = integrate_wave<true, PK>(P, t, X, j0, m, P.keys + at, P.keyW + at, n, stage[wave], f, te);
   else
-    ex = integrate_wave<false>(P, t, X, j0, m, P.keys + at, P.keyW + at, n, stage[wave], f, te);
+    ex = integrate_wave<false, PK>(P, t, X, j0, m, P.keys + at, P.keyW + at, n, stage[wave], f, te);
   publish_spikes(P, t + 1, j0 + lane, f, te, S.popCnt, S.count);  // LTP follows in k_ltp
   for (int o = 32; o > 0; o >>= 1) ex += __shfl_down(ex, o);
   if (lane == 0 && ex) atomicAdd(counter_slot(P.counters) + 1, ex);
@@ -1980,7 +1985,10 @@ static int one_step(dpsnn_net *n) {
   if (n->split) {
     hipLaunchKernelGGL(K.gather, dim3(n->gatherGrid), dim3(kWave), n->ldsBytes, n->stream, P, sp.t, sp.ltd);
     const uint32_t blocks = (P.N + kBlock - 1) / kBlock;
-    hipLaunchKernelGGL(k_integrate, dim3(blocks), dim3(kBlock), 0, n->stream, P, sp.t, sp.X, sp.S);
+    if (sp.t >= kPackT)
+      hipLaunchKernelGGL(k_integrate<1>, dim3(blocks), dim3(kBlock), 0, n->stream, P, sp.t, sp.X, sp.S);
+    else
+      hipLaunchKernelGGL(k_integrate<2>, dim3(blocks), dim3(kBlock), 0, n->stream, P, sp.t, sp.X, sp.S);
     if (sp.S.ltp) hipLaunchKernelGGL(k_ltp, dim3(n->stdpBlocks), dim3(kWaveLTP), 0, n->stream, P, sp.t + 1);
   } else {
     hipLaunchKernelGGL(K.step, dim3(n->grid), dim3(kWave), n->ldsBytes, n->stream, P, sp.t, sp.X, sp.ltd, sp.S);
