@@ -1306,6 +1306,7 @@ __global__ __launch_bounds__(kWave) void k_batch_gather(const DevNet *nets, cons
   if (threadIdx.x == 0 && evSum) atomicAdd(counter_slot(evCtr), evSum);
 }
 
+template <int PK>
 __global__ __launch_bounds__(kBlock) void k_batch_integrate(const DevNet *nets, const BatchItem *items,
                                                             uint32_t count, uint32_t g0, uint32_t g1, uint32_t cap,
                                                             const uint64_t *keys, const int16_t *keyW,
@@ -1326,9 +1327,9 @@ __global__ __launch_bounds__(kBlock) void k_batch_integrate(const DevNet *nets, 
   double te = 0.0;
   unsigned long long ex;
   if (P.drive == DPSNN_DRIVE_REPLAY)
-    ex = integrate_wave<true>(P, it.t, it.X, j0, m, keys + at, keyW + at, n, stage[wave], f, te);
+    ex = integrate_wave<true, PK>(P, it.t, it.X, j0, m, keys + at, keyW + at, n, stage[wave], f, te);
   else
-    ex = integrate_wave<false>(P, it.t, it.X, j0, m, keys + at, keyW + at, n, stage[wave], f, te);
+    ex = integrate_wave<false, PK>(P, it.t, it.X, j0, m, keys + at, keyW + at, n, stage[wave], f, te);
   publish_spikes(P, it.t + 1, j0 + lane, f, te, it.S.popCnt, it.S.count);  // LTP follows in k_batch_ltp
   for (int o = 32; o > 0; o >>= 1) ex += __shfl_down(ex, o);
   if (lane == 0 && ex) atomicAdd(counter_slot(P.counters) + 1, ex);
@@ -2320,7 +2321,7 @@ struct dpsnn_batch {
     hipGraphExec_t exec = nullptr;
     uint32_t count = 0, tiles = 0, chunkTiles = 0, wpc = 0;
     int shape = -1;
-    bool split = false;
+    bool split = false, packed = false;
   } graphs[kRing];
   bool profile = false;
   std::vector<std::pair<hipEvent_t, hipEvent_t>> ev;
@@ -2429,8 +2430,10 @@ uint32_t batch_waves(const dpsnn_batch *b, size_t lds) {
 }
 
 // The device work of one batched step: the items of ring slot r, then either
-// the persistent fused kernel or, chunk by chunk, phase 1 and phase 2.
-int batch_launch(dpsnn_batch *b, int r, uint32_t count, uint32_t tiles) {
+// the persistent fused kernel or, chunk by chunk, phase 1 and phase 2
+// (packed: every member is at t >= kPackT, so phase 2 runs its packed-only
+// instance).
+int batch_launch(dpsnn_batch *b, int r, uint32_t count, uint32_t tiles, bool packed) {
   HIP_TRY(hipMemcpyAsync(b->d_items[r], b->h_items[r], sizeof(BatchItem) * count, hipMemcpyHostToDevice, b->stream));
   const KernelShape &K = kShapes[b->shape];
   const uint32_t cap = b->scratchCap;
@@ -2450,8 +2453,12 @@ int batch_launch(dpsnn_batch *b, int r, uint32_t count, uint32_t tiles) {
       hipLaunchKernelGGL(K.batchGather, dim3(grid), dim3(kWave), lds, b->stream, b->d_nets, b->d_items[r], count, g0,
                          g1, cap, b->keys, b->keyW, b->keyPos, b->keyCount, b->workCtr);
       const uint32_t blocks = (g1 - g0 + kBlock / kWave - 1) / (kBlock / kWave);
-      hipLaunchKernelGGL(k_batch_integrate, dim3(blocks), dim3(kBlock), 0, b->stream, b->d_nets, b->d_items[r], count,
-                         g0, g1, cap, b->keys, b->keyW, b->keyCount, b->workCtr);
+      if (packed)
+        hipLaunchKernelGGL(k_batch_integrate<1>, dim3(blocks), dim3(kBlock), 0, b->stream, b->d_nets, b->d_items[r],
+                           count, g0, g1, cap, b->keys, b->keyW, b->keyCount, b->workCtr);
+      else
+        hipLaunchKernelGGL(k_batch_integrate<2>, dim3(blocks), dim3(kBlock), 0, b->stream, b->d_nets, b->d_items[r],
+                           count, g0, g1, cap, b->keys, b->keyW, b->keyCount, b->workCtr);
       if (b->anyPlastic)
         hipLaunchKernelGGL(k_batch_ltp, dim3(std::min<uint32_t>((g1 - g0) * (kWave * kWave / kWaveLTP), b->cus * 16u)),
                            dim3(kWaveLTP), 0, b->stream, b->d_nets, b->d_items[r], count, g0, g1);
@@ -2470,16 +2477,16 @@ void batch_drop_graphs(dpsnn_batch *b) {
 
 // The same work replayed from a graph captured once per ring slot and launch
 // configuration (the per-step arguments live in the slot's items).
-int batch_graph_launch(dpsnn_batch *b, int r, uint32_t count, uint32_t tiles) {
+int batch_graph_launch(dpsnn_batch *b, int r, uint32_t count, uint32_t tiles, bool packed) {
   dpsnn_batch::Graph &g = b->graphs[r];
   const uint32_t wpc = b->wavesPerCu;
   if (!g.exec || g.count != count || g.tiles != tiles || g.chunkTiles != b->chunkTiles || g.wpc != wpc ||
-      g.shape != b->shape || g.split != b->split) {
+      g.shape != b->shape || g.split != b->split || g.packed != packed) {
     if (g.exec) HIP_TRY(hipGraphExecDestroy(g.exec));
     g = dpsnn_batch::Graph{};
     hipGraph_t graph = nullptr;
     HIP_TRY(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
-    const int rc = batch_launch(b, r, count, tiles);
+    const int rc = batch_launch(b, r, count, tiles, packed);
     const hipError_t e = hipStreamEndCapture(b->stream, &graph);
     if (rc) {
       if (graph) (void)hipGraphDestroy(graph);
@@ -2498,6 +2505,7 @@ int batch_graph_launch(dpsnn_batch *b, int r, uint32_t count, uint32_t tiles) {
     g.wpc = wpc;
     g.shape = b->shape;
     g.split = b->split;
+    g.packed = packed;
   }
   HIP_TRY(hipGraphLaunch(g.exec, b->stream));
   return 0;
@@ -2518,10 +2526,12 @@ int batch_one_step(dpsnn_batch *b, bool *any) {
   if (b->pending[r]) HIP_TRY(hipEventSynchronize(b->done[r]));
   b->plans.resize(b->active.size());
   uint32_t tiles = 0;
+  bool packed = true;
   for (size_t i = 0; i < b->active.size(); ++i) {
     dpsnn_net *n = b->active[i];
     StepPlan &sp = b->plans[i];
     if ((rc = pre_step(n, sp))) return rc;
+    packed = packed && sp.t >= kPackT;
     const uint32_t idx = (uint32_t)(std::find(b->nets.begin(), b->nets.end(), n) - b->nets.begin());
     b->h_items[r][i] = BatchItem{sp.t, sp.ltd, sp.S, sp.X, tiles, idx};
     tiles += n->dev.NW;
@@ -2537,9 +2547,9 @@ int batch_one_step(dpsnn_batch *b, bool *any) {
     HIP_TRY(hipEventRecord(b->ev[b->evUsed].first, b->stream));
   }
   if (b->useGraph) {
-    if ((rc = batch_graph_launch(b, r, count, tiles))) return rc;
+    if ((rc = batch_graph_launch(b, r, count, tiles, packed))) return rc;
   } else {
-    if ((rc = batch_launch(b, r, count, tiles))) return rc;
+    if ((rc = batch_launch(b, r, count, tiles, packed))) return rc;
   }
   if (b->profile) HIP_TRY(hipEventRecord(b->ev[b->evUsed++].second, b->stream));
   HIP_TRY(hipEventRecord(b->done[r], b->stream));
