@@ -134,6 +134,8 @@ _SIGS = {
     "dpsnn_device_bytes_needed": (ctypes.c_uint64, [ctypes.c_void_p]),
     "dpsnn_upload": (ctypes.c_int, [ctypes.c_void_p]),
     "dpsnn_step": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint32]),
+    "dpsnn_checkpoint": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_char_p]),
+    "dpsnn_restore": (ctypes.c_int, [ctypes.c_char_p, _p(Options), ctypes.c_char_p, _p(ctypes.c_void_p)]),
     "dpsnn_sync": (ctypes.c_int, [ctypes.c_void_p]),
     "dpsnn_now_ms": (ctypes.c_uint32, [ctypes.c_void_p]),
     "dpsnn_total_ms": (ctypes.c_uint32, [ctypes.c_void_p]),
@@ -154,6 +156,8 @@ _SIGS = {
     "dpsnn_host_num_items": (ctypes.c_uint64, [ctypes.c_void_p]),
     "dpsnn_host_get_synapses": (ctypes.c_int, [ctypes.c_void_p] + [ctypes.c_void_p] * 4),
     "dpsnn_host_get_inlists": (ctypes.c_int, [ctypes.c_void_p] + [ctypes.c_void_p] * 4),
+    "dpsnn_host_window": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint32, ctypes.c_void_p, ctypes.c_void_p,
+                                         ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]),
     "dpsnn_host_get_external": (ctypes.c_int, [ctypes.c_void_p] + [ctypes.c_void_p] * 2),
     "dpsnn_host_write_synapse_dump": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_char_p]),
     "dpsnn_host_external_step": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
@@ -252,18 +256,28 @@ class Network:
     """One local network on one GPU (a dpsnn_net)."""
 
     def __init__(self, inidir, device=0, drive=DRIVE_REPLAY, philox_seed=1, output_dir=None,
-                 drain_every_ms=0, keep_spikes=False, env=None, drop_host_synapses=False):
+                 drain_every_ms=0, keep_spikes=False, env=None, drop_host_synapses=False, checkpoint=None):
         """env: {"env_name": value} settings that take precedence over the
-        process environment for this network only."""
+        process environment for this network only.  checkpoint: a file of
+        dpsnn_checkpoint to resume from (dpsnn_restore), taken on the
+        network of the same ini files and settings."""
         self._lib = library()
         self._out = output_dir.encode() if output_dir else None
         env_text = ";".join(f"{k}={v}" for k, v in env.items()).encode() if env else None
         opt = Options(device, drive, philox_seed, self._out, drain_every_ms, int(keep_spikes), env_text,
                       int(drop_host_synapses))
         h = ctypes.c_void_p()
-        _check("dpsnn_create", self._lib.dpsnn_create(inidir.encode(), ctypes.byref(opt), ctypes.byref(h)))
+        if checkpoint:
+            _check("dpsnn_restore", self._lib.dpsnn_restore(inidir.encode(), ctypes.byref(opt),
+                                                            str(checkpoint).encode(), ctypes.byref(h)))
+        else:
+            _check("dpsnn_create", self._lib.dpsnn_create(inidir.encode(), ctypes.byref(opt), ctypes.byref(h)))
         self._h = h
         self.n = self.stats()["neurons"]
+
+    def checkpoint(self, path):
+        """Writes the network's state to path (dpsnn_checkpoint)."""
+        _check("dpsnn_checkpoint", self._lib.dpsnn_checkpoint(self._h, str(path).encode()))
 
     @classmethod
     def _wrap(cls, handle):
@@ -382,6 +396,22 @@ class HostConnectome:
         _check("dpsnn_host_get_inlists", self._lib.dpsnn_host_get_inlists(self._h, _ptr(off), _ptr(pd),
                                                                           _ptr(w), _ptr(syn)))
         return {"off": off, "pre": pd & 0xFFFFFF, "delay": pd >> 24, "weight": w, "syn": syn}
+
+    def window(self, units_per_col=1):
+        """The stencil-window layout of the in-lists (dpsnn_host_window)."""
+        dims = np.zeros(6, np.uint32)
+        sizes = np.zeros(4, np.uint64)
+        null = ctypes.c_void_p()
+        _check("dpsnn_host_window", self._lib.dpsnn_host_window(self._h, units_per_col, _ptr(dims), _ptr(sizes),
+                                                                null, null, null, null))
+        item = np.empty(int(sizes[0]), np.uint32)
+        desc = np.empty(int(sizes[1]), np.uint16)
+        chunk = np.empty(int(sizes[2]), np.uint64)
+        cols = np.empty(int(sizes[3]), np.int32)
+        _check("dpsnn_host_window", self._lib.dpsnn_host_window(self._h, units_per_col, _ptr(dims), _ptr(sizes),
+                                                                _ptr(item), _ptr(desc), _ptr(chunk), _ptr(cols)))
+        keys = ("slots", "npc", "passes", "unit", "units_per_col", "units")
+        return dict(zip(keys, (int(v) for v in dims)), item=item, desc=desc, chunk=chunk, cols=cols)
 
     def external_params(self):
         inv = np.empty(self.n, np.float64)
@@ -554,25 +584,58 @@ class Server:
 
 
 # ---- the reference's process interface (DAL shape) ------------------------
+class _LocalStates(ctypes.Structure):
+    """struct _local_states (include/localNetProcess_hip.h)."""
+    _fields_ = [("parameters", Parameters), ("net", ctypes.c_void_p), ("id", ctypes.c_int),
+                ("name", ctypes.c_char * 32), ("fireCount", ctypes.c_int), ("fireSeconds", ctypes.c_double)]
+
+
+class DALProcess(ctypes.Structure):
+    """DALProcess (include/localNetProcess_hip.h; the reference's dal.h:25-42)."""
+    _fields_ = [("local", ctypes.POINTER(_LocalStates)), ("init", ctypes.c_void_p), ("fire", ctypes.c_void_p),
+                ("finish", ctypes.c_void_p), ("wptr", ctypes.c_void_p)]
+
+
 class LocalNetProcess:
-    """Python mirror of localNetProcess_{init,fire,finish}
-    (localNetProcess.c:40-173): init builds the network from the env and the
-    ini files in ``inidir``; fire advances one ms and returns 0, or 1 once
-    env_totalSimTime_ms is reached; finish syncs and releases the device."""
+    """localNetProcess_{init,fire,finish} of the library (the reference's
+    localNetProcess.c:40-173), called through ctypes on a DALProcess: init
+    builds the network from the env and the ini files in ``inidir`` (its
+    $DPSNN_INIDIR), fire advances one ms and returns 0, or 1 once
+    env_totalSimTime_ms is reached, finish drains the output files and frees
+    the device.  As in the reference, a configuration or device error makes
+    init/fire print the message and end the process."""
 
     def __init__(self, inidir=".", output_dir=".", device=0, drive=DRIVE_REPLAY):
         self.inidir, self.output_dir, self.device, self.drive = inidir, output_dir, device, drive
-        self.net = None
+        lib = library()
+        for name, res in (("localNetProcess_init", None), ("localNetProcess_fire", ctypes.c_int),
+                          ("localNetProcess_finish", None)):
+            fn = getattr(lib, name)
+            fn.restype, fn.argtypes = res, [ctypes.POINTER(DALProcess)]
+        self._lib = lib
+        self._state = _LocalStates()
+        self._proc = DALProcess(ctypes.pointer(self._state), None, None, None, None)
 
     def init(self):
-        self.net = Network(self.inidir, device=self.device, drive=self.drive, output_dir=self.output_dir)
+        os.environ.update(DPSNN_INIDIR=self.inidir, DPSNN_OUTDIR=self.output_dir, DPSNN_DEVICE=str(self.device),
+                          DPSNN_DRIVE="philox" if self.drive == DRIVE_PHILOX else "replay")
+        self._lib.localNetProcess_init(ctypes.byref(self._proc))
 
     def fire(self):
-        if self.net.now_ms >= self.net.total_ms:
-            return 1
-        self.net.step(1)
-        return 0
+        return self._lib.localNetProcess_fire(ctypes.byref(self._proc))
+
+    def stats(self):
+        st = Stats()
+        _check("dpsnn_get_stats", self._lib.dpsnn_get_stats(self._state.net, ctypes.byref(st)))
+        return st.as_dict()
+
+    @property
+    def parameters(self):
+        return self._state.parameters
+
+    @property
+    def fire_count(self):
+        return self._state.fireCount
 
     def finish(self):
-        self.net.sync()
-        self.net.close()
+        self._lib.localNetProcess_finish(ctypes.byref(self._proc))

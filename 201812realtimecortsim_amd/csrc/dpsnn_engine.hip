@@ -182,6 +182,19 @@ struct DevNet {
   const float *maxSyn, *revMax;      // [nsub][nsub]
   float f2w, drift, decayMul;
   int decayReset;
+  // Stencil-window form of phase 1 (k_collect + k_sort; null when the
+  // network has no window layout): the in-lists regrouped per unit of
+  // targets of one column, per pass (run delay, descending) and per wave,
+  // in padded chunks of kWindowChunk items (window index << 16 | weight).
+  const uint32_t *winItem;
+  const uint16_t *winDesc;   // per chunk: its target's index in the unit
+  const uint64_t *winChunk;  // [units][passes][kWindowWaves] + 1: first chunk
+  const int32_t *winCols;    // [columns][slots]: source column of each stencil slot, or -1
+  uint32_t winSlots, winPasses, winUnit, winUnitsPerCol, winUnits;
+  uint32_t *winCtr;          // [kXcds][kCtrStride] unit counters (zero between steps)
+  // Steps replayed from a captured graph read their t here (the kernel
+  // argument is then ignored); null: t is the argument.
+  const uint32_t *tDev;
 };
 
 // The step's counters are spread over kCounterSlots cache lines (a block
@@ -946,6 +959,54 @@ __host__ __device__ inline size_t phase1_lds(uint32_t cap, int B) {
   return (size_t)cap * sizeof(uint64_t) + (size_t)cap * sizeof(int16_t) + (size_t)(B + 1) * sizeof(uint32_t);
 }
 
+// Offset of event p of a neuron in the tile's blocked scratch, from the
+// neuron's [0][q][0].
+__device__ inline size_t run_offset(uint32_t p) { return (size_t)(p / kRun) * (kWave * kRun) + (p % kRun); }
+
+// Exclusive scan of the B bucket counters in place (R = B / 64 per lane);
+// bcnt[B] receives the total.  Whole wave.
+template <int B>
+__device__ __attribute__((always_inline)) inline void bucket_scan(uint32_t *bcnt) {
+  constexpr int R = B / kWave;
+  const uint32_t lane = threadIdx.x & 63;
+  uint32_t c[R], sum = 0;
+#pragma unroll
+  for (int r = 0; r < R; ++r) { c[r] = bcnt[lane * R + r]; sum += c[r]; }
+  uint32_t incl = sum;
+  for (int o = 1; o < kWave; o <<= 1) {
+    const uint32_t v = __shfl_up(incl, o);
+    if (lane >= (uint32_t)o) incl += v;
+  }
+  uint32_t run = incl - sum;
+  __syncthreads();
+#pragma unroll
+  for (int r = 0; r < R; ++r) { bcnt[lane * R + r] = run; run += c[r]; }
+  if (lane == kWave - 1) bcnt[B] = run;
+}
+
+// The n keys in sorted[] are scattered into their buckets (bcnt[b] = end of
+// bucket b).  Within a bucket each key's place is its rank (keys are
+// unique): the count of the bucket's smaller keys.  Calls put(key, place)
+// once per key.  Lanes take the keys in bucket order, so lanes of one
+// bucket read the same words together; the reads are independent of each
+// other.  Whole wave.
+template <int B, class Put>
+__device__ __attribute__((always_inline)) inline void rank_in_buckets(const uint64_t *sorted, const uint32_t *bcnt,
+                                                                      uint32_t n, int shift, Put &&put) {
+  for (uint32_t i = threadIdx.x & 63; i < n; i += kWave) {
+    const uint64_t x = sorted[i];
+    const uint32_t b = key_bucket<B>(x >> 12, shift);
+    const uint32_t lo = b ? bcnt[b - 1] : 0u, hi = bcnt[b];
+    uint32_t r = 0, y = lo;
+    for (; y + 4 <= hi; y += 4) {  // four independent reads per round
+      const uint64_t a0 = sorted[y], a1 = sorted[y + 1], a2 = sorted[y + 2], a3 = sorted[y + 3];
+      r += (a0 < x ? 1u : 0u) + (a1 < x ? 1u : 0u) + (a2 < x ? 1u : 0u) + (a3 < x ? 1u : 0u);
+    }
+    for (; y < hi; ++y) r += sorted[y] < x ? 1u : 0u;
+    put(x, lo + r);
+  }
+}
+
 // Phase 1 for neuron j, whole wave: live keys of its in-list, bucket-sorted
 // in LDS (lds: sorted[cap], ldsW[cap], bcnt[B + 1]), stored in arrival order
 // as ticks relative to bits(t) (outK) with their weights (outW) and, with
@@ -965,7 +1026,6 @@ template <int K, int B>
 __device__ __attribute__((always_inline)) inline uint32_t gather_sort_neuron(const DevNet &P, uint32_t t, uint32_t j, int plastic, uint8_t *lds,
                                               uint32_t cap, uint64_t *outK, int16_t *outW, uint16_t *outP) {
   static_assert(B % kWave == 0, "bucket count must be a multiple of 64");
-  constexpr int R = B / kWave;  // buckets per lane
   constexpr int LOG2B = B == 64 ? 6 : B == 128 ? 7 : B == 256 ? 8 : B == 512 ? 9 : 10;
   uint64_t *sorted = (uint64_t *)lds;
   int16_t *ldsW = (int16_t *)(lds + (size_t)cap * sizeof(uint64_t));
@@ -1038,22 +1098,7 @@ __device__ __attribute__((always_inline)) inline uint32_t gather_sort_neuron(con
       n += gather_keys<kSlowK, B>(P, t, j, beg, len, base, k8, bcnt, ldsW, shift, tBits, true, plastic);
   }
   __syncthreads();
-  // exclusive scan of the bucket counters (R per lane)
-  {
-    uint32_t c[R], sum = 0;
-#pragma unroll
-    for (int r = 0; r < R; ++r) { c[r] = bcnt[lane * R + r]; sum += c[r]; }
-    uint32_t incl = sum;
-    for (int o = 1; o < kWave; o <<= 1) {
-      const uint32_t v = __shfl_up(incl, o);
-      if (lane >= (uint32_t)o) incl += v;
-    }
-    uint32_t run = incl - sum;
-    __syncthreads();
-#pragma unroll
-    for (int r = 0; r < R; ++r) { bcnt[lane * R + r] = run; run += c[r]; }
-    if (lane == kWave - 1) bcnt[B] = run;
-  }
+  bucket_scan<B>(bcnt);
   __syncthreads();
   // scatter into buckets (order inside a bucket is fixed below); the general
   // path gathers its batches again (L2 hits)
@@ -1070,22 +1115,8 @@ __device__ __attribute__((always_inline)) inline uint32_t gather_sort_neuron(con
     }
   }
   __syncthreads();
-  // bcnt[b] now holds the end of bucket b.  Within a bucket, each key's
-  // place is its rank (keys are unique): count the bucket's smaller keys.
-  // Lanes take the keys in bucket order, so lanes of one bucket read the
-  // same words together; the reads are independent of each other.
-  for (uint32_t i = lane; i < n; i += kWave) {
-    const uint64_t x = sorted[i];
-    const uint32_t b = key_bucket<B>(x >> 12, shift);
-    const uint32_t lo = b ? bcnt[b - 1] : 0u, hi = bcnt[b];
-    uint32_t r = 0, y = lo;
-    for (; y + 4 <= hi; y += 4) {  // four independent reads per round
-      const uint64_t a0 = sorted[y], a1 = sorted[y + 1], a2 = sorted[y + 2], a3 = sorted[y + 3];
-      r += (a0 < x ? 1u : 0u) + (a1 < x ? 1u : 0u) + (a2 < x ? 1u : 0u) + (a3 < x ? 1u : 0u);
-    }
-    for (; y < hi; ++y) r += sorted[y] < x ? 1u : 0u;
-    const uint32_t p = lo + r;
-    const size_t at = (size_t)(p / kRun) * (kWave * kRun) + (p % kRun);
+  rank_in_buckets<B>(sorted, bcnt, n, shift, [&](uint64_t x, uint32_t p) {
+    const size_t at = run_offset(p);
     const int16_t wv = ldsW[x & 0xFFFu];
     if (t >= kPackT) {
       outK[at] = ((x >> 12) << 16) | (uint16_t)wv;
@@ -1094,7 +1125,7 @@ __device__ __attribute__((always_inline)) inline uint32_t gather_sort_neuron(con
       outW[at] = wv;
     }
     if (plastic && !fast) outP[at] = (uint16_t)(x & 0xFFFu);
-  }
+  });
   __syncthreads();
   if (plastic && !fast) ltd_neuron(P, t, j, ltd_increment(P, t, j), n, outP);
   return n;
@@ -1221,6 +1252,7 @@ __global__ __launch_bounds__(kWave) void k_batch_step(const DevNet *nets, const 
 template <int K, int B>
 __global__ __launch_bounds__(kWave) void k_gather(DevNet P, uint32_t t, int ltd) {
   extern __shared__ uint64_t lds[];
+  if (P.tDev) t = *P.tDev;
   unsigned long long evSum = 0;
   const uint32_t home = blockIdx.x % kXcds;
   uint32_t x = home;
@@ -1235,11 +1267,229 @@ __global__ __launch_bounds__(kWave) void k_gather(DevNet P, uint32_t t, int ltd)
   if (threadIdx.x == 0) atomicAdd(counter_slot(P.counters) + 0, evSum);
 }
 
+// ---------------------------------------------------------------------------
+// Stencil-window form of phase 1 (steps t >= kPackT without LTD).
+//
+// k_collect: one workgroup of kWindowWaves waves per unit (a range of the
+// targets of one column), handed out by an XCD-ordered queue.  For each
+// run delay d, in the in-lists' order (descending), the workgroup stages
+// the emission times of slot t - d of the column's stencil columns into
+// LDS, already turned into packed ticks (bits(t + frac(Te)) - bits(t)) << 16,
+// or kNoKey where the source did not fire.  Its waves then stream the
+// unit's items of delay d (window index << 16 | weight, coalesced chunks):
+// an item is live when its window entry holds a tick, and its record
+// (tick << 16 | weight) is appended to its target's run in the blocked
+// scratch.  Each target's records come out in the order of its live
+// in-list items, so a record's place in the run breaks equal ticks exactly
+// as the in-list position does.  No per-event global load: the only
+// per-event memory access besides the item stream is the LDS lookup.
+//
+// k_sort: one wave per target orders its records by (tick, place) in LDS
+// (bucket counts, scatter, rank within bucket) and writes them back in
+// place, as phase 2 reads them.
+// ---------------------------------------------------------------------------
+constexpr int kCollectThreads = kWindowWaves * kWave;
+constexpr uint32_t kWindowMinNeurons = 1u << 16;  // the window form by default from this size
+constexpr uint32_t kWindowUnitsWanted = 1024;     // units per step: about four per CU
+constexpr uint32_t kWindowMaxEntries = 19000;     // window entries (u64) in one CU's LDS
+
+// Block-wide work queue: the next unit for the whole workgroup (one unit
+// per counter update), XCD-ordered as next_item.  sh: two LDS words.
+__device__ inline uint32_t next_unit(uint32_t *ctr, uint32_t total, uint32_t &x, uint32_t home, uint32_t *sh) {
+  __syncthreads();  // every wave has read the previous hand-out
+  if (threadIdx.x == 0) {
+    uint32_t got = total;
+    for (;;) {
+      const uint32_t begin = (uint32_t)((uint64_t)total * x / kXcds);
+      const uint32_t end = (uint32_t)((uint64_t)total * (x + 1) / kXcds);
+      uint32_t *c = ctr + x * kCtrStride;
+      if (begin + __hip_atomic_load(c, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < end) {
+        const uint32_t k = atomicAdd(c, 1u);
+        if (begin + k < end) {
+          got = begin + k;
+          break;
+        }
+      }
+      x = (x + 1) % kXcds;
+      if (x == home) break;
+    }
+    sh[0] = got;
+    sh[1] = x;
+  }
+  __syncthreads();
+  x = sh[1];
+  return sh[0];
+}
+
+__host__ __device__ inline size_t collect_lds(uint32_t entries, uint32_t unit) {
+  return ((size_t)entries + 1) * sizeof(uint64_t) + (size_t)unit * sizeof(uint32_t) + 2 * sizeof(uint32_t);
+}
+
+__global__ __launch_bounds__(kCollectThreads) void k_collect(DevNet P, uint32_t t) {
+  extern __shared__ uint64_t lds[];
+  if (P.tDev) t = *P.tDev;
+  const uint32_t W = P.winSlots * P.npc;  // entry W: the padding items' (never live)
+  uint64_t *win = lds;
+  uint32_t *cnt = (uint32_t *)(win + W + 1);  // records so far, per target of the unit
+  uint32_t *sh = cnt + P.winUnit;
+  const uint32_t wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const double tF = (double)t;
+  const uint64_t tBits = dpsnn_asu64(tF);
+  const uint64_t below = lane_mask_lt();
+  const uint64_t upTo = lane == 63 ? ~0ull : (2ull << lane) - 1;
+  if (threadIdx.x == 0) win[W] = kNoKey;
+  unsigned long long evSum = 0;
+  const uint32_t home = blockIdx.x % kXcds;
+  uint32_t x = home;
+  for (uint32_t u; (u = next_unit(P.winCtr, P.winUnits, x, home, sh)) < P.winUnits;) {
+    const uint32_t col = u / P.winUnitsPerCol, part = u - col * P.winUnitsPerCol;
+    const uint32_t j0 = col * P.npc + part * P.winUnit;
+    const uint32_t m = min(P.winUnit, (col + 1) * P.npc - j0);
+    for (uint32_t i = threadIdx.x; i < m; i += blockDim.x) cnt[i] = 0;
+    for (uint32_t pass = 0; pass < P.winPasses; ++pass) {
+      const uint32_t d = P.winPasses - 1 - pass;
+      const double *slot = P.teHist + (size_t)((t - d) & (P.RS - 1)) * P.N;
+      __syncthreads();  // the previous pass's lookups are done
+      for (uint32_t k = 0; k < P.winSlots; ++k) {
+        const int32_t sc = P.winCols[(size_t)col * P.winSlots + k];
+        if (sc < 0) continue;  // no item refers to the slot
+        const double *te = slot + (size_t)sc * P.npc;
+        for (uint32_t i = threadIdx.x; i < P.npc; i += blockDim.x) {
+          const double v = te[i];
+          win[k * P.npc + i] = v >= 0.0 ? (dpsnn_asu64(tF + (v - trunc(v))) - tBits) << 16 : kNoKey;  // modf, exact
+        }
+      }
+      __syncthreads();
+      const uint64_t *co = P.winChunk + ((size_t)u * P.winPasses + pass) * kWindowWaves + wave;
+      const uint64_t cb = co[0], ce = co[1];
+      constexpr int U = 4;  // batches of 64 items (8 chunks) in flight per lane
+      for (uint64_t c0 = cb; c0 < ce; c0 += (uint64_t)kWindowChunk * U) {
+        uint32_t it[U], q[U];
+#pragma unroll
+        for (int v = 0; v < U; ++v) {
+          const uint64_t ch = c0 + (uint64_t)v * kWindowChunk + (lane >> 3);
+          const bool ok = ch < ce;
+          it[v] = ok ? P.winItem[ch * kWindowChunk + (lane & 7)] : W << 16;
+          q[v] = ok ? (uint32_t)P.winDesc[ch] : 0xFFFFu;
+        }
+#pragma unroll
+        for (int v = 0; v < U; ++v) {
+          if (c0 + (uint64_t)v * kWindowChunk >= ce) break;  // uniform
+          const uint64_t r = win[it[v] >> 16];
+          const bool live = r != kNoKey;  // never on padding
+          const uint32_t qv = q[v];
+          // lanes of one target are consecutive: a run of equal q; a live
+          // lane's place is its run's count so far plus the live lanes
+          // before it in the run
+          const uint64_t liveM = __ballot(live);
+          const uint32_t qPrev = (uint32_t)__shfl_up((int)qv, 1);
+          const uint64_t startM = __ballot(lane == 0 || qv != qPrev);
+          const uint32_t s = 63u - (uint32_t)__clzll((long long)(startM & upTo));
+          const uint64_t runBelow = below & ~((1ull << s) - 1ull);
+          const bool valid = qv != 0xFFFFu;
+          const uint32_t p = (valid ? cnt[qv] : 0u) + (uint32_t)__popcll(liveM & runBelow);
+          if (live) {
+            const uint32_t j = j0 + qv;
+            P.keys[(size_t)(j >> 6) * P.cap * kWave + (j & 63) * kRun + run_offset(p)] = r | (it[v] & 0xFFFFu);
+          }
+          const bool lastOfRun = lane == 63 || ((startM >> (lane + 1)) & 1ull);
+          if (valid && lastOfRun) cnt[qv] = p + (live ? 1u : 0u);
+          wave_sync();
+        }
+      }
+    }
+    __syncthreads();
+    for (uint32_t i = threadIdx.x; i < m; i += blockDim.x) {
+      P.keyCount[j0 + i] = cnt[i];
+      evSum += cnt[i];
+    }
+  }
+  for (int o = 32; o > 0; o >>= 1) evSum += __shfl_down(evSum, o);
+  if (lane == 0 && evSum) atomicAdd(counter_slot(P.counters) + 0, evSum);
+}
+
+// Orders neuron j's n records (tick << 16 | weight, in live in-list order)
+// in place at out (neuron j's [0][q][0] of the blocked scratch).  Sort key
+// tick << 12 | place: unique, ordered as (arrival time, in-list position).
+// LDS as phase1_lds.  Whole wave.
+template <int B>
+__device__ __attribute__((always_inline)) inline void sort_records(uint32_t t, uint32_t n, uint8_t *lds, uint32_t cap,
+                                                                   uint64_t *out) {
+  constexpr int LOG2B = B == 64 ? 6 : B == 128 ? 7 : B == 256 ? 8 : B == 512 ? 9 : 10;
+  uint64_t *sorted = (uint64_t *)lds;
+  int16_t *ldsW = (int16_t *)(lds + (size_t)cap * sizeof(uint64_t));
+  uint32_t *bcnt = (uint32_t *)(lds + (size_t)cap * (sizeof(uint64_t) + sizeof(int16_t)));
+  const uint32_t lane = threadIdx.x & 63;
+  const int shift = 52 - LOG2B - (31 - __clz((int)t));
+  for (uint32_t b = lane; b <= (uint32_t)B; b += kWave) bcnt[b] = 0;
+  __syncthreads();
+  const bool fast = n <= (uint32_t)kDense * kWave;
+  uint64_t key[kDense];
+  if (fast) {
+    uint64_t rec[kDense];
+#pragma unroll
+    for (int r = 0; r < kDense; ++r)
+      if ((uint32_t)r * kWave < n) {
+        const uint32_t p = r * kWave + lane;
+        rec[r] = out[run_offset(p < n ? p : 0u)];
+      }
+#pragma unroll
+    for (int r = 0; r < kDense; ++r)
+      if ((uint32_t)r * kWave < n) {
+        const uint32_t p = r * kWave + lane;
+        key[r] = ((rec[r] >> 16) << 12) | p;
+        if (p < n) {
+          ldsW[p] = (int16_t)(uint16_t)rec[r];
+          atomicAdd(&bcnt[key_bucket<B>(rec[r] >> 16, shift)], 1u);
+        }
+      }
+  } else {
+    for (uint32_t p = lane; p < n; p += kWave) {
+      const uint64_t rc = out[run_offset(p)];
+      ldsW[p] = (int16_t)(uint16_t)rc;
+      atomicAdd(&bcnt[key_bucket<B>(rc >> 16, shift)], 1u);
+    }
+  }
+  __syncthreads();
+  bucket_scan<B>(bcnt);
+  __syncthreads();
+  if (fast) {
+#pragma unroll
+    for (int r = 0; r < kDense; ++r)
+      if ((uint32_t)r * kWave < n && r * kWave + lane < n)
+        sorted[atomicAdd(&bcnt[key_bucket<B>(key[r] >> 12, shift)], 1u)] = key[r];
+  } else {
+    for (uint32_t p = lane; p < n; p += kWave) {
+      const uint64_t k = ((out[run_offset(p)] >> 16) << 12) | p;
+      sorted[atomicAdd(&bcnt[key_bucket<B>(k >> 12, shift)], 1u)] = k;
+    }
+  }
+  __syncthreads();  // every record has been read: the runs are rewritten in place
+  rank_in_buckets<B>(sorted, bcnt, n, shift, [&](uint64_t x, uint32_t p) {
+    out[run_offset(p)] = ((x >> 12) << 16) | (uint16_t)ldsW[x & 0xFFFu];
+  });
+  __syncthreads();
+}
+
+template <int B>
+__global__ __launch_bounds__(kWave) void k_sort(DevNet P, uint32_t t) {
+  extern __shared__ uint64_t lds[];
+  if (P.tDev) t = *P.tDev;
+  if (blockIdx.x == 0 && threadIdx.x < kXcds) P.winCtr[threadIdx.x * kCtrStride] = 0;  // for the next k_collect
+  const uint32_t home = blockIdx.x % kXcds;
+  uint32_t x = home;
+  for (uint32_t j0, j1; (j0 = next_item(P.workCtr, P.N, x, home, &j1)) < P.N;)
+    for (uint32_t j = j0; j < j1; ++j)
+      sort_records<B>(t, P.keyCount[j], (uint8_t *)lds, P.cap,
+                      P.keys + (size_t)(j >> 6) * P.cap * kWave + (j & 63) * kRun);
+}
+
 // Blocks of kBlock threads: each wave holds 64 consecutive neurons starting
 // at a multiple of 64, as publish_spikes requires.
 template <int PK>
 __global__ __launch_bounds__(kBlock) void k_integrate(DevNet P, uint32_t t, ExtView X, NextScan S) {
   __shared__ uint8_t stage[kBlock / kWave][kStageBytes];
+  if (P.tDev) t = *P.tDev;
   const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   if (blockIdx.x == 0 && threadIdx.x < kXcds) P.workCtr[threadIdx.x * kCtrStride] = 0;  // for the next k_gather
   const uint32_t j0 = blockIdx.x * kBlock + wave * kWave;
@@ -1353,6 +1603,11 @@ __global__ __launch_bounds__(kWaveLTP) void k_batch_ltp(const DevNet *nets, cons
   }
 }
 
+// The step counter of graph-replayed steps: set before a replay, advanced
+// by the last node of each step.
+__global__ void k_set_t(uint32_t *t, uint32_t v) { *t = v; }
+__global__ void k_tick(uint32_t *t) { *t += 1u; }
+
 // Fills n doubles with v.
 __global__ void k_fill_f64(double *p, double v, uint64_t n) {
   for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
@@ -1368,14 +1623,16 @@ using BatchFn = void (*)(const DevNet *, const BatchItem *, uint32_t, uint32_t, 
                          uint16_t *, uint32_t *);
 using BatchGatherFn = void (*)(const DevNet *, const BatchItem *, uint32_t, uint32_t, uint32_t, uint32_t, uint64_t *,
                                int16_t *, uint16_t *, uint32_t *, uint32_t *);
+using SortFn = void (*)(DevNet, uint32_t);
 struct KernelShape {
   int K, B;
   StepFn step;
   GatherFn gather;
   BatchFn batch;
   BatchGatherFn batchGather;
+  SortFn sort;
 };
-#define DPSNN_SHAPE(k, b) {k, b, k_step<k, b>, k_gather<k, b>, k_batch_step<k, b>, k_batch_gather<k, b>}
+#define DPSNN_SHAPE(k, b) {k, b, k_step<k, b>, k_gather<k, b>, k_batch_step<k, b>, k_batch_gather<k, b>, k_sort<b>}
 static const KernelShape kShapes[] = {
     DPSNN_SHAPE(8, 64),  DPSNN_SHAPE(8, 256),  DPSNN_SHAPE(8, 1024),
     DPSNN_SHAPE(16, 64), DPSNN_SHAPE(16, 256), DPSNN_SHAPE(16, 1024),
@@ -1428,6 +1685,8 @@ struct dpsnn_net {
   ExternalStep extStep;
   StatWriter writer;
   bool writing = false;
+  StdpEventWriter stdpWriter;  // the STDPevent file (host restatement from the spikes)
+  bool stdpEvents = false;
   dpsnn_options opt{};
   hipStream_t stream = nullptr;
   DevNet dev{};
@@ -1466,10 +1725,24 @@ struct dpsnn_net {
   bool split = false;        // k_gather + k_integrate instead of the fused k_step
   uint32_t gatherGrid = 0;
   std::vector<uint16_t> inSynLocal;
+  // stencil-window phase 1 (k_collect + k_sort)
+  WindowLayout window;       // host staging until upload
+  bool winOn = false;        // the device holds the layout
+  uint32_t winGrid = 0, sortGrid = 0;
+  size_t winLds = 0;
   // profiling
   bool profile = false;
-  std::vector<std::pair<hipEvent_t, hipEvent_t>> profEv;  // integrate launches
+  struct ProfEv {
+    hipEvent_t a, b;
+    uint32_t steps;  // steps between a and b
+  };
+  std::vector<ProfEv> profEv;
   size_t profUsed = 0;
+  // graph-replayed steps (kGraphSteps consecutive plain steps per replay)
+  hipGraphExec_t stepGraph = nullptr;
+  const uint64_t *graphKeys = nullptr;  // the key scratch the graph was captured with
+  uint32_t *d_t = nullptr;
+  bool graphs = true;        // DPSNN_GRAPH=0: plain launches only
   bool collect = false;  // raster / rates are being produced
 };
 
@@ -1504,12 +1777,50 @@ static void free_all(dpsnn_net *n) {
     s = ExtSlot{};
   }
   for (auto &e : n->profEv) {
-    hipEventDestroy(e.first);
-    hipEventDestroy(e.second);
+    hipEventDestroy(e.a);
+    hipEventDestroy(e.b);
   }
+  if (n->stepGraph) hipGraphExecDestroy(n->stepGraph);
+  n->stepGraph = nullptr;
   n->profEv.clear();
   if (n->stream) hipStreamDestroy(n->stream);
   n->stream = nullptr;
+}
+
+// Uploads the stencil-window layout and sizes k_collect (one workgroup of
+// kCollectThreads per CU: its LDS holds a whole window) and k_sort.
+static int upload_window(dpsnn_net *n, int cus) {
+  const WindowLayout &L = n->window;
+  DevNet &P = n->dev;
+  const size_t lds = collect_lds(L.slots * L.npc, L.unit);
+  int maxLds = 0;
+  HIP_TRY(hipDeviceGetAttribute(&maxLds, hipDeviceAttributeMaxSharedMemoryPerBlock, n->opt.device));
+  if (lds > (size_t)maxLds) return 0;  // the gather form stays
+  if (lds > (size_t)64 * 1024)
+    HIP_TRY(hipFuncSetAttribute((const void *)k_collect, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+  uint32_t *d_item;
+  uint16_t *d_desc;
+  uint64_t *d_chunk;
+  int32_t *d_cols;
+  int rc;
+  if ((rc = dev_alloc(n, &d_item, L.item.size())) || (rc = upload(n, d_item, L.item.data(), L.item.size())) ||
+      (rc = dev_alloc(n, &d_desc, L.desc.size())) || (rc = upload(n, d_desc, L.desc.data(), L.desc.size())) ||
+      (rc = dev_alloc(n, &d_chunk, L.chunk.size())) || (rc = upload(n, d_chunk, L.chunk.data(), L.chunk.size())) ||
+      (rc = dev_alloc(n, &d_cols, L.cols.size())) || (rc = upload(n, d_cols, L.cols.data(), L.cols.size())) ||
+      (rc = dev_alloc(n, &P.winCtr, kXcds * kCtrStride)))
+    return rc;
+  HIP_TRY(hipMemsetAsync(P.winCtr, 0, sizeof(uint32_t) * kXcds * kCtrStride, n->stream));
+  HIP_TRY(hipStreamSynchronize(n->stream));  // the host staging is dropped after upload
+  P.winItem = d_item; P.winDesc = d_desc; P.winChunk = d_chunk; P.winCols = d_cols;
+  P.winSlots = L.slots; P.winPasses = L.passes; P.winUnit = L.unit; P.winUnitsPerCol = L.unitsPerCol;
+  P.winUnits = L.units;
+  n->winLds = lds;
+  n->winGrid = round_up(std::min<uint32_t>(L.units, (uint32_t)cus), kXcds);
+  const KernelShape &K = kShapes[n->shape];
+  n->sortGrid = round_up(
+      std::min<uint32_t>(P.N, (uint32_t)cus * resident_waves((const void *)K.sort, phase1_lds(P.cap, K.B))), kXcds);
+  n->winOn = true;
+  return 0;
 }
 
 static int build_device(dpsnn_net *n) {
@@ -1627,12 +1938,8 @@ static int build_device(dpsnn_net *n) {
     std::vector<int32_t> lastSpike(N, -100000);  // DPSNN_localNet_init.c:217
     if ((rc = dev_alloc(n, &P.lastSpike, N)) || (rc = upload(n, P.lastSpike, lastSpike.data(), N))) return rc;
     // tables, initLongTermPlasticity DPSNN_neuron_sim.c:45-92
-    std::vector<float> causal(kStdpSpan), anti(kStdpSpan);
-    causal[0] = q.causalSTDPmaxAbsValue * q.STDPmultiplier;
-    causal[1] = causal[0];
-    anti[0] = q.antiCausalCoeff * q.antiCausalSTDPmaxAbsValue * q.STDPmultiplier;
-    for (uint32_t i = 1; i < kStdpSpan; ++i) causal[i] = (1 - 1 / q.tauSTDP_ms) * causal[i - 1];
-    for (uint32_t i = 1; i < kStdpSpan; ++i) anti[i] = (1 - 1 / q.tauSTDP_ms) * anti[i - 1];
+    std::vector<float> causal, anti;
+    stdp_tables(q, causal, anti);
     std::vector<float> maxSyn(size_t(P.nsub) * P.nsub), rev(size_t(P.nsub) * P.nsub);
     for (uint32_t a = 0; a < P.nsub; ++a)
       for (uint32_t b = 0; b < P.nsub; ++b) {
@@ -1669,10 +1976,12 @@ static int build_device(dpsnn_net *n) {
   n->stdpBlocks = std::min<uint32_t>((N + 3) / 4, (uint32_t)cus * 16);
   n->split = true;
   if (const char *e = std::getenv("DPSNN_SPLIT")) n->split = std::atoi(e) != 0;
+  if (const char *e = std::getenv("DPSNN_GRAPH")) n->graphs = std::atoi(e) != 0;
   n->gatherGrid = round_up(
       std::min<uint32_t>(N, (uint32_t)cus * resident_waves((const void *)kShapes[n->shape].gather,
                                                                phase1_lds(P.cap, kShapes[n->shape].B))),
       kXcds);
+  if (n->window.ok && (rc = upload_window(n, cus))) return rc;
   // the key scratch of stand-alone steps is allocated at the first
   // dpsnn_step (a network stepped only by a batch uses the batch's)
   if ((rc = dev_alloc(n, &P.workCtr, kXcds * kCtrStride))) return rc;
@@ -1684,7 +1993,7 @@ static int build_device(dpsnn_net *n) {
   const size_t popPer = size_t(P.ncols) * P.nsub;
   if ((rc = dev_alloc(n, &n->d_popCnt, popPer * n->drainEvery))) return rc;
   HIP_TRY(hipMemsetAsync(n->d_popCnt, 0, sizeof(uint32_t) * popPer * n->drainEvery, n->stream));
-  n->wantRaster = (n->writing && !q.spikes_writeDisable) || n->opt.keep_spikes;
+  n->wantRaster = (n->writing && !q.spikes_writeDisable) || n->opt.keep_spikes || n->stdpEvents;
   n->collect = n->writing || n->opt.keep_spikes;
   if (n->wantRaster) {
     n->rasterCap = N * std::min<uint32_t>(n->drainEvery, 1000) + 1;
@@ -1760,6 +2069,8 @@ static int drain(dpsnn_net *n) {
         !n->writer.step(n->pendingFrom + k, n->h_ids.data() + pos, n->h_te.data() + pos, cnt,
                         n->h_pop.data() + popPer * k))
       return fail(DPSNN_ERR_IO, "writing output files in %s failed", n->opt.output_dir);
+    if (n->stdpEvents && !n->stdpWriter.step(n->pendingFrom + k, n->h_ids.data() + pos, cnt))
+      return fail(DPSNN_ERR_IO, "writing the STDPevent file in %s failed", n->opt.output_dir);
     // last_spikes reports the scan of the last completed step (ms t - 1);
     // the record of ms t, already produced, waits for the next step
     const uint32_t m = n->pendingFrom + k;
@@ -1849,14 +2160,15 @@ static int prof_begin(dpsnn_net *n) {
     hipEvent_t a, b;
     HIP_TRY(hipEventCreate(&a));
     HIP_TRY(hipEventCreate(&b));
-    n->profEv.emplace_back(a, b);
+    n->profEv.push_back({a, b, 0});
   }
-  HIP_TRY(hipEventRecord(n->profEv[n->profUsed].first, n->stream));
+  HIP_TRY(hipEventRecord(n->profEv[n->profUsed].a, n->stream));
   return 0;
 }
-static int prof_end(dpsnn_net *n) {
+static int prof_end(dpsnn_net *n, uint32_t steps = 1) {
   if (!n->profile) return 0;
-  HIP_TRY(hipEventRecord(n->profEv[n->profUsed].second, n->stream));
+  HIP_TRY(hipEventRecord(n->profEv[n->profUsed].b, n->stream));
+  n->profEv[n->profUsed].steps = steps;
   n->profUsed++;
   return 0;
 }
@@ -1976,15 +2288,20 @@ static int post_step(dpsnn_net *n, StepPlan &sp) {
 
 // Step t of one network: delivery + dynamics of t, and the fired scan of
 // t+1 (which the reference runs at the head of step t+1; nothing reads the
-// scan's outputs in between).  The scan of step 0 finds no spikes.
-static int one_step(dpsnn_net *n) {
+// scan's outputs in between).  The scan of step 0 finds no spikes.  P is
+// the network's device view the kernels are launched with.
+static int one_step(dpsnn_net *n, const DevNet &P) {
   StepPlan sp;
   int rc;
   if ((rc = pre_step(n, sp)) || (rc = prof_begin(n))) return rc;
-  const DevNet &P = n->dev;
   const KernelShape &K = kShapes[n->shape];
   if (n->split) {
-    hipLaunchKernelGGL(K.gather, dim3(n->gatherGrid), dim3(kWave), n->ldsBytes, n->stream, P, sp.t, sp.ltd);
+    if (n->winOn && sp.t >= kPackT && !sp.ltd) {
+      hipLaunchKernelGGL(k_collect, dim3(n->winGrid), dim3(kCollectThreads), n->winLds, n->stream, P, sp.t);
+      hipLaunchKernelGGL(K.sort, dim3(n->sortGrid), dim3(kWave), phase1_lds(P.cap, K.B), n->stream, P, sp.t);
+    } else {
+      hipLaunchKernelGGL(K.gather, dim3(n->gatherGrid), dim3(kWave), n->ldsBytes, n->stream, P, sp.t, sp.ltd);
+    }
     const uint32_t blocks = (P.N + kBlock - 1) / kBlock;
     if (sp.t >= kPackT)
       hipLaunchKernelGGL(k_integrate<1>, dim3(blocks), dim3(kBlock), 0, n->stream, P, sp.t, sp.X, sp.S);
@@ -1996,6 +2313,100 @@ static int one_step(dpsnn_net *n) {
   }
   if ((rc = prof_end(n))) return rc;
   return post_step(n, sp);
+}
+
+// ----------------------------------------------------------------------------
+// Graph-replayed steps.  kGraphSteps consecutive plain steps are captured
+// once into a HIP graph whose kernels read the step's t from the device
+// counter d_t (set before each replay, advanced by a k_tick node after each
+// step), so that the same graph replays any run of such steps.  Steps
+// qualify when nothing the host does around a step is needed: split form,
+// Philox drive, no raster or rates collected, no plasticity, t >= kPackT
+// (the packed scratch), and the next step still inside the run.
+// ----------------------------------------------------------------------------
+constexpr uint32_t kGraphSteps = 10;
+
+static bool graph_run_ok(const dpsnn_net *n, uint32_t steps) {
+  const uint32_t t = n->t, T = n->par.pub.totalSimTime_ms;
+  if (!n->graphs || !n->split || n->dev.drive != DPSNN_DRIVE_PHILOX || n->collect) return false;
+  if (t < kPackT || (uint64_t)t + steps >= T) return false;
+  for (uint32_t k = 0; k <= steps; ++k)
+    if (plastic_at(n, t + k)) return false;
+  return true;
+}
+
+// Launches of one plain step in the graph form (t from P.tDev).
+static void launch_graph_step(dpsnn_net *n, const DevNet &G) {
+  const KernelShape &K = kShapes[n->shape];
+  if (n->winOn) {
+    hipLaunchKernelGGL(k_collect, dim3(n->winGrid), dim3(kCollectThreads), n->winLds, n->stream, G, 0u);
+    hipLaunchKernelGGL(K.sort, dim3(n->sortGrid), dim3(kWave), phase1_lds(G.cap, K.B), n->stream, G, 0u);
+  } else {
+    hipLaunchKernelGGL(K.gather, dim3(n->gatherGrid), dim3(kWave), n->ldsBytes, n->stream, G, 0u, 0);
+  }
+  const uint32_t blocks = (G.N + kBlock - 1) / kBlock;
+  hipLaunchKernelGGL(k_integrate<1>, dim3(blocks), dim3(kBlock), 0, n->stream, G, 0u, ExtView{nullptr, nullptr, nullptr},
+                     NextScan{nullptr, 0, 1});
+  hipLaunchKernelGGL(k_tick, dim3(1), dim3(1), 0, n->stream, n->d_t);
+}
+
+static int capture_step_graph(dpsnn_net *n, const DevNet &P) {
+  if (n->stepGraph) {
+    HIP_TRY(hipGraphExecDestroy(n->stepGraph));
+    n->stepGraph = nullptr;
+  }
+  if (!n->d_t) {
+    int rc = dev_alloc(n, &n->d_t, 1);
+    if (rc) return rc;
+  }
+  DevNet G = P;
+  G.tDev = n->d_t;
+  hipGraph_t g = nullptr;
+  HIP_TRY(hipStreamBeginCapture(n->stream, hipStreamCaptureModeThreadLocal));
+  for (uint32_t k = 0; k < kGraphSteps; ++k) launch_graph_step(n, G);
+  const hipError_t e = hipStreamEndCapture(n->stream, &g);
+  if (e != hipSuccess) return fail(DPSNN_ERR_DEVICE, "graph capture of the step failed: %s", hipGetErrorString(e));
+  const hipError_t ei = hipGraphInstantiate(&n->stepGraph, g, nullptr, nullptr, 0);
+  hipGraphDestroy(g);
+  if (ei != hipSuccess) {
+    n->stepGraph = nullptr;
+    return fail(DPSNN_ERR_DEVICE, "graph instantiation failed: %s", hipGetErrorString(ei));
+  }
+  n->graphKeys = P.keys;
+  return 0;
+}
+
+// kGraphSteps steps from n->t by one graph replay.
+static int graph_steps(dpsnn_net *n, const DevNet &P) {
+  int rc;
+  if ((!n->stepGraph || n->graphKeys != P.keys) && (rc = capture_step_graph(n, P))) return rc;
+  hipLaunchKernelGGL(k_set_t, dim3(1), dim3(1), 0, n->stream, n->d_t, n->t);
+  if ((rc = prof_begin(n))) return rc;
+  HIP_TRY(hipGraphLaunch(n->stepGraph, n->stream));
+  if ((rc = prof_end(n, kGraphSteps))) return rc;
+  HIP_TRY(hipGetLastError());
+  n->t += kGraphSteps;
+  return 0;
+}
+
+// Up to nsteps steps with the device view P (graph replays where the steps
+// qualify and `graphs` allows them).
+static int run_steps(dpsnn_net *n, const DevNet &P, uint32_t nsteps, bool graphs) {
+  int rc;
+  for (uint32_t k = 0; k < nsteps;) {
+    if (n->t >= n->par.pub.totalSimTime_ms) {
+      rc = drain(n);
+      return rc ? rc : DPSNN_DONE;
+    }
+    if (graphs && nsteps - k >= kGraphSteps && graph_run_ok(n, kGraphSteps)) {
+      if ((rc = graph_steps(n, P))) return rc;
+      k += kGraphSteps;
+      continue;
+    }
+    if ((rc = one_step(n, P))) return rc;
+    ++k;
+  }
+  return 0;
 }
 
 // ============================================================================
@@ -2062,12 +2473,30 @@ int dpsnn_prepare(const char *inidir, const dpsnn_options *opt, dpsnn_net **out)
   if (n->con.max_in >= kMaxInList)
     return fail(DPSNN_ERR_CONFIG, "a neuron has %u in-list items; this build handles up to %u", n->con.max_in,
                 kMaxInList - 1);
+  // stencil-window phase 1: on for large networks; DPSNN_WINDOW=1 forces
+  // it on (tests of small networks), 0 off.  A connectome whose stencil
+  // window does not fit the LDS keeps the gather form.
+  {
+    bool want = n->con.N >= kWindowMinNeurons;
+    if (const char *e = std::getenv("DPSNN_WINDOW")) want = std::atoi(e) != 0;
+    if (want) {
+      const uint32_t unitsPerCol = std::max<uint32_t>(1, (kWindowUnitsWanted + q.globCFT - 1) / q.globCFT);
+      build_window(n->par, n->con, unitsPerCol, kWindowMaxEntries, n->window, n->opt.host_threads);
+    }
+  }
   if (n->opt.drive == DPSNN_DRIVE_REPLAY) n->replay.init(n->con, &n->rng);
   n->writing = n->opt.output_dir && n->opt.output_dir[0];
   if (n->opt.drop_host_synapses && n->stdp && n->writing && !q.statSynPeriodicProbe_writeDisable)
     return fail(DPSNN_ERR_CONFIG, "synPeriodicProbe files need the host synapse list (drop_host_synapses is set)");
   if (n->writing && !n->writer.init(n->par, n->opt.output_dir))
     return fail(DPSNN_ERR_IO, "cannot write output files in %s", n->opt.output_dir);
+  n->stdpEvents = n->writing && n->stdp && !n->par.stdpEventDisable;
+  if (n->stdpEvents) {
+    if (n->opt.drop_host_synapses)
+      return fail(DPSNN_ERR_CONFIG, "the STDPevent file needs the host synapse list (drop_host_synapses is set)");
+    if (!n->stdpWriter.init(n->par, n->con, n->opt.output_dir))
+      return fail(DPSNN_ERR_CONFIG, "cannot set up the STDPevent writer");
+  }
   n->initSeconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
   *out = n.release();
   return 0;
@@ -2100,6 +2529,10 @@ uint64_t dpsnn_device_bytes_needed(const dpsnn_net *n) {
     b += r(4 * cap) + r(8 * cap) + r(4) + r(4 * drainEvery);
   }
   if (n->opt.drive == DPSNN_DRIVE_REPLAY) b += kExtRing * r(4 * (N + 1));
+  const WindowLayout &L = n->window;
+  if (L.ok)
+    b += r(4 * L.item.size()) + r(2 * L.desc.size()) + r(8 * L.chunk.size()) + r(4 * L.cols.size()) +
+         r(4 * kXcds * kCtrStride);
   return b;
 }
 
@@ -2122,6 +2555,7 @@ int dpsnn_upload(dpsnn_net *n) {
   std::vector<int16_t>().swap(n->con.in_w);
   std::vector<uint32_t>().swap(n->con.in_syn);
   std::vector<uint16_t>().swap(n->inSynLocal);
+  n->window = WindowLayout{};
   if (n->opt.drop_host_synapses) {
     std::vector<uint32_t>().swap(n->con.pre);
     std::vector<uint32_t>().swap(n->con.post);
@@ -2157,15 +2591,7 @@ int dpsnn_step(dpsnn_net *n, uint32_t nsteps) {
   HIP_TRY(hipSetDevice(n->opt.device));
   int rc0 = ensure_scratch(n);
   if (rc0) return rc0;
-  for (uint32_t k = 0; k < nsteps; ++k) {
-    if (n->t >= n->par.pub.totalSimTime_ms) {
-      int rc = drain(n);
-      return rc ? rc : DPSNN_DONE;
-    }
-    int rc = one_step(n);
-    if (rc) return rc;
-  }
-  return 0;
+  return run_steps(n, n->dev, nsteps, true);
 }
 
 int dpsnn_sync(dpsnn_net *n) {
@@ -2260,13 +2686,15 @@ int dpsnn_kernel_time(dpsnn_net *n, int kernel, double *avg_ms, uint64_t *launch
   if (kernel != 0) return fail(DPSNN_ERR_ARG, "kernel 0 (k_step) is the only timed kernel");
   HIP_TRY(hipStreamSynchronize(n->stream));
   double total = 0;
+  uint64_t steps = 0;
   for (size_t k = 0; k < n->profUsed; ++k) {
     float ms = 0;
-    HIP_TRY(hipEventElapsedTime(&ms, n->profEv[k].first, n->profEv[k].second));
+    HIP_TRY(hipEventElapsedTime(&ms, n->profEv[k].a, n->profEv[k].b));
     total += ms;
+    steps += n->profEv[k].steps;
   }
-  if (avg_ms) *avg_ms = n->profUsed ? total / (double)n->profUsed : 0.0;
-  if (launches) *launches = n->profUsed;
+  if (avg_ms) *avg_ms = steps ? total / (double)steps : 0.0;
+  if (launches) *launches = steps;
   n->profUsed = 0;
   return 0;
 }
@@ -2276,8 +2704,146 @@ void dpsnn_destroy(dpsnn_net *n) {
   if (n->batch) dpsnn_batch_remove(n->batch, n);
   drain(n);
   n->writer.close();
+  n->stdpWriter.close();
   free_all(n);
   delete n;
+}
+
+}  // extern "C"
+
+// ============================================================================
+// Checkpoint / restore of a network between steps: the device-resident
+// state (neurons, bath cursors, Philox counters, spike history, counters,
+// STDP synapse state) and the host stream state (ran3 with its NormDev
+// cache, the replayed bath's next times, the open output files and the
+// counts of the open rates window).  The connectome is not stored: restore
+// rebuilds it from the same ini files (the generator is deterministic) and
+// checks the sizes recorded in the file.
+// ============================================================================
+namespace {
+constexpr char kCkptMagic[8] = {'D', 'P', 'S', 'N', 'N', 'C', 'K', '1'};
+struct CkptHeader {
+  char magic[8];
+  uint32_t N, RS, NW, drive, stdp, t, nsub, ncols, writer, spikesOpen, ratesOpen, pad;
+  uint64_t S, items;
+};
+
+struct CkptFile {
+  FILE *f = nullptr;
+  ~CkptFile() {
+    if (f) std::fclose(f);
+  }
+};
+
+template <class T>
+int ck_write(FILE *f, const T *h, size_t n) {
+  if (n && std::fwrite(h, sizeof(T), n, f) != n) return fail(DPSNN_ERR_IO, "checkpoint write failed");
+  return 0;
+}
+template <class T>
+int ck_read(FILE *f, T *h, size_t n) {
+  if (n && std::fread(h, sizeof(T), n, f) != n) return fail(DPSNN_ERR_IO, "checkpoint file truncated");
+  return 0;
+}
+template <class T>
+int ck_dev_write(FILE *f, const T *d, size_t n) {
+  std::vector<T> h(n);
+  HIP_TRY(hipMemcpy(h.data(), d, n * sizeof(T), hipMemcpyDeviceToHost));
+  return ck_write(f, h.data(), n);
+}
+template <class T>
+int ck_dev_read(FILE *f, T *d, size_t n) {
+  std::vector<T> h(n);
+  int rc = ck_read(f, h.data(), n);
+  if (rc) return rc;
+  HIP_TRY(hipMemcpy(d, h.data(), n * sizeof(T), hipMemcpyHostToDevice));
+  return 0;
+}
+
+// The device arrays of the state, in file order (the STDP ones last).
+template <class Fn>
+int ck_device_arrays(dpsnn_net *n, Fn &&fn) {
+  DevNet &P = n->dev;
+  const size_t N = P.N, S = n->con.S;
+  int rc;
+  if ((rc = fn(P.v, N)) || (rc = fn(P.c, N)) || (rc = fn(P.Tr, N)) || (rc = fn(P.Te, N)) ||
+      (rc = fn(P.assigned, N)) || (rc = fn(P.philoxCtr, N)) || (rc = fn(P.bits, size_t(P.RS) * P.NW)) ||
+      (rc = fn(P.teHist, size_t(P.RS) * N)) || (rc = fn(P.counters, size_t(kCounterSlots) * kCounterStride)))
+    return rc;
+  if (n->stdp && ((rc = fn(P.lastSpike, N)) || (rc = fn(P.wT, S)) || (rc = fn(P.tdT, S)) || (rc = fn(P.lastT, S))))
+    return rc;
+  return 0;
+}
+}  // namespace
+
+extern "C" {
+
+int dpsnn_checkpoint(dpsnn_net *n, const char *path) {
+  if (!n || !path) return fail(DPSNN_ERR_ARG, "null argument");
+  if (!n->uploaded) return fail(DPSNN_ERR_ARG, "network not uploaded (dpsnn_upload)");
+  if (n->batch) return fail(DPSNN_ERR_ARG, "network is a member of a batch: remove it first");
+  if (n->stdpEvents) return fail(DPSNN_ERR_ARG, "a network writing the STDPevent file cannot be checkpointed");
+  HIP_TRY(hipSetDevice(n->opt.device));
+  int rc = dpsnn_sync(n);  // every step's files are written up to now
+  if (rc) return rc;
+  n->writer.flush();
+  CkptFile cf;
+  if (!(cf.f = std::fopen(path, "wb"))) return fail(DPSNN_ERR_IO, "cannot open %s", path);
+  const StatWriter::State ws = n->writer.state();
+  CkptHeader h{};
+  std::memcpy(h.magic, kCkptMagic, sizeof h.magic);
+  h.N = n->dev.N; h.RS = n->dev.RS; h.NW = n->dev.NW; h.drive = n->dev.drive; h.stdp = n->stdp;
+  h.t = n->t; h.nsub = n->dev.nsub; h.ncols = n->dev.ncols; h.writer = n->writing;
+  h.spikesOpen = ws.spikesOpen; h.ratesOpen = ws.ratesOpen;
+  h.S = n->con.S; h.items = n->items;
+  if ((rc = ck_write(cf.f, &h, 1))) return rc;
+  if ((rc = ck_device_arrays(n, [&](auto *d, size_t cnt) { return ck_dev_write(cf.f, d, cnt); }))) return rc;
+  const Ran3::State rs = n->rng.state();
+  if ((rc = ck_write(cf.f, &rs, 1))) return rc;
+  if (n->dev.drive == DPSNN_DRIVE_REPLAY && (rc = ck_write(cf.f, n->replay.next_times().data(), h.N))) return rc;
+  if (n->writing && (rc = ck_write(cf.f, ws.perPop.data(), ws.perPop.size()))) return rc;
+  if (std::fflush(cf.f) != 0) return fail(DPSNN_ERR_IO, "checkpoint write failed");
+  return 0;
+}
+
+int dpsnn_restore(const char *inidir, const dpsnn_options *opt, const char *path, dpsnn_net **out) {
+  if (!inidir || !path || !out) return fail(DPSNN_ERR_ARG, "null argument");
+  *out = nullptr;
+  CkptFile cf;
+  if (!(cf.f = std::fopen(path, "rb"))) return fail(DPSNN_ERR_IO, "cannot open %s", path);
+  CkptHeader h{};
+  int rc = ck_read(cf.f, &h, 1);
+  if (rc) return rc;
+  if (std::memcmp(h.magic, kCkptMagic, sizeof h.magic) != 0)
+    return fail(DPSNN_ERR_IO, "%s is not a network checkpoint", path);
+  dpsnn_net *n = nullptr;
+  if ((rc = dpsnn_create(inidir, opt, &n))) return rc;
+  std::unique_ptr<dpsnn_net, void (*)(dpsnn_net *)> guard(n, dpsnn_destroy);
+  if (h.N != n->dev.N || h.S != n->con.S || h.items != n->items || h.RS != n->dev.RS || h.NW != n->dev.NW ||
+      h.nsub != n->dev.nsub || h.ncols != n->dev.ncols || h.stdp != (uint32_t)n->stdp)
+    return fail(DPSNN_ERR_CONFIG, "the checkpoint was taken on another network (sizes differ)");
+  if (h.drive != n->dev.drive) return fail(DPSNN_ERR_CONFIG, "the checkpoint was taken with another drive");
+  if (h.t > n->par.pub.totalSimTime_ms)
+    return fail(DPSNN_ERR_CONFIG, "the checkpoint is at %u ms, past env_totalSimTime_ms", h.t);
+  HIP_TRY(hipSetDevice(n->opt.device));
+  HIP_TRY(hipStreamSynchronize(n->stream));
+  if ((rc = ck_device_arrays(n, [&](auto *d, size_t cnt) { return ck_dev_read(cf.f, d, cnt); }))) return rc;
+  Ran3::State rs{};
+  if ((rc = ck_read(cf.f, &rs, 1))) return rc;
+  n->rng.set_state(rs);
+  if (h.drive == DPSNN_DRIVE_REPLAY && (rc = ck_read(cf.f, n->replay.next_times().data(), h.N))) return rc;
+  if (h.writer) {
+    StatWriter::State ws;
+    ws.perPop.resize(size_t(h.ncols) * h.nsub);
+    ws.spikesOpen = h.spikesOpen;
+    ws.ratesOpen = h.ratesOpen;
+    if ((rc = ck_read(cf.f, ws.perPop.data(), ws.perPop.size()))) return rc;
+    if (n->writing && !n->writer.resume(ws, h.t / 1000))
+      return fail(DPSNN_ERR_IO, "cannot reopen the output files in %s", n->opt.output_dir);
+  }
+  n->t = h.t;
+  *out = guard.release();
+  return 0;
 }
 
 }  // extern "C"
@@ -2773,6 +3339,8 @@ struct dpsnn_host_net {
   ExternalReplay replay;
   ExternalStep step;
   uint32_t t = 0;
+  WindowLayout window;
+  uint32_t windowUnitsPerCol = 0;
 };
 
 namespace {
@@ -2845,6 +3413,24 @@ int dpsnn_host_get_inlists(const dpsnn_host_net *h, uint64_t *off, uint32_t *pre
   if (pre_delay) std::memcpy(pre_delay, c.in_pre_d.data(), 4 * items);
   if (weight) std::memcpy(weight, c.in_w.data(), 2 * items);
   if (syn) std::memcpy(syn, c.in_syn.data(), 4 * items);
+  return 0;
+}
+int dpsnn_host_window(dpsnn_host_net *h, uint32_t units_per_col, uint32_t dims[6], uint64_t sizes[4],
+                      uint32_t *item, uint16_t *desc, uint64_t *chunk, int32_t *cols) {
+  if (!h || !dims || !sizes) return fail(DPSNN_ERR_ARG, "null argument");
+  WindowLayout &L = h->window;
+  if (!L.ok || h->windowUnitsPerCol != units_per_col) {
+    build_window(h->par, h->con, units_per_col, kWindowMaxEntries, L);
+    h->windowUnitsPerCol = units_per_col;
+    if (!L.ok) return fail(DPSNN_ERR_CONFIG, "no stencil-window layout: %s", L.why.c_str());
+  }
+  const uint32_t d[6] = {L.slots, L.npc, L.passes, L.unit, L.unitsPerCol, L.units};
+  std::memcpy(dims, d, sizeof d);
+  sizes[0] = L.item.size(); sizes[1] = L.desc.size(); sizes[2] = L.chunk.size(); sizes[3] = L.cols.size();
+  if (item) std::memcpy(item, L.item.data(), 4 * L.item.size());
+  if (desc) std::memcpy(desc, L.desc.data(), 2 * L.desc.size());
+  if (chunk) std::memcpy(chunk, L.chunk.data(), 8 * L.chunk.size());
+  if (cols) std::memcpy(cols, L.cols.data(), 4 * L.cols.size());
   return 0;
 }
 int dpsnn_host_get_external(const dpsnn_host_net *h, double *invNuExt, uint32_t *maxCount) {

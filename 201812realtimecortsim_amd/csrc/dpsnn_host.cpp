@@ -82,6 +82,22 @@ double Ran3::norm() {
   return v2 * fac;
 }
 
+Ran3::State Ran3::state() const {
+  State st{};
+  st.idum = idum_; st.inext = inext_; st.inextp = inextp_; st.iff = iff_; st.set = set_;
+  std::memcpy(st.ma, ma_, sizeof ma_);
+  st.cached = cached_;
+  st.count = count_;
+  return st;
+}
+
+void Ran3::set_state(const State &st) {
+  idum_ = st.idum; inext_ = st.inext; inextp_ = st.inextp; iff_ = st.iff; set_ = st.set;
+  std::memcpy(ma_, st.ma, sizeof ma_);
+  cached_ = st.cached;
+  count_ = st.count;
+}
+
 // --------------------------------------------------------------------------
 // erflib pieces: roundr2i (erflib.c:172-181) and the truncated Gaussian
 // LUT of the bath (erflib.c:41-169).
@@ -396,15 +412,18 @@ bool read_params(const std::string &dir, Params &p, std::string &err, const EnvM
   q.startStatFiles_s = env.u32("env_startStatFiles_s");
   q.stopStatFiles_s = env.u32("env_stopStatFiles_s");
   q.spikes_writeDisable = env.flag("env_spikes_writeDisable");
-  (void)env.flag("env_statSTDPevent_writeDisable");
+  p.stdpEventDisable = env.flag("env_statSTDPevent_writeDisable");
   q.statSynPeriodicProbe_writeDisable = env.flag("env_statSynPeriodicProbe_writeDisable");
   q.statRatesPerPop_writeDisable = env.flag("env_statRatesPerPop_writeDisable");
   q.startRatesPerPop_s = env.u32("env_startRatesPerPop_s");
   q.stopRatesPerPop_s = env.u32("env_stopRatesPerPop_s");
   q.startSynPeriodicProbe_s = env.u32("env_startSynPeriodicProbe_s");
   q.stopSynPeriodicProbe_s = env.u32("env_stopSynPeriodicProbe_s");
-  (void)env.u32("env_startSTDPevent_s");
-  (void)env.u32("env_stopSTDPevent_s");
+  p.startSTDPevent_s = env.u32("env_startSTDPevent_s");
+  p.stopSTDPevent_s = env.u32("env_stopSTDPevent_s");
+  // optional (DPSNN_getParameters.c:754-760): 0 = no debug window
+  const char *dbg = env.get("env_debugPrintEnable_ms");
+  p.debugPrintEnable_ms = dbg ? (uint32_t)std::atoi(dbg) : 0;
   q.ratesSampling = env.u32("env_ratesSampling");
   if (!env.ok) return false;
   if (q.moduloSec == 0) { err = "env_moduloSec must be > 0"; return false; }
@@ -842,6 +861,134 @@ void build_delivery(const Params &p, Connectome &c, bool with_incoming, unsigned
 }
 
 // --------------------------------------------------------------------------
+// Stencil-window layout of the in-lists (see dpsnn_host.h).  The in-lists
+// are those of build_delivery: per target, items in delivery insertion
+// order, i.e. run delay descending, then source ascending.
+// --------------------------------------------------------------------------
+void build_window(const Params &p, const Connectome &c, uint32_t unitsPerCol, uint32_t maxEntries,
+                  WindowLayout &L, unsigned threads) {
+  L = WindowLayout{};
+  const uint32_t N = c.N, npc = p.pub.neuronsPerCM, ncols = p.pub.globCFT;
+  if (N == 0 || c.in_off.size() != size_t(N) + 1 || c.in_pre_d.size() != c.in_off[N]) {
+    L.why = "the in-lists are not on the host";
+    return;
+  }
+  if (npc == 0 || uint64_t(npc) * ncols != N) {
+    L.why = "neurons are not whole columns";
+    return;
+  }
+  if (threads == 0) threads = default_threads();
+  threads = std::max(1u, std::min<unsigned>(threads, ncols));
+  auto parallel = [&](uint32_t count, auto &&body) {
+    std::vector<std::thread> pool;
+    std::atomic<uint32_t> next{0};
+    for (unsigned th = 0; th < threads; ++th)
+      pool.emplace_back([&]() {
+        for (uint32_t i; (i = next.fetch_add(1)) < count;) body(i);
+      });
+    for (auto &t : pool) t.join();
+  };
+  // stencil of each column: the distinct columns its targets' sources lie in
+  std::vector<std::vector<uint32_t>> stencil(ncols);
+  std::vector<uint32_t> maxD(ncols, 0);
+  parallel(ncols, [&](uint32_t col) {
+    std::vector<uint8_t> seen(ncols, 0);
+    uint32_t md = 0;
+    for (uint64_t k = c.in_off[uint64_t(col) * npc]; k < c.in_off[uint64_t(col + 1) * npc]; ++k) {
+      const uint32_t pd = c.in_pre_d[k];
+      seen[(pd & 0xFFFFFFu) / npc] = 1;
+      md = std::max(md, pd >> 24);
+    }
+    for (uint32_t s = 0; s < ncols; ++s)
+      if (seen[s]) stencil[col].push_back(s);
+    maxD[col] = md;
+  });
+  uint32_t slots = 1, passes = 1;
+  for (uint32_t col = 0; col < ncols; ++col) {
+    slots = std::max<uint32_t>(slots, (uint32_t)stencil[col].size());
+    passes = std::max(passes, maxD[col] + 1);
+  }
+  if (uint64_t(slots) * npc + 1 > maxEntries || uint64_t(slots) * npc >= 0xFFFFu) {
+    char buf[160];
+    std::snprintf(buf, sizeof buf, "a stencil window of %u columns x %u neurons does not fit", slots, npc);
+    L.why = buf;
+    return;
+  }
+  L.slots = slots;
+  L.npc = npc;
+  L.passes = passes;
+  L.unitsPerCol = std::max<uint32_t>(std::max<uint32_t>(unitsPerCol, 1), (npc + kWindowMaxUnit - 1) / kWindowMaxUnit);
+  L.unitsPerCol = std::min(L.unitsPerCol, npc);
+  L.unit = (npc + L.unitsPerCol - 1) / L.unitsPerCol;
+  L.unitsPerCol = (npc + L.unit - 1) / L.unit;
+  L.units = ncols * L.unitsPerCol;
+  L.cols.assign(size_t(ncols) * slots, -1);
+  for (uint32_t col = 0; col < ncols; ++col)
+    for (uint32_t k = 0; k < stencil[col].size(); ++k) L.cols[size_t(col) * slots + k] = (int32_t)stencil[col][k];
+  const uint32_t W = kWindowWaves, C = kWindowChunk;
+  // segment bounds of each target of a unit: cum[q][d] = items of run
+  // delay >= d (the in-list is run delay descending)
+  auto unit_bounds = [&](uint32_t u, uint32_t &j0, uint32_t &m, std::vector<uint32_t> &cum) {
+    const uint32_t col = u / L.unitsPerCol, part = u % L.unitsPerCol;
+    j0 = col * npc + part * L.unit;
+    m = std::min(L.unit, (col + 1) * npc - j0);
+    cum.assign(size_t(m) * (passes + 1), 0);
+    for (uint32_t q = 0; q < m; ++q) {
+      const uint64_t b = c.in_off[j0 + q], e = c.in_off[j0 + q + 1];
+      uint32_t *cq = &cum[size_t(q) * (passes + 1)];
+      for (uint64_t k = b; k < e; ++k) ++cq[c.in_pre_d[k] >> 24];
+      for (uint32_t d = passes; d-- > 0;) cq[d] += cq[d + 1];
+    }
+  };
+  const size_t streams = size_t(L.units) * passes * W;
+  L.chunk.assign(streams + 1, 0);
+  parallel(L.units, [&](uint32_t u) {
+    uint32_t j0, m;
+    std::vector<uint32_t> cum;
+    unit_bounds(u, j0, m, cum);
+    for (uint32_t pass = 0; pass < passes; ++pass) {
+      const uint32_t d = passes - 1 - pass;
+      for (uint32_t q = 0; q < m; ++q) {
+        const uint32_t len = cum[size_t(q) * (passes + 1) + d] - cum[size_t(q) * (passes + 1) + d + 1];
+        L.chunk[(size_t(u) * passes + pass) * W + q % W + 1] += (len + C - 1) / C;
+      }
+    }
+  });
+  for (size_t s = 0; s < streams; ++s) L.chunk[s + 1] += L.chunk[s];
+  L.item.assign(L.chunk[streams] * C, (uint32_t)(slots * npc) << 16);
+  L.desc.assign(L.chunk[streams], 0);
+  parallel(L.units, [&](uint32_t u) {
+    uint32_t j0, m;
+    std::vector<uint32_t> cum;
+    unit_bounds(u, j0, m, cum);
+    const uint32_t col = j0 / npc;
+    std::vector<int32_t> slotOf(ncols, -1);
+    for (uint32_t k = 0; k < stencil[col].size(); ++k) slotOf[stencil[col][k]] = (int32_t)k;
+    for (uint32_t pass = 0; pass < passes; ++pass) {
+      const uint32_t d = passes - 1 - pass;
+      for (uint32_t w = 0; w < W; ++w) {
+        uint64_t ch = L.chunk[(size_t(u) * passes + pass) * W + w];
+        for (uint32_t q = w; q < m; q += W) {
+          const uint32_t *cq = &cum[size_t(q) * (passes + 1)];
+          // run delays above d come first in the in-list
+          const uint64_t b = c.in_off[j0 + q] + cq[d + 1], e = c.in_off[j0 + q] + cq[d];
+          uint64_t at = ch * C;
+          for (uint64_t k = b; k < e; ++k, ++at) {
+            const uint32_t src = c.in_pre_d[k] & 0xFFFFFFu, scol = src / npc;
+            const uint32_t widx = (uint32_t)slotOf[scol] * npc + (src - scol * npc);
+            L.item[at] = widx << 16 | (uint16_t)c.in_w[k];
+          }
+          const uint64_t nch = (e - b + C - 1) / C;
+          for (uint64_t x = 0; x < nch; ++x) L.desc[ch + x] = (uint16_t)q;
+          ch += nch;
+        }
+      }
+    }
+  });
+  L.ok = true;
+}
+
+// --------------------------------------------------------------------------
 // External drive replay: regularDynamicTimeStep's bath loop
 // (DPSNN_neuron_sim.c:604-640) with getPoissonLapse (:844-851) -- a LUT
 // index draw then a lapse draw per event, lapses-only past the per-ms cap --
@@ -906,7 +1053,7 @@ bool StatWriter::window(uint32_t sec) const {
   return sec >= q.startStatFiles_s && sec <= q.stopStatFiles_s && sec % q.moduloSec == 0;
 }
 
-FILE *StatWriter::open_file(const char *prefix, uint32_t sec) {
+FILE *StatWriter::open_file(const char *prefix, uint32_t sec, const char *mode) {
   const dpsnn_parameters &q = p_->pub;
   char name[512];
   if (q.howManyOutputFiles == 1)
@@ -917,7 +1064,21 @@ FILE *StatWriter::open_file(const char *prefix, uint32_t sec) {
     std::snprintf(name, sizeof name, "%s-lasteven-C%u-H%u-h%u.dat", prefix, q.globCFT, q.globH, 0u);
   else
     std::snprintf(name, sizeof name, "%s-lastodd-C%u-H%u-h%u.dat", prefix, q.globCFT, q.globH, 0u);
-  return std::fopen((dir_ + "/" + name).c_str(), "w");
+  return std::fopen((dir_ + "/" + name).c_str(), mode);
+}
+
+bool StatWriter::resume(const State &s, uint32_t sec) {
+  close();
+  if (s.perPop.size() != perPop_.size()) return false;
+  perPop_ = s.perPop;
+  if (s.spikesOpen && !(spikes_ = open_file("spikes", sec, "a"))) return false;
+  if (s.ratesOpen && !(rates_ = open_file("ratesPerPop", sec, "a"))) return false;
+  return true;
+}
+
+void StatWriter::flush() {
+  if (spikes_) std::fflush(spikes_);
+  if (rates_) std::fflush(rates_);
 }
 
 bool StatWriter::step(uint32_t t, const uint32_t *ids, const double *te, uint32_t n,
@@ -958,6 +1119,129 @@ bool StatWriter::step(uint32_t t, const uint32_t *ids, const double *te, uint32_
     if (rates_ && ratesSec) { std::fclose(rates_); rates_ = nullptr; }
   }
   return true;
+}
+
+// --------------------------------------------------------------------------
+// STDP tables, initLongTermPlasticity DPSNN_neuron_sim.c:45-92.
+// --------------------------------------------------------------------------
+void stdp_tables(const dpsnn_parameters &q, std::vector<float> &causal, std::vector<float> &anti) {
+  causal.assign(kStdpSpan, 0.0f);
+  anti.assign(kStdpSpan, 0.0f);
+  causal[0] = q.causalSTDPmaxAbsValue * q.STDPmultiplier;
+  causal[1] = causal[0];
+  anti[0] = q.antiCausalCoeff * q.antiCausalSTDPmaxAbsValue * q.STDPmultiplier;
+  for (uint32_t i = 1; i < kStdpSpan; ++i) causal[i] = (1 - 1 / q.tauSTDP_ms) * causal[i - 1];
+  for (uint32_t i = 1; i < kStdpSpan; ++i) anti[i] = (1 - 1 / q.tauSTDP_ms) * anti[i - 1];
+}
+
+// --------------------------------------------------------------------------
+// STDPevent file (statSTDPeventClass::write, DPSNN_stat.c:451-470): one line
+// "%8u %7d %7d %2.7f" (ms, pre, post, contribution) per non-zero LTP
+// contribution at the fired scan (causalSTDP_ms_ofBackwardSynapses,
+// DPSNN_neuron_sim.c:139-177: neurons ascending, each one's incoming
+// synapses in backward-list order) and per non-zero LTD contribution at
+// delivery (synapseSetLastActiveTime_antiCausalSTDP, :207-277, in the
+// delivery loop's order DPSNN_localNet_sim.c:837-999: run delays
+// descending, sources ascending, each source's run continued while the
+// source matches).  The contributions depend on spike times only, so they
+// are restated here from the spikes of each step.
+// --------------------------------------------------------------------------
+bool StdpEventWriter::init(const Params &p, const Connectome &c, const std::string &dir) {
+  p_ = &p;
+  c_ = &c;
+  dir_ = dir;
+  stdp_tables(p.pub, causal_, anti_);
+  const uint32_t N = c.N, D = std::min<uint32_t>(p.pub.D, kDelaySlots);
+  D_ = D;
+  if (c.pre.size() != c.S || c.inc_off.size() != size_t(N) + 1) return false;
+  // first synapse of each (source, delay) run: the backward list is sorted
+  // by (delay, source)
+  first_.assign(size_t(N) * D, UINT32_MAX);
+  for (uint64_t s = c.S; s-- > 0;)
+    if (c.delay[s] < D) first_[size_t(c.pre[s]) * D + c.delay[s]] = (uint32_t)s;
+  lastSpike_.assign(N, -100000);  // DPSNN_localNet_init.c:217
+  lastAct_.assign(c.S, 0);
+  ring_.assign(D, {});
+  buf_.resize(256);
+  return true;
+}
+
+bool StdpEventWriter::writes(uint32_t t) const {
+  const dpsnn_parameters &q = p_->pub;
+  const uint32_t sec = t / 1000, dbg = p_->debugPrintEnable_ms;
+  return fp_ && t >= dbg && (dbg == 0 || t <= dbg + 20) && sec >= q.startStatFiles_s && sec <= q.stopStatFiles_s &&
+         sec % q.moduloSec == 0 && sec >= p_->startSTDPevent_s && sec <= p_->stopSTDPevent_s;
+}
+
+void StdpEventWriter::line(uint32_t t, uint32_t pre, uint32_t post, float v) {
+  const int len = std::snprintf(buf_.data(), buf_.size(), "%8u %7d %7d %2.7f \n", t, (int)pre, (int)post, v);
+  std::fwrite(buf_.data(), 1, (size_t)len, fp_);
+}
+
+bool StdpEventWriter::step(uint32_t t, const uint32_t *ids, uint32_t n) {
+  const dpsnn_parameters &q = p_->pub;
+  const Connectome &c = *c_;
+  const uint32_t sec = t / 1000;
+  if (t % 1000 == 0 && sec >= q.startStatFiles_s && sec <= q.stopStatFiles_s && sec % q.moduloSec == 0 &&
+      sec >= p_->startSTDPevent_s && sec <= p_->stopSTDPevent_s && !dir_.empty()) {
+    if (fp_) std::fclose(fp_);
+    char name[512];
+    if (q.howManyOutputFiles == 1)
+      std::snprintf(name, sizeof name, "STDPevent-sec%u-C%u-H%u-h%03u.dat", sec, q.globCFT, q.globH, 0u);
+    else if (sec == 0)
+      std::snprintf(name, sizeof name, "STDPevent-start-C%u-H%u-h%u.dat", q.globCFT, q.globH, 0u);
+    else
+      std::snprintf(name, sizeof name, "STDPevent-%s-C%u-H%u-h%u.dat", sec % 2 == 0 ? "lasteven" : "lastodd",
+                    q.globCFT, q.globH, 0u);
+    if (!(fp_ = std::fopen((dir_ + "/" + name).c_str(), "w"))) return false;
+  }
+  const bool plastic = t >= q.startPlasticity_ms && t <= q.stopPlasticity_ms;
+  const bool out = writes(t);
+  std::vector<uint32_t> &cur = ring_[t % D_];
+  cur.assign(ids, ids + n);
+  // the fired scan: last emitted spike, LTP
+  for (uint32_t k = 0; k < n; ++k) {
+    const uint32_t i = ids[k];
+    lastSpike_[i] = (int32_t)t;
+    if (!plastic) continue;
+    for (uint64_t x = c.inc_off[i]; x < c.inc_off[i + 1]; ++x) {
+      const uint32_t s = c.inc_syn[x];
+      const int32_t last = lastAct_[s];
+      if (last == 0) continue;
+      const int dt = (int)t - 1 - last;
+      const float f = dt < (int)kStdpSpan ? causal_[dt] : 0.0f;
+      if (out && f != 0.0f && t != 0) line(t, c.pre[s], c.post[s], f);
+    }
+  }
+  // delivery: LTD
+  if (plastic) {
+    for (uint32_t d = D_; d-- > 0;) {
+      if (d > t) continue;
+      const std::vector<uint32_t> &src = ring_[(t - d) % D_];
+      for (uint32_t pre : src) {
+        uint32_t s = first_[size_t(pre) * D_ + d];
+        if (s == UINT32_MAX) continue;
+        for (; s < c.S && c.pre[s] == pre; ++s) {
+          const uint32_t post = c.post[s];
+          lastAct_[s] = (int32_t)t;
+          const int dt = (int)t - lastSpike_[post];
+          const float f = dt < (int)kStdpSpan ? anti_[dt] : 0.0f;
+          if (out && f != 0.0f) line(t, pre, post, f);
+        }
+      }
+    }
+  }
+  if (fp_) std::fflush(fp_);
+  if (t % 1000 == 999 && fp_ && sec >= q.startStatFiles_s && sec <= q.stopStatFiles_s && sec % q.moduloSec == 0) {
+    std::fclose(fp_);
+    fp_ = nullptr;
+  }
+  return true;
+}
+
+void StdpEventWriter::close() {
+  if (fp_) std::fclose(fp_);
+  fp_ = nullptr;
 }
 
 void StatWriter::close() {

@@ -37,6 +37,10 @@ struct Params {
   SubPop sp[kMaxSubPop];
   std::vector<float> prob;        // [src][tgt][x][y] over kMaxStencil^2
   std::vector<uint32_t> bath;     // [x][y]
+  // STDPevent file settings (env_statSTDPevent_writeDisable,
+  // env_start/stopSTDPevent_s, env_debugPrintEnable_ms)
+  int32_t stdpEventDisable = 1;
+  uint32_t startSTDPevent_s = 0, stopSTDPevent_s = 0, debugPrintEnable_ms = 0;
   float prob_at(uint32_t s, uint32_t t, uint32_t x, uint32_t y) const {
     return prob[((size_t(s) * kMaxSubPop + t) * kMaxStencil + x) * kMaxStencil + y];
   }
@@ -64,6 +68,14 @@ class Ran3 {
   int32_t norm_set() const { return set_; }
   double norm_cached() const { return cached_; }
   void set_norm_state(int32_t set, double cached) { set_ = set; cached_ = cached; }
+  // the whole stream position (checkpoints)
+  struct State {
+    int32_t idum, inext, inextp, ma[56], iff, set;
+    double cached;
+    uint64_t count;
+  };
+  State state() const;
+  void set_state(const State &s);
  private:
   int32_t idum_ = -77531, inext_ = 0, inextp_ = 0, ma_[56] = {}, iff_ = 0;
   int32_t set_ = 0;
@@ -108,6 +120,42 @@ bool generate_connectome(const Params &p, Ran3 &rng, Connectome &c, std::string 
 // with_incoming: also build the per-target incoming synapse lists (STDP).
 void build_delivery(const Params &p, Connectome &c, bool with_incoming = true, unsigned threads = 0);
 
+// The in-lists regrouped for the stencil-window form of phase 1.  Every
+// source of a target lies in a few columns around the target's column (its
+// stencil).  A unit is a range of at most kWindowMaxUnit targets of one
+// column; for each run delay d (a pass, descending as in the in-lists),
+// the emission times of the unit's stencil columns in slot t - d fit the
+// device's LDS, and the unit's items of that delay are streamed against
+// them.  Within a (unit, pass), wave w of the workgroup owns the unit's
+// targets q = w, w + kWindowWaves, ... and streams their segments of delay
+// d in target order, each padded to whole chunks of kWindowChunk items.
+//   item:  window index << 16 | (uint16) weight, window index =
+//          slot * npc + (source - first neuron of the slot's column);
+//          padding items carry window index slots * npc (never live)
+//   desc:  per chunk, the index of its target within the unit
+//   chunk: [units][passes][kWindowWaves] + 1, first chunk of each stream
+//   cols:  [columns][slots] source column of each stencil slot, or -1
+// A target's live items of all passes, taken in stream order, are its live
+// in-list items in in-list order.
+constexpr uint32_t kWindowWaves = 16;
+constexpr uint32_t kWindowChunk = 8;
+constexpr uint32_t kWindowMaxUnit = 1024;
+struct WindowLayout {
+  bool ok = false;
+  std::string why;               // why the layout was not built
+  uint32_t slots = 0, npc = 0, passes = 0, unit = 0, unitsPerCol = 0, units = 0;
+  std::vector<int32_t> cols;
+  std::vector<uint32_t> item;
+  std::vector<uint16_t> desc;
+  std::vector<uint64_t> chunk;
+};
+// Builds the layout from c's in-lists with at most `maxSlots` stencil slots
+// (window entries slots * npc + 1 <= maxEntries) and units of
+// ceil(npc / unitsPerCol) targets; sets ok = false (and why) when the
+// connectome does not fit.
+void build_window(const Params &p, const Connectome &c, uint32_t unitsPerCol, uint32_t maxEntries,
+                  WindowLayout &L, unsigned threads = 0);
+
 // Host-side replay of the external Poisson drive (DPSNN_neuron_sim.c:
 // 604-640, :844-865): the reference's ran3 draws in neuron order, step by
 // step, recorded as per-neuron (time, value) lists.
@@ -119,10 +167,42 @@ class ExternalReplay {
  public:
   void init(const Connectome &c, Ran3 *rng);
   void generate(uint32_t t, ExternalStep &out);
+  // next bath event time of each neuron (checkpoints)
+  std::vector<double> &next_times() { return assigned_; }
  private:
   const Connectome *c_ = nullptr;
   Ran3 *rng_ = nullptr;
   std::vector<double> assigned_;
+};
+
+// STDP contribution tables (initLongTermPlasticity, DPSNN_neuron_sim.c:
+// 45-92): causal (LTP) and anti-causal (LTD), kStdpSpan entries each.
+void stdp_tables(const dpsnn_parameters &q, std::vector<float> &causal, std::vector<float> &anti);
+
+// The STDPevent file (DPSNN_stat.c:451-470): every non-zero LTP and LTD
+// contribution, restated on the host from each step's spikes (the
+// contributions depend on spike times only).  Needs the host backward list
+// and incoming lists.  step() is called once per ms, in order, with the
+// neurons whose spikes the fired scan of that ms found.
+class StdpEventWriter {
+ public:
+  bool init(const Params &p, const Connectome &c, const std::string &dir);
+  bool step(uint32_t t, const uint32_t *ids, uint32_t n);
+  void close();
+  ~StdpEventWriter() { close(); }
+ private:
+  bool writes(uint32_t t) const;
+  void line(uint32_t t, uint32_t pre, uint32_t post, float v);
+  const Params *p_ = nullptr;
+  const Connectome *c_ = nullptr;
+  std::string dir_;
+  FILE *fp_ = nullptr;
+  uint32_t D_ = 1;
+  std::vector<float> causal_, anti_;
+  std::vector<uint32_t> first_;              // [N][D] first synapse of each run
+  std::vector<int32_t> lastSpike_, lastAct_;
+  std::vector<std::vector<uint32_t>> ring_;  // spikes of the last D ms
+  std::vector<char> buf_;
 };
 
 // Raster and per-population rate files (DPSNN_stat.c:55-107, :225-296).
@@ -134,9 +214,18 @@ class StatWriter {
   bool step(uint32_t t, const uint32_t *ids, const double *te, uint32_t n,
             const uint32_t *popCounts);
   void close();
+  void flush();
+  // Checkpoints: the counts of the open rates window and which of the
+  // second's files are open; resume reopens those files for appending.
+  struct State {
+    std::vector<uint32_t> perPop;
+    int spikesOpen = 0, ratesOpen = 0;
+  };
+  State state() const { return State{perPop_, spikes_ != nullptr, rates_ != nullptr}; }
+  bool resume(const State &s, uint32_t sec);
   ~StatWriter() { close(); }
  private:
-  FILE *open_file(const char *prefix, uint32_t sec);
+  FILE *open_file(const char *prefix, uint32_t sec, const char *mode = "w");
   bool window(uint32_t sec) const;
   const Params *p_ = nullptr;
   std::string dir_;

@@ -61,6 +61,10 @@ struct dpsnn_server {
   uint64_t nextId = 1, used = 0, budget = 0;
   uint64_t eventsDone = 0, eventsRunning = 0;
   uint32_t uploading = 0;                 // admitted, in uploadQ or being uploaded
+  // bumped whenever admission could change (a job waits, memory or a
+  // running slot frees up); the scheduler sleeps on waiting jobs until the
+  // generation moves past the one it last tried
+  uint64_t admitGen = 0, triedGen = ~uint64_t(0);
   double stepSeconds = 0, uploadSeconds = 0;
   uint32_t hostThreads = 0;
   dpsnn_batch *batch = nullptr;
@@ -114,6 +118,7 @@ void builder_loop(dpsnn_server *s) {
       j->info.built_s = s->now();
       j->info.state = DPSNN_JOB_WAITING;
       s->waiting.push_back(j);
+      s->admitGen++;
       s->cvSched.notify_one();
     }
   }
@@ -137,6 +142,7 @@ void uploader_loop(dpsnn_server *s) {
     std::lock_guard<std::mutex> lk(s->mu);
     s->uploadSeconds += std::chrono::duration<double>(Clock::now() - t0).count();
     s->uploading--;
+    s->admitGen++;
     if (rc) {
       j->net = nullptr;
       s->used -= j->info.device_bytes;
@@ -154,9 +160,11 @@ void scheduler_loop(dpsnn_server *s) {
     std::vector<Job *> admit;
     {
       std::unique_lock<std::mutex> lk(s->mu);
-      // with nothing running and uploads pending, wait for them rather than spin
+      // with nothing running, wait for uploads, or for a change that could
+      // admit a waiting job, rather than spin
       s->cvSched.wait(lk, [&] {
-        return s->stop || !s->ready.empty() || !s->running.empty() || (!s->waiting.empty() && !s->uploading);
+        return s->stop || !s->ready.empty() || !s->running.empty() ||
+               (!s->waiting.empty() && !s->uploading && s->admitGen != s->triedGen);
       });
       if (s->stop) return;
       // first fit in arrival order; admitted jobs go to the uploader
@@ -174,6 +182,7 @@ void scheduler_loop(dpsnn_server *s) {
           ++it;
         }
       }
+      s->triedGen = s->admitGen;
       admit.assign(s->ready.begin(), s->ready.end());
       s->ready.clear();
     }
@@ -185,6 +194,7 @@ void scheduler_loop(dpsnn_server *s) {
         std::lock_guard<std::mutex> lk(s->mu);
         j->net = nullptr;
         s->used -= j->info.device_bytes;
+        s->admitGen++;
         finish_locked(s, j, DPSNN_JOB_FAILED, rc, err);
         continue;
       }
@@ -208,15 +218,19 @@ void scheduler_loop(dpsnn_server *s) {
       }
       dpsnn_batch_remove(s->batch, j->net);
       int frc = rc < 0 ? rc : dpsnn_sync(j->net);
-      if (!frc) frc = dpsnn_get_stats(j->net, &j->info.stats);
+      dpsnn_stats fst{};
+      if (!frc) frc = dpsnn_get_stats(j->net, &fst);
       const std::string ferr = frc ? (rc < 0 ? err : std::string(dpsnn_last_error())) : "";
-      if (frc || !s->o.keep_finished) {
-        dpsnn_destroy(j->net);
-        j->net = nullptr;
-      }
+      const bool drop = frc || !s->o.keep_finished;
+      if (drop) dpsnn_destroy(j->net);
       std::lock_guard<std::mutex> lk(s->mu);
-      if (!j->net) s->used -= j->info.device_bytes;
+      j->info.stats = fst;
+      if (drop) {
+        j->net = nullptr;
+        s->used -= j->info.device_bytes;
+      }
       s->running.erase(std::find(s->running.begin(), s->running.end(), j));
+      s->admitGen++;
       if (frc) {
         finish_locked(s, j, DPSNN_JOB_FAILED, frc, ferr);
       } else {
@@ -331,6 +345,7 @@ int dpsnn_server_take(dpsnn_server *s, uint64_t id, dpsnn_net **net) {
   *net = j->net;
   j->net = nullptr;
   s->used -= j->info.device_bytes;
+  s->admitGen++;
   s->cvSched.notify_one();
   return 0;
 }

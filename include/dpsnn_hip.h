@@ -153,6 +153,21 @@ int dpsnn_get_synapses(dpsnn_net *net, uint32_t *pre, uint32_t *post,
  * DPSNN_synapse.h:63-91) of the device's current synapses. */
 int dpsnn_write_synapse_dump(dpsnn_net *net, const char *path);
 int dpsnn_get_stats(dpsnn_net *net, dpsnn_stats *out);
+
+/* Checkpoint / resume of a network between steps -- the process-level
+ * checkpoint hooks the reference's DAL declares but never implements
+ * (dal.h:52-55).  dpsnn_checkpoint waits for queued work, writes the
+ * pending output, and stores the device-resident state (neuron state, bath
+ * cursors, Philox counters, spike history, counters, STDP synapse state)
+ * and the host stream state (ran3 position and NormDev cache, replayed
+ * bath, open output files and the open rates window) to `path`.
+ * dpsnn_restore builds the network of `inidir` with `opt` (the same ini
+ * files and env settings as the checkpointed one: the connectome is
+ * regenerated, not stored) and loads the state into it; the run then
+ * continues as if never interrupted, appending to the second's output
+ * files.  A network of a batch must leave it before a checkpoint. */
+int dpsnn_checkpoint(dpsnn_net *net, const char *path);
+int dpsnn_restore(const char *inidir, const dpsnn_options *opt, const char *path, dpsnn_net **out);
 /* HIP stream the network runs on (hipStream_t as void*). */
 void *dpsnn_stream(dpsnn_net *net);
 void dpsnn_destroy(dpsnn_net *net);
@@ -280,6 +295,19 @@ int dpsnn_host_get_synapses(const dpsnn_host_net *h, uint32_t *pre,
 int dpsnn_host_get_inlists(const dpsnn_host_net *h, uint64_t *off,
                            uint32_t *pre_delay, int16_t *weight,
                            uint32_t *syn);
+/* The in-lists regrouped for the stencil-window form of phase 1 (the
+ * device's k_collect input), units of ceil(npc / units_per_col) targets:
+ * dims = slots, npc, passes, unit, units per column, units; sizes = the
+ * lengths of item, desc, chunk and cols (null arrays: sizes only).
+ *   item  window index << 16 | (uint16) weight, padded per target segment
+ *         to whole chunks of 8 (padding: window index slots * npc)
+ *   desc  per chunk, its target's index within the unit
+ *   chunk [units][passes][16] + 1: first chunk of each (unit, pass, wave)
+ *         stream; pass p holds run delay passes - 1 - p; wave w streams the
+ *         unit's targets w, w + 16, ...
+ *   cols  [columns][slots]: the source column of each window slot, or -1 */
+int dpsnn_host_window(dpsnn_host_net *h, uint32_t units_per_col, uint32_t dims[6], uint64_t sizes[4],
+                      uint32_t *item, uint16_t *desc, uint64_t *chunk, int32_t *cols);
 /* per-neuron external bath: 1000/(NuExt*CExt) and the per-ms cap */
 int dpsnn_host_get_external(const dpsnn_host_net *h, double *invNuExt,
                             uint32_t *maxCount);

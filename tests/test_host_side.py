@@ -206,3 +206,78 @@ def test_column_parallel_generator_matches_oracle_on_many_columns(pkg, oracle, t
     host.close()
     ora.close()
     pkg.apply_env(pkg.config_dir("c0"))
+
+
+def _check_window(pkg, host, units_per_col):
+    """The stencil-window layout streams, per (unit, pass, wave), each owned
+    target's in-list items of the pass's run delay in in-list order, padded
+    to whole chunks; concatenated over the passes they are the in-list."""
+    L = host.window(units_per_col)
+    inl = host.inlists()
+    off, pre, dly, w = inl["off"], inl["pre"], inl["delay"], inl["weight"]
+    slots, npc, passes, unit, upc = L["slots"], L["npc"], L["passes"], L["unit"], L["units_per_col"]
+    item, desc, chunk, cols = L["item"], L["desc"], L["chunk"], L["cols"]
+    ncols = host.n // npc
+    assert L["units"] == ncols * upc and unit * upc >= npc and unit <= 1024
+    assert passes == int(dly.max()) + 1
+    pad = np.uint32((slots * npc) << 16)
+    # every column's sources lie in its stencil slots
+    for col in range(ncols):
+        srcs = np.unique(pre[off[col * npc]:off[(col + 1) * npc]] // npc)
+        assert set(srcs) <= set(cols[col * slots:(col + 1) * slots][cols[col * slots:(col + 1) * slots] >= 0])
+    seen = np.zeros(host.n, np.int64)  # items streamed per target
+    for u in range(L["units"]):
+        col, part = divmod(u, upc)
+        j0 = col * npc + part * unit
+        m = min(unit, (col + 1) * npc - j0)
+        stencil = cols[col * slots:(col + 1) * slots]
+        for p in range(passes):
+            d = passes - 1 - p
+            for wv in range(16):
+                s = (u * passes + p) * 16 + wv
+                ch = int(chunk[s])
+                for q in range(wv, m, 16):
+                    j = j0 + q
+                    b, e = int(off[j]), int(off[j + 1])
+                    seg = np.arange(b, e)[dly[b:e] == d]
+                    nch = (len(seg) + 7) // 8
+                    assert (desc[ch:ch + nch] == q).all()
+                    got = item[ch * 8:ch * 8 + len(seg)]
+                    widx = got >> 16
+                    src = stencil[widx // npc] * npc + widx % npc
+                    assert np.array_equal(src, pre[seg]), (u, p, q)
+                    assert np.array_equal((got & 0xFFFF).astype(np.uint16).view(np.int16), w[seg])
+                    assert (item[ch * 8 + len(seg):(ch + nch) * 8] == pad).all()
+                    seen[j] += len(seg)
+                    ch += nch
+                assert ch == int(chunk[s + 1])
+    assert np.array_equal(seen, np.diff(off).astype(np.int64))
+
+
+def test_window_layout_streams_the_inlists(c0_pair, pkg):
+    host, _ = c0_pair
+    _check_window(pkg, host, 1)   # units of 625 targets (at most 1024 per unit)
+    _check_window(pkg, host, 5)   # units of 250
+
+
+def test_window_layout_on_grid_edges(pkg, tmp_path):
+    """A 3x3 grid of small columns: corner, edge and centre columns have
+    stencils of 4, 6 and 9 columns."""
+    src = pkg.config_dir("c1")
+    d = tmp_path / "g3"
+    d.mkdir()
+    (d / "column.txt").write_text("0 40 0.5 0.0 3 20 20 15 2 1 500 0.05\n"
+                                  "1 40 0.5 0.0 3 20 20 15 2 1 500 0.05\n"
+                                  "2 20 0.5 0.0 3 10 20 15 2 0 500 0\n")
+    for f in ("connectivity.txt", "bathEfficacy.txt"):
+        (d / f).write_text(open(os.path.join(src, f)).read())
+    pkg.apply_env(src, env_CFX=3, env_CFY=3)
+    host = pkg.HostConnectome(str(d))
+    try:
+        L = host.window(2)
+        assert L["slots"] == 9
+        sizes = sorted(int((L["cols"][c * 9:(c + 1) * 9] >= 0).sum()) for c in range(9))
+        assert sizes == [4, 4, 4, 4, 6, 6, 6, 6, 9]
+        _check_window(pkg, host, 2)
+    finally:
+        host.close()
