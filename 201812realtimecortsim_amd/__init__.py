@@ -134,6 +134,7 @@ _SIGS = {
     "dpsnn_device_bytes_needed": (ctypes.c_uint64, [ctypes.c_void_p]),
     "dpsnn_upload": (ctypes.c_int, [ctypes.c_void_p]),
     "dpsnn_step": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint32]),
+    "dpsnn_step_noalloc": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint32]),
     "dpsnn_checkpoint": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_char_p]),
     "dpsnn_restore": (ctypes.c_int, [ctypes.c_char_p, _p(Options), ctypes.c_char_p, _p(ctypes.c_void_p)]),
     "dpsnn_sync": (ctypes.c_int, [ctypes.c_void_p]),
@@ -305,6 +306,11 @@ class Network:
     def step(self, nsteps=1):
         """Advances nsteps ms; returns True once the programmed time is reached."""
         return _check("dpsnn_step", self._lib.dpsnn_step(self._h, nsteps)) == DONE
+
+    def step_noalloc(self, nsteps=1):
+        """step() without device allocations: the key scratch is the
+        library's static per-device array (dpsnn_step_noalloc)."""
+        return _check("dpsnn_step_noalloc", self._lib.dpsnn_step_noalloc(self._h, nsteps)) == DONE
 
     def sync(self):
         _check("dpsnn_sync", self._lib.dpsnn_sync(self._h))

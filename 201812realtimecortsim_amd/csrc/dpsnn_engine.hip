@@ -39,6 +39,7 @@
 #include <cstdio>
 #include <cstring>
 #include <memory>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -2209,6 +2210,57 @@ static int ensure_scratch(dpsnn_net *n) {
   return 0;
 }
 
+// Key scratch of the allocation-free step (dpsnn_step_noalloc): one static
+// array per device, held from the library's load on that device until the
+// process ends.  Every allocation-free step on a device uses the same
+// array, so such steps of two networks that overlap in time on one device
+// race on it.  Nothing in it needs to start at zero: each step's phase 1
+// writes every count and key its phase 2 reads.
+constexpr size_t kStaticScratchBytes = size_t(128) << 20;
+static __device__ __align__(256) uint8_t scratch[kStaticScratchBytes];
+constexpr int kMaxDevices = 64;
+static std::mutex g_scratchMu;
+static void *g_scratchPtr[kMaxDevices];
+
+// The device address of `scratch` on the current device (set up once per
+// device: the name is a host-side stand-in, not a device address).
+static int static_scratch_ptr(int device, uint8_t **out) {
+  if (device < 0 || device >= kMaxDevices) return fail(DPSNN_ERR_ARG, "device %d out of range", device);
+  std::lock_guard<std::mutex> lock(g_scratchMu);
+  if (!g_scratchPtr[device]) {
+    void *p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p)
+      return fail(DPSNN_ERR_DEVICE, "hipGetSymbolAddress(scratch) failed: %s", hipGetErrorString(e));
+    g_scratchPtr[device] = p;
+  }
+  *out = (uint8_t *)g_scratchPtr[device];
+  return 0;
+}
+
+// n's device view with its key scratch carved from the static array, in
+// the layout ensure_scratch allocates; refuses networks whose scratch is
+// larger than the array.
+static int static_scratch_view(dpsnn_net *n, DevNet &P) {
+  P = n->dev;
+  const size_t slots = n->split ? size_t(P.NW) * kWave : size_t(n->grid) * kWave;
+  auto al = [](size_t b) { return (b + 255) / 256 * 256; };
+  const size_t kb = al(slots * P.cap * sizeof(uint64_t)), wb = al(slots * P.cap * sizeof(int16_t)),
+               cb = al(slots * sizeof(uint32_t)), pb = n->stdp ? al(slots * P.cap * sizeof(uint16_t)) : 0;
+  const size_t need = kb + wb + cb + pb;
+  if (need > kStaticScratchBytes)
+    return fail(DPSNN_ERR_ARG, "the network's step needs %zu bytes of key scratch; the allocation-free step holds %zu",
+                need, kStaticScratchBytes);
+  uint8_t *base;
+  int rc = static_scratch_ptr(n->opt.device, &base);
+  if (rc) return rc;
+  P.keys = (uint64_t *)base;
+  P.keyW = (int16_t *)(base + kb);
+  P.keyCount = (uint32_t *)(base + kb + wb);
+  P.keyPos = n->stdp ? (uint16_t *)(base + kb + wb + cb) : nullptr;
+  return 0;
+}
+
 // What one step of a network needs around its kernel launch.
 struct StepPlan {
   uint32_t t = 0;
@@ -2289,7 +2341,8 @@ static int post_step(dpsnn_net *n, StepPlan &sp) {
 // Step t of one network: delivery + dynamics of t, and the fired scan of
 // t+1 (which the reference runs at the head of step t+1; nothing reads the
 // scan's outputs in between).  The scan of step 0 finds no spikes.  P is
-// the network's device view the kernels are launched with.
+// the network's device view with the key scratch the kernels use (the
+// network's own, or the allocation-free step's static one).
 static int one_step(dpsnn_net *n, const DevNet &P) {
   StepPlan sp;
   int rc;
@@ -2592,6 +2645,21 @@ int dpsnn_step(dpsnn_net *n, uint32_t nsteps) {
   int rc0 = ensure_scratch(n);
   if (rc0) return rc0;
   return run_steps(n, n->dev, nsteps, true);
+}
+
+int dpsnn_step_noalloc(dpsnn_net *n, uint32_t nsteps) {
+  if (!n) return fail(DPSNN_ERR_ARG, "null network");
+  if (n->batch) return fail(DPSNN_ERR_ARG, "network is a member of a batch: step it through dpsnn_batch_step");
+  if (!n->uploaded) return fail(DPSNN_ERR_ARG, "network not uploaded (dpsnn_upload)");
+  if (n->dev.drive == DPSNN_DRIVE_REPLAY)
+    // the replayed bath's staging buffers grow with the step's event count
+    return fail(DPSNN_ERR_ARG, "the allocation-free step runs the Philox drive only");
+  HIP_TRY(hipSetDevice(n->opt.device));
+  DevNet P;
+  int rc = static_scratch_view(n, P);
+  if (rc) return rc;
+  // graph instantiation allocates: plain launches only
+  return run_steps(n, P, nsteps, false);
 }
 
 int dpsnn_sync(dpsnn_net *n) {

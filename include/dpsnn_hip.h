@@ -131,6 +131,18 @@ int dpsnn_upload(dpsnn_net *net);
  * returning 1, localNetProcess.c:138-157).  Work is queued on the
  * network's stream; files are written at each drain. */
 int dpsnn_step(dpsnn_net *net, uint32_t nsteps);
+/* dpsnn_step without any device allocation.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory inside
+ * their loop (code under HIP graph capture, tight serving loops, harnesses
+ * that forbid hipMalloc).  The step's key scratch comes from a static
+ * device array of 128 MiB per device (held from the library's load on that
+ * device until the process ends) instead of the network's own buffers, so
+ * networks whose step needs more scratch than that are refused
+ * (DPSNN_ERR_ARG); so is the replayed drive, whose host-fed buffers grow
+ * with the bath.  All allocation-free steps on one device share the array:
+ * calls for two networks that overlap in time on one device race.
+ * Results are bit-identical to dpsnn_step's. */
+int dpsnn_step_noalloc(dpsnn_net *net, uint32_t nsteps);
 /* Waits for queued work, drains raster/rates to files. */
 int dpsnn_sync(dpsnn_net *net);
 uint32_t dpsnn_now_ms(const dpsnn_net *net);
