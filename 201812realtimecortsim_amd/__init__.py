@@ -76,7 +76,8 @@ class Stats(ctypes.Structure):
                 ("synaptic_events", ctypes.c_uint64), ("external_events", ctypes.c_uint64),
                 ("synapses", ctypes.c_uint64), ("delivery_items", ctypes.c_uint64),
                 ("neurons", ctypes.c_uint32), ("max_in_items", ctypes.c_uint32),
-                ("init_seconds", ctypes.c_double), ("device_bytes", ctypes.c_uint64)]
+                ("init_seconds", ctypes.c_double), ("device_bytes", ctypes.c_uint64),
+                ("window_form", ctypes.c_uint32), ("reserved", ctypes.c_uint32), ("graph_steps", ctypes.c_uint64)]
 
     def as_dict(self):
         return {name: getattr(self, name) for name, _ in self._fields_}

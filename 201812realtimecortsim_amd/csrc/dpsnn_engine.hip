@@ -134,7 +134,9 @@ __device__ inline uint32_t next_item(uint32_t *ctr, uint32_t total, uint32_t &x,
 }
 
 // LDS written by some lanes of a wave and read by others: order the wave's
-// accesses around this point.
+// accesses around this point.  The one-wave workgroups of phase 1 use it in
+// place of __syncthreads(), whose workgroup-scope fence would also wait for
+// every global load in flight (the records loaded ahead for the next target).
 __device__ inline void wave_sync() {
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
   __builtin_amdgcn_wave_barrier();
@@ -193,6 +195,9 @@ struct DevNet {
   const int32_t *winCols;    // [columns][slots]: source column of each stencil slot, or -1
   uint32_t winSlots, winPasses, winUnit, winUnitsPerCol, winUnits;
   uint32_t *winCtr;          // [kXcds][kCtrStride] unit counters (zero between steps)
+  uint32_t *bigList;         // [N] targets k_sort leaves to k_sort_big
+  uint32_t *bigCount;        // their count (zero between steps)
+  uint32_t sortCap;          // k_sort keeps targets of up to this many records (<= kSortCap)
   // Steps replayed from a captured graph read their t here (the kernel
   // argument is then ignored); null: t is the argument.
   const uint32_t *tDev;
@@ -979,7 +984,7 @@ __device__ __attribute__((always_inline)) inline void bucket_scan(uint32_t *bcnt
     if (lane >= (uint32_t)o) incl += v;
   }
   uint32_t run = incl - sum;
-  __syncthreads();
+  wave_sync();
 #pragma unroll
   for (int r = 0; r < R; ++r) { bcnt[lane * R + r] = run; run += c[r]; }
   if (lane == kWave - 1) bcnt[B] = run;
@@ -988,9 +993,9 @@ __device__ __attribute__((always_inline)) inline void bucket_scan(uint32_t *bcnt
 // The n keys in sorted[] are scattered into their buckets (bcnt[b] = end of
 // bucket b).  Within a bucket each key's place is its rank (keys are
 // unique): the count of the bucket's smaller keys.  Calls put(key, place)
-// once per key.  Lanes take the keys in bucket order, so lanes of one
-// bucket read the same words together; the reads are independent of each
-// other.  Whole wave.
+// once per key.  Lanes take the keys in bucket order, so
+// lanes of one bucket read the same words together; the reads are
+// independent of each other.  Whole wave.
 template <int B, class Put>
 __device__ __attribute__((always_inline)) inline void rank_in_buckets(const uint64_t *sorted, const uint32_t *bcnt,
                                                                       uint32_t n, int shift, Put &&put) {
@@ -1042,7 +1047,7 @@ __device__ __attribute__((always_inline)) inline uint32_t gather_sort_neuron(con
   for (uint32_t b = lane; b <= (uint32_t)B; b += kWave) bcnt[b] = 0;
   uint32_t n = 0;
   if (t > 0) n = compact_live<K>(P, t, beg, len, sorted, plastic);
-  __syncthreads();
+  wave_sync();
   const bool fast = n <= P.denseMax;
   uint64_t key[kDense];
   if (fast) {
@@ -1098,9 +1103,9 @@ __device__ __attribute__((always_inline)) inline uint32_t gather_sort_neuron(con
     for (uint32_t base = 0; base < len; base += kSuper)
       n += gather_keys<kSlowK, B>(P, t, j, beg, len, base, k8, bcnt, ldsW, shift, tBits, true, plastic);
   }
-  __syncthreads();
+  wave_sync();
   bucket_scan<B>(bcnt);
-  __syncthreads();
+  wave_sync();
   // scatter into buckets (order inside a bucket is fixed below); the general
   // path gathers its batches again (L2 hits)
   if (fast) {
@@ -1115,7 +1120,7 @@ __device__ __attribute__((always_inline)) inline uint32_t gather_sort_neuron(con
       scatter_keys<kSlowK, B>(k8, bcnt, sorted, shift);
     }
   }
-  __syncthreads();
+  wave_sync();
   rank_in_buckets<B>(sorted, bcnt, n, shift, [&](uint64_t x, uint32_t p) {
     const size_t at = run_offset(p);
     const int16_t wv = ldsW[x & 0xFFFu];
@@ -1127,7 +1132,7 @@ __device__ __attribute__((always_inline)) inline uint32_t gather_sort_neuron(con
     }
     if (plastic && !fast) outP[at] = (uint16_t)(x & 0xFFFu);
   });
-  __syncthreads();
+  wave_sync();
   if (plastic && !fast) ltd_neuron(P, t, j, ltd_increment(P, t, j), n, outP);
   return n;
 }
@@ -1292,7 +1297,7 @@ __global__ __launch_bounds__(kWave) void k_gather(DevNet P, uint32_t t, int ltd)
 constexpr int kCollectThreads = kWindowWaves * kWave;
 constexpr uint32_t kWindowMinNeurons = 1u << 16;  // the window form by default from this size
 constexpr uint32_t kWindowUnitsWanted = 1024;     // units per step: about four per CU
-constexpr uint32_t kWindowMaxEntries = 19000;     // window entries (u64) in one CU's LDS
+constexpr uint32_t kWindowMaxEntries = 12 * 1024;  // window entries: kWinPerThread per thread of k_collect
 
 // Block-wide work queue: the next unit for the whole workgroup (one unit
 // per counter update), XCD-ordered as next_item.  sh: two LDS words.
@@ -1326,6 +1331,12 @@ __host__ __device__ inline size_t collect_lds(uint32_t entries, uint32_t unit) {
   return ((size_t)entries + 1) * sizeof(uint64_t) + (size_t)unit * sizeof(uint32_t) + 2 * sizeof(uint32_t);
 }
 
+// Window entries per thread of k_collect (W <= kWinPerThread * 1024): each
+// thread keeps its entries' history offsets in registers and loads all of
+// a pass's emission times at once.
+constexpr int kWinPerThread = 12;
+constexpr int kCollectBatches = 4;  // 64-item batches per round of loads (the next round is in flight too)
+
 __global__ __launch_bounds__(kCollectThreads) void k_collect(DevNet P, uint32_t t) {
   extern __shared__ uint64_t lds[];
   if (P.tDev) t = *P.tDev;
@@ -1347,55 +1358,88 @@ __global__ __launch_bounds__(kCollectThreads) void k_collect(DevNet P, uint32_t 
     const uint32_t j0 = col * P.npc + part * P.winUnit;
     const uint32_t m = min(P.winUnit, (col + 1) * P.npc - j0);
     for (uint32_t i = threadIdx.x; i < m; i += blockDim.x) cnt[i] = 0;
-    for (uint32_t pass = 0; pass < P.winPasses; ++pass) {
-      const uint32_t d = P.winPasses - 1 - pass;
-      const double *slot = P.teHist + (size_t)((t - d) & (P.RS - 1)) * P.N;
-      __syncthreads();  // the previous pass's lookups are done
-      for (uint32_t k = 0; k < P.winSlots; ++k) {
+    // this thread's window entries e = tid + r * blockDim: the history
+    // offset of their source (or ~0u: no source column in that slot)
+    uint32_t off[kWinPerThread];
+#pragma unroll
+    for (int r = 0; r < kWinPerThread; ++r) {
+      const uint32_t e = threadIdx.x + (uint32_t)r * blockDim.x;
+      off[r] = ~0u;
+      if (e < W) {
+        const uint32_t k = e / P.npc;
         const int32_t sc = P.winCols[(size_t)col * P.winSlots + k];
-        if (sc < 0) continue;  // no item refers to the slot
-        const double *te = slot + (size_t)sc * P.npc;
-        for (uint32_t i = threadIdx.x; i < P.npc; i += blockDim.x) {
-          const double v = te[i];
-          win[k * P.npc + i] = v >= 0.0 ? (dpsnn_asu64(tF + (v - trunc(v))) - tBits) << 16 : kNoKey;  // modf, exact
-        }
+        if (sc >= 0) off[r] = (uint32_t)sc * P.npc + (e - k * P.npc);
+      }
+    }
+    for (uint32_t pass = 0; pass < P.winPasses; ++pass) {
+      // emission times of the pass's slot (run delay d: slot t - d)
+      const double *slot = P.teHist + (size_t)((t - (P.winPasses - 1 - pass)) & (P.RS - 1)) * P.N;
+      double wv[kWinPerThread];
+#pragma unroll
+      for (int r = 0; r < kWinPerThread; ++r) wv[r] = off[r] != ~0u ? slot[off[r]] : kNotFired;
+      __syncthreads();  // the previous pass's lookups are done
+#pragma unroll
+      for (int r = 0; r < kWinPerThread; ++r) {
+        const uint32_t e = threadIdx.x + (uint32_t)r * blockDim.x;
+        const double v = wv[r];
+        if (e < W) win[e] = v >= 0.0 ? (dpsnn_asu64(tF + (v - trunc(v))) - tBits) << 16 : kNoKey;  // modf, exact
       }
       __syncthreads();
       const uint64_t *co = P.winChunk + ((size_t)u * P.winPasses + pass) * kWindowWaves + wave;
       const uint64_t cb = co[0], ce = co[1];
-      constexpr int U = 4;  // batches of 64 items (8 chunks) in flight per lane
-      for (uint64_t c0 = cb; c0 < ce; c0 += (uint64_t)kWindowChunk * U) {
-        uint32_t it[U], q[U];
+      // The wave's targets come in order, each one's items contiguous: a
+      // live item's place is its target's count at the start of the pass
+      // (cnt, read where the target's segment starts) or, for the target
+      // carried over from the previous batch, the carried count, plus the
+      // live items before it in its run of lanes.
+      uint32_t lastQ = 0xFFFFu, carry = 0;
+      uint32_t it[kCollectBatches], q[kCollectBatches], itN[kCollectBatches], qN[kCollectBatches];
+      auto load = [&](uint64_t c0, uint32_t (&I)[kCollectBatches], uint32_t (&Q)[kCollectBatches]) {
 #pragma unroll
-        for (int v = 0; v < U; ++v) {
+        for (int v = 0; v < kCollectBatches; ++v) {
           const uint64_t ch = c0 + (uint64_t)v * kWindowChunk + (lane >> 3);
           const bool ok = ch < ce;
-          it[v] = ok ? P.winItem[ch * kWindowChunk + (lane & 7)] : W << 16;
-          q[v] = ok ? (uint32_t)P.winDesc[ch] : 0xFFFFu;
+          I[v] = ok ? __builtin_nontemporal_load(P.winItem + ch * kWindowChunk + (lane & 7)) : W << 16;
+          Q[v] = ok ? (uint32_t)P.winDesc[ch] : 0xFFFFu;
+        }
+      };
+      if (cb < ce) load(cb, it, q);
+      for (uint64_t c0 = cb; c0 < ce; c0 += (uint64_t)kWindowChunk * kCollectBatches) {
+        const uint64_t cn = c0 + (uint64_t)kWindowChunk * kCollectBatches;
+        if (cn < ce) load(cn, itN, qN);
+        uint64_t r[kCollectBatches];
+        uint32_t base[kCollectBatches];
+#pragma unroll
+        for (int v = 0; v < kCollectBatches; ++v) {
+          r[v] = win[it[v] >> 16];
+          base[v] = q[v] != 0xFFFFu ? cnt[q[v]] : 0u;
         }
 #pragma unroll
-        for (int v = 0; v < U; ++v) {
+        for (int v = 0; v < kCollectBatches; ++v) {
           if (c0 + (uint64_t)v * kWindowChunk >= ce) break;  // uniform
-          const uint64_t r = win[it[v] >> 16];
-          const bool live = r != kNoKey;  // never on padding
           const uint32_t qv = q[v];
-          // lanes of one target are consecutive: a run of equal q; a live
-          // lane's place is its run's count so far plus the live lanes
-          // before it in the run
+          const bool live = r[v] != kNoKey;  // never on padding
           const uint64_t liveM = __ballot(live);
           const uint32_t qPrev = (uint32_t)__shfl_up((int)qv, 1);
-          const uint64_t startM = __ballot(lane == 0 || qv != qPrev);
-          const uint32_t s = 63u - (uint32_t)__clzll((long long)(startM & upTo));
-          const uint64_t runBelow = below & ~((1ull << s) - 1ull);
-          const bool valid = qv != 0xFFFFu;
-          const uint32_t p = (valid ? cnt[qv] : 0u) + (uint32_t)__popcll(liveM & runBelow);
+          const uint64_t startM = __ballot(lane == 0 ? qv != lastQ : qv != qPrev);
+          const uint64_t upToStart = startM & upTo;
+          const uint32_t s = upToStart ? 63u - (uint32_t)__clzll((long long)upToStart) : 0u;
+          const uint32_t b0 = upToStart ? base[v] : carry;  // no start at or below: lastQ's run
+          const uint32_t p = b0 + (uint32_t)__popcll(liveM & below & ~((1ull << s) - 1ull));
           if (live) {
             const uint32_t j = j0 + qv;
-            P.keys[(size_t)(j >> 6) * P.cap * kWave + (j & 63) * kRun + run_offset(p)] = r | (it[v] & 0xFFFFu);
+            P.keys[(size_t)(j >> 6) * P.cap * kWave + (j & 63) * kRun + run_offset(p)] = r[v] | (it[v] & 0xFFFFu);
           }
+          const uint32_t after = p + (live ? 1u : 0u);
           const bool lastOfRun = lane == 63 || ((startM >> (lane + 1)) & 1ull);
-          if (valid && lastOfRun) cnt[qv] = p + (live ? 1u : 0u);
-          wave_sync();
+          if (qv != 0xFFFFu && lastOfRun) cnt[qv] = after;
+          carry = (uint32_t)__builtin_amdgcn_readlane((int)after, 63);
+          lastQ = (uint32_t)__builtin_amdgcn_readlane((int)qv, 63);
+        }
+#pragma unroll
+        for (int v = 0; v < kCollectBatches; ++v) {
+          it[v] = itN[v];
+          q[v] = qN[v];
         }
       }
     }
@@ -1412,8 +1456,9 @@ __global__ __launch_bounds__(kCollectThreads) void k_collect(DevNet P, uint32_t 
 // Orders neuron j's n records (tick << 16 | weight, in live in-list order)
 // in place at out (neuron j's [0][q][0] of the blocked scratch).  Sort key
 // tick << 12 | place: unique, ordered as (arrival time, in-list position).
-// LDS as phase1_lds.  Whole wave.
-template <int B>
+// LDS: phase1_lds(cap, B), cap >= n.  Up to CH * 64 records are held in
+// registers; more are read again from out.  Whole wave.
+template <int B, int CH>
 __device__ __attribute__((always_inline)) inline void sort_records(uint32_t t, uint32_t n, uint8_t *lds, uint32_t cap,
                                                                    uint64_t *out) {
   constexpr int LOG2B = B == 64 ? 6 : B == 128 ? 7 : B == 256 ? 8 : B == 512 ? 9 : 10;
@@ -1423,19 +1468,19 @@ __device__ __attribute__((always_inline)) inline void sort_records(uint32_t t, u
   const uint32_t lane = threadIdx.x & 63;
   const int shift = 52 - LOG2B - (31 - __clz((int)t));
   for (uint32_t b = lane; b <= (uint32_t)B; b += kWave) bcnt[b] = 0;
-  __syncthreads();
-  const bool fast = n <= (uint32_t)kDense * kWave;
-  uint64_t key[kDense];
+  wave_sync();
+  const bool fast = n <= (uint32_t)CH * kWave;
+  uint64_t key[CH];
   if (fast) {
-    uint64_t rec[kDense];
+    uint64_t rec[CH];
 #pragma unroll
-    for (int r = 0; r < kDense; ++r)
+    for (int r = 0; r < CH; ++r)
       if ((uint32_t)r * kWave < n) {
         const uint32_t p = r * kWave + lane;
         rec[r] = out[run_offset(p < n ? p : 0u)];
       }
 #pragma unroll
-    for (int r = 0; r < kDense; ++r)
+    for (int r = 0; r < CH; ++r)
       if ((uint32_t)r * kWave < n) {
         const uint32_t p = r * kWave + lane;
         key[r] = ((rec[r] >> 16) << 12) | p;
@@ -1451,12 +1496,12 @@ __device__ __attribute__((always_inline)) inline void sort_records(uint32_t t, u
       atomicAdd(&bcnt[key_bucket<B>(rc >> 16, shift)], 1u);
     }
   }
-  __syncthreads();
+  wave_sync();
   bucket_scan<B>(bcnt);
-  __syncthreads();
+  wave_sync();
   if (fast) {
 #pragma unroll
-    for (int r = 0; r < kDense; ++r)
+    for (int r = 0; r < CH; ++r)
       if ((uint32_t)r * kWave < n && r * kWave + lane < n)
         sorted[atomicAdd(&bcnt[key_bucket<B>(key[r] >> 12, shift)], 1u)] = key[r];
   } else {
@@ -1465,13 +1510,25 @@ __device__ __attribute__((always_inline)) inline void sort_records(uint32_t t, u
       sorted[atomicAdd(&bcnt[key_bucket<B>(k >> 12, shift)], 1u)] = k;
     }
   }
-  __syncthreads();  // every record has been read: the runs are rewritten in place
+  wave_sync();  // every record has been read: the runs are rewritten in place
   rank_in_buckets<B>(sorted, bcnt, n, shift, [&](uint64_t x, uint32_t p) {
     out[run_offset(p)] = ((x >> 12) << 16) | (uint16_t)ldsW[x & 0xFFFu];
   });
-  __syncthreads();
+  wave_sync();
 }
 
+// k_sort sizes each wave's LDS for kSortCap records, far above a step's
+// live events per target (a few hundred at the saturated rate) but below
+// the longest in-list, so that more waves fit a CU; a target with more
+// records is listed for k_sort_big, whose LDS holds the longest in-list.
+constexpr uint32_t kSortCap = 768;
+constexpr int kSortChunks = (int)(kSortCap / kWave);
+
+__device__ inline uint64_t *record_runs(const DevNet &P, uint32_t j) {
+  return P.keys + (size_t)(j >> 6) * P.cap * kWave + (j & 63) * kRun;
+}
+
+// One wave per target from the XCD-ordered queue.
 template <int B>
 __global__ __launch_bounds__(kWave) void k_sort(DevNet P, uint32_t t) {
   extern __shared__ uint64_t lds[];
@@ -1480,9 +1537,27 @@ __global__ __launch_bounds__(kWave) void k_sort(DevNet P, uint32_t t) {
   const uint32_t home = blockIdx.x % kXcds;
   uint32_t x = home;
   for (uint32_t j0, j1; (j0 = next_item(P.workCtr, P.N, x, home, &j1)) < P.N;)
-    for (uint32_t j = j0; j < j1; ++j)
-      sort_records<B>(t, P.keyCount[j], (uint8_t *)lds, P.cap,
-                      P.keys + (size_t)(j >> 6) * P.cap * kWave + (j & 63) * kRun);
+    for (uint32_t j = j0; j < j1; ++j) {
+      const uint32_t n = P.keyCount[j];
+      if (n > P.sortCap) {
+        if ((threadIdx.x & 63) == 0) P.bigList[atomicAdd(P.bigCount, 1u)] = j;
+        continue;
+      }
+      sort_records<B, kSortChunks>(t, n, (uint8_t *)lds, kSortCap, record_runs(P, j));
+    }
+}
+
+// The targets k_sort left (more than kSortCap records), one wave each;
+// bigCount is cleared by the next k_integrate.
+template <int B>
+__global__ __launch_bounds__(kWave) void k_sort_big(DevNet P, uint32_t t) {
+  extern __shared__ uint64_t lds[];
+  if (P.tDev) t = *P.tDev;
+  const uint32_t count = *P.bigCount;
+  for (uint32_t i = blockIdx.x; i < count; i += gridDim.x) {
+    const uint32_t j = P.bigList[i];
+    sort_records<B, kDense>(t, P.keyCount[j], (uint8_t *)lds, P.cap, record_runs(P, j));
+  }
 }
 
 // Blocks of kBlock threads: each wave holds 64 consecutive neurons starting
@@ -1493,6 +1568,7 @@ __global__ __launch_bounds__(kBlock) void k_integrate(DevNet P, uint32_t t, ExtV
   if (P.tDev) t = *P.tDev;
   const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   if (blockIdx.x == 0 && threadIdx.x < kXcds) P.workCtr[threadIdx.x * kCtrStride] = 0;  // for the next k_gather
+  if (blockIdx.x == 0 && threadIdx.x == 0 && P.bigCount) *P.bigCount = 0;                 // for the next k_sort
   const uint32_t j0 = blockIdx.x * kBlock + wave * kWave;
   const uint32_t m = j0 < P.N ? min((uint32_t)kWave, P.N - j0) : 0u;
   const uint32_t n = lane < m ? P.keyCount[j0 + lane] : 0u;
@@ -1631,9 +1707,10 @@ struct KernelShape {
   GatherFn gather;
   BatchFn batch;
   BatchGatherFn batchGather;
-  SortFn sort;
+  SortFn sort, sortBig;
 };
-#define DPSNN_SHAPE(k, b) {k, b, k_step<k, b>, k_gather<k, b>, k_batch_step<k, b>, k_batch_gather<k, b>, k_sort<b>}
+#define DPSNN_SHAPE(k, b) \
+  {k, b, k_step<k, b>, k_gather<k, b>, k_batch_step<k, b>, k_batch_gather<k, b>, k_sort<b>, k_sort_big<b>}
 static const KernelShape kShapes[] = {
     DPSNN_SHAPE(8, 64),  DPSNN_SHAPE(8, 256),  DPSNN_SHAPE(8, 1024),
     DPSNN_SHAPE(16, 64), DPSNN_SHAPE(16, 256), DPSNN_SHAPE(16, 1024),
@@ -1729,7 +1806,7 @@ struct dpsnn_net {
   // stencil-window phase 1 (k_collect + k_sort)
   WindowLayout window;       // host staging until upload
   bool winOn = false;        // the device holds the layout
-  uint32_t winGrid = 0, sortGrid = 0;
+  uint32_t winGrid = 0, sortGrid = 0, sortBigGrid = 0;
   size_t winLds = 0;
   // profiling
   bool profile = false;
@@ -1744,6 +1821,7 @@ struct dpsnn_net {
   const uint64_t *graphKeys = nullptr;  // the key scratch the graph was captured with
   uint32_t *d_t = nullptr;
   bool graphs = true;        // DPSNN_GRAPH=0: plain launches only
+  uint64_t graphSteps = 0;   // steps replayed from the graph so far
   bool collect = false;  // raster / rates are being produced
 };
 
@@ -1808,18 +1886,25 @@ static int upload_window(dpsnn_net *n, int cus) {
       (rc = dev_alloc(n, &d_desc, L.desc.size())) || (rc = upload(n, d_desc, L.desc.data(), L.desc.size())) ||
       (rc = dev_alloc(n, &d_chunk, L.chunk.size())) || (rc = upload(n, d_chunk, L.chunk.data(), L.chunk.size())) ||
       (rc = dev_alloc(n, &d_cols, L.cols.size())) || (rc = upload(n, d_cols, L.cols.data(), L.cols.size())) ||
-      (rc = dev_alloc(n, &P.winCtr, kXcds * kCtrStride)))
+      (rc = dev_alloc(n, &P.winCtr, kXcds * kCtrStride)) || (rc = dev_alloc(n, &P.bigList, P.N)) ||
+      (rc = dev_alloc(n, &P.bigCount, 1)))
     return rc;
   HIP_TRY(hipMemsetAsync(P.winCtr, 0, sizeof(uint32_t) * kXcds * kCtrStride, n->stream));
+  HIP_TRY(hipMemsetAsync(P.bigCount, 0, sizeof(uint32_t), n->stream));
   HIP_TRY(hipStreamSynchronize(n->stream));  // the host staging is dropped after upload
   P.winItem = d_item; P.winDesc = d_desc; P.winChunk = d_chunk; P.winCols = d_cols;
   P.winSlots = L.slots; P.winPasses = L.passes; P.winUnit = L.unit; P.winUnitsPerCol = L.unitsPerCol;
   P.winUnits = L.units;
+  // DPSNN_SORT_CAP lowers k_sort's limit, so that tests send targets through
+  // k_sort_big
+  P.sortCap = kSortCap;
+  if (const char *e = getenv("DPSNN_SORT_CAP")) P.sortCap = std::min<uint32_t>(kSortCap, (uint32_t)atoi(e));
   n->winLds = lds;
   n->winGrid = round_up(std::min<uint32_t>(L.units, (uint32_t)cus), kXcds);
   const KernelShape &K = kShapes[n->shape];
   n->sortGrid = round_up(
-      std::min<uint32_t>(P.N, (uint32_t)cus * resident_waves((const void *)K.sort, phase1_lds(P.cap, K.B))), kXcds);
+      std::min<uint32_t>(P.N, (uint32_t)cus * resident_waves((const void *)K.sort, phase1_lds(kSortCap, K.B))), kXcds);
+  n->sortBigGrid = std::min<uint32_t>(P.N, (uint32_t)cus * 4);
   n->winOn = true;
   return 0;
 }
@@ -2351,7 +2436,8 @@ static int one_step(dpsnn_net *n, const DevNet &P) {
   if (n->split) {
     if (n->winOn && sp.t >= kPackT && !sp.ltd) {
       hipLaunchKernelGGL(k_collect, dim3(n->winGrid), dim3(kCollectThreads), n->winLds, n->stream, P, sp.t);
-      hipLaunchKernelGGL(K.sort, dim3(n->sortGrid), dim3(kWave), phase1_lds(P.cap, K.B), n->stream, P, sp.t);
+      hipLaunchKernelGGL(K.sort, dim3(n->sortGrid), dim3(kWave), phase1_lds(kSortCap, K.B), n->stream, P, sp.t);
+      hipLaunchKernelGGL(K.sortBig, dim3(n->sortBigGrid), dim3(kWave), phase1_lds(P.cap, K.B), n->stream, P, sp.t);
     } else {
       hipLaunchKernelGGL(K.gather, dim3(n->gatherGrid), dim3(kWave), n->ldsBytes, n->stream, P, sp.t, sp.ltd);
     }
@@ -2393,7 +2479,8 @@ static void launch_graph_step(dpsnn_net *n, const DevNet &G) {
   const KernelShape &K = kShapes[n->shape];
   if (n->winOn) {
     hipLaunchKernelGGL(k_collect, dim3(n->winGrid), dim3(kCollectThreads), n->winLds, n->stream, G, 0u);
-    hipLaunchKernelGGL(K.sort, dim3(n->sortGrid), dim3(kWave), phase1_lds(G.cap, K.B), n->stream, G, 0u);
+    hipLaunchKernelGGL(K.sort, dim3(n->sortGrid), dim3(kWave), phase1_lds(kSortCap, K.B), n->stream, G, 0u);
+    hipLaunchKernelGGL(K.sortBig, dim3(n->sortBigGrid), dim3(kWave), phase1_lds(G.cap, K.B), n->stream, G, 0u);
   } else {
     hipLaunchKernelGGL(K.gather, dim3(n->gatherGrid), dim3(kWave), n->ldsBytes, n->stream, G, 0u, 0);
   }
@@ -2439,6 +2526,7 @@ static int graph_steps(dpsnn_net *n, const DevNet &P) {
   if ((rc = prof_end(n, kGraphSteps))) return rc;
   HIP_TRY(hipGetLastError());
   n->t += kGraphSteps;
+  n->graphSteps += kGraphSteps;
   return 0;
 }
 
@@ -2585,7 +2673,7 @@ uint64_t dpsnn_device_bytes_needed(const dpsnn_net *n) {
   const WindowLayout &L = n->window;
   if (L.ok)
     b += r(4 * L.item.size()) + r(2 * L.desc.size()) + r(8 * L.chunk.size()) + r(4 * L.cols.size()) +
-         r(4 * kXcds * kCtrStride);
+         r(4 * kXcds * kCtrStride) + r(4 * N) + r(4);
   return b;
 }
 
@@ -2736,6 +2824,9 @@ int dpsnn_get_stats(dpsnn_net *n, dpsnn_stats *out) {
   out->max_in_items = n->con.max_in;
   out->init_seconds = n->initSeconds;
   out->device_bytes = n->devBytes;
+  out->window_form = n->winOn ? 1u : 0u;
+  out->reserved = 0;
+  out->graph_steps = n->graphSteps;
   return 0;
 }
 

@@ -256,14 +256,18 @@ def bench_copy(args):
 
 
 def traffic_from_profile(config):
-    """HBM bytes per step of the measured workload from the committed counter
-    summary (two rocprofv3 --pmc passes, FETCH_SIZE then WRITE_SIZE, scaled
-    by the same counters read for a plain copy), or None."""
+    """HBM bytes per step of this workload from the committed counter summary
+    (two rocprofv3 --pmc passes, FETCH_SIZE then WRITE_SIZE, scaled by the
+    same counters read for a plain copy) -- measured by a profiled run of
+    this build, not by this run -- with where and over which steps it was
+    measured; or None."""
     path = os.path.join(ROOT, "profiles", f"step_{config}_traffic.json")
     if not os.path.exists(path):
         return None
     with open(path) as f:
-        return json.load(f).get("hbm_bytes_per_step")
+        j = json.load(f)
+    return {"bytes": j.get("hbm_bytes_per_step"),
+            "source": f"profiles/step_{config}_traffic.json (counter passes over {j.get('workload', '?')})"}
 
 
 def main():
@@ -271,6 +275,9 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--warmup", type=int, default=100)
+    ap.add_argument("--settle", type=int, default=100,
+                    help="untimed ms run before the warm-up steps: the start-up transient of the "
+                         "network (rates still rising) is not part of the timed regime")
     ap.add_argument("--config", default="c1")
     ap.add_argument("--mode", default="plain", choices=["plain", "batch", "serve", "copy"])
     ap.add_argument("--batch-count", type=int, default=64)
@@ -299,7 +306,7 @@ def main():
         print(json.dumps(bench_copy(args)), flush=True)
         return 0
     cfg = pkg.config_dir(args.config)
-    total = args.warmup + args.steps
+    total = args.settle + args.warmup + args.steps
     pkg.apply_env(cfg, env_totalSimTime_ms=max(total, 1), env_spikes_writeDisable=1)
     drive = pkg.DRIVE_PHILOX if args.drive == "philox" else pkg.DRIVE_REPLAY
     print(f"[bench] building {args.config} ...", file=sys.stderr, flush=True)
@@ -309,9 +316,11 @@ def main():
     st0 = net.stats()
     print(f"[bench] init {init_s:.1f} s: {st0['neurons']} neurons, {st0['synapses']} synapses, "
           f"{st0['device_bytes'] / 1e9:.1f} GB on device", file=sys.stderr, flush=True)
+    net.step(args.settle)
     net.step(args.warmup)
     net.sync()
     a = net.stats()
+    t_first = args.settle + args.warmup
     net.profile(True)
     t0 = time.perf_counter()
     net.step(args.steps)
@@ -320,6 +329,7 @@ def main():
     b = net.stats()
     kms, launches = net.kernel_time(0)
     net.close()
+    traffic = traffic_from_profile(args.config)
     events = b["synaptic_events"] - a["synaptic_events"]
     ext = b["external_events"] - a["external_events"]
     spikes = b["spikes"] - a["spikes"]
@@ -347,10 +357,16 @@ def main():
                    "neurons": neurons, "synapses": st0["synapses"], "drive": args.drive,
                    "spikes_per_step": spikes / args.steps,
                    "mean_rate_hz": spikes / args.steps * 1000.0 / neurons,
-                   "init_seconds": init_s},
+                   "init_seconds": init_s,
+                   "timed_ms": [t_first, t_first + args.steps],
+                   "phase1": ("stencil window (k_collect + k_sort)" if b["window_form"] else
+                              "in-list gather (k_gather)") if t_first >= 32 else "mixed (steps before 32 gather)",
+                   "graph_replayed_steps": b["graph_steps"] - a["graph_steps"]},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic_from_profile(args.config),
-                     "kernel": "one step's device work (k_gather + k_integrate, or k_step)",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic["bytes"] if traffic else None,
+                     "traffic_source": traffic["source"] if traffic else None,
+                     "kernel": "one step's device work (phase 1 + k_integrate), HIP events on the "
+                               "network's stream",
                      "kernel_ms": kms, "launches": launches,
                      "algorithmic_bytes_per_launch": per_launch},
         "cpu_baseline": None,

@@ -105,6 +105,10 @@ typedef struct {
   uint32_t max_in_items;     /* longest per-neuron in-list */
   double init_seconds;       /* connectome generation + upload */
   uint64_t device_bytes;     /* device memory held by this network */
+  uint32_t window_form;      /* 1: phase 1 runs in the stencil-window form
+                                (k_collect + k_sort) where it applies */
+  uint32_t reserved;
+  uint64_t graph_steps;      /* steps replayed from captured graphs */
 } dpsnn_stats;
 
 /* getParameters equivalent: reads the env vars of the calling process and

@@ -32,9 +32,13 @@ def _columns(tmp, pkg, name, counts, src="c0", nu_ext=3.0):
     return str(d)
 
 
-@pytest.mark.parametrize("window", ["1", "0"])
-def test_c0_full_second_identical_to_reference(pkg, tmp_path, monkeypatch, window):
+@pytest.mark.parametrize("window,sort_cap", [("1", None), ("1", "200"), ("0", None)])
+def test_c0_full_second_identical_to_reference(pkg, tmp_path, monkeypatch, window, sort_cap):
+    """sort_cap 200: targets with more than 200 records of a step (nearly all
+    of configs[0]'s) are sorted by the second sort launch."""
     monkeypatch.setenv("DPSNN_WINDOW", window)
+    if sort_cap:
+        monkeypatch.setenv("DPSNN_SORT_CAP", sort_cap)
     g = golden("c0_raster.json")
     cfg = pkg.config_dir("c0")
     pkg.apply_env(cfg, env_totalSimTime_ms=1000)
