@@ -181,7 +181,26 @@ int dpsnn_get_stats(dpsnn_net *net, dpsnn_stats *out);
  * files and env settings as the checkpointed one: the connectome is
  * regenerated, not stored) and loads the state into it; the run then
  * continues as if never interrupted, appending to the second's output
- * files.  A network of a batch must leave it before a checkpoint. */
+ * files.  A network of a batch must leave it before a checkpoint.
+ *
+ * File layout (little endian, no padding between items):
+ *   header: char magic[8] = "DPSNNCK1"; uint32 N, RS, NW, drive, stdp, t,
+ *           nsub, ncols, writer, spikesOpen, ratesOpen, pad; uint64 S, items
+ *   double v[N], c[N], Tr[N], Te[N], next external event time[N]
+ *   uint64 Philox counter[N]
+ *   uint64 fired bitmap[RS][NW]   (slot of ms m: m % RS; bit i % 64 of
+ *                                  word i / 64: neuron i fired in step m-1)
+ *   double emission time[RS][N]   (same slots; -1: did not fire)
+ *   uint64 counters[16][16]       (sum over rows: [0] synaptic events,
+ *                                  [1] external events, [2] spikes)
+ *   with stdp: int32 last spike ms[N]; then per synapse in target-major
+ *           order (each target's incoming synapses in backward-list order):
+ *           int16 weight[S], float timeDerivative[S], int32 lastActive[S]
+ *   ran3 state: int32 idum, inext, inextp, ma[56], iff, normSet, pad;
+ *           double normCached; uint64 draws
+ *   replayed drive: double next external event time[N] (host stream)
+ *   writer: uint32 per-(column, sub-population) counts of the open rates
+ *           window [ncols * nsub] */
 int dpsnn_checkpoint(dpsnn_net *net, const char *path);
 int dpsnn_restore(const char *inidir, const dpsnn_options *opt, const char *path, dpsnn_net **out);
 /* HIP stream the network runs on (hipStream_t as void*). */

@@ -566,6 +566,28 @@ static uint32_t delay_uni(ora_net *n, uint32_t s, uint32_t t) {
 /* simpleCM_invokeIndivSynGenerators DPSNN_LIFCAcorticalModuleGen.cpp:62-220
  * with generateTargetNeuList DPSNN_LIFCAconnectome.cpp:513-823 (Bernoulli
  * branch, non-periodic boundaries) and generateSourceNeuCExt :871-887. */
+/* The external bath of neuron i (DPSNN_LIFCAcorticalModuleGen.cpp:203-211,
+ * generateSourceNeuCExt: mirrored index into the bath template) and its
+ * sub-population. */
+static void neuron_bath(ora_net *n, uint32_t i) {
+  par_t *p = &n->p;
+  uint32_t npc = p->npc, cfy = (i / npc) / p->CFX, cfx = (i / npc) % p->CFX;
+  uint32_t src = sub_pop_of(n, i % npc), bx = cfx, by = cfy;
+  float nuExt = (float)n->sp[src].NuExt, cExt;
+  n->subPop[i] = (uint8_t)src;
+  if (bx >= p->CFX / 2) bx = -1 * (bx - p->CFX + 1);
+  if (by >= p->CFY / 2) by = -1 * (by - p->CFY + 1);
+  if (bx >= p->bathX_Max) bx = p->bathX_Max - 1;
+  if (by >= p->bathY_Max) by = p->bathY_Max - 1;
+  cExt = (float)n->bathEff[bx][by];
+  if (cExt != 0) {
+    n->invNuExt[i] = 1000.0 / (nuExt * cExt);
+    n->maxExt[i] = (uint32_t)ceil(20 / n->invNuExt[i]);
+  } else {
+    n->invNuExt[i] = 0; n->maxExt[i] = 0;
+  }
+}
+
 static int generate(ora_net *n, char *err, int errlen) {
   par_t *p = &n->p;
   uint32_t i, N = p->N, npc = p->npc;
@@ -640,20 +662,8 @@ static int generate(ora_net *n, char *err, int errlen) {
       n->g_w[S] = (int16_t)(weightF * p->f2w);
       S++;
     }
-    nuExt = (float)n->sp[src].NuExt;
-    /* generateSourceNeuCExt: mirrored index into the bath template */
-    bx = cfx; by = cfy;
-    if (bx >= p->CFX / 2) bx = -1 * (bx - p->CFX + 1);
-    if (by >= p->CFY / 2) by = -1 * (by - p->CFY + 1);
-    if (bx >= p->bathX_Max) bx = p->bathX_Max - 1;
-    if (by >= p->bathY_Max) by = p->bathY_Max - 1;
-    cExt = (float)n->bathEff[bx][by];
-    if (cExt != 0) {
-      n->invNuExt[i] = 1000.0 / (nuExt * cExt);
-      n->maxExt[i] = (uint32_t)ceil(20 / n->invNuExt[i]);
-    } else {
-      n->invNuExt[i] = 0; n->maxExt[i] = 0;
-    }
+    (void)nuExt; (void)cExt; (void)bx; (void)by;
+    neuron_bath(n, i);
   }
   n->S = S;
   free(tlist); free(tsub);
@@ -717,7 +727,8 @@ static void init_stdp_tables(ora_net *n) {
   for (i = 1; i < STDP_SPAN; i++) n->antiCausalTab[i] = (1 - 1 / s->tauSTDP_ms) * n->antiCausalTab[i - 1];
 }
 
-ora_net *ora_create(const char *dir, char *err, int errlen) {
+static ora_net *create(const char *dir, uint64_t S, const uint32_t *pre, const uint32_t *post,
+                       const uint8_t *delay, const int16_t *weight, char *err, int errlen) {
   ora_net *n = (ora_net *)calloc(1, sizeof(ora_net));
   uint32_t i, N;
   if (!n) { seterr(err, errlen, "out of memory"); return NULL; }
@@ -738,7 +749,17 @@ ora_net *ora_create(const char *dir, char *err, int errlen) {
   /* localNet init seeds with env_globalSeed, DPSNN_localNet_init.c:119 */
   rng_seed(&n->rng, (int32_t)n->p.globalSeed);
   init_lut(n);
-  if (generate(n, err, errlen)) { ora_destroy(n); return NULL; }
+  if (!pre) {
+    if (generate(n, err, errlen)) { ora_destroy(n); return NULL; }
+  } else {
+    /* a given backward list: its stable sort by delay is itself */
+    n->S = S;
+    n->g_pre = (uint32_t *)malloc(S * 4 + 4); n->g_post = (uint32_t *)malloc(S * 4 + 4);
+    n->g_delay = (uint8_t *)malloc(S + 1); n->g_w = (int16_t *)malloc(S * 2 + 2);
+    memcpy(n->g_pre, pre, S * 4); memcpy(n->g_post, post, S * 4);
+    memcpy(n->g_delay, delay, S); memcpy(n->g_w, weight, S * 2);
+    for (i = 0; i < N; i++) neuron_bath(n, i);
+  }
   build_layout(n);
   for (i = 0; i < N; i++) {
     /* initNeuralKind DPSNN_neuron_init.c:99-115 */
@@ -769,6 +790,106 @@ ora_net *ora_create(const char *dir, char *err, int errlen) {
   rng_seed(&n->rng, 0);
   n->ms = 0;
   return n;
+}
+
+ora_net *ora_create(const char *dir, char *err, int errlen) {
+  return create(dir, 0, NULL, NULL, NULL, NULL, err, errlen);
+}
+
+ora_net *ora_create_from_backward(const char *dir, uint64_t S, const uint32_t *pre, const uint32_t *post,
+                                  const uint8_t *delay, const int16_t *weight, char *err, int errlen) {
+  uint64_t i;
+  for (i = 0; i < S; i++)
+    if (delay[i] >= RING || (i && delay[i] < delay[i - 1])) {
+      seterr(err, errlen, "the synapse list is not sorted by delay");
+      return NULL;
+    }
+  return create(dir, S, pre, post, delay, weight, err, errlen);
+}
+
+/* Reads an engine checkpoint (layout documented with dpsnn_checkpoint in
+ * include/dpsnn_hip.h) into the oracle's state: the neuron state, the
+ * spikes of the last ring slots (the oracle's axonal ring), the fired flags
+ * the next scan reads, the ran3 stream and the replayed drive's next event
+ * times (or, for a Philox checkpoint, the device's next bath times, from
+ * which this oracle continues with its own ran3 stream). */
+int ora_load_checkpoint(ora_net *n, const char *path, char *err, int errlen) {
+  struct {
+    char magic[8];
+    uint32_t N, RS, NW, drive, stdp, t, nsub, ncols, writer, spikesOpen, ratesOpen, pad;
+    uint64_t S, items;
+  } h;
+  struct {
+    int32_t idum, inext, inextp, ma[56], iff, set, pad;
+    double cached;
+    uint64_t count;
+  } rs;
+  FILE *f = fopen(path, "rb");
+  uint32_t N = n->p.N, i, m, RS;
+  uint64_t k, *bits = NULL;
+  double *skip = NULL, *hist = NULL;
+  int ok = 0;
+  if (!f) { seterr(err, errlen, "cannot open %s", path); return -1; }
+  if (fread(&h, sizeof h, 1, f) != 1 || memcmp(h.magic, "DPSNNCK1", 8) != 0) {
+    seterr(err, errlen, "not a checkpoint"); fclose(f); return -1;
+  }
+  if (h.N != N || h.S != n->S) {
+    seterr(err, errlen, "checkpoint of another network"); fclose(f); return -1;
+  }
+  RS = h.RS;
+  skip = (double *)malloc((uint64_t)N * 8);
+  bits = (uint64_t *)malloc((uint64_t)RS * h.NW * 8);
+  hist = (double *)malloc((uint64_t)RS * N * 8);
+  if (fread(n->v, 8, N, f) != N || fread(n->c, 8, N, f) != N || fread(n->Tr, 8, N, f) != N ||
+      fread(n->Te, 8, N, f) != N || fread(n->assignedExt, 8, N, f) != N || fread(skip, 8, N, f) != N ||
+      fread(bits, 8, (uint64_t)RS * h.NW, f) != (uint64_t)RS * h.NW ||
+      fread(hist, 8, (uint64_t)RS * N, f) != (uint64_t)RS * N)
+    goto out;
+  {
+    uint64_t counters[256];
+    if (fread(counters, 8, 256, f) != 256) goto out;
+  }
+  if (h.stdp) {
+    int16_t *w = (int16_t *)malloc(h.S * 2);
+    float *td = (float *)malloc(h.S * 4);
+    int32_t *last = (int32_t *)malloc(h.S * 4);
+    int good = fread(n->lastSpike_ms, 4, N, f) == N && fread(w, 2, h.S, f) == h.S && fread(td, 4, h.S, f) == h.S &&
+               fread(last, 4, h.S, f) == h.S;
+    if (good)
+      for (k = 0; k < h.S; k++) {  /* target-major -> backward order */
+        uint32_t s = n->incoming_idx[k];
+        n->b_w[s] = w[k]; n->b_td[s] = td[k]; n->b_last[s] = last[k];
+      }
+    free(w); free(td); free(last);
+    if (!good) goto out;
+  }
+  if (fread(&rs, sizeof rs, 1, f) != 1) goto out;
+  n->rng.idum = rs.idum; n->rng.inext = rs.inext; n->rng.inextp = rs.inextp;
+  memcpy(n->rng.ma, rs.ma, sizeof rs.ma);
+  n->rng.iff = rs.iff; n->rng.normSet = rs.set; n->rng.normII = rs.cached; n->rng.counter = rs.count;
+  if (h.drive == 0 && fread(n->assignedExt, 8, N, f) != N) goto out;
+  /* ring slots of ms t-1 .. t-RS+1 and the fired flags of ms t */
+  n->ms = h.t;
+  for (m = 0; m < RING; m++) n->ring_cnt[m] = 0;
+  for (m = 1; m < RS && m <= h.t; m++) {
+    uint32_t ms = h.t - m, slot = ms % RING, hs = ms & (RS - 1);
+    for (i = 0; i < N; i++) {
+      double te = hist[(uint64_t)hs * N + i];
+      if (te >= 0) {
+        n->ring_ids[slot][n->ring_cnt[slot]] = i;
+        n->ring_te[slot][n->ring_cnt[slot]] = te;
+        n->ring_cnt[slot]++;
+      }
+    }
+  }
+  for (i = 0; i < N; i++)
+    n->didItFire[i] = (uint8_t)((bits[(uint64_t)(h.t & (RS - 1)) * h.NW + (i >> 6)] >> (i & 63)) & 1u);
+  ok = 1;
+out:
+  if (!ok) seterr(err, errlen, "checkpoint file truncated");
+  free(skip); free(bits); free(hist);
+  fclose(f);
+  return ok ? 0 : -1;
 }
 
 void ora_destroy(ora_net *n) {

@@ -29,6 +29,15 @@ typedef struct ora_net ora_net;
  * on a configuration error. */
 ora_net *ora_create(const char *dir, char *err, int errlen);
 void ora_destroy(ora_net *n);
+/* ora_create with a given backward (delivery-ordered) synapse list instead
+ * of the generator's (the engine's host generator is checked against this
+ * oracle's, so a large network need not be generated twice). */
+ora_net *ora_create_from_backward(const char *dir, uint64_t S, const uint32_t *pre, const uint32_t *post,
+                                  const uint8_t *delay, const int16_t *weight, char *err, int errlen);
+/* Loads an engine checkpoint file (dpsnn_checkpoint; layout in
+ * include/dpsnn_hip.h) of the same network into the oracle's state: the
+ * oracle then continues the run from that ms. */
+int ora_load_checkpoint(ora_net *n, const char *path, char *err, int errlen);
 
 /* Sizes. */
 uint32_t ora_num_neurons(const ora_net *n);

@@ -37,6 +37,8 @@ class Oracle:
         vp, u32, u64 = ctypes.c_void_p, ctypes.c_uint32, ctypes.c_uint64
         sigs = {
             "ora_create": (vp, [ctypes.c_char_p, ctypes.c_char_p, ctypes.c_int]),
+            "ora_create_from_backward": (vp, [ctypes.c_char_p, u64, vp, vp, vp, vp, ctypes.c_char_p, ctypes.c_int]),
+            "ora_load_checkpoint": (ctypes.c_int, [vp, ctypes.c_char_p, ctypes.c_char_p, ctypes.c_int]),
             "ora_destroy": (None, [vp]),
             "ora_num_neurons": (u32, [vp]), "ora_num_synapses": (u64, [vp]),
             "ora_total_ms": (u32, [vp]), "ora_now_ms": (u32, [vp]),

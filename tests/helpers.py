@@ -22,14 +22,26 @@ def ptr(a):
 class OracleNet:
     """One oracle network (CPU restatement)."""
 
-    def __init__(self, oracle, inidir):
+    def __init__(self, oracle, inidir, backward=None):
+        """backward: {"pre", "post", "delay", "weight"} arrays of a backward
+        synapse list to take instead of generating the connectome."""
         self.lib = oracle.lib
         err = ctypes.create_string_buffer(512)
-        self.h = self.lib.ora_create(inidir.encode(), err, 512)
+        if backward is None:
+            self.h = self.lib.ora_create(inidir.encode(), err, 512)
+        else:
+            b = {k: np.ascontiguousarray(backward[k]) for k in ("pre", "post", "delay", "weight")}
+            self.h = self.lib.ora_create_from_backward(inidir.encode(), len(b["pre"]), ptr(b["pre"]), ptr(b["post"]),
+                                                       ptr(b["delay"]), ptr(b["weight"]), err, 512)
         if not self.h:
             raise RuntimeError("oracle create failed: " + err.value.decode())
         self.n = self.lib.ora_num_neurons(self.h)
         self.s = self.lib.ora_num_synapses(self.h)
+
+    def load_checkpoint(self, path):
+        err = ctypes.create_string_buffer(512)
+        if self.lib.ora_load_checkpoint(self.h, str(path).encode(), err, 512) != 0:
+            raise RuntimeError("oracle checkpoint load failed: " + err.value.decode())
 
     def close(self):
         if self.h:

@@ -144,3 +144,27 @@ def test_oracle_stdp_c2small_identical_to_reference(pkg, oracle, tmp_path):
     for probe in ("A", "B"):
         assert (sha256_file(tmp_path / f"synPeriodicProbe_{probe}-sec0-C4-H1-h000.dat") ==
                 g[f"synPeriodicProbe_{probe}"]["sha256"])
+
+
+def test_oracle_from_given_backward_list_matches_generated(oracle, pkg):
+    """An oracle built from the engine host generator's backward list (the
+    import used at sizes the oracle's own generator is too slow for; the
+    run then continues from an engine checkpoint, which carries the ran3
+    stream) holds the same synapses and bath as one that generated its
+    connectome itself."""
+    cfg = pkg.config_dir("c0")
+    pkg.apply_env(cfg, env_totalSimTime_ms=60)
+    host = pkg.HostConnectome(cfg)
+    syn = host.synapses()
+    host.close()
+    a = OracleNet(oracle, cfg)
+    b = OracleNet(oracle, cfg, backward=syn)
+    try:
+        ba, bb = a.backward(), b.backward()
+        for k in ("pre", "post", "delay", "kind", "weight"):
+            assert np.array_equal(ba[k], bb[k]), k
+        for x, y in zip(a.external_params(), b.external_params()):
+            assert np.array_equal(x, y)
+    finally:
+        a.close()
+        b.close()
