@@ -2170,6 +2170,9 @@ static int drain(dpsnn_net *n) {
     pos += cnt;
   }
   n->pendingSteps = 0;
+  // the files hold every drained step once a drain returns (dpsnn_sync and
+  // the end of the run read them complete)
+  if (n->writing) n->writer.flush();
   return 0;
 }
 

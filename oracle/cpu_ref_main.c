@@ -46,8 +46,9 @@ int main(int argc, char **argv) {
     nspk += ora_last_spikes(n, NULL, NULL);
   }
   t2 = now_s();
-  printf("steps %u in %.3f s (%.6f s/step), spikes %llu\n", ora_now_ms(n), t2 - t1,
-         (t2 - t1) / (ora_now_ms(n) ? ora_now_ms(n) : 1), (unsigned long long)nspk);
+  printf("steps %u in %.3f s (%.6f s/step), spikes %llu, synaptic events %llu\n", ora_now_ms(n), t2 - t1,
+         (t2 - t1) / (ora_now_ms(n) ? ora_now_ms(n) : 1), (unsigned long long)nspk,
+         (unsigned long long)ora_synaptic_events(n));
   ora_destroy(n);
   return 0;
 }

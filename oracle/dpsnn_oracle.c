@@ -221,6 +221,7 @@ struct ora_net {
   uint32_t ms;
   /* last step spikes */
   uint32_t nspk;
+  uint64_t events; /* synaptic events delivered since create */
   /* rates per pop */
   uint32_t *firingsPerPop;
   /* output */
@@ -813,6 +814,8 @@ ora_net *ora_create_from_backward(const char *dir, uint64_t S, const uint32_t *p
  * the next scan reads, the ran3 stream and the replayed drive's next event
  * times (or, for a Philox checkpoint, the device's next bath times, from
  * which this oracle continues with its own ran3 stream). */
+static double poisson_lapse(ora_net *n, uint32_t i);
+
 int ora_load_checkpoint(ora_net *n, const char *path, char *err, int errlen) {
   struct {
     char magic[8];
@@ -868,6 +871,13 @@ int ora_load_checkpoint(ora_net *n, const char *path, char *err, int errlen) {
   memcpy(n->rng.ma, rs.ma, sizeof rs.ma);
   n->rng.iff = rs.iff; n->rng.normSet = rs.set; n->rng.normII = rs.cached; n->rng.counter = rs.count;
   if (h.drive == 0 && fread(n->assignedExt, 8, N, f) != N) goto out;
+  /* A checkpoint of the on-device Philox drive carries no replayed stream:
+   * the oracle's own ran3 drive takes over from ms t, each neuron's first
+   * external event one Poisson lapse after t.  The continuation is then
+   * statistical, not bit-exact (used to time the oracle in the steady
+   * state of a large network). */
+  if (h.drive != 0)
+    for (i = 0; i < N; i++) n->assignedExt[i] = (double)h.t + poisson_lapse(n, i);
   /* ring slots of ms t-1 .. t-RS+1 and the fired flags of ms t */
   n->ms = h.t;
   for (m = 0; m < RING; m++) n->ring_cnt[m] = 0;
@@ -917,6 +927,7 @@ uint32_t ora_num_neurons(const ora_net *n) { return n->p.N; }
 uint64_t ora_num_synapses(const ora_net *n) { return n->S; }
 uint32_t ora_total_ms(const ora_net *n) { return n->p.totalSimTime_ms; }
 uint32_t ora_now_ms(const ora_net *n) { return n->ms; }
+uint64_t ora_synaptic_events(const ora_net *n) { return n->events; }
 uint32_t ora_num_subpops(const ora_net *n) { return n->p.subPopNumber; }
 uint32_t ora_neurons_per_cm(const ora_net *n) { return n->p.npc; }
 
@@ -1227,6 +1238,7 @@ int ora_fire(ora_net *n) {
         n->in_cur[n->in_off[post] + c] = wF;
         n->in_frac[n->in_off[post] + c] = modf(te, &ip);
         n->in_cnt[post] = c + 1;
+        n->events++;
         if (plastic) ltd(n, s, post, t);
       }
     }

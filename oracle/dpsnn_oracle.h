@@ -44,6 +44,8 @@ uint32_t ora_num_neurons(const ora_net *n);
 uint64_t ora_num_synapses(const ora_net *n);
 uint32_t ora_total_ms(const ora_net *n);
 uint32_t ora_now_ms(const ora_net *n);
+/* synaptic events delivered since create (each delivered synapse once) */
+uint64_t ora_synaptic_events(const ora_net *n);
 
 /* Backward (delivery-ordered) synapse list, as the reference keeps it after
  * inflateIntermToBackwardSynList.  Arrays of length ora_num_synapses(). */

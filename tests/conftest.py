@@ -41,7 +41,7 @@ class Oracle:
             "ora_load_checkpoint": (ctypes.c_int, [vp, ctypes.c_char_p, ctypes.c_char_p, ctypes.c_int]),
             "ora_destroy": (None, [vp]),
             "ora_num_neurons": (u32, [vp]), "ora_num_synapses": (u64, [vp]),
-            "ora_total_ms": (u32, [vp]), "ora_now_ms": (u32, [vp]),
+            "ora_total_ms": (u32, [vp]), "ora_now_ms": (u32, [vp]), "ora_synaptic_events": (u64, [vp]),
             "ora_get_backward": (None, [vp] + [vp] * 7),
             "ora_get_generated": (None, [vp] + [vp] * 4),
             "ora_write_synapse_dump": (ctypes.c_int, [vp, ctypes.c_char_p]),

@@ -3,6 +3,7 @@ import ctypes
 import hashlib
 import json
 import os
+import sys
 
 import numpy as np
 
@@ -13,6 +14,16 @@ GOLDEN = os.path.join(ROOT, "tests", "golden")
 def golden(name):
     with open(os.path.join(GOLDEN, name)) as f:
         return json.load(f)
+
+
+def small_network(kind, d):
+    """Writes the ini files of one of the networks of small_golden.json
+    ("djext" or "stdp_small", tests/golden/make_small_golden.py) into d;
+    returns the run's env settings."""
+    if GOLDEN not in sys.path:
+        sys.path.insert(0, GOLDEN)
+    import make_small_golden as m
+    return {"djext": m.write_djext, "stdp_small": m.write_stdp_small}[kind](str(d))
 
 
 def ptr(a):
@@ -115,3 +126,58 @@ def prefix_sha(path, nlines):
 def raster_lines(ids, te):
     """Raster text of one step, formatted like statSpikesClass::write."""
     return "".join("%d %.7g\n" % (int(i), float(t)) for i, t in zip(ids, te))
+
+
+def run_lengths(pre, delay, n, D):
+    """[n, D] events per delivery of (source, run delay d): the run starts at
+    the source's first synapse in delay bucket d of the backward list and
+    continues while the source matches, past the bucket's end too
+    (findAndGetNextSyn, DPSNN_synapse.c:48-136)."""
+    S = len(pre)
+    starts = np.flatnonzero(np.r_[True, (pre[1:] != pre[:-1]) | (delay[1:] != delay[:-1])])
+    first = np.flatnonzero(np.r_[True, pre[1:] != pre[:-1]])
+    end = np.r_[first[1:], S]
+    which = np.searchsorted(first, starts, side="right") - 1
+    deg = np.zeros((n, D), np.int64)
+    deg[pre[starts], delay[starts]] = end[which] - starts
+    return deg
+
+
+def delivered_events(syn, n, spikes):
+    """Synaptic events of steps 0..len(spikes)-1 from the backward list and
+    the spike ids of each step's scan: step t delivers, for each run delay
+    d <= t, the runs of the sources found by the scan of step t - d
+    (DPSNN_localNet_sim.c:837-999)."""
+    D = int(syn["delay"].max()) + 1 if len(syn["delay"]) else 1
+    deg = run_lengths(syn["pre"], syn["delay"], n, D)
+    return sum(int(deg[spikes[t - d], d].sum()) for t in range(len(spikes)) for d in range(min(D, t + 1)))
+
+
+def check_raster(steps, n, tarp_ms=2.0):
+    """Invariants of a raster given per step as (ids, te): ids ascend within
+    each step, every te lies in [t - 1, t), and no neuron fires twice within
+    its refractory period.  Returns the spike count."""
+    last = np.full(n, -1.0e9)
+    total = 0
+    for t, (ids, te) in enumerate(steps):
+        ids = np.asarray(ids, np.int64)
+        assert np.all(np.diff(ids) > 0), t
+        assert np.all((te >= t - 1) & (te < t)), t
+        assert ids.size == 0 or (ids[0] >= 0 and ids[-1] < n), t
+        assert np.all(te - last[ids] >= tarp_ms - 1e-9), t
+        last[ids] = te
+        total += ids.size
+    return total
+
+
+def read_raster(path, steps):
+    """A raster file ("id te" lines, statSpikesClass::write) as per-step
+    (ids, te) lists; a spike with te in [t - 1, t) belongs to step t."""
+    a = np.loadtxt(path, dtype=np.float64, ndmin=2) if os.path.getsize(path) else np.zeros((0, 2))
+    ids, te = a[:, 0].astype(np.int64), a[:, 1]
+    step = np.floor(te).astype(np.int64) + 1
+    order = np.argsort(step, kind="stable")
+    ids, te, step = ids[order], te[order], step[order]
+    bounds = np.searchsorted(step, np.arange(steps + 1))
+    assert bounds[-1] == len(step), "spikes past the last step"
+    return [(ids[bounds[t]:bounds[t + 1]], te[bounds[t]:bounds[t + 1]]) for t in range(steps)]

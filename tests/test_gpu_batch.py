@@ -8,7 +8,7 @@ import os
 import numpy as np
 import pytest
 
-from helpers import sha256_file
+from helpers import GOLDEN, golden, prefix_sha, sha256_file
 
 pytestmark = pytest.mark.gpu
 
@@ -97,6 +97,13 @@ def test_batch_members_match_alone(pkg, tmp_path):
             n.close()
     for i, (a, g) in enumerate(zip(alone, got)):
         _same(a, g)
+    # member 2 is configs[0] (2x2 columns, seed 1, replay drive) cut at 320 ms:
+    # its batched raster and rates are the reference's
+    g = golden("c0_raster.json")
+    assert prefix_sha(outs[2] / "spikes-sec0-C4-H1-h000.dat", g["prefix"]["200"]["lines"]) == \
+        g["prefix"]["200"]["sha256"]
+    rates = open(outs[2] / "ratesPerPop-sec0-C4-H1-h000.dat").read().splitlines()
+    assert rates == open(os.path.join(GOLDEN, "c0_ratesPerPop-sec0.dat")).read().splitlines()[:64]
 
 
 def test_removed_member_steps_alone_again(pkg, tmp_path):

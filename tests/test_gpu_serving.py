@@ -1,14 +1,15 @@
 """Serving: jobs submitted while others run are admitted into the running
 batch (continuous batching); each job's results -- raster/rates files,
 counters, final state, STDP synapses -- are bit-identical to the same
-network run alone, and the jobs overlap on the device."""
+network run alone and, for one job, to the oracle's run; the jobs overlap
+on the device."""
 import os
 import time
 
 import numpy as np
 import pytest
 
-from helpers import sha256_file
+from helpers import OracleNet, sha256_file
 
 pytestmark = pytest.mark.gpu
 
@@ -50,7 +51,7 @@ def _same(a, b):
         assert np.array_equal(a["syn"][k].view(np.uint8), b["syn"][k].view(np.uint8)), k
 
 
-def test_served_jobs_match_alone_and_overlap(pkg, tmp_path):
+def test_served_jobs_match_alone_and_overlap(pkg, oracle, tmp_path, monkeypatch):
     W = __import__("importlib").import_module("201812realtimecortsim_amd.workloads")
     dirs, envs = [], []
     for i, (side, seed, dur, stdp, _) in enumerate(JOBS):
@@ -99,6 +100,22 @@ def test_served_jobs_match_alone_and_overlap(pkg, tmp_path):
     assert overlap
     assert st["finished"] == len(JOBS) and st["failed"] == 0
     assert st["synaptic_events"] == sum(g["stats"]["synaptic_events"] for g in got)
+    # one served job against the oracle: job 4 (2x2 columns, replay drive)
+    for k, v in envs[4].items():
+        monkeypatch.setenv(k, str(v))
+    od = tmp_path / "oracle4"
+    od.mkdir()
+    ora = OracleNet(oracle, dirs[4])
+    try:
+        ora.set_output_dir(str(od))
+        for _ in range(JOBS[4][2]):
+            ora.fire()
+        o = ora.state()
+    finally:
+        ora.close()  # flushes the files
+    for k in ("v", "c", "Tr", "Te", "fired"):
+        assert np.array_equal(got[4]["state"][k].view(np.uint8), o[k].view(np.uint8)), k
+    assert {f: sha256_file(od / f) for f in sorted(os.listdir(od))} == got[4]["files"]
 
 
 def test_bad_job_fails_alone(pkg, tmp_path):

@@ -7,7 +7,7 @@ import subprocess
 import numpy as np
 import pytest
 
-from helpers import GOLDEN, OracleNet, golden, prefix_sha, sha256_file
+from helpers import GOLDEN, OracleNet, check_raster, delivered_events, golden, prefix_sha, read_raster, sha256_file, small_network
 
 
 @pytest.fixture()
@@ -42,6 +42,8 @@ def test_raster_first_30ms_matches_reference(c0, tmp_path):
     c0.set_output_dir(str(tmp_path))
     for _ in range(30):
         c0.fire()
+    events = c0.lib.ora_synaptic_events(c0.h)
+    backward = c0.backward()
     c0.close()  # flushes the files
     path = tmp_path / "spikes-sec0-C4-H1-h000.dat"
     assert prefix_sha(path, g["prefix"]["30"]["lines"]) == g["prefix"]["30"]["sha256"]
@@ -50,6 +52,13 @@ def test_raster_first_30ms_matches_reference(c0, tmp_path):
     rates = open(tmp_path / "ratesPerPop-sec0-C4-H1-h000.dat").read().splitlines()
     ref = open(os.path.join(GOLDEN, "c0_ratesPerPop-sec0.dat")).read().splitlines()
     assert rates == ref[:len(rates)] and len(rates) == 6  # written at t = 0, 5, ..., 25
+    # the raster invariants the full-size GPU tests check hold on the reference's raster
+    steps = read_raster(path, 30)
+    assert [len(s[0]) for s in steps] == g["per_ms_counts"][:30]
+    assert check_raster(steps, 5000) == g["prefix"]["30"]["lines"]
+    # the event count restated from the backward list and the raster is the
+    # oracle's count of delivered synapses
+    assert delivered_events(backward, 5000, [s[0] for s in steps]) == events > 0
 
 
 def _ran3_stream(seed, n):
@@ -168,3 +177,45 @@ def test_oracle_from_given_backward_list_matches_generated(oracle, pkg):
     finally:
         a.close()
         b.close()
+
+
+def _small(oracle, kind, tmp_path, monkeypatch):
+    d = tmp_path / kind
+    for k, v in small_network(kind, d).items():
+        monkeypatch.setenv(k, v)
+    out = tmp_path / "out"
+    out.mkdir()
+    net = OracleNet(oracle, str(d))
+    net.set_output_dir(str(out))
+    return net, out
+
+
+def test_oracle_djext_bath_matches_reference(oracle, tmp_path, monkeypatch):
+    """The Gaussian bath LUT's DJExt > 0 branch (erflib.c:141-169): configs[0]
+    with DJExt = 0.1, raster and rates of the first 50 ms."""
+    g = golden("small_golden.json")["djext"]
+    net, out = _small(oracle, "djext", tmp_path, monkeypatch)
+    for _ in range(50):
+        net.fire()
+    net.close()
+    path = out / "spikes-sec0-C4-H1-h000.dat"
+    assert prefix_sha(path, g["raster"]["prefix"]["50"]["lines"]) == g["raster"]["prefix"]["50"]["sha256"]
+    rates = open(out / "ratesPerPop-sec0-C4-H1-h000.dat").read().splitlines()
+    assert len(rates) == 10 and rates == g["rates"].splitlines()[:10]
+
+
+def test_oracle_stdp_small_across_commit_matches_reference(oracle, tmp_path, monkeypatch):
+    """configs[2]'s STDP on 2 x 2 columns of 125 neurons for 1.1 s: both
+    seconds' rasters and rates and the synapse probes around the
+    end-of-second commit (DPSNN_localNet_sim.c:255-262) equal the
+    reference's."""
+    g = golden("small_golden.json")["stdp_small"]
+    net, out = _small(oracle, "stdp_small", tmp_path, monkeypatch)
+    while net.fire() == 0:
+        pass
+    net.close()
+    for sec in (0, 1):
+        assert sha256_file(out / f"spikes-sec{sec}-C4-H1-h000.dat") == g[f"spikes-sec{sec}"]["sha256"], sec
+        assert open(out / f"ratesPerPop-sec{sec}-C4-H1-h000.dat").read() == g[f"rates-sec{sec}"], sec
+    for probe in ("A", "B"):
+        assert sha256_file(out / f"synPeriodicProbe_{probe}-sec0-C4-H1-h000.dat") == g[f"synPeriodicProbe_{probe}"]
