@@ -130,6 +130,7 @@ class BatchShape(ctypes.Structure):
 
 _SIGS = {
     "dpsnn_get_parameters": (ctypes.c_int, [ctypes.c_char_p, _p(Parameters)]),
+    "dpsnn_write_ini_files": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_char_p, ctypes.c_char_p]),
     "dpsnn_create": (ctypes.c_int, [ctypes.c_char_p, _p(Options), _p(ctypes.c_void_p)]),
     "dpsnn_prepare": (ctypes.c_int, [ctypes.c_char_p, _p(Options), _p(ctypes.c_void_p)]),
     "dpsnn_device_bytes_needed": (ctypes.c_uint64, [ctypes.c_void_p]),
@@ -254,6 +255,23 @@ def get_parameters(inidir):
     return out.as_dict()
 
 
+def _options(device=0, drive=DRIVE_REPLAY, philox_seed=1, output_dir=None, drain_every_ms=0, keep_spikes=False,
+             env=None, drop_host_synapses=False):
+    """dpsnn_options of the Network keyword arguments (the strings stay
+    referenced by the structure)."""
+    out = output_dir.encode() if output_dir else None
+    env_text = ";".join(f"{k}={v}" for k, v in env.items()).encode() if env else None
+    return Options(device, drive, philox_seed, out, drain_every_ms, int(keep_spikes), env_text,
+                   int(drop_host_synapses))
+
+
+def write_ini_files(inidir, outdir, env=None):
+    """Perseo .ini files of the network of inidir into outdir
+    (dpsnn_write_ini_files; localNetClass::writeIniFiles)."""
+    env_text = ";".join(f"{k}={v}" for k, v in env.items()).encode() if env else None
+    _check("dpsnn_write_ini_files", library().dpsnn_write_ini_files(inidir.encode(), outdir.encode(), env_text))
+
+
 class Network:
     """One local network on one GPU (a dpsnn_net)."""
 
@@ -264,10 +282,9 @@ class Network:
         dpsnn_checkpoint to resume from (dpsnn_restore), taken on the
         network of the same ini files and settings."""
         self._lib = library()
-        self._out = output_dir.encode() if output_dir else None
-        env_text = ";".join(f"{k}={v}" for k, v in env.items()).encode() if env else None
-        opt = Options(device, drive, philox_seed, self._out, drain_every_ms, int(keep_spikes), env_text,
-                      int(drop_host_synapses))
+        opt = _options(device, drive, philox_seed, output_dir, drain_every_ms, keep_spikes, env,
+                       drop_host_synapses)
+        self._out = opt.output_dir
         h = ctypes.c_void_p()
         if checkpoint:
             _check("dpsnn_restore", self._lib.dpsnn_restore(inidir.encode(), ctypes.byref(opt),
@@ -280,6 +297,44 @@ class Network:
     def checkpoint(self, path):
         """Writes the network's state to path (dpsnn_checkpoint)."""
         _check("dpsnn_checkpoint", self._lib.dpsnn_checkpoint(self._h, str(path).encode()))
+
+    @classmethod
+    def build_many(cls, specs, threads=0, host_threads=0):
+        """Builds several networks: their host builds (dpsnn_prepare: ini
+        files, connectome, delivery layout; no device call) run on `threads`
+        Python threads at once (0: one per host core, at most 16), each
+        build on `host_threads` threads of its own (0: the library's
+        default); the uploads (dpsnn_upload) follow in order.  specs: a list
+        of (inidir, {Network keyword arguments}) pairs.  Returns the
+        networks in the order of specs."""
+        from concurrent.futures import ThreadPoolExecutor
+        lib = library()
+        threads = threads or min(16, os.cpu_count() or 1)
+        opts = []
+        for _, kw in specs:
+            o = _options(**kw)
+            o.host_threads = host_threads
+            opts.append(o)
+        handles = [ctypes.c_void_p() for _ in specs]
+
+        def prep(i):
+            # ctypes drops the GIL for the call: the builds overlap
+            return lib.dpsnn_prepare(specs[i][0].encode(), ctypes.byref(opts[i]), ctypes.byref(handles[i]))
+
+        with ThreadPoolExecutor(max_workers=threads) as ex:
+            codes = list(ex.map(prep, range(len(specs))))
+        nets = []
+        try:
+            for i, rc in enumerate(codes):
+                _check("dpsnn_prepare", rc)
+                _check("dpsnn_upload", lib.dpsnn_upload(handles[i]))
+                nets.append(cls._wrap(handles[i]))
+                handles[i] = None
+        finally:
+            for h in handles:
+                if h is not None and h.value:
+                    lib.dpsnn_destroy(h)
+        return nets
 
     @classmethod
     def _wrap(cls, handle):

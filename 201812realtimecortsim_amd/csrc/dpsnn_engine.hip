@@ -2569,6 +2569,17 @@ int dpsnn_get_parameters(const char *inidir, dpsnn_parameters *out) {
   return 0;
 }
 
+int dpsnn_write_ini_files(const char *inidir, const char *outdir, const char *env) {
+  if (!inidir || !outdir) return fail(DPSNN_ERR_ARG, "null argument");
+  Params p;
+  std::string err;
+  EnvMap over;
+  if (env && !parse_env_overrides(env, over, err)) return fail(DPSNN_ERR_ARG, "%s", err.c_str());
+  if (!read_params(inidir, p, err, env ? &over : nullptr)) return fail(DPSNN_ERR_CONFIG, "%s", err.c_str());
+  if (!write_ini_files(p, outdir, err)) return fail(DPSNN_ERR_IO, "%s", err.c_str());
+  return 0;
+}
+
 int dpsnn_prepare(const char *inidir, const dpsnn_options *opt, dpsnn_net **out) {
   if (!inidir || !out) return fail(DPSNN_ERR_ARG, "null argument");
   *out = nullptr;
@@ -2601,6 +2612,12 @@ int dpsnn_prepare(const char *inidir, const dpsnn_options *opt, dpsnn_net **out)
     return fail(DPSNN_ERR_CONFIG, "%s", err.c_str());
   build_delivery(n->par, n->con, /*with_incoming=*/n->stdp, n->opt.host_threads);
   n->rng.seed(0);
+  // DPSNN_REPLAY_STREAM=k > 0: the replayed bath continues from
+  // SetRandomSeed(k) instead, the stream process loc_h = k would draw --
+  // the reference's own drive with another stream, to measure its
+  // run-to-run spread; unset or 0 is the reference's stream
+  if (const char *e = std::getenv("DPSNN_REPLAY_STREAM"))
+    if (const int k = std::atoi(e)) n->rng.seed(k);
   if (n->stdp) {
     // each in-list item names its synapse by its rank in the target's
     // incoming list (backward-list order)

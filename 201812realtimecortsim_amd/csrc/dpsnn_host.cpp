@@ -1251,3 +1251,65 @@ void StatWriter::close() {
 }
 
 }  // namespace dpsnn
+
+namespace dpsnn {
+
+// --------------------------------------------------------------------------
+// Perseo .ini files (localNetClass::writeIniFiles, DPSNN_writeIniFiles.c:
+// 31-158) at world size 1 (loc_h 0, every column local).  Per column (x
+// outer, y inner): PerseoConFile gets one line per (stencil cell, target
+// sub-population, source sub-population) whose connection probability is
+// positive -- over the reference's whole compile-time stencil range, the
+// target column being the cell's offset from the source (explicit stencil)
+// or the cell itself (homogeneous) when it lies on the grid; PerseoModFile
+// gets one line per sub-population with the column's external bath
+// (generateSourceNeuCExt, DPSNN_LIFCAconnectome.cpp:871-887).
+// --------------------------------------------------------------------------
+bool write_ini_files(const Params &p, const std::string &outdir, std::string &err) {
+  const dpsnn_parameters &q = p.pub;
+  const std::string con = outdir + "/PerseoConFile_h0.txt", mod = outdir + "/PerseoModFile_h0.txt";
+  FILE *fc = std::fopen(con.c_str(), "w");
+  FILE *fm = fc ? std::fopen(mod.c_str(), "w") : nullptr;
+  if (!fc || !fm) {
+    if (fc) std::fclose(fc);
+    err = "cannot create the Perseo files in " + outdir;
+    return false;
+  }
+  const uint32_t nsub = q.subPopNumber;
+  for (uint32_t cx = 0; cx < q.globCFX; ++cx)
+    for (uint32_t cy = 0; cy < q.globCFY; ++cy) {
+      const uint32_t src = cx + cy * q.globCFX;
+      for (uint32_t sx = 0; sx < kMaxStencil; ++sx)
+        for (uint32_t sy = 0; sy < kMaxStencil; ++sy) {
+          const int32_t tx = q.overallConnectivity == 0 ? (int32_t)(cx + sx) - (int32_t)p.maxModDeltaX : (int32_t)sx;
+          const int32_t ty = q.overallConnectivity == 0 ? (int32_t)(cy + sy) - (int32_t)p.maxModDeltaY : (int32_t)sy;
+          if (tx < 0 || tx >= (int32_t)q.globCFX || ty < 0 || ty >= (int32_t)q.globCFY) continue;
+          const uint32_t tgt = (uint32_t)tx + q.globCFX * (uint32_t)ty;
+          for (uint32_t ts = 0; ts < nsub; ++ts)
+            for (uint32_t ss = 0; ss < nsub; ++ss) {
+              const float prob = p.prob_at(ss, ts, sx, sy);
+              if (!(prob > 0.0f)) continue;
+              const SubPop &s = p.sp[ss];
+              std::fprintf(fc, " %5d %5d     %g   %g   %g   '%s'   %8g   %g\n", (int)(nsub * tgt + ts),
+                           (int)(nsub * src + ss), (double)prob, s.DMin[ts] + 0.5, s.DMax[ts] + 0.5, "Fixed",
+                           s.J[ts], s.DJ[ts]);
+            }
+        }
+      uint32_t bx = cx, by = cy;
+      if (bx >= q.globCFX / 2) bx = -1 * (bx - q.globCFX + 1);
+      if (by >= q.globCFY / 2) by = -1 * (by - q.globCFY + 1);
+      if (bx >= q.stencilDim) bx = q.stencilDim - 1;
+      if (by >= q.stencilDim) by = q.stencilDim - 1;
+      const float cext = (float)p.bath[size_t(bx) * kMaxStencil + by];
+      for (uint32_t ss = 0; ss < nsub; ++ss) {
+        const SubPop &s = p.sp[ss];
+        std::fprintf(fm, " %6d   %g   %g   %g   %7g   %g   %g   %g   %g   %g   %4g   %4g   %d \n", (int)s.count, s.JExt,
+                     s.DJExt, (double)cext, s.NuExt, s.Tau, s.Theta, s.H, s.Tarp, s.AlphaC, s.TauC, s.gC, 0);
+      }
+    }
+  const bool ok = std::fclose(fc) == 0 && std::fclose(fm) == 0;
+  if (!ok) err = "writing the Perseo files in " + outdir + " failed";
+  return ok;
+}
+
+}  // namespace dpsnn

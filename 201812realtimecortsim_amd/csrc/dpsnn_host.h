@@ -234,4 +234,8 @@ class StatWriter {
   std::vector<char> buf_;
 };
 
+// Perseo .ini files of the network (localNetClass::writeIniFiles,
+// DPSNN_writeIniFiles.c:31-158): PerseoConFile_h0.txt, PerseoModFile_h0.txt.
+bool write_ini_files(const Params &p, const std::string &outdir, std::string &err);
+
 }  // namespace dpsnn

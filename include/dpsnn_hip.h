@@ -115,6 +115,16 @@ typedef struct {
  * the ini files in `inidir`.  Replaces DPSNN_getParameters.c:23-831. */
 int dpsnn_get_parameters(const char *inidir, dpsnn_parameters *out);
 
+/* Perseo simulator .ini files of the network of `inidir`
+ * (localNetClass::writeIniFiles, DPSNN_writeIniFiles.c:31-158), at world
+ * size 1: PerseoConFile_h0.txt (one line per target/source sub-population
+ * pair of each column and stencil cell with a positive connection
+ * probability) and PerseoModFile_h0.txt (one line per sub-population of each
+ * column) in `outdir`.  The reference compiles its call out
+ * (DPSNN_localNet_init.c:247-249, an if(false) region); here it is a call of
+ * its own, host only.  env: as dpsnn_options.env (NULL: the process's). */
+int dpsnn_write_ini_files(const char *inidir, const char *outdir, const char *env);
+
 /* Builds a network: reads parameters, runs the reference connectome
  * generator on the host (DPSNN_LIFCAcorticalModuleGen.cpp:62-220), builds
  * the delivery layout (DPSNN_localNet_init.c:631-740) and uploads state and

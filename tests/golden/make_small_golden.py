@@ -10,6 +10,10 @@ the other fixtures leave open:
               DJExt > 0 branch (erflib.c:141-169, used by the external drive
               at DPSNN_neuron_sim.c:604-640).  Raster summary (prefixes at
               50 and 100 ms) and the ratesPerPop file verbatim.
+              Also the connection-probability matrices the reference
+              prints at init (probConnectivityMatrix, DPSNN_LIFCAconnectome
+              .cpp:181-192, "%f", rows y, columns x) for configs[0]'s
+              connectivity: the values writeIniFiles puts in PerseoConFile.
   stdp_small  configs[2]'s STDP on a 2 x 2 grid of 125-neuron columns for
               1100 ms, across the end-of-second weight commit, with the
               synPeriodicProbe dumps on: raster summaries of both seconds,
@@ -92,6 +96,24 @@ def _run(d, env_over):
                        check=True)
 
 
+def parse_prob_matrices(stdout_path):
+    """{"s,t": rows of the printed matrix} from the reference's stdout."""
+    out, key, rows = {}, None, []
+    for line in open(stdout_path):
+        if "probConnectivityMatrix for sourceSubPop=" in line:
+            f = line.replace("=", " ").replace(":", " ").split()
+            key = f"{int(f[f.index('sourceSubPop') + 1])},{int(f[f.index('targetSubPop') + 1])}"
+            rows = []
+            out[key] = rows
+        elif key is not None:
+            vals = line.split()
+            if vals and all(v.replace(".", "", 1).isdigit() for v in vals):
+                rows.append(vals)
+            elif rows:
+                key = None
+    return out
+
+
 def main():
     sys.path.insert(0, HERE)
     from make_golden import raster_summary, sha256_file  # noqa: E402
@@ -101,7 +123,8 @@ def main():
     d = os.path.join(tmp, "djext")
     _run(d, write_djext(d))
     out["djext"] = {"raster": raster_summary(os.path.join(d, "spikes-sec0-C4-H1-h000.dat"), [50, 100]),
-                    "rates": open(os.path.join(d, "ratesPerPop-sec0-C4-H1-h000.dat")).read()}
+                    "rates": open(os.path.join(d, "ratesPerPop-sec0-C4-H1-h000.dat")).read(),
+                    "prob_matrices": parse_prob_matrices(os.path.join(d, "stdout.txt"))}
     d = os.path.join(tmp, "stdp_small")
     _run(d, write_stdp_small(d))
     s = {}

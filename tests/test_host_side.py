@@ -281,3 +281,64 @@ def test_window_layout_on_grid_edges(pkg, tmp_path):
         _check_window(pkg, host, 2)
     finally:
         host.close()
+
+
+def test_perseo_ini_files(pkg, tmp_path):
+    """writeIniFiles (DPSNN_writeIniFiles.c:31-158) on configs[0]: every
+    PerseoConFile probability equals the matrix the reference prints at
+    init (its stdout, "%f"; rows y, columns x), every column pair within the
+    stencil appears once per sub-population pair, sources in column order
+    (x outer, y inner); PerseoModFile carries column.txt's values and the
+    bath efficacy.  Format strings follow the source; no reference run
+    writes these files (its call is compiled out), so the text itself is
+    parity unpinned."""
+    g = golden("small_golden.json")["djext"]["prob_matrices"]
+    cfg = pkg.config_dir("c0")
+    pkg.apply_env(cfg)
+    pkg.write_ini_files(cfg, str(tmp_path))
+    par = pkg.get_parameters(cfg)
+    CFX, CFY, nsub = par["globCFX"], par["globCFY"], par["subPopNumber"]
+    half = len(g["0,0"]) // 2
+    conn = {}
+    for line in open(os.path.join(cfg, "connectivity.txt")):
+        f = line.split()
+        conn[(int(f[0]), int(f[1]))] = [float(np.float32(x)) for x in f[2:6]]
+    lines = open(tmp_path / "PerseoConFile_h0.txt").read().splitlines()
+    seen, src_order = [], []
+    for ln in lines:
+        f = ln.split()
+        tp, sp_, prob, dmin, dmax, kind, J, DJ = int(f[0]), int(f[1]), float(f[2]), float(f[3]), float(f[4]), f[5], \
+            float(f[6]), float(f[7])
+        (tc, ts), (sc, ss) = divmod(tp, nsub), divmod(sp_, nsub)
+        dx, dy = tc % CFX - sc % CFX, tc // CFX - sc // CFX
+        ref = float(g[f"{ss},{ts}"][dy + half][dx + half])
+        assert abs(prob - ref) <= 6e-7 and kind == "'Fixed'", ln
+        Jr, DJr, dminr, dmaxr = conn[(ss, ts)]
+        assert (dmin, dmax) == (dminr + 0.5, dmaxr + 0.5) and abs(J / Jr - 1) < 1e-5 and abs(DJ / DJr - 1) < 1e-5, ln
+        seen.append((sc, tc, ss, ts))
+        if not src_order or src_order[-1] != sc:
+            src_order.append(sc)
+    want = set()
+    for scx in range(CFX):
+        for scy in range(CFY):
+            for tcx in range(CFX):
+                for tcy in range(CFY):
+                    dx, dy = tcx - scx, tcy - scy
+                    if abs(dx) > half or abs(dy) > half:
+                        continue
+                    for ss in range(nsub):
+                        for ts in range(nsub):
+                            if float(g[f"{ss},{ts}"][dy + half][dx + half]) > 0:
+                                want.add((scx + scy * CFX, tcx + tcy * CFX, ss, ts))
+    assert len(seen) == len(set(seen)) and set(seen) == want
+    assert src_order == [cx + cy * CFX for cx in range(CFX) for cy in range(CFY)]
+    col = [ln.split() for ln in open(os.path.join(cfg, "column.txt")) if ln.strip()]
+    bath = float(open(os.path.join(cfg, "bathEfficacy.txt")).read().split()[0])
+    mods = open(tmp_path / "PerseoModFile_h0.txt").read().splitlines()
+    assert len(mods) == CFX * CFY * nsub
+    for k, ln in enumerate(mods):
+        f, c = ln.split(), col[k % nsub]
+        assert ln.endswith(" 0 ") and int(f[0]) == int(c[1]) and float(f[3]) == bath
+        # JExt DJExt | NuExt Tau Theta H Tarp AlphaC TauC gC
+        for got, want_ in zip(f[1:3] + f[4:12], c[2:4] + c[4:12]):
+            assert abs(float(got) - float(want_)) <= 1e-6 * max(1.0, abs(float(want_))), (ln, got, want_)
