@@ -153,31 +153,45 @@ def delivered_events(syn, n, spikes):
     return sum(int(deg[spikes[t - d], d].sum()) for t in range(len(spikes)) for d in range(min(D, t + 1)))
 
 
-def check_raster(steps, n, tarp_ms=2.0):
+def check_raster(steps, n, tarp_ms=2.0, te_tol=0.0):
     """Invariants of a raster given per step as (ids, te): ids ascend within
     each step, every te lies in [t - 1, t), and no neuron fires twice within
-    its refractory period.  Returns the spike count."""
+    its refractory period.  te_tol widens both time checks (emission times
+    read back from the raster file's 7 significant digits).  Returns the
+    spike count."""
     last = np.full(n, -1.0e9)
     total = 0
     for t, (ids, te) in enumerate(steps):
         ids = np.asarray(ids, np.int64)
         assert np.all(np.diff(ids) > 0), t
-        assert np.all((te >= t - 1) & (te < t)), t
+        if te_tol:
+            assert np.all((te >= t - 1 - te_tol) & (te <= t + te_tol)), t
+        else:
+            assert np.all((te >= t - 1) & (te < t)), t
         assert ids.size == 0 or (ids[0] >= 0 and ids[-1] < n), t
-        assert np.all(te - last[ids] >= tarp_ms - 1e-9), t
+        assert np.all(te - last[ids] >= tarp_ms - 1e-9 - te_tol), t
         last[ids] = te
         total += ids.size
     return total
 
 
 def read_raster(path, steps):
-    """A raster file ("id te" lines, statSpikesClass::write) as per-step
-    (ids, te) lists; a spike with te in [t - 1, t) belongs to step t."""
+    """A raster file ("id te" lines, statSpikesClass::write: each step's
+    spikes in ascending id order, steps in order) as per-step (ids, te).  A
+    spike of step t has te in [t - 1, t), but "%.7g" can print a te just
+    below t as t itself: such a line stays in step t while the file's order
+    allows it (ids still ascending), else it opens step t + 1."""
     a = np.loadtxt(path, dtype=np.float64, ndmin=2) if os.path.getsize(path) else np.zeros((0, 2))
     ids, te = a[:, 0].astype(np.int64), a[:, 1]
     step = np.floor(te).astype(np.int64) + 1
-    order = np.argsort(step, kind="stable")
-    ids, te, step = ids[order], te[order], step[order]
+    cur, last = -1, -1
+    for k in range(len(ids)):
+        s_ = step[k]
+        if te[k] == np.floor(te[k]) and (cur < s_ - 1 or (cur == s_ - 1 and ids[k] > last)):
+            s_ -= 1
+        assert s_ >= cur, ("raster out of step order", k)
+        cur, last = s_, ids[k]
+        step[k] = s_
     bounds = np.searchsorted(step, np.arange(steps + 1))
     assert bounds[-1] == len(step), "spikes past the last step"
     return [(ids[bounds[t]:bounds[t + 1]], te[bounds[t]:bounds[t + 1]]) for t in range(steps)]
