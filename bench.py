@@ -19,8 +19,12 @@ the reference generator (no external data).  Prints one JSON line:
                 by its average time from HIP events on the network's stream;
                 traffic = HBM bytes per step from the committed counter
                 summary (profiles/step_<config>_traffic.json)
-  cpu_baseline  the oracle (scalar C restatement of DPSNN) on the host, on a
-                bounded sample (configs[0]'s network, a few tens of ms)
+  cpu_baseline  the oracle (scalar C restatement of DPSNN's step) on one host
+                core, on the benchmarked network continued from the
+                engine's state after the timed steps (a few steps); beside
+                it configs[0] on the host: one oracle process per core of
+                this process's share, and the reference itself (built from
+                its sources into oracle/_ref) under mpirun -np 1
 """
 import argparse
 import ctypes
@@ -38,34 +42,115 @@ sys.path.insert(0, ROOT)
 HBM_PEAK_GBS = 8000.0
 
 
-def cpu_baseline(pkg, sample_ms):
+def _oracle():
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     from conftest import Oracle
     from helpers import OracleNet
-    cfg = pkg.config_dir("c0")
-    pkg.apply_env(cfg)
-    ora = OracleNet(Oracle(), cfg)
-    # synaptic events = sum over delivered runs; count them from the spikes
-    # and the out-degree per delay, exactly as delivered
-    import numpy as np
-    b = ora.backward()
-    D = 20
-    deg = np.zeros((ora.n, D), np.int64)
-    np.add.at(deg, (b["pre"], b["delay"]), 1)
-    hist = []
-    events = 0
+    return Oracle, OracleNet
+
+
+def cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_share():
+    """Host cores this process may use: its affinity mask, capped by
+    OMP_NUM_THREADS where set (the box's CPU share)."""
+    n = len(os.sched_getaffinity(0))
+    k = int(os.environ.get("OMP_NUM_THREADS", "0") or 0)
+    return min(n, k) if k > 0 else n
+
+
+def cpu_baseline_port(pkg, cfg, ckpt, backward, t_ckpt, steps):
+    """The oracle (scalar C restatement of the reference's step) on the
+    same network, continued from the engine's checkpoint of ms t_ckpt (the
+    network's state after the timed region; the oracle's own ran3 drive
+    replaces the Philox drive from there), timed over `steps` steps on one
+    host core."""
+    Oracle, OracleNet = _oracle()
     t0 = time.perf_counter()
-    for t in range(sample_ms):
+    ora = OracleNet(Oracle(), cfg, backward=backward)
+    ora.load_checkpoint(ckpt)
+    build_s = time.perf_counter() - t0
+    e0 = ora.lib.ora_synaptic_events(ora.h)
+    t0 = time.perf_counter()
+    for _ in range(steps):
         ora.fire()
-        ids, _ = ora.last_spikes()
-        hist.append(ids)
-        for d in range(min(D, t + 1)):
-            events += int(deg[hist[t - d], d].sum())
     el = time.perf_counter() - t0
+    ev = ora.lib.ora_synaptic_events(ora.h) - e0
     ora.close()
-    return {"value": events / el, "unit": "synaptic events/s", "cores": 1, "kind": "port",
-            "sample": f"oracle (scalar C port of DPSNN) on configs[0]'s 5000-neuron network, "
-                      f"{sample_ms} ms simulated after init, {el:.1f} s wall"}
+    return {"value": ev / el, "unit": "synaptic events/s", "cores": 1, "kind": "port",
+            "sample": f"oracle (scalar C restatement of DPSNN's step, one host core) on this workload's network "
+                      f"(built from the engine's backward list in {build_s:.0f} s, not timed), continued from the "
+                      f"engine's state at ms {t_ckpt} with its own ran3 bath: {steps} steps, {ev} synaptic "
+                      f"events in {el:.1f} s ({el / steps:.2f} s per simulated ms)",
+            "cpu_model": cpu_model()}
+
+
+def cpu_baseline_c0(pkg, ms):
+    """configs[0] on the host: (a) the oracle's driver, one process per host
+    core of this process's share, all at once; (b) the reference itself
+    (oracle/_ref, built from the reference's sources) at MPI world size 1,
+    where that binary and mpirun are present.  Both run `ms` simulated ms;
+    the event count of (b) is (a)'s (the same network and raster)."""
+    import subprocess
+    import tempfile
+    exe = os.path.join(ROOT, "oracle", "_build", "dpsnn_cpu_ref")
+    if not os.path.exists(exe):
+        subprocess.run(["make", "-s", "-C", os.path.join(ROOT, "oracle")], check=True)
+    cfg = pkg.config_dir("c0")
+    env = dict(os.environ)
+    env.update(pkg.read_env_file(os.path.join(cfg, "env.txt")))
+    env["env_spikes_writeDisable"] = "1"
+    P = cpu_share()
+    tmp = tempfile.mkdtemp(prefix="dpsnn_cpu_")
+    procs = []
+    for k in range(P):
+        out = os.path.join(tmp, f"p{k}")
+        os.makedirs(out)
+        procs.append(subprocess.Popen([exe, cfg, out, "--steps", str(ms)], env=env, stdout=subprocess.PIPE,
+                                      stderr=subprocess.STDOUT, text=True))
+    runs = []
+    for p in procs:
+        text = p.communicate()[0]
+        if p.returncode != 0:
+            raise RuntimeError(f"dpsnn_cpu_ref failed: {text[-400:]}")
+        line = [x for x in text.splitlines() if x.startswith("steps ")][0]
+        runs.append((float(line.split(" in ")[1].split(" s")[0]), int(line.rsplit("synaptic events ", 1)[1])))
+    ev = runs[0][1]
+    assert all(r[1] == ev for r in runs)
+    wall = max(r[0] for r in runs)
+    res = {"per_core": {"value": P * ev / wall, "unit": "synaptic events/s", "cores": P, "kind": "port",
+                        "sample": f"{P} concurrent oracle processes (dpsnn_cpu_ref), each configs[0] "
+                                  f"(5000 neurons) for {ms} ms after init; slowest step loop {wall:.1f} s"}}
+    ref = os.path.join(ROOT, "oracle", "_ref", "LIFCA-DPSNN.out")
+    mpirun = "/opt/conda/bin/mpirun"
+    if os.path.exists(ref) and os.path.exists(mpirun):
+        d = os.path.join(tmp, "ref")
+        os.makedirs(d)
+        for name in ("column.txt", "connectivity.txt", "bathEfficacy.txt"):
+            with open(os.path.join(cfg, name)) as f, open(os.path.join(d, name), "w") as g:
+                g.write(f.read())
+        renv = dict(env, env_totalSimTime_ms=str(ms))
+        text = subprocess.run([mpirun, "-np", "1", ref], cwd=d, env=renv, capture_output=True, text=True,
+                              timeout=600).stdout
+        line = [x for x in text.splitlines() if "mean localNetProcess_fire" in x]
+        if line:
+            fire_s = float(line[0].rsplit("total fire time =", 1)[1].split()[0])
+            res["reference"] = {"value": ev / fire_s, "unit": "synaptic events/s", "cores": 1, "kind": "reference",
+                                "sample": f"the reference (oracle/_ref, built from its sources) under mpirun -np 1 on "
+                                          f"configs[0] for {ms} ms: {fire_s:.1f} s of localNetProcess_fire "
+                                          f"(its own CHRONO line), events as counted by the oracle on the same "
+                                          f"raster"}
+    else:
+        res["reference"] = None
+    return res
 
 
 def log(msg):
@@ -287,7 +372,10 @@ def main():
     ap.add_argument("--build-threads", type=int, default=4)
     ap.add_argument("--no-tune", action="store_true")
     ap.add_argument("--drive", default="philox", choices=["philox", "replay"])
-    ap.add_argument("--cpu-sample-ms", type=int, default=200)
+    ap.add_argument("--cpu-steps", type=int, default=2,
+                    help="steps of the oracle on the benchmarked network (from the engine's checkpoint)")
+    ap.add_argument("--cpu-c0-ms", type=int, default=40,
+                    help="simulated ms of the configs[0] host runs (one process per core; the reference)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -328,6 +416,16 @@ def main():
     wall = time.perf_counter() - t0
     b = net.stats()
     kms, launches = net.kernel_time(0)
+    if not args.no_cpu_baseline:
+        # the network's state after the timed region, for the host baseline
+        import tempfile
+        ck_dir = tempfile.mkdtemp(prefix="dpsnn_bench_")
+        ck = os.path.join(ck_dir, "net.ckpt")
+        net.checkpoint(ck)
+        t_ck = net.now_ms
+        syn = net.synapses()
+        backward = {k: syn[k] for k in ("pre", "post", "delay", "weight")}
+        del syn
     net.close()
     traffic = traffic_from_profile(args.config)
     events = b["synaptic_events"] - a["synaptic_events"]
@@ -372,8 +470,16 @@ def main():
         "cpu_baseline": None,
     }
     if not args.no_cpu_baseline:
-        print("[bench] cpu baseline ...", file=sys.stderr, flush=True)
-        line["cpu_baseline"] = cpu_baseline(pkg, args.cpu_sample_ms)
+        log(f"cpu baseline: oracle on {args.config}, {args.cpu_steps} steps from ms {t_ck} ...")
+        pkg.apply_env(cfg, env_totalSimTime_ms=t_ck + args.cpu_steps + 1, env_spikes_writeDisable=1)
+        base = cpu_baseline_port(pkg, cfg, ck, backward, t_ck, args.cpu_steps)
+        del backward
+        os.remove(ck)
+        os.rmdir(ck_dir)
+        log(f"cpu baseline: {base['sample']}")
+        log(f"cpu baseline: configs[0], {args.cpu_c0_ms} ms, one process per core and the reference ...")
+        base.update(cpu_baseline_c0(pkg, args.cpu_c0_ms))
+        line["cpu_baseline"] = base
     print(json.dumps(line), flush=True)
     return 0
 

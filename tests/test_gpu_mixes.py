@@ -1,0 +1,147 @@
+"""The batching and serving workloads at their own sizes.
+
+* The 64-network batching mix (grid side 2-12 columns of 1250 neurons, own
+  seeds, bath 2-6 Hz per input, STDP on half; 5.2 M neurons and 5.2 G
+  synapses in all) for 30
+  batched steps: every member's raster invariants and spike counter, the
+  event counter of three members recomputed from their backward lists, and
+  two members built again and stepped alone, bit for bit.
+* A slice of the serving mix (the first 16 arrivals of its job stream, grid
+  side 1-16, 30 ms each) through the server: every job's raster
+  file through the same invariants and its event counter recomputed; two
+  jobs run alone give the same files and state.
+"""
+import importlib
+import os
+import time
+
+import numpy as np
+import pytest
+
+from helpers import check_raster, delivered_events, read_raster, sha256_file
+
+pytestmark = pytest.mark.gpu
+
+T = 30
+
+
+def _w():
+    return importlib.import_module("201812realtimecortsim_amd.workloads")
+
+
+def _state_equal(a, b):
+    for k in ("v", "c", "Tr", "Te", "fired"):
+        assert np.array_equal(a[k].view(np.uint8), b[k].view(np.uint8)), k
+
+
+def test_batching_mix_64_networks(pkg, tmp_path):
+    W = _w()
+    mix = W.batch_mix(count=64)
+    small = [i for i, m in enumerate(mix) if m["side"] <= 4]
+    sample = [next(i for i in small if mix[i]["stdp"]), next(i for i in small if not mix[i]["stdp"])]
+    largest = max(range(len(mix)), key=lambda i: mix[i]["side"])
+    with_syn = set(sample) | {largest}
+    specs = []
+    for i, spec in enumerate(mix):
+        d = str(tmp_path / f"n{i:02d}")
+        env = W.write_instance(d, spec)
+        specs.append((d, dict(drive=pkg.DRIVE_PHILOX, philox_seed=7000 + i, env=env, keep_spikes=True,
+                              drop_host_synapses=i not in with_syn)))
+    t0 = time.time()
+    nets = pkg.Network.build_many(specs, host_threads=2)
+    sizes = [n.n for n in nets]
+    print(f"64 networks built in {time.time() - t0:.0f} s: {sum(sizes)} neurons, "
+          f"{sum(n.stats()['synapses'] for n in nets)} synapses", flush=True)
+    records = [[] for _ in nets]
+    try:
+        with pkg.Batch() as b:
+            for n in nets:
+                b.add(n)
+            for t in range(T):
+                b.step(1)
+                for i, n in enumerate(nets):
+                    records[i].append(n.last_spikes())
+            b.sync()
+            for n in nets:
+                b.remove(n)
+        print(f"{T} batched steps done", flush=True)
+        stats = [n.stats() for n in nets]
+        # spikes of the last dynamics: counted, reported by the next scan
+        pending = [int(n.state()["fired"].sum()) for n in nets]
+        states = {i: nets[i].state() for i in sample}
+        syns = {i: nets[i].synapses() for i in with_syn}
+    finally:
+        for n in nets:
+            n.close()
+    for i in range(len(nets)):
+        assert stats[i]["steps"] == T
+        assert check_raster(records[i], sizes[i]) + pending[i] == stats[i]["spikes"], i
+    assert sum(s["spikes"] for s in stats) > 0
+    for i in sorted(with_syn):
+        ev = delivered_events(syns[i], sizes[i], [r[0] for r in records[i]])
+        assert ev == stats[i]["synaptic_events"], i
+        print(f"member {i} (side {mix[i]['side']}): {ev} events recomputed", flush=True)
+    del syns
+    for i in sample:
+        d, kw = specs[i]
+        with pkg.Network(d, **kw) as alone:
+            alone.step(T)
+            alone.sync()
+            st = alone.stats()
+            for k in ("spikes", "synaptic_events", "external_events", "steps"):
+                assert st[k] == stats[i][k], (i, k)
+            _state_equal(alone.state(), states[i])
+        print(f"member {i} (side {mix[i]['side']}, stdp {mix[i]['stdp']}) alone: identical", flush=True)
+
+
+def test_serving_mix_slice(pkg, tmp_path):
+    W = _w()
+    jobs = [dict(j, duration_ms=T) for j in W.serve_jobs()[:16]]
+    dirs, envs, outs = [], [], []
+    for i, spec in enumerate(jobs):
+        d = str(tmp_path / f"j{i:02d}")
+        envs.append(W.write_instance(d, spec, raster=True))
+        dirs.append(d)
+        outs.append(tmp_path / f"out{i:02d}")
+        outs[-1].mkdir()
+    print(f"{len(jobs)} jobs, sides {[j['side'] for j in jobs]}", flush=True)
+    results = []
+    with pkg.Server(build_threads=8, steps_per_round=10, keep_finished=True) as srv:
+        t0 = time.perf_counter()
+        ids = []
+        for i, j in enumerate(jobs):
+            while time.perf_counter() - t0 < j["arrival_s"]:
+                time.sleep(0.002)
+            ids.append(srv.submit(dirs[i], drive=pkg.DRIVE_PHILOX, philox_seed=9000 + i, output_dir=str(outs[i]),
+                                  env=envs[i]))
+        assert srv.wait_all(900.0)
+        infos = [srv.job(j) for j in ids]
+        for inf in infos:
+            assert inf["state"] == "done", inf["message"]
+        print(f"served in {time.perf_counter() - t0:.1f} s", flush=True)
+        for i, jid in enumerate(ids):
+            net = srv.take(jid)
+            try:
+                st = net.stats()
+                steps = read_raster(outs[i] / f"spikes-sec0-C{jobs[i]['side'] ** 2}-H1-h000.dat", T)
+                assert check_raster(steps, st["neurons"], te_tol=1e-4) == st["spikes"], i
+                assert delivered_events(net.synapses(), st["neurons"], [s[0] for s in steps]) == \
+                    st["synaptic_events"], i
+                results.append({"stats": st, "state": net.state() if i < 2 else None,
+                                "files": {f: sha256_file(outs[i] / f) for f in sorted(os.listdir(outs[i]))}})
+            finally:
+                net.close()
+    assert all(inf["stats"]["steps"] == T for inf in infos)
+    for i in (0, 1):
+        out = tmp_path / f"alone{i}"
+        out.mkdir()
+        with pkg.Network(dirs[i], drive=pkg.DRIVE_PHILOX, philox_seed=9000 + i, output_dir=str(out),
+                         env=envs[i]) as net:
+            while not net.step(100):
+                pass
+            st = net.stats()
+            _state_equal(net.state(), results[i]["state"])
+        for k in ("spikes", "synaptic_events", "external_events", "steps"):
+            assert st[k] == results[i]["stats"][k], (i, k)
+        assert {f: sha256_file(out / f) for f in sorted(os.listdir(out))} == results[i]["files"]
+        print(f"job {i} (side {jobs[i]['side']}) alone: identical", flush=True)
