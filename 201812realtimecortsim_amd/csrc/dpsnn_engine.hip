@@ -1327,8 +1327,10 @@ __device__ inline uint32_t next_unit(uint32_t *ctr, uint32_t total, uint32_t &x,
   return sh[0];
 }
 
+// LDS of k_collect: the window (entries + 1 u64), the per-target record
+// counts of a unit plus one slot for padding chunks, two queue words.
 __host__ __device__ inline size_t collect_lds(uint32_t entries, uint32_t unit) {
-  return ((size_t)entries + 1) * sizeof(uint64_t) + (size_t)unit * sizeof(uint32_t) + 2 * sizeof(uint32_t);
+  return ((size_t)entries + 1) * sizeof(uint64_t) + ((size_t)unit + 1) * sizeof(uint32_t) + 2 * sizeof(uint32_t);
 }
 
 // Window entries per thread of k_collect (W <= kWinPerThread * 1024): each
@@ -1342,8 +1344,9 @@ __global__ __launch_bounds__(kCollectThreads) void k_collect(DevNet P, uint32_t 
   if (P.tDev) t = *P.tDev;
   const uint32_t W = P.winSlots * P.npc;  // entry W: the padding items' (never live)
   uint64_t *win = lds;
-  uint32_t *cnt = (uint32_t *)(win + W + 1);  // records so far, per target of the unit
-  uint32_t *sh = cnt + P.winUnit;
+  uint32_t *cnt = (uint32_t *)(win + W + 1);  // records so far, per target of the unit (+ padding's slot)
+  uint32_t *sh = cnt + P.winUnit + 1;
+  const uint32_t pad = P.winUnit;  // the target index of padding chunks past a stream's end
   const uint32_t wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const double tF = (double)t;
   const uint64_t tBits = dpsnn_asu64(tF);
@@ -1358,6 +1361,10 @@ __global__ __launch_bounds__(kCollectThreads) void k_collect(DevNet P, uint32_t 
     const uint32_t j0 = col * P.npc + part * P.winUnit;
     const uint32_t m = min(P.winUnit, (col + 1) * P.npc - j0);
     for (uint32_t i = threadIdx.x; i < m; i += blockDim.x) cnt[i] = 0;
+    // the unit's records: tile (j >> 6) of the blocked scratch relative to
+    // the unit's first tile, so that a record's offset fits 32 bits
+    uint64_t *unitKeys = P.keys + (size_t)(j0 >> 6) * P.cap * kWave;
+    const uint32_t j0lo = j0 & 63, tileElems = P.cap * kWave;
     // this thread's window entries e = tid + r * blockDim: the history
     // offset of their source (or ~0u: no source column in that slot)
     uint32_t off[kWinPerThread];
@@ -1392,7 +1399,7 @@ __global__ __launch_bounds__(kCollectThreads) void k_collect(DevNet P, uint32_t 
       // (cnt, read where the target's segment starts) or, for the target
       // carried over from the previous batch, the carried count, plus the
       // live items before it in its run of lanes.
-      uint32_t lastQ = 0xFFFFu, carry = 0;
+      uint32_t lastQ = ~0u, carry = 0;
       uint32_t it[kCollectBatches], q[kCollectBatches], itN[kCollectBatches], qN[kCollectBatches];
       auto load = [&](uint64_t c0, uint32_t (&I)[kCollectBatches], uint32_t (&Q)[kCollectBatches]) {
 #pragma unroll
@@ -1400,7 +1407,7 @@ __global__ __launch_bounds__(kCollectThreads) void k_collect(DevNet P, uint32_t 
           const uint64_t ch = c0 + (uint64_t)v * kWindowChunk + (lane >> 3);
           const bool ok = ch < ce;
           I[v] = ok ? __builtin_nontemporal_load(P.winItem + ch * kWindowChunk + (lane & 7)) : W << 16;
-          Q[v] = ok ? (uint32_t)P.winDesc[ch] : 0xFFFFu;
+          Q[v] = ok ? (uint32_t)P.winDesc[ch] : pad;
         }
       };
       if (cb < ce) load(cb, it, q);
@@ -1412,7 +1419,7 @@ __global__ __launch_bounds__(kCollectThreads) void k_collect(DevNet P, uint32_t 
 #pragma unroll
         for (int v = 0; v < kCollectBatches; ++v) {
           r[v] = win[it[v] >> 16];
-          base[v] = q[v] != 0xFFFFu ? cnt[q[v]] : 0u;
+          base[v] = cnt[q[v]];
         }
 #pragma unroll
         for (int v = 0; v < kCollectBatches; ++v) {
@@ -1427,12 +1434,13 @@ __global__ __launch_bounds__(kCollectThreads) void k_collect(DevNet P, uint32_t 
           const uint32_t b0 = upToStart ? base[v] : carry;  // no start at or below: lastQ's run
           const uint32_t p = b0 + (uint32_t)__popcll(liveM & below & ~((1ull << s) - 1ull));
           if (live) {
-            const uint32_t j = j0 + qv;
-            P.keys[(size_t)(j >> 6) * P.cap * kWave + (j & 63) * kRun + run_offset(p)] = r[v] | (it[v] & 0xFFFFu);
+            const uint32_t jj = j0lo + qv;
+            const uint32_t at = (jj >> 6) * tileElems + (jj & 63) * kRun + (p / kRun) * (kWave * kRun) + p % kRun;
+            unitKeys[at] = r[v] | (it[v] & 0xFFFFu);
           }
           const uint32_t after = p + (live ? 1u : 0u);
           const bool lastOfRun = lane == 63 || ((startM >> (lane + 1)) & 1ull);
-          if (qv != 0xFFFFu && lastOfRun) cnt[qv] = after;
+          if (lastOfRun) cnt[qv] = after;
           carry = (uint32_t)__builtin_amdgcn_readlane((int)after, 63);
           lastQ = (uint32_t)__builtin_amdgcn_readlane((int)qv, 63);
         }
@@ -1528,23 +1536,22 @@ __device__ inline uint64_t *record_runs(const DevNet &P, uint32_t j) {
   return P.keys + (size_t)(j >> 6) * P.cap * kWave + (j & 63) * kRun;
 }
 
-// One wave per target from the XCD-ordered queue.
+// One wave per block, targets j = blockIdx.x + k * gridDim.x: each target's
+// records are its own lines, so there is no locality for a work queue to
+// keep (its counter round trips measured slower than this static order).
 template <int B>
 __global__ __launch_bounds__(kWave) void k_sort(DevNet P, uint32_t t) {
   extern __shared__ uint64_t lds[];
   if (P.tDev) t = *P.tDev;
   if (blockIdx.x == 0 && threadIdx.x < kXcds) P.winCtr[threadIdx.x * kCtrStride] = 0;  // for the next k_collect
-  const uint32_t home = blockIdx.x % kXcds;
-  uint32_t x = home;
-  for (uint32_t j0, j1; (j0 = next_item(P.workCtr, P.N, x, home, &j1)) < P.N;)
-    for (uint32_t j = j0; j < j1; ++j) {
-      const uint32_t n = P.keyCount[j];
-      if (n > P.sortCap) {
-        if ((threadIdx.x & 63) == 0) P.bigList[atomicAdd(P.bigCount, 1u)] = j;
-        continue;
-      }
-      sort_records<B, kSortChunks>(t, n, (uint8_t *)lds, kSortCap, record_runs(P, j));
+  for (uint32_t j = blockIdx.x; j < P.N; j += gridDim.x) {
+    const uint32_t n = P.keyCount[j];
+    if (n > P.sortCap) {
+      if ((threadIdx.x & 63) == 0) P.bigList[atomicAdd(P.bigCount, 1u)] = j;
+      continue;
     }
+    sort_records<B, kSortChunks>(t, n, (uint8_t *)lds, kSortCap, record_runs(P, j));
+  }
 }
 
 // The targets k_sort left (more than kSortCap records), one wave each;

@@ -165,15 +165,16 @@ def bench_batch(pkg, args):
     W = importlib.import_module("201812realtimecortsim_amd.workloads")
     mix = W.batch_mix(count=args.batch_count)
     tmp = tempfile.mkdtemp(prefix="dpsnn_batch_")
-    nets = []
-    t0 = time.perf_counter()
+    specs = []
     for i, spec in enumerate(mix):
         d = os.path.join(tmp, f"n{i:02d}")
         env = W.write_instance(d, spec)
-        nets.append(pkg.Network(d, drive=pkg.DRIVE_PHILOX, philox_seed=7000 + i, env=env, drop_host_synapses=True))
-        if i % 8 == 7 or i == len(mix) - 1:
-            log(f"built {i + 1}/{len(mix)} networks, {time.perf_counter() - t0:.0f} s")
+        specs.append((d, dict(drive=pkg.DRIVE_PHILOX, philox_seed=7000 + i, env=env, drop_host_synapses=True)))
+    log(f"building {len(mix)} networks (host builds on parallel threads) ...")
+    t0 = time.perf_counter()
+    nets = pkg.Network.build_many(specs, host_threads=2)
     init_s = time.perf_counter() - t0
+    log(f"built {len(nets)} networks in {init_s:.0f} s")
     neurons = sum(n.n for n in nets)
     synapses = sum(n.stats()["synapses"] for n in nets)
     dev_bytes = sum(n.stats()["device_bytes"] for n in nets)
@@ -205,6 +206,7 @@ def bench_batch(pkg, args):
         b.remove(n)
     b.close()
     pe, pw = 0, 0.0
+    per_net = []
     for n in nets:
         a = n.stats()["synaptic_events"]
         n.step(1)  # stand-alone scratch
@@ -213,10 +215,41 @@ def bench_batch(pkg, args):
         t0 = time.perf_counter()
         n.step(args.steps)
         n.sync()
-        pw += time.perf_counter() - t0
-        pe += n.stats()["synaptic_events"] - a
+        dt = time.perf_counter() - t0
+        de = n.stats()["synaptic_events"] - a
+        per_net.append((de, dt))
+        pw += dt
+        pe += de
     plain = pe / pw
     log(f"plain (one network at a time): {plain:.3e} events/s")
+    # the same comparison per size class: a batch of only that class's members
+    classes = {}
+    for lo, hi in ((2, 4), (5, 8), (9, 12)):
+        idx = [i for i, m in enumerate(mix) if lo <= m["side"] <= hi]
+        if not idx:
+            continue
+        with pkg.Batch() as bc:
+            for i in idx:
+                bc.add(nets[i])
+            bc.step(2)
+            bc.sync()
+            ev0 = sum(nets[i].stats()["synaptic_events"] for i in idx)
+            t0 = time.perf_counter()
+            bc.step(args.steps)
+            bc.sync()
+            wall_c = time.perf_counter() - t0
+            ev1 = sum(nets[i].stats()["synaptic_events"] for i in idx)
+            for i in idx:
+                bc.remove(nets[i])
+        b_rate = (ev1 - ev0) / wall_c
+        p_rate = sum(per_net[i][0] for i in idx) / sum(per_net[i][1] for i in idx)
+        classes[f"side {lo}-{hi}"] = {"networks": len(idx), "neurons": sum(nets[i].n for i in idx),
+                                      "batched_events_per_s": b_rate, "plain_events_per_s": p_rate,
+                                      "batched_vs_plain": b_rate / p_rate,
+                                      "batched_ms_per_step": 1000.0 * wall_c / args.steps,
+                                      "plain_ms_per_step": 1000.0 * sum(per_net[i][1] for i in idx) / args.steps}
+        log(f"side {lo}-{hi}: {len(idx)} networks, batched {b_rate:.3e} vs plain {p_rate:.3e} events/s "
+            f"({b_rate / p_rate:.2f}x)")
     for n in nets:
         n.close()
     per_launch = (25.0 * (e1 - e0) + 72.0 * neurons * args.steps) / args.steps
@@ -237,6 +270,7 @@ def bench_batch(pkg, args):
                    "mean_rate_hz": (s1 - s0) / args.steps * 1000.0 / neurons},
         "batched_vs_plain": batched / plain,
         "plain_events_per_s": plain,
+        "by_size_class": classes,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": None,
                      "kernel": "batched step (graph of k_batch_gather + k_batch_integrate per chunk)",
@@ -369,7 +403,7 @@ def main():
     ap.add_argument("--serve-rate", type=float, default=20.0)
     ap.add_argument("--serve-window", type=float, default=120.0)
     ap.add_argument("--serve-budget-gb", type=float, default=200.0)
-    ap.add_argument("--build-threads", type=int, default=4)
+    ap.add_argument("--build-threads", type=int, default=8)
     ap.add_argument("--no-tune", action="store_true")
     ap.add_argument("--drive", default="philox", choices=["philox", "replay"])
     ap.add_argument("--cpu-steps", type=int, default=2,
