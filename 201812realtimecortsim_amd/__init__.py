@@ -26,6 +26,8 @@ CONFIGS = os.path.join(HERE, "configs")
 
 DRIVE_REPLAY = 0
 DRIVE_PHILOX = 1
+GENERATOR_HOST = 0
+GENERATOR_DEVICE = 1
 DONE = 1
 # dpsnn_status codes
 ERR_CONFIG, ERR_DEVICE, ERR_MEMORY, ERR_ARG, ERR_OVERFLOW, ERR_IO = -1, -2, -3, -4, -5, -6
@@ -46,7 +48,8 @@ class Options(ctypes.Structure):
                 ("philox_seed", ctypes.c_uint64), ("output_dir", ctypes.c_char_p),
                 ("drain_every_ms", ctypes.c_uint32), ("keep_spikes", ctypes.c_int),
                 ("env", ctypes.c_char_p), ("drop_host_synapses", ctypes.c_int),
-                ("host_threads", ctypes.c_uint32)]
+                ("host_threads", ctypes.c_uint32), ("generator", ctypes.c_uint32),
+                ("generator_seed", ctypes.c_uint64)]
 
 
 _PARAM_U32 = ["globH", "locN", "globN", "D", "totalSimTime_ms", "globCFX", "globCFY", "globCFT",
@@ -256,13 +259,13 @@ def get_parameters(inidir):
 
 
 def _options(device=0, drive=DRIVE_REPLAY, philox_seed=1, output_dir=None, drain_every_ms=0, keep_spikes=False,
-             env=None, drop_host_synapses=False):
+             env=None, drop_host_synapses=False, generator=GENERATOR_HOST, generator_seed=1):
     """dpsnn_options of the Network keyword arguments (the strings stay
     referenced by the structure)."""
     out = output_dir.encode() if output_dir else None
     env_text = ";".join(f"{k}={v}" for k, v in env.items()).encode() if env else None
     return Options(device, drive, philox_seed, out, drain_every_ms, int(keep_spikes), env_text,
-                   int(drop_host_synapses))
+                   int(drop_host_synapses), 0, generator, generator_seed)
 
 
 def write_ini_files(inidir, outdir, env=None):
@@ -276,14 +279,17 @@ class Network:
     """One local network on one GPU (a dpsnn_net)."""
 
     def __init__(self, inidir, device=0, drive=DRIVE_REPLAY, philox_seed=1, output_dir=None,
-                 drain_every_ms=0, keep_spikes=False, env=None, drop_host_synapses=False, checkpoint=None):
+                 drain_every_ms=0, keep_spikes=False, env=None, drop_host_synapses=False, checkpoint=None,
+                 generator=GENERATOR_HOST, generator_seed=1):
         """env: {"env_name": value} settings that take precedence over the
         process environment for this network only.  checkpoint: a file of
         dpsnn_checkpoint to resume from (dpsnn_restore), taken on the
-        network of the same ini files and settings."""
+        network of the same ini files and settings.  generator:
+        GENERATOR_HOST (the reference's, bit-exact) or GENERATOR_DEVICE
+        (statistically equivalent, drawn on the device with generator_seed)."""
         self._lib = library()
         opt = _options(device, drive, philox_seed, output_dir, drain_every_ms, keep_spikes, env,
-                       drop_host_synapses)
+                       drop_host_synapses, generator, generator_seed)
         self._out = opt.output_dir
         h = ctypes.c_void_p()
         if checkpoint:
@@ -597,10 +603,11 @@ class Server:
         self._h = h
 
     def submit(self, inidir, drive=DRIVE_PHILOX, philox_seed=1, output_dir=None, env=None,
-               drop_host_synapses=False, keep_spikes=False):
+               drop_host_synapses=False, keep_spikes=False, generator=GENERATOR_HOST, generator_seed=1):
         env_text = ";".join(f"{k}={v}" for k, v in env.items()).encode() if env else None
         out = output_dir.encode() if output_dir else None
-        opt = Options(0, drive, philox_seed, out, 0, int(keep_spikes), env_text, int(drop_host_synapses), 0)
+        opt = Options(0, drive, philox_seed, out, 0, int(keep_spikes), env_text, int(drop_host_synapses), 0,
+                      generator, generator_seed)
         jid = ctypes.c_uint64()
         _check("dpsnn_server_submit", self._lib.dpsnn_server_submit(self._h, inidir.encode(), ctypes.byref(opt),
                                                                     ctypes.byref(jid)))

@@ -2563,6 +2563,232 @@ static int run_steps(dpsnn_net *n, const DevNet &P, uint32_t nsteps, bool graphs
 // ============================================================================
 // C ABI
 // ============================================================================
+// --------------------------------------------------------------------------
+// Device connectome generator (dpsnn_options.generator = device): the
+// distributions of the reference's generator (generate_column;
+// DPSNN_LIFCAcorticalModuleGen.cpp:62-220, DPSNN_LIFCAconnectome.cpp:
+// 1079-1133) drawn from Philox4x32-10, one thread per source neuron, instead
+// of the sequential ran3 stream: for each target sub-population and stencil
+// cell with connection probability q, targets at geometric gaps over the
+// cell's column (smallest n >= 1 with 1 - (1 - q)^n >= r, at most span + 1,
+// as the reference's linear search), then per target a uniform delay
+// (uniform_delay's level arithmetic) and a truncated Gaussian weight (the
+// reference's rejection bounds), int16-quantised.  Pass 0 counts each
+// source's synapses, pass 1 writes them at their offsets in generation
+// order; the host then sorts them into the backward list as for the
+// reference's.  Statistically equivalent, not bit-exact.
+// --------------------------------------------------------------------------
+struct GenView {
+  uint32_t N, npc, CFX, CFY, nsub, sx, sy, mdx, mdy, overall, D;
+  float f2w, minW, delayLo, delayHi;
+  uint64_t seed;
+  const float *prob;                 // [nsub][nsub][sx][sy]
+  const float *J, *DJ, *DMin, *DMax;  // [nsub][nsub], source-major
+  const uint32_t *subOff, *subCnt;   // [nsub], within a column
+  uint32_t *count;
+  const uint64_t *off;
+  uint32_t *post;
+  uint8_t *delay;
+  int16_t *w;
+  uint32_t *error;                   // bit 0: q in (0, 1e-5]; 1: 7500 or more
+                                     // synapses; 2: weight bound; 3: delay >= env_D;
+                                     // 4: no weight within the draw bounds
+};
+
+__device__ inline uint32_t gen_round_half_up(double r) {
+  const double f = floor(r);
+  return (f + 0.5 > r) ? (uint32_t)f : (uint32_t)(f + 1.0);
+}
+
+template <int PASS>
+__global__ __launch_bounds__(256) void k_generate(GenView G) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= G.N) return;
+  const uint32_t col = i / G.npc, inCM = i - col * G.npc, cfx = col % G.CFX, cfy = col / G.CFX;
+  uint32_t src = 0;
+  for (uint32_t s = 0; s < G.nsub; ++s)
+    if (inCM >= G.subOff[s] && inCM < G.subOff[s] + G.subCnt[s]) src = s;
+  PhiloxStream A{(uint32_t)G.seed, (uint32_t)(G.seed >> 32) ^ i, 0ull, 0.0, 0};
+  PhiloxStream Bw{(uint32_t)G.seed ^ 0x5bd1e995u, (uint32_t)(G.seed >> 32) ^ i, 0ull, 0.0, 0};
+  const uint64_t at0 = PASS ? G.off[i] : 0ull;
+  uint32_t n = 0, err = 0;
+  for (uint32_t ts = 0; ts < G.nsub; ++ts)
+    for (uint32_t x = 0; x < G.sx; ++x)
+      for (uint32_t y = 0; y < G.sy; ++y) {
+        int32_t tx, ty;
+        if (G.overall == 0) {
+          tx = (int32_t)cfx + (int32_t)x - (int32_t)G.mdx;
+          ty = (int32_t)cfy + (int32_t)y - (int32_t)G.mdy;
+        } else {
+          tx = (int32_t)x;
+          ty = (int32_t)y;
+        }
+        if (tx < 0 || tx >= (int32_t)G.CFX || ty < 0 || ty >= (int32_t)G.CFY) continue;
+        const float q = G.prob[((src * G.nsub + ts) * G.sx + x) * G.sy + y];
+        if (!(q > 0.0f)) continue;
+        if (!(q > 0.00001f)) {
+          err |= 1u;
+          continue;
+        }
+        const uint32_t first = G.subOff[ts], end = first + G.subCnt[ts];
+        const uint32_t colBase = (uint32_t)tx * G.npc + (uint32_t)ty * G.npc * G.CFX;
+        uint32_t cur = first - 1;  // wraps as the reference's uint32 arithmetic
+        const uint32_t span = end - cur;
+        const double lq = log1p(-(double)q);
+        const float mean = G.J[src * G.nsub + ts], sigma = G.DJ[src * G.nsub + ts] * mean;
+        for (;;) {
+          const double r = A.next();
+          double g = q >= 1.0f ? 1.0 : ceil(log1p(-r) / lq);
+          g = fmin(fmax(g, 1.0), (double)span + 1.0);
+          cur += (uint32_t)g;
+          if (cur >= end) break;
+          if (n + 1 >= kMaxOutDegree) {
+            err |= 2u;
+            break;
+          }
+          if (PASS) {
+            // delay: uniform_delay's levels
+            float DMin = G.DMin[src * G.nsub + ts], DMax = G.DMax[src * G.nsub + ts];
+            float lo = G.delayLo, hi = G.delayHi;
+            const float step = (hi - lo) / (float)kDelaySlots;
+            hi -= step / 2;
+            lo += step / 2;
+            DMax = gen_round_half_up((DMax - lo) / step) * step + lo;
+            DMin = gen_round_half_up((DMin - lo) / step) * step + lo;
+            const float level = (float)((Bw.next() * (DMax - DMin) + DMin - lo) / step);
+            const uint32_t d = gen_round_half_up(level * step);
+            // weight: Gaussian (Box-Muller) within the reference's bounds
+            // (at most 64 draws: a bound the draws cannot meet, such as a
+            // zero mean, is reported instead of looping)
+            float wv = mean;
+            bool inBounds = false;
+            for (int tries = 0; tries < 64 && !inBounds; ++tries) {
+              const double u1 = 1.0 - Bw.next(), u2 = Bw.next();
+              const double z = sqrt(-2.0 * log(u1)) * cos(6.283185307179586 * u2);
+              wv = (float)(mean + z * sigma);
+              inBounds = mean > 0.0f ? !(wv < 0.0f || wv >= 2.0f * mean) : !(wv <= 2.0f * mean || wv > 0.0f);
+            }
+            if (!inBounds) err |= 16u;
+            if (wv < G.minW || wv > -G.minW) err |= 4u;
+            if (d >= kDelaySlots || d >= G.D) err |= 8u;
+            G.post[at0 + n] = colBase + cur;
+            G.delay[at0 + n] = (uint8_t)min(d, kDelaySlots - 1);
+            G.w[at0 + n] = (int16_t)(wv * G.f2w);
+          }
+          ++n;
+        }
+      }
+  if (!PASS) G.count[i] = n;
+  if (err) atomicOr(G.error, err);
+}
+
+// The connectome of n drawn on its device (k_generate), then sorted into the
+// backward list on the host.
+static int generate_on_device(dpsnn_net *n, std::string &err) {
+  const Params &p = n->par;
+  const dpsnn_parameters &q = p.pub;
+  Connectome &c = n->con;
+  connectome_neurons(p, c);
+  const uint32_t N = c.N, npc = q.neuronsPerCM, nsub = q.subPopNumber;
+  for (uint32_t i = 0; i < N; ++i) {
+    const uint32_t inCM = i % npc;
+    uint32_t src = 0;
+    for (uint32_t s = 0; s < nsub; ++s)
+      if (inCM >= p.sp[s].offset && inCM < p.sp[s].offset + p.sp[s].count) src = s;
+    c.subPop[i] = (uint8_t)src;
+    neuron_bath(p, i, src, c);
+  }
+  std::vector<float> prob(size_t(nsub) * nsub * p.stencilX * p.stencilY), J(nsub * nsub), DJ(nsub * nsub),
+      DMin(nsub * nsub), DMax(nsub * nsub);
+  std::vector<uint32_t> off(nsub), cnt(nsub);
+  for (uint32_t s = 0; s < nsub; ++s) {
+    off[s] = p.sp[s].offset;
+    cnt[s] = p.sp[s].count;
+    for (uint32_t t = 0; t < nsub; ++t) {
+      J[s * nsub + t] = (float)p.sp[s].J[t];
+      DJ[s * nsub + t] = (float)p.sp[s].DJ[t];
+      DMin[s * nsub + t] = (float)p.sp[s].DMin[t];
+      DMax[s * nsub + t] = (float)p.sp[s].DMax[t];
+      for (uint32_t x = 0; x < p.stencilX; ++x)
+        for (uint32_t y = 0; y < p.stencilY; ++y)
+          prob[((s * nsub + t) * p.stencilX + x) * p.stencilY + y] = p.prob_at(s, t, x, y);
+    }
+  }
+  HIP_TRY(hipSetDevice(n->opt.device));
+  hipStream_t st;
+  HIP_TRY(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+  std::vector<void *> bufs;
+  auto fin = [&](int rc) {
+    (void)hipStreamSynchronize(st);
+    for (void *b : bufs) (void)hipFree(b);
+    (void)hipStreamDestroy(st);
+    return rc;
+  };
+  auto alloc = [&](void **ptr, size_t bytes) -> bool {
+    if (hipMalloc(ptr, bytes ? bytes : 1) != hipSuccess) return false;
+    bufs.push_back(*ptr);
+    return true;
+  };
+  auto put = [&](void **ptr, const void *src, size_t bytes) -> bool {
+    return alloc(ptr, bytes) && hipMemcpyAsync(*ptr, src, bytes, hipMemcpyHostToDevice, st) == hipSuccess;
+  };
+  GenView G{};
+  G.N = N; G.npc = npc; G.CFX = q.globCFX; G.CFY = q.globCFY; G.nsub = nsub;
+  G.sx = p.stencilX; G.sy = p.stencilY; G.mdx = p.maxModDeltaX; G.mdy = p.maxModDeltaY;
+  G.overall = q.overallConnectivity; G.D = q.D;
+  G.f2w = q.factorFloat_2_weightType; G.minW = q.minSynWeight_f;
+  G.delayLo = (float)q.delayMin; G.delayHi = (float)q.delayMax;
+  G.seed = n->opt.generator_seed;
+  void *dProb, *dJ, *dDJ, *dDMin, *dDMax, *dOff, *dCnt, *dCount, *dErr;
+  if (!put(&dProb, prob.data(), 4 * prob.size()) || !put(&dJ, J.data(), 4 * J.size()) ||
+      !put(&dDJ, DJ.data(), 4 * DJ.size()) || !put(&dDMin, DMin.data(), 4 * DMin.size()) ||
+      !put(&dDMax, DMax.data(), 4 * DMax.size()) || !put(&dOff, off.data(), 4 * nsub) ||
+      !put(&dCnt, cnt.data(), 4 * nsub) || !alloc(&dCount, 4 * (size_t)N) || !alloc(&dErr, 4))
+    return fin(fail(DPSNN_ERR_MEMORY, "device generator: out of device memory"));
+  G.prob = (const float *)dProb; G.J = (const float *)dJ; G.DJ = (const float *)dDJ;
+  G.DMin = (const float *)dDMin; G.DMax = (const float *)dDMax;
+  G.subOff = (const uint32_t *)dOff; G.subCnt = (const uint32_t *)dCnt;
+  G.count = (uint32_t *)dCount; G.error = (uint32_t *)dErr;
+  if (hipMemsetAsync(G.error, 0, 4, st) != hipSuccess) return fin(fail(DPSNN_ERR_DEVICE, "device generator: memset failed"));
+  const dim3 grid((N + 255) / 256);
+  hipLaunchKernelGGL(k_generate<0>, grid, dim3(256), 0, st, G);
+  if (hipGetLastError() != hipSuccess) return fin(fail(DPSNN_ERR_DEVICE, "device generator: launch failed"));
+  std::vector<uint32_t> counts(N);
+  uint32_t errBits = 0;
+  if (hipMemcpyAsync(counts.data(), dCount, 4 * (size_t)N, hipMemcpyDeviceToHost, st) != hipSuccess ||
+      hipMemcpyAsync(&errBits, dErr, 4, hipMemcpyDeviceToHost, st) != hipSuccess ||
+      hipStreamSynchronize(st) != hipSuccess)
+    return fin(fail(DPSNN_ERR_DEVICE, "device generator: count pass failed"));
+  std::vector<uint64_t> gen_off(N + 1, 0);
+  for (uint32_t i = 0; i < N; ++i) gen_off[i + 1] = gen_off[i] + counts[i];
+  const uint64_t S = gen_off[N];
+  void *dGenOff, *dPost, *dDelay, *dW;
+  if (!put(&dGenOff, gen_off.data(), 8 * (size_t)N) || !alloc(&dPost, 4 * S) || !alloc(&dDelay, S) ||
+      !alloc(&dW, 2 * S))
+    return fin(fail(DPSNN_ERR_MEMORY, "device generator: out of device memory for %llu synapses",
+                    (unsigned long long)S));
+  G.off = (const uint64_t *)dGenOff; G.post = (uint32_t *)dPost; G.delay = (uint8_t *)dDelay; G.w = (int16_t *)dW;
+  hipLaunchKernelGGL(k_generate<1>, grid, dim3(256), 0, st, G);
+  if (hipGetLastError() != hipSuccess) return fin(fail(DPSNN_ERR_DEVICE, "device generator: launch failed"));
+  std::vector<uint32_t> g_post(S);
+  std::vector<uint8_t> g_delay(S);
+  std::vector<int16_t> g_w(S);
+  if (hipMemcpyAsync(g_post.data(), dPost, 4 * S, hipMemcpyDeviceToHost, st) != hipSuccess ||
+      hipMemcpyAsync(g_delay.data(), dDelay, S, hipMemcpyDeviceToHost, st) != hipSuccess ||
+      hipMemcpyAsync(g_w.data(), dW, 2 * S, hipMemcpyDeviceToHost, st) != hipSuccess ||
+      hipMemcpyAsync(&errBits, dErr, 4, hipMemcpyDeviceToHost, st) != hipSuccess ||
+      hipStreamSynchronize(st) != hipSuccess)
+    return fin(fail(DPSNN_ERR_DEVICE, "device generator: fill pass failed"));
+  fin(0);
+  if (errBits & 1u) { err = "connection probabilities in (0, 1e-5] are not supported"; return DPSNN_ERR_CONFIG; }
+  if (errBits & 2u) { err = "a neuron would get 7500 or more synapses (DSD__maxM)"; return DPSNN_ERR_CONFIG; }
+  if (errBits & 4u) { err = "synaptic weight outside [env_minSynWeight_f, -env_minSynWeight_f]"; return DPSNN_ERR_CONFIG; }
+  if (errBits & 8u) { err = "a stored delay is not below env_D"; return DPSNN_ERR_CONFIG; }
+  if (errBits & 16u) { err = "synaptic weights cannot be drawn within their bounds (zero mean?)"; return DPSNN_ERR_CONFIG; }
+  inflate_backward(c, gen_off, g_post, g_delay, g_w, n->opt.host_threads);
+  return 0;
+}
+
 extern "C" {
 
 const char *dpsnn_last_error(void) { return g_err.c_str(); }
@@ -2615,8 +2841,14 @@ int dpsnn_prepare(const char *inidir, const dpsnn_options *opt, dpsnn_net **out)
   // localNet init: seed with env_globalSeed (DPSNN_localNet_init.c:119),
   // generate, then continue the stream from SetRandomSeed(loc_h=0) (:283)
   n->rng.seed((int32_t)q.globalSeed);
-  if (!generate_connectome(n->par, n->rng, n->con, err, n->opt.host_threads))
+  if (n->opt.generator == DPSNN_GENERATOR_DEVICE) {
+    const int grc = generate_on_device(n.get(), err);
+    if (grc) return grc == DPSNN_ERR_CONFIG ? fail(DPSNN_ERR_CONFIG, "%s", err.c_str()) : grc;
+  } else if (n->opt.generator != DPSNN_GENERATOR_HOST) {
+    return fail(DPSNN_ERR_ARG, "unknown generator %u", n->opt.generator);
+  } else if (!generate_connectome(n->par, n->rng, n->con, err, n->opt.host_threads)) {
     return fail(DPSNN_ERR_CONFIG, "%s", err.c_str());
+  }
   build_delivery(n->par, n->con, /*with_incoming=*/n->stdp, n->opt.host_threads);
   n->rng.seed(0);
   // DPSNN_REPLAY_STREAM=k > 0: the replayed bath continues from

@@ -464,6 +464,28 @@ bool read_params(const std::string &dir, Params &p, std::string &err, const EnvM
 // (:871-887).  The random stream is reseeded with the column index at each
 // column's first neuron; the NormDev cache and the seed-0 quirk carry over.
 // --------------------------------------------------------------------------
+// External bath of neuron i of sub-population src, in column i / npc
+// (generateSourceNeuCExt's mirrored index, DPSNN_LIFCAconnectome.cpp:871-887):
+// its mean inter-event time and per-ms event cap.
+void neuron_bath(const Params &p, uint32_t i, uint32_t src, Connectome &c) {
+  const dpsnn_parameters &q = p.pub;
+  const uint32_t col = i / q.neuronsPerCM;
+  const float nu = (float)p.sp[src].NuExt;
+  uint32_t bx = col % q.globCFX, by = col / q.globCFX;
+  if (bx >= q.globCFX / 2) bx = -1 * (bx - q.globCFX + 1);
+  if (by >= q.globCFY / 2) by = -1 * (by - q.globCFY + 1);
+  if (bx >= q.stencilDim) bx = q.stencilDim - 1;
+  if (by >= q.stencilDim) by = q.stencilDim - 1;
+  const float cext = (float)p.bath[size_t(bx) * kMaxStencil + by];
+  if (cext != 0) {
+    c.invNuExt[i] = 1000.0 / (nu * cext);
+    c.maxExt[i] = (uint32_t)std::ceil(20 / c.invNuExt[i]);
+  } else {
+    c.invNuExt[i] = 0;
+    c.maxExt[i] = 0;
+  }
+}
+
 namespace {
 
 struct Target {
@@ -608,21 +630,7 @@ void generate_column(const Params &p, uint32_t col, Ran3 &rng, bool placeholderE
       ++out.count[inCM];
     }
     if (!writeNeuronInfo) continue;
-    // external bath of this neuron (generateSourceNeuCExt's mirrored index)
-    const float nu = (float)p.sp[src].NuExt;
-    uint32_t bx = cfx, by = cfy;
-    if (bx >= q.globCFX / 2) bx = -1 * (bx - q.globCFX + 1);
-    if (by >= q.globCFY / 2) by = -1 * (by - q.globCFY + 1);
-    if (bx >= q.stencilDim) bx = q.stencilDim - 1;
-    if (by >= q.stencilDim) by = q.stencilDim - 1;
-    const float cext = (float)p.bath[size_t(bx) * kMaxStencil + by];
-    if (cext != 0) {
-      c.invNuExt[i] = 1000.0 / (nu * cext);
-      c.maxExt[i] = (uint32_t)std::ceil(20 / c.invNuExt[i]);
-    } else {
-      c.invNuExt[i] = 0;
-      c.maxExt[i] = 0;
-    }
+    neuron_bath(p, i, src, c);
   }
   out.exitSet = rng.norm_set();
   out.exitCached = rng.norm_cached();
@@ -631,9 +639,9 @@ void generate_column(const Params &p, uint32_t col, Ran3 &rng, bool placeholderE
 
 }  // namespace
 
-bool generate_connectome(const Params &p, Ran3 &rng, Connectome &c, std::string &err, unsigned threads) {
+void connectome_neurons(const Params &p, Connectome &c) {
   const dpsnn_parameters &q = p.pub;
-  const uint32_t N = q.globN, npc = q.neuronsPerCM, cols = q.globCFT;
+  const uint32_t N = q.globN;
   c.N = N;
   c.subPop.assign(N, 0);
   c.invNuExt.assign(N, 0.0);
@@ -643,6 +651,12 @@ bool generate_connectome(const Params &p, Ran3 &rng, Connectome &c, std::string 
     const float J = (float)p.sp[s].JExt, DJ = (float)p.sp[s].DJExt;
     gaussian_lut(&c.bathLut[size_t(s) * kLutDepth], kLutDepth, J, J * DJ, 0.0, 2.0 * J);
   }
+}
+
+bool generate_connectome(const Params &p, Ran3 &rng, Connectome &c, std::string &err, unsigned threads) {
+  const dpsnn_parameters &q = p.pub;
+  const uint32_t N = q.globN, npc = q.neuronsPerCM, cols = q.globCFT;
+  connectome_neurons(p, c);
   if (q.tabNeuron != npc) {
     err = "reseeding period (tabNeuron) differs from the column size";
     return false;
@@ -735,9 +749,17 @@ bool generate_connectome(const Params &p, Ran3 &rng, Connectome &c, std::string 
       }
     }
   }
+  inflate_backward(c, gen_off, g_post, g_delay, g_w, threads);
+  return true;
+}
+
+void inflate_backward(Connectome &c, const std::vector<uint64_t> &gen_off, const std::vector<uint32_t> &g_post,
+                      const std::vector<uint8_t> &g_delay, const std::vector<int16_t> &g_w, unsigned threads) {
   // inflate: stable counting sort by delay into the backward list,
   // DPSNN_localNet_init.c:651-680 (neuron chunks in parallel, each writing
   // its own sub-range of every delay bucket, so the order stays stable)
+  const uint32_t N = c.N;
+  if (threads == 0) threads = default_threads();
   const uint64_t S = g_post.size();
   c.S = S;
   c.pre.resize(S); c.post.resize(S); c.delay.resize(S); c.kind.resize(S); c.weight.resize(S);
@@ -781,7 +803,6 @@ bool generate_connectome(const Params &p, Ran3 &rng, Connectome &c, std::string 
     for (auto &t : pool) t.join();
   }
   c.max_delay = maxd;
-  return true;
 }
 
 // --------------------------------------------------------------------------

@@ -115,6 +115,15 @@ struct Connectome {
 unsigned default_threads();
 
 // threads = 0: default_threads().
+// The per-neuron part of a connectome (sub-population, bath rate and cap,
+// the bath LUT) before any synapse; generate_connectome starts with it.
+void connectome_neurons(const Params &p, Connectome &c);
+// Bath rate and per-ms cap of neuron i (sub-population src), in c.
+void neuron_bath(const Params &p, uint32_t i, uint32_t src, Connectome &c);
+// The backward list from synapses in generation order (source-major; source
+// i's are [gen_off[i], gen_off[i + 1])): a stable counting sort by delay.
+void inflate_backward(Connectome &c, const std::vector<uint64_t> &gen_off, const std::vector<uint32_t> &g_post,
+                      const std::vector<uint8_t> &g_delay, const std::vector<int16_t> &g_w, unsigned threads = 0);
 bool generate_connectome(const Params &p, Ran3 &rng, Connectome &c, std::string &err,
                          unsigned threads = 0);
 // with_incoming: also build the per-target incoming synapse lists (STDP).

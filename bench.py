@@ -282,6 +282,7 @@ def bench_batch(pkg, args):
 def run_server(pkg, jobs, dirs, envs, args, max_running):
     """Replays the job arrivals against a server for the window; returns
     what it finished and delivered by the window's end."""
+    gen = pkg.GENERATOR_DEVICE if args.generator == "device" else pkg.GENERATOR_HOST
     srv = pkg.Server(hbm_budget_bytes=int(args.serve_budget_gb * 1e9), build_threads=args.build_threads,
                      steps_per_round=10, max_running=max_running)
     t0 = time.perf_counter()
@@ -290,7 +291,7 @@ def run_server(pkg, jobs, dirs, envs, args, max_running):
         now = time.perf_counter() - t0
         while k < len(jobs) and jobs[k]["arrival_s"] <= now:
             ids.append(srv.submit(dirs[k], drive=pkg.DRIVE_PHILOX, philox_seed=9000 + k, env=envs[k],
-                                  drop_host_synapses=True))
+                                  drop_host_synapses=True, generator=gen, generator_seed=9000 + k))
             k += 1
         if now >= args.serve_window:
             break
@@ -350,7 +351,8 @@ def bench_serve(pkg, args):
         "config": {"workload": f"configs[4] serving: Poisson arrivals {args.serve_rate:g} jobs/s for "
                                f"{args.serve_window:g} s, grid side 1-16 x 1250 neurons, 0.5-5 s simulated, "
                                f"STDP p=0.3, {args.serve_budget_gb:g} GB HBM admission budget, continuous batching",
-                   "jobs_arrived": len(jobs), "offered_events_per_s": offered, "drive": "philox"},
+                   "jobs_arrived": len(jobs), "offered_events_per_s": offered, "drive": "philox",
+                   "generator": args.generator, "build_threads": args.build_threads},
         "served": served,
         "plain": plain,
         "served_vs_plain": served["events_per_s"] / plain["events_per_s"] if plain["events_per_s"] else None,
@@ -406,6 +408,8 @@ def main():
     ap.add_argument("--build-threads", type=int, default=8)
     ap.add_argument("--no-tune", action="store_true")
     ap.add_argument("--drive", default="philox", choices=["philox", "replay"])
+    ap.add_argument("--generator", default="host", choices=["host", "device"],
+                    help="connectome generator: the reference's (bit-exact) or the device one (statistical)")
     ap.add_argument("--cpu-steps", type=int, default=2,
                     help="steps of the oracle on the benchmarked network (from the engine's checkpoint)")
     ap.add_argument("--cpu-c0-ms", type=int, default=40,
@@ -433,7 +437,8 @@ def main():
     drive = pkg.DRIVE_PHILOX if args.drive == "philox" else pkg.DRIVE_REPLAY
     print(f"[bench] building {args.config} ...", file=sys.stderr, flush=True)
     t0 = time.perf_counter()
-    net = pkg.Network(cfg, drive=drive, philox_seed=20181201)
+    net = pkg.Network(cfg, drive=drive, philox_seed=20181201,
+                      generator=pkg.GENERATOR_DEVICE if args.generator == "device" else pkg.GENERATOR_HOST)
     init_s = time.perf_counter() - t0
     st0 = net.stats()
     print(f"[bench] init {init_s:.1f} s: {st0['neurons']} neurons, {st0['synapses']} synapses, "
@@ -487,6 +492,7 @@ def main():
                                f"{st0['neurons']} neurons, {st0['synapses']} synapses, "
                                f"{args.drive} external drive",
                    "neurons": neurons, "synapses": st0["synapses"], "drive": args.drive,
+                   "generator": args.generator,
                    "spikes_per_step": spikes / args.steps,
                    "mean_rate_hz": spikes / args.steps * 1000.0 / neurons,
                    "init_seconds": init_s,

@@ -72,7 +72,23 @@ typedef struct {
   uint32_t host_threads;     /* host threads of the connectome generator and
                                 delivery layout (0 = the hardware threads,
                                 capped by OMP_NUM_THREADS where set) */
+  uint32_t generator;        /* dpsnn_generator: who draws the connectome */
+  uint64_t generator_seed;   /* seed of the device generator */
 } dpsnn_options;
+
+/* Connectome generator (DPSNN_LIFCAcorticalModuleGen.cpp:62-220). */
+typedef enum {
+  DPSNN_GENERATOR_HOST = 0,    /* the reference's generator and ran3 stream on
+                                  the host: bit-exact with the reference */
+  DPSNN_GENERATOR_DEVICE = 1   /* the same distributions (connection
+                                  probabilities per stencil cell, geometric
+                                  target gaps, uniform delays, truncated
+                                  Gaussian weights) drawn on the device from
+                                  Philox4x32-10 keyed by generator_seed and
+                                  the source neuron: statistically
+                                  equivalent, not bit-exact.  dpsnn_prepare
+                                  then uses the device named in the options */
+} dpsnn_generator;
 
 /* Parameters as getParameters returns them (the subset this path reads;
  * field names follow struct DPSNN_parameters, DPSNN_parameters.h:72-146). */
