@@ -93,7 +93,8 @@ def _p(t):
 class ServerOptions(ctypes.Structure):
     _fields_ = [("device", ctypes.c_int), ("hbm_budget_bytes", ctypes.c_uint64),
                 ("build_threads", ctypes.c_uint32), ("steps_per_round", ctypes.c_uint32),
-                ("keep_finished", ctypes.c_int), ("max_running", ctypes.c_uint32)]
+                ("keep_finished", ctypes.c_int), ("max_running", ctypes.c_uint32),
+                ("host_backlog_bytes", ctypes.c_uint64)]
 
 
 class JobInfo(ctypes.Structure):
@@ -137,6 +138,7 @@ _SIGS = {
     "dpsnn_create": (ctypes.c_int, [ctypes.c_char_p, _p(Options), _p(ctypes.c_void_p)]),
     "dpsnn_prepare": (ctypes.c_int, [ctypes.c_char_p, _p(Options), _p(ctypes.c_void_p)]),
     "dpsnn_device_bytes_needed": (ctypes.c_uint64, [ctypes.c_void_p]),
+    "dpsnn_host_bytes": (ctypes.c_uint64, [ctypes.c_void_p]),
     "dpsnn_upload": (ctypes.c_int, [ctypes.c_void_p]),
     "dpsnn_step": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint32]),
     "dpsnn_step_noalloc": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint32]),
@@ -595,9 +597,10 @@ class Server:
     when its device footprint fits the HBM budget."""
 
     def __init__(self, device=0, hbm_budget_bytes=0, build_threads=0, steps_per_round=0, keep_finished=False,
-                 max_running=0):
+                 max_running=0, host_backlog_bytes=0):
         self._lib = library()
-        o = ServerOptions(device, hbm_budget_bytes, build_threads, steps_per_round, int(keep_finished), max_running)
+        o = ServerOptions(device, hbm_budget_bytes, build_threads, steps_per_round, int(keep_finished), max_running,
+                          host_backlog_bytes)
         h = ctypes.c_void_p()
         _check("dpsnn_server_create", self._lib.dpsnn_server_create(ctypes.byref(o), ctypes.byref(h)))
         self._h = h

@@ -198,6 +198,7 @@ struct DevNet {
   uint32_t *bigList;         // [N] targets k_sort leaves to k_sort_big
   uint32_t *bigCount;        // their count (zero between steps)
   uint32_t sortCap;          // k_sort keeps targets of up to this many records (<= kSortCap)
+  uint32_t fuseSort;         // 1: k_collect sorts each unit's targets itself (no k_sort)
   // Steps replayed from a captured graph read their t here (the kernel
   // argument is then ignored); null: t is the argument.
   const uint32_t *tDev;
@@ -1339,6 +1340,18 @@ __host__ __device__ inline size_t collect_lds(uint32_t entries, uint32_t unit) {
 constexpr int kWinPerThread = 12;
 constexpr int kCollectBatches = 4;  // 64-item batches per round of loads (the next round is in flight too)
 
+template <int B, int CH>
+__device__ __attribute__((always_inline)) inline void sort_records(uint32_t t, uint32_t n, uint8_t *lds, uint32_t cap,
+                                                                   uint64_t *out);
+__device__ inline uint64_t *record_runs(const DevNet &P, uint32_t j);
+constexpr uint32_t kSortCap = 768;
+constexpr int kSortChunks = (int)(kSortCap / kWave);
+constexpr int kFusedSortB = 256;  // buckets of the sort inside k_collect
+constexpr int kFusedSortChunks = 8;  // records per lane it holds in registers (more: read twice)
+constexpr bool kFusedSortDefault = false;
+// one wave's sort area inside k_collect, 16-byte aligned (it holds u64s)
+__host__ __device__ inline size_t fused_sort_stride() { return (phase1_lds(kSortCap, kFusedSortB) + 15) / 16 * 16; }
+
 __global__ __launch_bounds__(kCollectThreads) void k_collect(DevNet P, uint32_t t) {
   extern __shared__ uint64_t lds[];
   if (P.tDev) t = *P.tDev;
@@ -1352,7 +1365,6 @@ __global__ __launch_bounds__(kCollectThreads) void k_collect(DevNet P, uint32_t 
   const uint64_t tBits = dpsnn_asu64(tF);
   const uint64_t below = lane_mask_lt();
   const uint64_t upTo = lane == 63 ? ~0ull : (2ull << lane) - 1;
-  if (threadIdx.x == 0) win[W] = kNoKey;
   unsigned long long evSum = 0;
   const uint32_t home = blockIdx.x % kXcds;
   uint32_t x = home;
@@ -1361,6 +1373,7 @@ __global__ __launch_bounds__(kCollectThreads) void k_collect(DevNet P, uint32_t 
     const uint32_t j0 = col * P.npc + part * P.winUnit;
     const uint32_t m = min(P.winUnit, (col + 1) * P.npc - j0);
     for (uint32_t i = threadIdx.x; i < m; i += blockDim.x) cnt[i] = 0;
+    if (threadIdx.x == 0) win[W] = kNoKey;  // the fused sort's LDS may have overlaid it
     // the unit's records: tile (j >> 6) of the blocked scratch relative to
     // the unit's first tile, so that a record's offset fits 32 bits
     uint64_t *unitKeys = P.keys + (size_t)(j0 >> 6) * P.cap * kWave;
@@ -1456,6 +1469,24 @@ __global__ __launch_bounds__(kCollectThreads) void k_collect(DevNet P, uint32_t 
       P.keyCount[j0 + i] = cnt[i];
       evSum += cnt[i];
     }
+    if (P.fuseSort) {
+      // Sort the unit's targets while their records are recent: wave w
+      // takes targets w + 16 k (k < 64, as m <= 1024), its count held in
+      // lane k, since the waves' sort areas overlay the window and counts.
+      const uint32_t mi = wave + kWindowWaves * lane;
+      const uint32_t held = mi < m ? cnt[mi] : 0u;
+      __syncthreads();
+      uint8_t *area = (uint8_t *)lds + (size_t)wave * fused_sort_stride();
+      for (uint32_t k = 0; wave + k * kWindowWaves < m; ++k) {
+        const uint32_t j = j0 + wave + k * kWindowWaves;
+        const uint32_t nrec = (uint32_t)__builtin_amdgcn_readlane((int)held, (int)k);
+        if (nrec > P.sortCap) {
+          if (lane == 0) P.bigList[atomicAdd(P.bigCount, 1u)] = j;
+          continue;
+        }
+        sort_records<kFusedSortB, kFusedSortChunks>(t, nrec, area, kSortCap, record_runs(P, j));
+      }
+    }
   }
   for (int o = 32; o > 0; o >>= 1) evSum += __shfl_down(evSum, o);
   if (lane == 0 && evSum) atomicAdd(counter_slot(P.counters) + 0, evSum);
@@ -1529,8 +1560,6 @@ __device__ __attribute__((always_inline)) inline void sort_records(uint32_t t, u
 // live events per target (a few hundred at the saturated rate) but below
 // the longest in-list, so that more waves fit a CU; a target with more
 // records is listed for k_sort_big, whose LDS holds the longest in-list.
-constexpr uint32_t kSortCap = 768;
-constexpr int kSortChunks = (int)(kSortCap / kWave);
 
 __device__ inline uint64_t *record_runs(const DevNet &P, uint32_t j) {
   return P.keys + (size_t)(j >> 6) * P.cap * kWave + (j & 63) * kRun;
@@ -1543,7 +1572,6 @@ template <int B>
 __global__ __launch_bounds__(kWave) void k_sort(DevNet P, uint32_t t) {
   extern __shared__ uint64_t lds[];
   if (P.tDev) t = *P.tDev;
-  if (blockIdx.x == 0 && threadIdx.x < kXcds) P.winCtr[threadIdx.x * kCtrStride] = 0;  // for the next k_collect
   for (uint32_t j = blockIdx.x; j < P.N; j += gridDim.x) {
     const uint32_t n = P.keyCount[j];
     if (n > P.sortCap) {
@@ -1555,11 +1583,12 @@ __global__ __launch_bounds__(kWave) void k_sort(DevNet P, uint32_t t) {
 }
 
 // The targets k_sort left (more than kSortCap records), one wave each;
-// bigCount is cleared by the next k_integrate.
+// bigCount is cleared by the next k_integrate.  Runs after every k_collect.
 template <int B>
 __global__ __launch_bounds__(kWave) void k_sort_big(DevNet P, uint32_t t) {
   extern __shared__ uint64_t lds[];
   if (P.tDev) t = *P.tDev;
+  if (blockIdx.x == 0 && threadIdx.x < kXcds) P.winCtr[threadIdx.x * kCtrStride] = 0;  // for the next k_collect
   const uint32_t count = *P.bigCount;
   for (uint32_t i = blockIdx.x; i < count; i += gridDim.x) {
     const uint32_t j = P.bigList[i];
@@ -1878,12 +1907,22 @@ static void free_all(dpsnn_net *n) {
 static int upload_window(dpsnn_net *n, int cus) {
   const WindowLayout &L = n->window;
   DevNet &P = n->dev;
-  const size_t lds = collect_lds(L.slots * L.npc, L.unit);
+  size_t lds = collect_lds(L.slots * L.npc, L.unit);
   int maxLds = 0;
   HIP_TRY(hipDeviceGetAttribute(&maxLds, hipDeviceAttributeMaxSharedMemoryPerBlock, n->opt.device));
   if (lds > (size_t)maxLds) return 0;  // the gather form stays
+  // the sort fused into k_collect: its waves' sort areas overlay the window
+  const KernelShape &K = kShapes[n->shape];
+  const size_t fusedLds = std::max(lds, (size_t)kWindowWaves * fused_sort_stride());
+  // (DPSNN_FUSED_SORT=0/1 overrides the default, for tests and timing)
+  const char *fe = getenv("DPSNN_FUSED_SORT");
+  const bool fuse = fe ? atoi(fe) != 0 : kFusedSortDefault;
+  P.fuseSort = fuse && K.B == kFusedSortB && L.unit <= (uint32_t)kCollectThreads && fusedLds <= (size_t)maxLds;
+  if (P.fuseSort) lds = fusedLds;
+  // the device's limit, not this network's size: networks of other sizes
+  // launch the same kernel
   if (lds > (size_t)64 * 1024)
-    HIP_TRY(hipFuncSetAttribute((const void *)k_collect, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    HIP_TRY(hipFuncSetAttribute((const void *)k_collect, hipFuncAttributeMaxDynamicSharedMemorySize, maxLds));
   uint32_t *d_item;
   uint16_t *d_desc;
   uint64_t *d_chunk;
@@ -1908,7 +1947,6 @@ static int upload_window(dpsnn_net *n, int cus) {
   if (const char *e = getenv("DPSNN_SORT_CAP")) P.sortCap = std::min<uint32_t>(kSortCap, (uint32_t)atoi(e));
   n->winLds = lds;
   n->winGrid = round_up(std::min<uint32_t>(L.units, (uint32_t)cus), kXcds);
-  const KernelShape &K = kShapes[n->shape];
   n->sortGrid = round_up(
       std::min<uint32_t>(P.N, (uint32_t)cus * resident_waves((const void *)K.sort, phase1_lds(kSortCap, K.B))), kXcds);
   n->sortBigGrid = std::min<uint32_t>(P.N, (uint32_t)cus * 4);
@@ -2446,7 +2484,8 @@ static int one_step(dpsnn_net *n, const DevNet &P) {
   if (n->split) {
     if (n->winOn && sp.t >= kPackT && !sp.ltd) {
       hipLaunchKernelGGL(k_collect, dim3(n->winGrid), dim3(kCollectThreads), n->winLds, n->stream, P, sp.t);
-      hipLaunchKernelGGL(K.sort, dim3(n->sortGrid), dim3(kWave), phase1_lds(kSortCap, K.B), n->stream, P, sp.t);
+      if (!P.fuseSort)
+        hipLaunchKernelGGL(K.sort, dim3(n->sortGrid), dim3(kWave), phase1_lds(kSortCap, K.B), n->stream, P, sp.t);
       hipLaunchKernelGGL(K.sortBig, dim3(n->sortBigGrid), dim3(kWave), phase1_lds(P.cap, K.B), n->stream, P, sp.t);
     } else {
       hipLaunchKernelGGL(K.gather, dim3(n->gatherGrid), dim3(kWave), n->ldsBytes, n->stream, P, sp.t, sp.ltd);
@@ -2489,7 +2528,8 @@ static void launch_graph_step(dpsnn_net *n, const DevNet &G) {
   const KernelShape &K = kShapes[n->shape];
   if (n->winOn) {
     hipLaunchKernelGGL(k_collect, dim3(n->winGrid), dim3(kCollectThreads), n->winLds, n->stream, G, 0u);
-    hipLaunchKernelGGL(K.sort, dim3(n->sortGrid), dim3(kWave), phase1_lds(kSortCap, K.B), n->stream, G, 0u);
+    if (!G.fuseSort)
+      hipLaunchKernelGGL(K.sort, dim3(n->sortGrid), dim3(kWave), phase1_lds(kSortCap, K.B), n->stream, G, 0u);
     hipLaunchKernelGGL(K.sortBig, dim3(n->sortBigGrid), dim3(kWave), phase1_lds(G.cap, K.B), n->stream, G, 0u);
   } else {
     hipLaunchKernelGGL(K.gather, dim3(n->gatherGrid), dim3(kWave), n->ldsBytes, n->stream, G, 0u, 0);
@@ -2934,6 +2974,16 @@ uint64_t dpsnn_device_bytes_needed(const dpsnn_net *n) {
     b += r(4 * L.item.size()) + r(2 * L.desc.size()) + r(8 * L.chunk.size()) + r(4 * L.cols.size()) +
          r(4 * kXcds * kCtrStride) + r(4 * N) + r(4);
   return b;
+}
+
+uint64_t dpsnn_host_bytes(const dpsnn_net *n) {
+  if (!n) return 0;
+  auto v = [](const auto &x) { return uint64_t(x.capacity()) * sizeof(x[0]); };
+  const Connectome &c = n->con;
+  const WindowLayout &L = n->window;
+  return v(c.pre) + v(c.post) + v(c.delay) + v(c.kind) + v(c.weight) + v(c.in_off) + v(c.in_pre_d) + v(c.in_w) +
+         v(c.in_syn) + v(c.inc_off) + v(c.inc_syn) + v(n->inSynLocal) + v(L.cols) + v(L.item) + v(L.desc) +
+         v(L.chunk);
 }
 
 int dpsnn_upload(dpsnn_net *n) {

@@ -2,7 +2,8 @@
 // (continuous batching over the engine's C ABI).
 //
 // Builder threads turn submitted jobs into host-side networks
-// (dpsnn_prepare: parameters, generator, delivery layout).  The scheduler
+// (dpsnn_prepare: parameters, generator, delivery layout) while the host
+// lists of built jobs not yet uploaded stay under a bound.  The scheduler
 // thread admits built jobs first-fit in arrival order while their exact
 // device footprint (dpsnn_device_bytes_needed) fits the HBM budget, and an
 // uploader thread copies each admitted job to the device (dpsnn_upload, on
@@ -26,6 +27,8 @@
 #include <thread>
 #include <vector>
 
+#include <unistd.h>
+
 #include "dpsnn_hip.h"
 #include "dpsnn_host.h"
 
@@ -42,6 +45,7 @@ struct Job {
   dpsnn_options opt{};
   dpsnn_job_info info{};
   dpsnn_net *net = nullptr;
+  uint64_t hostBytes = 0;  // counted in the server's backlog until uploaded
   std::string message;
 };
 
@@ -59,6 +63,10 @@ struct dpsnn_server {
   std::thread sched, uploader;
   bool stop = false;
   uint64_t nextId = 1, used = 0, budget = 0;
+  // host bytes of built jobs not yet uploaded, and their bound: a fast
+  // generator would otherwise build every queued job while the device is
+  // busy and hold all their host lists at once
+  uint64_t backlog = 0, backlogLimit = 0;
   uint64_t eventsDone = 0, eventsRunning = 0;
   uint32_t uploading = 0;                 // admitted, in uploadQ or being uploaded
   // bumped whenever admission could change (a job waits, memory or a
@@ -90,7 +98,7 @@ void builder_loop(dpsnn_server *s) {
     Job *j = nullptr;
     {
       std::unique_lock<std::mutex> lk(s->mu);
-      s->cvBuild.wait(lk, [&] { return s->stop || !s->buildQ.empty(); });
+      s->cvBuild.wait(lk, [&] { return s->stop || (!s->buildQ.empty() && s->backlog < s->backlogLimit); });
       if (s->stop) return;
       j = s->buildQ.front();
       s->buildQ.pop_front();
@@ -104,6 +112,7 @@ void builder_loop(dpsnn_server *s) {
     const int rc = dpsnn_prepare(j->inidir.c_str(), &opt, &n);
     const std::string err = rc ? dpsnn_last_error() : "";
     const uint64_t need = rc ? 0 : dpsnn_device_bytes_needed(n);
+    const uint64_t host = rc ? 0 : dpsnn_host_bytes(n);
     std::unique_lock<std::mutex> lk(s->mu);
     if (rc) {
       finish_locked(s, j, DPSNN_JOB_FAILED, rc, err);
@@ -114,6 +123,8 @@ void builder_loop(dpsnn_server *s) {
       finish_locked(s, j, DPSNN_JOB_FAILED, DPSNN_ERR_MEMORY, "job needs more device memory than the budget");
     } else {
       j->net = n;
+      j->hostBytes = host;
+      s->backlog += host;
       j->info.device_bytes = need;
       j->info.built_s = s->now();
       j->info.state = DPSNN_JOB_WAITING;
@@ -142,6 +153,9 @@ void uploader_loop(dpsnn_server *s) {
     std::lock_guard<std::mutex> lk(s->mu);
     s->uploadSeconds += std::chrono::duration<double>(Clock::now() - t0).count();
     s->uploading--;
+    s->backlog -= j->hostBytes;
+    j->hostBytes = 0;
+    s->cvBuild.notify_all();
     s->admitGen++;
     if (rc) {
       j->net = nullptr;
@@ -264,6 +278,8 @@ int dpsnn_server_create(const dpsnn_server_options *opt, dpsnn_server **out) {
   if (budget <= kBatchReserve) return DPSNN_ERR_MEMORY;
   s->budget = budget - kBatchReserve;
   if ((rc = dpsnn_batch_create(s->o.device, &s->batch))) return rc;
+  const uint64_t phys = uint64_t(sysconf(_SC_PHYS_PAGES)) * uint64_t(sysconf(_SC_PAGE_SIZE));
+  s->backlogLimit = s->o.host_backlog_bytes ? s->o.host_backlog_bytes : std::min(uint64_t(32) << 30, phys / 4);
   const uint32_t nb = s->o.build_threads ? s->o.build_threads : 2;
   s->hostThreads = std::max(1u, dpsnn::default_threads() / nb);
   dpsnn_server *p = s.get();

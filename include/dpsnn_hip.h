@@ -154,6 +154,10 @@ int dpsnn_create(const char *inidir, const dpsnn_options *opt, dpsnn_net **out);
  * destroyed without ever being uploaded. */
 int dpsnn_prepare(const char *inidir, const dpsnn_options *opt, dpsnn_net **out);
 uint64_t dpsnn_device_bytes_needed(const dpsnn_net *net);
+/* Host bytes the network's synapse and delivery lists hold (the backward
+ * list and, until upload, the in-lists and window layout).  No counterpart
+ * in the reference: the server bounds its backlog of built jobs by it. */
+uint64_t dpsnn_host_bytes(const dpsnn_net *net);
 int dpsnn_upload(dpsnn_net *net);
 
 /* Advances `nsteps` milliseconds (each = one completeTimeStep_ms).  Returns
@@ -297,6 +301,10 @@ typedef struct {
                                  its memory) until dpsnn_server_take */
   uint32_t max_running;       /* jobs on the device at once (0 = as many as
                                  the budget holds; 1 = one job at a time) */
+  uint64_t host_backlog_bytes; /* host bytes (dpsnn_host_bytes) that built
+                                 jobs not yet uploaded may hold; builders
+                                 start no build beyond it (0 = 32 GiB, at
+                                 most a quarter of physical memory) */
 } dpsnn_server_options;
 typedef enum {
   DPSNN_JOB_QUEUED = 0,       /* waiting for a host builder */

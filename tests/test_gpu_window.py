@@ -1,7 +1,7 @@
 """GPU parity of the stencil-window form of phase 1 (k_collect + k_sort,
-steps t >= 32 without LTD): forced onto configs[0] and small networks it
-must give the reference's raster and the oracle's state bit for bit, like
-the gather form."""
+or the sort fused into k_collect; steps t >= 32 without LTD): forced onto
+configs[0] and small networks it must give the reference's raster and the
+oracle's state bit for bit, like the gather form."""
 import os
 
 import numpy as np
@@ -32,17 +32,20 @@ def _columns(tmp, pkg, name, counts, src="c0", nu_ext=3.0):
     return str(d)
 
 
-@pytest.mark.parametrize("window,sort_cap", [("1", None), ("1", "200"), ("0", None)])
-def test_c0_full_second_identical_to_reference(pkg, tmp_path, monkeypatch, window, sort_cap):
+@pytest.mark.parametrize("window,sort_cap,fused", [("1", None, "0"), ("1", "200", "0"), ("1", None, "1"),
+                                                  ("1", "200", "1"), ("0", None, "0")])
+def test_c0_full_second_identical_to_reference(pkg, tmp_path, monkeypatch, window, sort_cap, fused):
     """sort_cap 200: targets with more than 200 records of a step (nearly all
-    of configs[0]'s) are sorted by the second sort launch."""
+    of configs[0]'s) are sorted by the second sort launch.  fused 1: the sort
+    runs inside k_collect, per unit."""
     monkeypatch.setenv("DPSNN_WINDOW", window)
+    monkeypatch.setenv("DPSNN_FUSED_SORT", fused)
     if sort_cap:
         monkeypatch.setenv("DPSNN_SORT_CAP", sort_cap)
     g = golden("c0_raster.json")
     cfg = pkg.config_dir("c0")
     pkg.apply_env(cfg, env_totalSimTime_ms=1000)
-    print(f"window={window}: configs[0] 1000 ms", flush=True)
+    print(f"window={window} sort_cap={sort_cap} fused={fused}: configs[0] 1000 ms", flush=True)
     with pkg.Network(cfg, output_dir=str(tmp_path)) as net:
         while not net.step(100):
             pass
@@ -74,23 +77,26 @@ def _trace_vs_oracle(pkg, oracle, cfg, steps, every):
     return st
 
 
-def test_window_state_trace_bit_exact_vs_oracle(pkg, oracle, tmp_path, monkeypatch):
+@pytest.mark.parametrize("fused", ["0", "1"])
+def test_window_state_trace_bit_exact_vs_oracle(pkg, oracle, tmp_path, monkeypatch, fused):
     monkeypatch.setenv("DPSNN_WINDOW", "1")
+    monkeypatch.setenv("DPSNN_FUSED_SORT", fused)
     cfg = _columns(tmp_path, pkg, "tiny", (50, 50, 25))
     pkg.apply_env(pkg.config_dir("c0"), env_CFX=2, env_CFY=2, env_totalSimTime_ms=200)
-    print("window=1: 2x2 x 125 neurons, 200 ms vs oracle", flush=True)
+    print(f"window=1 fused={fused}: 2x2 x 125 neurons, 200 ms vs oracle", flush=True)
     st = _trace_vs_oracle(pkg, oracle, cfg, 200, 1)
     assert st["spikes"] > 0
 
 
-@pytest.mark.parametrize("window", ["1", "0"])
-def test_grid24_small_columns_bit_exact_vs_oracle(pkg, oracle, tmp_path, monkeypatch, window):
+@pytest.mark.parametrize("window,fused", [("1", "0"), ("1", "1"), ("0", "0")])
+def test_grid24_small_columns_bit_exact_vs_oracle(pkg, oracle, tmp_path, monkeypatch, window, fused):
     """configs[1]'s 24 x 24 grid and stencil (corner, edge and interior
     columns; 576 columns through the work queues) with 40-neuron columns,
     past the packed-scratch step: state and spikes are the oracle's."""
     monkeypatch.setenv("DPSNN_WINDOW", window)
+    monkeypatch.setenv("DPSNN_FUSED_SORT", fused)
     cfg = _columns(tmp_path, pkg, "g24", (16, 16, 8), src="c1")
     pkg.apply_env(pkg.config_dir("c1"), env_totalSimTime_ms=80)
-    print(f"window={window}: 24x24 x 40 neurons, 80 ms vs oracle", flush=True)
+    print(f"window={window} fused={fused}: 24x24 x 40 neurons, 80 ms vs oracle", flush=True)
     st = _trace_vs_oracle(pkg, oracle, cfg, 80, 8)
     assert st["neurons"] == 576 * 40 and st["spikes"] > 0 and st["synaptic_events"] > 0
