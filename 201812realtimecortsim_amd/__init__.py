@@ -115,7 +115,9 @@ class ServerStats(ctypes.Structure):
     _fields_ = [("submitted", ctypes.c_uint64), ("finished", ctypes.c_uint64), ("failed", ctypes.c_uint64),
                 ("running", ctypes.c_uint64), ("waiting", ctypes.c_uint64), ("building", ctypes.c_uint64),
                 ("synaptic_events", ctypes.c_uint64), ("device_bytes", ctypes.c_uint64),
-                ("elapsed_s", ctypes.c_double), ("step_s", ctypes.c_double), ("upload_s", ctypes.c_double)]
+                ("elapsed_s", ctypes.c_double), ("step_s", ctypes.c_double), ("upload_s", ctypes.c_double),
+                ("queued", ctypes.c_uint64), ("admit_s", ctypes.c_double), ("account_s", ctypes.c_double),
+                ("retire_s", ctypes.c_double)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}

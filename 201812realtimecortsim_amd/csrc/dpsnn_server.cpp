@@ -73,7 +73,7 @@ struct dpsnn_server {
   // running slot frees up); the scheduler sleeps on waiting jobs until the
   // generation moves past the one it last tried
   uint64_t admitGen = 0, triedGen = ~uint64_t(0);
-  double stepSeconds = 0, uploadSeconds = 0;
+  double stepSeconds = 0, uploadSeconds = 0, admitSeconds = 0, accountSeconds = 0, retireSeconds = 0;
   uint32_t hostThreads = 0;
   dpsnn_batch *batch = nullptr;
 
@@ -200,6 +200,7 @@ void scheduler_loop(dpsnn_server *s) {
       admit.assign(s->ready.begin(), s->ready.end());
       s->ready.clear();
     }
+    const Clock::time_point ta = Clock::now();
     for (Job *j : admit) {
       const int rc = dpsnn_batch_add(s->batch, j->net);
       if (rc) {
@@ -217,17 +218,24 @@ void scheduler_loop(dpsnn_server *s) {
       j->info.state = DPSNN_JOB_RUNNING;
       s->running.push_back(j);
     }
-    if (s->running.empty()) continue;
+    const double admitDt = std::chrono::duration<double>(Clock::now() - ta).count();
+    if (s->running.empty()) {
+      std::lock_guard<std::mutex> lk(s->mu);
+      s->admitSeconds += admitDt;
+      continue;
+    }
     const Clock::time_point t0 = Clock::now();
     const int rc = dpsnn_batch_step(s->batch, round);
     const double dt = std::chrono::duration<double>(Clock::now() - t0).count();
     const std::string err = rc < 0 ? dpsnn_last_error() : "";
-    std::vector<Job *> done;
     uint64_t events = 0;
+    double accountDt = 0, retireDt = 0;
     for (Job *j : std::vector<Job *>(s->running)) {
+      const Clock::time_point tj = Clock::now();
       if (rc >= 0 && dpsnn_now_ms(j->net) < dpsnn_total_ms(j->net)) {
         dpsnn_stats st{};
         if (dpsnn_get_stats(j->net, &st) == 0) events += st.synaptic_events;
+        accountDt += std::chrono::duration<double>(Clock::now() - tj).count();
         continue;
       }
       dpsnn_batch_remove(s->batch, j->net);
@@ -237,6 +245,7 @@ void scheduler_loop(dpsnn_server *s) {
       const std::string ferr = frc ? (rc < 0 ? err : std::string(dpsnn_last_error())) : "";
       const bool drop = frc || !s->o.keep_finished;
       if (drop) dpsnn_destroy(j->net);
+      retireDt += std::chrono::duration<double>(Clock::now() - tj).count();
       std::lock_guard<std::mutex> lk(s->mu);
       j->info.stats = fst;
       if (drop) {
@@ -255,6 +264,9 @@ void scheduler_loop(dpsnn_server *s) {
     std::lock_guard<std::mutex> lk(s->mu);
     s->eventsRunning = events;
     s->stepSeconds += dt;
+    s->admitSeconds += admitDt;
+    s->accountSeconds += accountDt;
+    s->retireSeconds += retireDt;
   }
 }
 
@@ -377,13 +389,17 @@ int dpsnn_server_get_stats(dpsnn_server *s, dpsnn_server_stats *st) {
     if (state == DPSNN_JOB_FAILED) st->failed++;
     if (state == DPSNN_JOB_RUNNING) st->running++;
     if (state == DPSNN_JOB_WAITING) st->waiting++;
-    if (state == DPSNN_JOB_BUILDING || state == DPSNN_JOB_QUEUED) st->building++;
+    if (state == DPSNN_JOB_BUILDING) st->building++;
+    if (state == DPSNN_JOB_QUEUED) st->queued++;
   }
   st->synaptic_events = s->eventsDone + s->eventsRunning;
   st->device_bytes = s->used + kBatchReserve;
   st->elapsed_s = s->now();
   st->step_s = s->stepSeconds;
   st->upload_s = s->uploadSeconds;
+  st->admit_s = s->admitSeconds;
+  st->account_s = s->accountSeconds;
+  st->retire_s = s->retireSeconds;
   return 0;
 }
 

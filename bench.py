@@ -298,7 +298,7 @@ def run_server(pkg, jobs, dirs, envs, args, max_running):
         if time.perf_counter() - last >= 10.0:
             st = srv.stats()
             log(f"{'plain' if max_running == 1 else 'served'} t={now:5.0f} s: submitted {st['submitted']} "
-                f"building {st['building']} waiting {st['waiting']} running {st['running']} "
+                f"queued {st['queued']} building {st['building']} waiting {st['waiting']} running {st['running']} "
                 f"finished {st['finished']} events {st['synaptic_events']:.3e}")
             last = time.perf_counter()
         time.sleep(0.005)
@@ -310,11 +310,13 @@ def run_server(pkg, jobs, dirs, envs, args, max_running):
     pct = (lambda q: float(np.percentile(lat, q)) if lat.size else None)
     return {"submitted": len(ids), "finished": len(done), "failed": sum(i["state"] == "failed" for i in infos),
             "running_at_end": st["running"], "waiting_at_end": st["waiting"], "building_at_end": st["building"],
+            "queued_at_end": st["queued"],
             "events": st["synaptic_events"], "elapsed_s": st["elapsed_s"],
             "events_per_s": st["synaptic_events"] / st["elapsed_s"],
             "latency_p50_s": pct(50), "latency_p99_s": pct(99),
             "finished_sim_ms": sum(i["stats"]["steps"] for i in done),
-            "step_s": st["step_s"], "upload_s": st["upload_s"]}
+            "step_s": st["step_s"], "upload_s": st["upload_s"], "admit_s": st["admit_s"],
+            "account_s": st["account_s"], "retire_s": st["retire_s"]}
 
 
 def bench_serve(pkg, args):

@@ -330,6 +330,10 @@ typedef struct {
   double elapsed_s;           /* server clock */
   double step_s;              /* scheduler time in batched steps */
   double upload_s;            /* uploader time copying admitted jobs (beside the steps) */
+  uint64_t queued;            /* submitted, not yet taken by a builder (not in `building`) */
+  double admit_s;             /* scheduler time adding uploaded jobs to the batch */
+  double account_s;           /* scheduler time reading running jobs' counters */
+  double retire_s;            /* scheduler time retiring finished jobs (sync, drain, free) */
 } dpsnn_server_stats;
 int dpsnn_server_create(const dpsnn_server_options *opt, dpsnn_server **out);
 /* Queues a job: the network of `inidir` with options `opt` (copied; its
