@@ -1329,9 +1329,10 @@ __device__ inline uint32_t next_unit(uint32_t *ctr, uint32_t total, uint32_t &x,
 }
 
 // LDS of k_collect: the window (entries + 1 u64), the per-target record
-// counts of a unit plus one slot for padding chunks, two queue words.
+// counts and run offsets of a unit plus one slot for padding chunks, two
+// queue words.
 __host__ __device__ inline size_t collect_lds(uint32_t entries, uint32_t unit) {
-  return ((size_t)entries + 1) * sizeof(uint64_t) + ((size_t)unit + 1) * sizeof(uint32_t) + 2 * sizeof(uint32_t);
+  return ((size_t)entries + 1) * sizeof(uint64_t) + ((size_t)unit + 1) * sizeof(uint2) + 2 * sizeof(uint32_t);
 }
 
 // Window entries per thread of k_collect (W <= kWinPerThread * 1024): each
@@ -1339,6 +1340,15 @@ __host__ __device__ inline size_t collect_lds(uint32_t entries, uint32_t unit) {
 // a pass's emission times at once.
 constexpr int kWinPerThread = 12;
 constexpr int kCollectBatches = 4;  // 64-item batches per round of loads (the next round is in flight too)
+
+// A buffer view of `bytes` bytes at base (raw: stride 0, byte offsets;
+// the gfx9 descriptor word).  A store at an offset past `bytes` is dropped
+// by the hardware.
+typedef unsigned int u32x2 __attribute__((ext_vector_type(2)));
+constexpr uint32_t kDropOffset = 0xFFFFFFFFu;
+__device__ inline __amdgpu_buffer_rsrc_t buffer_view(void *base, uint32_t bytes) {
+  return __builtin_amdgcn_make_buffer_rsrc(base, (short)0, (int)bytes, 0x00020000);
+}
 
 template <int B, int CH>
 __device__ __attribute__((always_inline)) inline void sort_records(uint32_t t, uint32_t n, uint8_t *lds, uint32_t cap,
@@ -1357,27 +1367,38 @@ __global__ __launch_bounds__(kCollectThreads) void k_collect(DevNet P, uint32_t 
   if (P.tDev) t = *P.tDev;
   const uint32_t W = P.winSlots * P.npc;  // entry W: the padding items' (never live)
   uint64_t *win = lds;
-  uint32_t *cnt = (uint32_t *)(win + W + 1);  // records so far, per target of the unit (+ padding's slot)
-  uint32_t *sh = cnt + P.winUnit + 1;
+  // per target of the unit (+ padding's slot): records so far (.x) and the
+  // element offset of its run in the unit's scratch (.y)
+  uint2 *ct = (uint2 *)(win + W + 1);
+  uint32_t *sh = (uint32_t *)(ct + P.winUnit + 1);
   const uint32_t pad = P.winUnit;  // the target index of padding chunks past a stream's end
-  const uint32_t wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const uint32_t wave = (uint32_t)__builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6)), lane = threadIdx.x & 63;
+  const uint32_t field = (lane >> 3) * 4;  // this lane's chunk's field in a batch's run-start table
+  const bool chunkEnd = (lane & 7) == 7;
   const double tF = (double)t;
   const uint64_t tBits = dpsnn_asu64(tF);
-  const uint64_t below = lane_mask_lt();
-  const uint64_t upTo = lane == 63 ? ~0ull : (2ull << lane) - 1;
   unsigned long long evSum = 0;
   const uint32_t home = blockIdx.x % kXcds;
   uint32_t x = home;
-  for (uint32_t u; (u = next_unit(P.winCtr, P.winUnits, x, home, sh)) < P.winUnits;) {
+  for (;;) {
+    const uint32_t u = (uint32_t)__builtin_amdgcn_readfirstlane((int)next_unit(P.winCtr, P.winUnits, x, home, sh));
+    if (u >= P.winUnits) break;
     const uint32_t col = u / P.winUnitsPerCol, part = u - col * P.winUnitsPerCol;
     const uint32_t j0 = col * P.npc + part * P.winUnit;
     const uint32_t m = min(P.winUnit, (col + 1) * P.npc - j0);
-    for (uint32_t i = threadIdx.x; i < m; i += blockDim.x) cnt[i] = 0;
-    if (threadIdx.x == 0) win[W] = kNoKey;  // the fused sort's LDS may have overlaid it
     // the unit's records: tile (j >> 6) of the blocked scratch relative to
     // the unit's first tile, so that a record's offset fits 32 bits
     uint64_t *unitKeys = P.keys + (size_t)(j0 >> 6) * P.cap * kWave;
     const uint32_t j0lo = j0 & 63, tileElems = P.cap * kWave;
+    const __amdgpu_buffer_rsrc_t keyRsrc = buffer_view(unitKeys, ((j0lo + m + 63) >> 6) * tileElems * 8u);
+    for (uint32_t i = threadIdx.x; i < m; i += blockDim.x) {
+      const uint32_t jj = j0lo + i;
+      ct[i] = make_uint2(0u, (jj >> 6) * tileElems + (jj & 63) * kRun);
+    }
+    if (threadIdx.x == 0) {
+      win[W] = kNoKey;  // the fused sort's LDS may have overlaid it
+      ct[pad] = make_uint2(0u, 0u);
+    }
     // this thread's window entries e = tid + r * blockDim: the history
     // offset of their source (or ~0u: no source column in that slot)
     uint32_t off[kWinPerThread];
@@ -1406,56 +1427,79 @@ __global__ __launch_bounds__(kCollectThreads) void k_collect(DevNet P, uint32_t 
       }
       __syncthreads();
       const uint64_t *co = P.winChunk + ((size_t)u * P.winPasses + pass) * kWindowWaves + wave;
-      const uint64_t cb = co[0], ce = co[1];
-      // The wave's targets come in order, each one's items contiguous: a
-      // live item's place is its target's count at the start of the pass
-      // (cnt, read where the target's segment starts) or, for the target
-      // carried over from the previous batch, the carried count, plus the
-      // live items before it in its run of lanes.
-      uint32_t lastQ = ~0u, carry = 0;
+      const uint32_t cb = (uint32_t)co[0], ce = (uint32_t)co[1];
+      // The wave's targets come in order, each one's chunks contiguous, the
+      // first flagged.  A live item's place is the count of its target at
+      // the start of the pass (read where the target's first chunk is) or,
+      // for the target carried over from the previous batch, the carried
+      // count, plus the live items before it in its run of chunks: the
+      // live lanes below it (mbcnt) less those below its run's first lane,
+      // taken from that lane in one permute.
+      uint32_t carry = 0;
       uint32_t it[kCollectBatches], q[kCollectBatches], itN[kCollectBatches], qN[kCollectBatches];
-      auto load = [&](uint64_t c0, uint32_t (&I)[kCollectBatches], uint32_t (&Q)[kCollectBatches]) {
+      // the stream's items and descriptors, addressed by 32-bit offsets.
+      // Every load is issued (lanes past the stream's end read its last
+      // chunk and take padding instead), and records are stored through a
+      // buffer view of the unit's scratch with an out-of-range offset where
+      // an item is not live (the store is dropped): a round issues a fixed
+      // number of memory operations, so the next round's loads stay in
+      // flight while this one is processed.
+      const uint32_t *sItem = P.winItem + (size_t)cb * kWindowChunk;
+      const uint16_t *sDesc = P.winDesc + cb;
+      const uint32_t last = ce - cb - 1;  // used only when the stream is not empty
+      auto load = [&](uint32_t c0, uint32_t (&I)[kCollectBatches], uint32_t (&Q)[kCollectBatches]) {
 #pragma unroll
         for (int v = 0; v < kCollectBatches; ++v) {
-          const uint64_t ch = c0 + (uint64_t)v * kWindowChunk + (lane >> 3);
-          const bool ok = ch < ce;
-          I[v] = ok ? __builtin_nontemporal_load(P.winItem + ch * kWindowChunk + (lane & 7)) : W << 16;
-          Q[v] = ok ? (uint32_t)P.winDesc[ch] : pad;
+          const uint32_t rel = c0 - cb + (uint32_t)v * kWindowChunk + (lane >> 3);
+          const uint32_t at = min(rel, last);
+          const uint32_t i = __builtin_nontemporal_load(
+              (const uint32_t *)((const char *)sItem + (at * kWindowChunk + (lane & 7)) * 4u));
+          const uint32_t d = (uint32_t)load_at(sDesc, at * 2u);
+          I[v] = rel <= last ? i : W << 16;
+          Q[v] = rel <= last ? d : (pad | kWindowFirst);
         }
       };
       if (cb < ce) load(cb, it, q);
-      for (uint64_t c0 = cb; c0 < ce; c0 += (uint64_t)kWindowChunk * kCollectBatches) {
-        const uint64_t cn = c0 + (uint64_t)kWindowChunk * kCollectBatches;
-        if (cn < ce) load(cn, itN, qN);
+      for (uint32_t c0 = cb; c0 < ce; c0 += kWindowChunk * kCollectBatches) {
+        const uint32_t cn = c0 + kWindowChunk * kCollectBatches;
+        load(cn, itN, qN);  // past the end on the last round: the last chunk again, unused
         uint64_t r[kCollectBatches];
-        uint32_t base[kCollectBatches];
+        uint2 base[kCollectBatches];
 #pragma unroll
         for (int v = 0; v < kCollectBatches; ++v) {
           r[v] = win[it[v] >> 16];
-          base[v] = cnt[q[v]];
+          base[v] = ct[q[v] & kWindowTarget];
         }
 #pragma unroll
         for (int v = 0; v < kCollectBatches; ++v) {
-          if (c0 + (uint64_t)v * kWindowChunk >= ce) break;  // uniform
-          const uint32_t qv = q[v];
+          if (c0 + (uint32_t)v * kWindowChunk >= ce) break;  // uniform
+          const uint32_t qd = q[v];
           const bool live = r[v] != kNoKey;  // never on padding
+          const bool first = qd >= kWindowFirst;
           const uint64_t liveM = __ballot(live);
-          const uint32_t qPrev = (uint32_t)__shfl_up((int)qv, 1);
-          const uint64_t startM = __ballot(lane == 0 ? qv != lastQ : qv != qPrev);
-          const uint64_t upToStart = startM & upTo;
-          const uint32_t s = upToStart ? 63u - (uint32_t)__clzll((long long)upToStart) : 0u;
-          const uint32_t b0 = upToStart ? base[v] : carry;  // no start at or below: lastQ's run
-          const uint32_t p = b0 + (uint32_t)__popcll(liveM & below & ~((1ull << s) - 1ull));
-          if (live) {
-            const uint32_t jj = j0lo + qv;
-            const uint32_t at = (jj >> 6) * tileElems + (jj & 63) * kRun + (p / kRun) * (kWave * kRun) + p % kRun;
-            unitKeys[at] = r[v] | (it[v] & 0xFFFFu);
+          const uint64_t startM = __ballot(first) & 0x0101010101010101ull;  // chunks' first lanes
+          // the run-start chunk of every chunk of the batch (4-bit fields):
+          // the last chunk at or before it that is a target's first; 0 before
+          // the first such chunk (the carried target's run)
+          uint32_t T = 0;
+          for (uint64_t sm = startM; sm; sm &= sm - 1) {
+            const uint32_t k = (uint32_t)__builtin_ctzll(sm) >> 3;
+            const uint32_t fill = ~0u << (4 * k);
+            T = (T & ~fill) | ((k * 0x11111111u) & fill);
           }
+          const uint32_t pre =
+              __builtin_amdgcn_mbcnt_hi((uint32_t)(liveM >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)liveM, 0u));
+          const uint32_t val = (first ? base[v].x : carry) - pre;
+          const uint32_t p = (uint32_t)__builtin_amdgcn_ds_bpermute((int)(((T >> field) & 15u) << 5), (int)val) + pre;
+          const uint32_t at = base[v].y + (p / kRun) * (kWave * kRun) + p % kRun;
+          const uint64_t rec = r[v] | (it[v] & 0xFFFFu);
+          __builtin_amdgcn_raw_buffer_store_b64(u32x2{(uint32_t)rec, (uint32_t)(rec >> 32)}, keyRsrc,
+                                                live ? at * 8u : kDropOffset, 0, 0);
           const uint32_t after = p + (live ? 1u : 0u);
-          const bool lastOfRun = lane == 63 || ((startM >> (lane + 1)) & 1ull);
-          if (lastOfRun) cnt[qv] = after;
+          // a run's count grows along it: the largest of its chunks' last
+          // lanes is the count after this batch
+          if (chunkEnd) atomicMax(&ct[qd & kWindowTarget].x, after);
           carry = (uint32_t)__builtin_amdgcn_readlane((int)after, 63);
-          lastQ = (uint32_t)__builtin_amdgcn_readlane((int)qv, 63);
         }
 #pragma unroll
         for (int v = 0; v < kCollectBatches; ++v) {
@@ -1466,15 +1510,15 @@ __global__ __launch_bounds__(kCollectThreads) void k_collect(DevNet P, uint32_t 
     }
     __syncthreads();
     for (uint32_t i = threadIdx.x; i < m; i += blockDim.x) {
-      P.keyCount[j0 + i] = cnt[i];
-      evSum += cnt[i];
+      P.keyCount[j0 + i] = ct[i].x;
+      evSum += ct[i].x;
     }
     if (P.fuseSort) {
       // Sort the unit's targets while their records are recent: wave w
       // takes targets w + 16 k (k < 64, as m <= 1024), its count held in
       // lane k, since the waves' sort areas overlay the window and counts.
       const uint32_t mi = wave + kWindowWaves * lane;
-      const uint32_t held = mi < m ? cnt[mi] : 0u;
+      const uint32_t held = mi < m ? ct[mi].x : 0u;
       __syncthreads();
       uint8_t *area = (uint8_t *)lds + (size_t)wave * fused_sort_stride();
       for (uint32_t k = 0; wave + k * kWindowWaves < m; ++k) {

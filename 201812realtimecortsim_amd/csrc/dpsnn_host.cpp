@@ -976,6 +976,11 @@ void build_window(const Params &p, const Connectome &c, uint32_t unitsPerCol, ui
     }
   });
   for (size_t s = 0; s < streams; ++s) L.chunk[s + 1] += L.chunk[s];
+  if (L.chunk[streams] * C >= (uint64_t(1) << 32)) {  // the device indexes items with 32 bits
+    L.why = "more than 2^32 window items";
+    L.chunk.clear();
+    return;
+  }
   L.item.assign(L.chunk[streams] * C, (uint32_t)(slots * npc) << 16);
   L.desc.assign(L.chunk[streams], 0);
   parallel(L.units, [&](uint32_t u) {
@@ -1000,7 +1005,7 @@ void build_window(const Params &p, const Connectome &c, uint32_t unitsPerCol, ui
             L.item[at] = widx << 16 | (uint16_t)c.in_w[k];
           }
           const uint64_t nch = (e - b + C - 1) / C;
-          for (uint64_t x = 0; x < nch; ++x) L.desc[ch + x] = (uint16_t)q;
+          for (uint64_t x = 0; x < nch; ++x) L.desc[ch + x] = (uint16_t)(q | (x == 0 ? kWindowFirst : 0u));
           ch += nch;
         }
       }

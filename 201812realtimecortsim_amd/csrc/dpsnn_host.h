@@ -141,7 +141,8 @@ void build_delivery(const Params &p, Connectome &c, bool with_incoming = true, u
 //   item:  window index << 16 | (uint16) weight, window index =
 //          slot * npc + (source - first neuron of the slot's column);
 //          padding items carry window index slots * npc (never live)
-//   desc:  per chunk, the index of its target within the unit
+//   desc:  per chunk, the index of its target within the unit, plus
+//          kWindowFirst on the target's first chunk of the stream
 //   chunk: [units][passes][kWindowWaves] + 1, first chunk of each stream
 //   cols:  [columns][slots] source column of each stencil slot, or -1
 // A target's live items of all passes, taken in stream order, are its live
@@ -149,6 +150,8 @@ void build_delivery(const Params &p, Connectome &c, bool with_incoming = true, u
 constexpr uint32_t kWindowWaves = 16;
 constexpr uint32_t kWindowChunk = 8;
 constexpr uint32_t kWindowMaxUnit = 1024;
+constexpr uint32_t kWindowFirst = 0x8000;   // desc flag: the target's first chunk in the stream
+constexpr uint32_t kWindowTarget = 0x7FFF;  // desc mask: the target's index within the unit
 struct WindowLayout {
   bool ok = false;
   std::string why;               // why the layout was not built

@@ -241,7 +241,9 @@ def _check_window(pkg, host, units_per_col):
                     b, e = int(off[j]), int(off[j + 1])
                     seg = np.arange(b, e)[dly[b:e] == d]
                     nch = (len(seg) + 7) // 8
-                    assert (desc[ch:ch + nch] == q).all()
+                    assert ((desc[ch:ch + nch] & 0x7FFF) == q).all()
+                    # the target's first chunk in the stream is flagged, the others not
+                    assert nch == 0 or (desc[ch] & 0x8000 and not (desc[ch + 1:ch + nch] & 0x8000).any())
                     got = item[ch * 8:ch * 8 + len(seg)]
                     widx = got >> 16
                     src = stencil[widx // npc] * npc + widx % npc
