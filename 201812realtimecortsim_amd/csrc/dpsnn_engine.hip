@@ -623,18 +623,44 @@ constexpr int kRun = 16;   // events per neuron run in the blocked scratch
 // queue (times, values: [kQueue][64] doubles each)
 constexpr size_t kStageBytes = kChunk * kWave * sizeof(uint64_t) + 2 * kQueue * kWave * sizeof(double);
 
+// A buffer view of `bytes` (< 2^31) bytes at base (raw: stride 0, byte
+// offsets; the gfx9 descriptor word).  An access at kDropOffset is out of
+// range: a load returns zero and a store is dropped, without touching
+// memory and without a branch.
+typedef unsigned int u32x2 __attribute__((ext_vector_type(2)));
+constexpr uint32_t kDropOffset = 0x80000000u;
+__device__ inline __amdgpu_buffer_rsrc_t buffer_view(const void *base, uint32_t bytes) {
+  return __builtin_amdgcn_make_buffer_rsrc(const_cast<void *>(base), (short)0, (int)bytes, 0x00020000);
+}
+// A pointer that is the same in every lane of the wave, as a scalar value
+// (a buffer view needs its base in scalar registers).
+template <class T>
+__device__ inline T *wave_uniform(T *p) {
+  const uint64_t v = (uint64_t)p;
+  const uint32_t lo = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)v);
+  const uint32_t hi = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)(v >> 32));
+  return (T *)(((uint64_t)hi << 32) | lo);
+}
+
 // Chunk c (events 8c..8c+7 of every lane) from the tile's blocked scratch
 // [i / kRun][q][i % kRun]: each neuron's 8 ticks are one 64-byte piece of
 // its run, loaded by the wave in 8 coalesced instructions (instruction r:
 // neurons 8r..8r+7) and transposed through LDS; each lane's 8 weights are
-// one 16-byte load.  Entries past a neuron's count are never consumed.
+// one 16-byte load.  Only the pieces of neurons with events in the chunk
+// (bit q of act) are fetched; the others read as zero from an
+// out-of-range offset.  Entries past a neuron's count are never consumed.
 static_assert(kChunk == 8 && kRun % kChunk == 0, "chunks are whole pieces of a run");
-__device__ inline void load_chunk(const uint64_t *tileK, const int16_t *tileW, uint32_t c, uint64_t (&rk)[kChunk],
-                                  uint4 &rw, bool packed) {
+__device__ inline void load_chunk(const __amdgpu_buffer_rsrc_t &tileK, const int16_t *tileW, uint32_t c,
+                                  uint64_t act, uint64_t (&rk)[kChunk], uint4 &rw, bool packed) {
   const uint32_t lane = threadIdx.x & 63;
-  const size_t blk = (size_t)(c * kChunk / kRun) * (kWave * kRun) + (c * kChunk) % kRun;
+  const uint32_t blk = (c * kChunk / kRun) * (kWave * kRun) + (c * kChunk) % kRun;
+  const uint64_t mine = act >> (lane >> 3);  // bit 8r: neuron 8r + lane / 8 has events in chunk c
 #pragma unroll
-  for (int r = 0; r < kChunk; ++r) rk[r] = tileK[blk + (size_t)(r * 8 + (lane >> 3)) * kRun + (lane & 7)];
+  for (int r = 0; r < kChunk; ++r) {
+    const uint32_t off = (blk + (uint32_t)(r * 8 + (lane >> 3)) * kRun + (lane & 7)) * 8u;
+    const u32x2 v = __builtin_amdgcn_raw_buffer_load_b64(tileK, ((mine >> (8 * r)) & 1u) ? off : kDropOffset, 0, 0);
+    rk[r] = (uint64_t)v.x | ((uint64_t)v.y << 32);
+  }
   if (!packed) rw = *(const uint4 *)(tileW + blk + (size_t)lane * kRun);
 }
 __device__ inline void store_chunk(uint64_t *stageK, const uint64_t (&rk)[kChunk]) {
@@ -710,15 +736,16 @@ __device__ __attribute__((always_inline)) inline uint32_t integrate_wave(const D
   uint64_t rk[kChunk];
   uint4 wCur = make_uint4(0, 0, 0, 0), wNext = wCur;
   const bool packed = PK == 2 ? t >= kPackT : PK == 1;
+  const __amdgpu_buffer_rsrc_t tileR = buffer_view(wave_uniform(tileK), P.cap * kWave * (uint32_t)sizeof(uint64_t));
   if (chunks) {
-    load_chunk(tileK, tileW, 0, rk, wCur, packed);
+    load_chunk(tileR, tileW, 0, __ballot(n > 0), rk, wCur, packed);
     store_chunk(stageK, rk);
     wave_sync();
   }
   for (uint32_t c = 0; c < iters; ++c) {
     const uint64_t *curK = stageK;
     const bool more = c + 1 < chunks;
-    if (more) load_chunk(tileK, tileW, c + 1, rk, wNext, packed);
+    if (more) load_chunk(tileR, tileW, c + 1, __ballot(n > (c + 1) * kChunk), rk, wNext, packed);
     uint64_t wlo = (uint64_t)wCur.x | ((uint64_t)wCur.y << 32), whi = (uint64_t)wCur.z | ((uint64_t)wCur.w << 32);
     const uint32_t left = n > c * kChunk ? min((uint32_t)kChunk, n - c * kChunk) : 0u;
     const bool last = c * kChunk + left >= n;
@@ -1297,7 +1324,7 @@ __global__ __launch_bounds__(kWave) void k_gather(DevNet P, uint32_t t, int ltd)
 // ---------------------------------------------------------------------------
 constexpr int kCollectThreads = kWindowWaves * kWave;
 constexpr uint32_t kWindowMinNeurons = 1u << 16;  // the window form by default from this size
-constexpr uint32_t kWindowUnitsWanted = 1024;     // units per step: about four per CU
+constexpr uint32_t kWindowUnitsWanted = 2304;     // units per step: about nine per CU (a short queue tail)
 constexpr uint32_t kWindowMaxEntries = 12 * 1024;  // window entries: kWinPerThread per thread of k_collect
 
 // Block-wide work queue: the next unit for the whole workgroup (one unit
@@ -1340,15 +1367,6 @@ __host__ __device__ inline size_t collect_lds(uint32_t entries, uint32_t unit) {
 // a pass's emission times at once.
 constexpr int kWinPerThread = 12;
 constexpr int kCollectBatches = 4;  // 64-item batches per round of loads (the next round is in flight too)
-
-// A buffer view of `bytes` bytes at base (raw: stride 0, byte offsets;
-// the gfx9 descriptor word).  A store at an offset past `bytes` is dropped
-// by the hardware.
-typedef unsigned int u32x2 __attribute__((ext_vector_type(2)));
-constexpr uint32_t kDropOffset = 0xFFFFFFFFu;
-__device__ inline __amdgpu_buffer_rsrc_t buffer_view(void *base, uint32_t bytes) {
-  return __builtin_amdgcn_make_buffer_rsrc(base, (short)0, (int)bytes, 0x00020000);
-}
 
 template <int B, int CH>
 __device__ __attribute__((always_inline)) inline void sort_records(uint32_t t, uint32_t n, uint8_t *lds, uint32_t cap,
@@ -2964,7 +2982,10 @@ int dpsnn_prepare(const char *inidir, const dpsnn_options *opt, dpsnn_net **out)
     bool want = n->con.N >= kWindowMinNeurons;
     if (const char *e = std::getenv("DPSNN_WINDOW")) want = std::atoi(e) != 0;
     if (want) {
-      const uint32_t unitsPerCol = std::max<uint32_t>(1, (kWindowUnitsWanted + q.globCFT - 1) / q.globCFT);
+      // DPSNN_WINDOW_UNITS: units wanted per step (tuning; results do not change)
+      const char *ue = getenv("DPSNN_WINDOW_UNITS");
+      const uint32_t wanted = ue && atoi(ue) > 0 ? (uint32_t)atoi(ue) : kWindowUnitsWanted;
+      const uint32_t unitsPerCol = std::max<uint32_t>(1, (wanted + q.globCFT - 1) / q.globCFT);
       build_window(n->par, n->con, unitsPerCol, kWindowMaxEntries, n->window, n->opt.host_threads);
     }
   }
