@@ -2538,20 +2538,34 @@ static int post_step(dpsnn_net *n, StepPlan &sp) {
 // scan's outputs in between).  The scan of step 0 finds no spikes.  P is
 // the network's device view with the key scratch the kernels use (the
 // network's own, or the allocation-free step's static one).
+// Whether step sp of n runs phase 1 in the window form: split form, a
+// window layout on the device, packed records (t >= kPackT), no LTD.
+static bool window_step(const dpsnn_net *n, const StepPlan &sp) {
+  return n->split && n->winOn && sp.t >= kPackT && !sp.ltd;
+}
+
+// The launches of a window-form step of n on `stream`, with the device view
+// P (the network's own, or one whose key scratch is a batch's).
+static void launch_window_step(const dpsnn_net *n, const DevNet &P, const StepPlan &sp, hipStream_t stream) {
+  const KernelShape &K = kShapes[n->shape];
+  hipLaunchKernelGGL(k_collect, dim3(n->winGrid), dim3(kCollectThreads), n->winLds, stream, P, sp.t);
+  if (!P.fuseSort)
+    hipLaunchKernelGGL(K.sort, dim3(n->sortGrid), dim3(kWave), phase1_lds(kSortCap, K.B), stream, P, sp.t);
+  hipLaunchKernelGGL(K.sortBig, dim3(n->sortBigGrid), dim3(kWave), phase1_lds(P.cap, K.B), stream, P, sp.t);
+  const uint32_t blocks = (P.N + kBlock - 1) / kBlock;
+  hipLaunchKernelGGL(k_integrate<1>, dim3(blocks), dim3(kBlock), 0, stream, P, sp.t, sp.X, sp.S);
+  if (sp.S.ltp) hipLaunchKernelGGL(k_ltp, dim3(n->stdpBlocks), dim3(kWaveLTP), 0, stream, P, sp.t + 1);
+}
+
 static int one_step(dpsnn_net *n, const DevNet &P) {
   StepPlan sp;
   int rc;
   if ((rc = pre_step(n, sp)) || (rc = prof_begin(n))) return rc;
   const KernelShape &K = kShapes[n->shape];
-  if (n->split) {
-    if (n->winOn && sp.t >= kPackT && !sp.ltd) {
-      hipLaunchKernelGGL(k_collect, dim3(n->winGrid), dim3(kCollectThreads), n->winLds, n->stream, P, sp.t);
-      if (!P.fuseSort)
-        hipLaunchKernelGGL(K.sort, dim3(n->sortGrid), dim3(kWave), phase1_lds(kSortCap, K.B), n->stream, P, sp.t);
-      hipLaunchKernelGGL(K.sortBig, dim3(n->sortBigGrid), dim3(kWave), phase1_lds(P.cap, K.B), n->stream, P, sp.t);
-    } else {
-      hipLaunchKernelGGL(K.gather, dim3(n->gatherGrid), dim3(kWave), n->ldsBytes, n->stream, P, sp.t, sp.ltd);
-    }
+  if (window_step(n, sp)) {
+    launch_window_step(n, P, sp, n->stream);
+  } else if (n->split) {
+    hipLaunchKernelGGL(K.gather, dim3(n->gatherGrid), dim3(kWave), n->ldsBytes, n->stream, P, sp.t, sp.ltd);
     const uint32_t blocks = (P.N + kBlock - 1) / kBlock;
     if (sp.t >= kPackT)
       hipLaunchKernelGGL(k_integrate<1>, dim3(blocks), dim3(kBlock), 0, n->stream, P, sp.t, sp.X, sp.S);
@@ -3427,6 +3441,7 @@ struct dpsnn_batch {
   size_t evUsed = 0;
   std::vector<StepPlan> plans;
   std::vector<dpsnn_net *> active;
+  std::vector<uint32_t> windowed;  // this step's members (indices into active) in the window form
 };
 
 // Scratch the split form may use for one chunk of tiles.
@@ -3624,18 +3639,24 @@ int batch_one_step(dpsnn_batch *b, bool *any) {
   b->ringNext = (r + 1) % dpsnn_batch::kRing;
   if (b->pending[r]) HIP_TRY(hipEventSynchronize(b->done[r]));
   b->plans.resize(b->active.size());
-  uint32_t tiles = 0;
+  b->windowed.clear();
+  uint32_t tiles = 0, count = 0;
   bool packed = true;
   for (size_t i = 0; i < b->active.size(); ++i) {
     dpsnn_net *n = b->active[i];
     StepPlan &sp = b->plans[i];
     if ((rc = pre_step(n, sp))) return rc;
+    // a member whose step runs in the window form, and whose tiles fit the
+    // batch's key scratch, steps on its own launches (below)
+    if (window_step(n, sp) && (size_t)n->dev.NW * kWave <= b->scratchSlots) {
+      b->windowed.push_back((uint32_t)i);
+      continue;
+    }
     packed = packed && sp.t >= kPackT;
     const uint32_t idx = (uint32_t)(std::find(b->nets.begin(), b->nets.end(), n) - b->nets.begin());
-    b->h_items[r][i] = BatchItem{sp.t, sp.ltd, sp.S, sp.X, tiles, idx};
+    b->h_items[r][count++] = BatchItem{sp.t, sp.ltd, sp.S, sp.X, tiles, idx};
     tiles += n->dev.NW;
   }
-  const uint32_t count = (uint32_t)b->active.size();
   if (b->profile) {
     if (b->evUsed == b->ev.size()) {
       hipEvent_t x, y;
@@ -3645,11 +3666,24 @@ int batch_one_step(dpsnn_batch *b, bool *any) {
     }
     HIP_TRY(hipEventRecord(b->ev[b->evUsed].first, b->stream));
   }
-  if (b->useGraph) {
+  if (count && b->useGraph) {
     if ((rc = batch_graph_launch(b, r, count, tiles, packed))) return rc;
-  } else {
+  } else if (count) {
     if ((rc = batch_launch(b, r, count, tiles, packed))) return rc;
   }
+  // window-form members, one after another on the batch's stream and key
+  // scratch (its cap is the largest member's, so their tiles fit it)
+  for (uint32_t i : b->windowed) {
+    const dpsnn_net *n = b->active[i];
+    DevNet P = n->dev;
+    P.keys = b->keys;
+    P.keyW = b->keyW;
+    P.keyCount = b->keyCount;
+    P.cap = b->scratchCap;
+    P.tDev = nullptr;
+    launch_window_step(n, P, b->plans[i], b->stream);
+  }
+  HIP_TRY(hipGetLastError());
   if (b->profile) HIP_TRY(hipEventRecord(b->ev[b->evUsed++].second, b->stream));
   HIP_TRY(hipEventRecord(b->done[r], b->stream));
   b->pending[r] = true;

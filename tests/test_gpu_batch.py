@@ -1,8 +1,9 @@
 """Batching: independent networks of uneven size advancing in one launch per
 step give, member by member, the same results as each network stepped alone
 (bit-exact state, counters, raster/rates files, STDP synapses), also when
-members join at different ms, when the launch shape changes mid-run and
-after the autotuner's trial steps."""
+members join at different ms, when the launch shape changes mid-run, after
+the autotuner's trial steps, and when members step in the window form
+inside the batch."""
 import os
 
 import numpy as np
@@ -53,7 +54,13 @@ def _same(a, b):
             assert np.array_equal(a["syn"][k].view(np.uint8), b["syn"][k].view(np.uint8)), k
 
 
-def test_batch_members_match_alone(pkg, tmp_path):
+@pytest.mark.parametrize("window", ["default", "1"])
+def test_batch_members_match_alone(pkg, tmp_path, monkeypatch, window):
+    """window 1: every member has a window layout, so from ms 32 on the
+    members without LTD in a step run it in the window form on the batch's key
+    scratch, beside the batched gather form of the others."""
+    if window == "1":
+        monkeypatch.setenv("DPSNN_WINDOW", "1")
     alone = []
     for i, spec in enumerate(MEMBERS):
         out = tmp_path / f"alone{i}"

@@ -2,10 +2,11 @@
 
 * The 64-network batching mix (grid side 2-12 columns of 1250 neurons, own
   seeds, bath 2-6 Hz per input, STDP on half; 5.2 M neurons and 5.2 G
-  synapses in all) for 30
-  batched steps: every member's raster invariants and spike counter, the
-  event counter of three members recomputed from their backward lists, and
-  two members built again and stepped alone, bit for bit.
+  synapses in all) for 40 batched steps (past ms 32, where large members
+  without LTD switch to the window form): every member's raster invariants
+  and spike counter, the event counter of four members recomputed from their
+  backward lists, and three members (two small, the largest non-plastic one)
+  built again and stepped alone, bit for bit.
 * A slice of the serving mix (the first 16 arrivals of its job stream, grid
   side 1-16, 30 ms each) through the server: every job's raster
   file through the same invariants and its event counter recomputed; two
@@ -23,6 +24,7 @@ from helpers import check_raster, delivered_events, read_raster, sha256_file
 pytestmark = pytest.mark.gpu
 
 T = 30
+T_BATCH = 40  # past ms 32: members without LTD step in the window form from there
 
 
 def _w():
@@ -39,6 +41,8 @@ def test_batching_mix_64_networks(pkg, tmp_path):
     mix = W.batch_mix(count=64)
     small = [i for i, m in enumerate(mix) if m["side"] <= 4]
     sample = [next(i for i in small if mix[i]["stdp"]), next(i for i in small if not mix[i]["stdp"])]
+    # the largest non-plastic member: the window form inside the batch
+    sample.append(max((i for i in range(len(mix)) if not mix[i]["stdp"]), key=lambda i: mix[i]["side"]))
     largest = max(range(len(mix)), key=lambda i: mix[i]["side"])
     with_syn = set(sample) | {largest}
     specs = []
@@ -57,14 +61,14 @@ def test_batching_mix_64_networks(pkg, tmp_path):
         with pkg.Batch() as b:
             for n in nets:
                 b.add(n)
-            for t in range(T):
+            for t in range(T_BATCH):
                 b.step(1)
                 for i, n in enumerate(nets):
                     records[i].append(n.last_spikes())
             b.sync()
             for n in nets:
                 b.remove(n)
-        print(f"{T} batched steps done", flush=True)
+        print(f"{T_BATCH} batched steps done", flush=True)
         stats = [n.stats() for n in nets]
         # spikes of the last dynamics: counted, reported by the next scan
         pending = [int(n.state()["fired"].sum()) for n in nets]
@@ -74,7 +78,7 @@ def test_batching_mix_64_networks(pkg, tmp_path):
         for n in nets:
             n.close()
     for i in range(len(nets)):
-        assert stats[i]["steps"] == T
+        assert stats[i]["steps"] == T_BATCH
         assert check_raster(records[i], sizes[i]) + pending[i] == stats[i]["spikes"], i
     assert sum(s["spikes"] for s in stats) > 0
     for i in sorted(with_syn):
@@ -85,7 +89,7 @@ def test_batching_mix_64_networks(pkg, tmp_path):
     for i in sample:
         d, kw = specs[i]
         with pkg.Network(d, **kw) as alone:
-            alone.step(T)
+            alone.step(T_BATCH)
             alone.sync()
             st = alone.stats()
             for k in ("spikes", "synaptic_events", "external_events", "steps"):
