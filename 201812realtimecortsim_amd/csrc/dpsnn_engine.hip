@@ -3445,7 +3445,7 @@ struct dpsnn_batch {
 };
 
 // Scratch the split form may use for one chunk of tiles.
-constexpr size_t kBatchScratchBytes = size_t(4) << 30;
+constexpr size_t kBatchScratchBytes = size_t(5) << 30;  // also holds a 16 x 16-column member in the window form
 
 namespace {
 

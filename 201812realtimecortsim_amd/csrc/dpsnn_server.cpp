@@ -38,7 +38,7 @@ using Clock = std::chrono::steady_clock;
 
 // Device memory of the batch itself (key scratch of the longest in-list at
 // the largest chunk, items, graphs), kept out of the jobs' budget.
-constexpr uint64_t kBatchReserve = uint64_t(9) << 29;  // 4.5 GiB
+constexpr uint64_t kBatchReserve = uint64_t(11) << 29;  // 5.5 GiB
 
 struct Job {
   std::string inidir, outdir, env;
