@@ -572,6 +572,7 @@ struct ExtPhiloxCursor {
 };
 
 constexpr int kQueue = 4;  // Philox bath events per lane drawn ahead of the merge
+typedef __attribute__((address_space(3))) double lds_f64;
 
 // The Philox bath with the step's first events drawn ahead (fill, all lanes
 // of the wave together, before the merge): up to kQueue (time, value) pairs
@@ -580,7 +581,7 @@ constexpr int kQueue = 4;  // Philox bath events per lane drawn ahead of the mer
 // consumed in the same order either way, so the events are the same.
 struct ExtQueuedCursor {
   ExtPhiloxCursor xp;
-  double *qt, *qv;
+  lds_f64 *qt, *qv;  // LDS-typed: a generic pointer here compiled to flat loads, which wait on global loads
   uint32_t qi, qn;
   double head;  // time of the next due event, or kNoEvent
   __device__ inline void fill(bool on) {
@@ -721,7 +722,7 @@ __device__ __attribute__((always_inline)) inline uint32_t integrate_wave(const D
     }
   }
   if (!REPLAY) {
-    double *q = (double *)(stage + kChunk * kWave * sizeof(uint64_t));
+    lds_f64 *q = (lds_f64 *)(stage + kChunk * kWave * sizeof(uint64_t));
     xq.qt = q + lane;
     xq.qv = q + kQueue * kWave + lane;
     xq.fill(on);
