@@ -15,9 +15,12 @@ Common neuron / synapse parameters:
   J ~ N(0.6, 0.06) mV from excitatory, N(-2.4, 0.24) mV from inhibitory
   sources; stored integer delays 0..9 (1-10 ms, uniform) exc, 0 (1 ms) inh;
   connection probability exp(-d) over a 3x3 column stencil (lambda = 1
-  column), normalised so that the expected out-degree is 1125: a 1x1 grid
-  keeps it inside the column, in a 2x2 grid every column is a corner (4
-  reachable columns), in larger grids an interior column sees all 9 cells.
+  column), normalised so that the expected out-degree averaged over the
+  grid's columns is 1125: a 1x1 grid keeps it inside the column, in a 2x2
+  grid every column is a corner (4 reachable columns), in larger grids
+  corner, edge and interior columns reach 4, 6 and 9 cells (the reference's
+  generator drops stencil cells past the grid's edges), so an interior
+  column's out-degree is somewhat above 1125 and an edge column's below.
 """
 import math
 import os
@@ -63,7 +66,15 @@ def connectivity_txt(side, npc=1250, out_degree=1125.0):
         if side == 2:
             mass = sum(p.values())
         else:
-            mass = p[(0, 0)] + 2 * p[(1, 0)] + 2 * p[(0, 1)] + 4 * p[(1, 1)]
+            # the stencil mass a column reaches, averaged over the grid
+            reach = 0.0
+            for x in range(side):
+                for y in range(side):
+                    for dx in (-1, 0, 1):
+                        for dy in (-1, 0, 1):
+                            if 0 <= x + dx < side and 0 <= y + dy < side:
+                                reach += p[(abs(dx), abs(dy))]
+            mass = reach / (side * side)
     scale = out_degree / npc / mass
     vals = [cells[(x, y)] * scale for y in range(2) for x in range(2)]
     lines = []
