@@ -1,7 +1,7 @@
 """The batching and serving workloads at their own sizes.
 
 * The 64-network batching mix (grid side 2-12 columns of 1250 neurons, own
-  seeds, bath 2-6 Hz per input, STDP on half; 5.2 M neurons and 5.2 G
+  seeds, bath 2-6 Hz per input, STDP on half; 5.2 M neurons and 5.8 G
   synapses in all) for 40 batched steps (past ms 32, where large members
   without LTD switch to the window form): every member's raster invariants
   and spike counter, the event counter of four members recomputed from their
