@@ -103,7 +103,7 @@ __device__ inline XcdWalk xcd_walk(uint32_t total) {
 // kXcds counters that are zero at the launch.  Each counter sits on its own
 // 128-byte line (ctr[x * kCtrStride]); an exhausted counter is recognised by
 // a plain read before any atomic.  The counters live in device-coherent
-// memory, where each update costs a round trip: with runs of 4, configs[1]'s
+// memory, where each update costs a round trip: with runs of 4, the 24x24-column network's
 // phase 1 was 9% slower than with the guided runs, with runs of 1 twice as
 // slow.
 constexpr uint32_t kGrabMin = 4, kGrabMax = 16;
@@ -2029,7 +2029,7 @@ static int build_device(dpsnn_net *n) {
   while (P.RS < c.max_delay + 2) P.RS <<= 1;
   // phase 1 addresses the spike history with 32-bit byte offsets and 24-bit
   // neuron indices
-  if (c.N >= (1u << 24)) return fail(DPSNN_ERR_CONFIG, "%u neurons; this build handles fewer than 2^24", c.N);
+  if (c.N >= (1u << 24)) return fail(DPSNN_ERR_CONFIG, "%u neurons; the engine handles fewer than 2^24", c.N);
   if (uint64_t(P.RS) * c.N * sizeof(double) > (1ull << 32))
     return fail(DPSNN_ERR_CONFIG, "spike history (%u slots x %u neurons) larger than 4 GiB", P.RS, c.N);
   P.nsub = q.subPopNumber;
@@ -2087,7 +2087,7 @@ static int build_device(dpsnn_net *n) {
   HIP_TRY(hipGetLastError());
   // in-lists (CSR in delivery insertion order)
   if (c.max_in >= kMaxInList)
-    return fail(DPSNN_ERR_CONFIG, "a neuron has %u in-list items; this build handles up to %u", c.max_in,
+    return fail(DPSNN_ERR_CONFIG, "a neuron has %u in-list items; the engine handles up to %u", c.max_in,
                 kMaxInList - 1);
   n->items = c.in_off[N];
   {
@@ -2988,7 +2988,7 @@ int dpsnn_prepare(const char *inidir, const dpsnn_options *opt, dpsnn_net **out)
     for (uint64_t it = 0; it < c.in_off[c.N]; ++it) n->inSynLocal[it] = (uint16_t)rank[c.in_syn[it]];
   }
   if (n->con.max_in >= kMaxInList)
-    return fail(DPSNN_ERR_CONFIG, "a neuron has %u in-list items; this build handles up to %u", n->con.max_in,
+    return fail(DPSNN_ERR_CONFIG, "a neuron has %u in-list items; the engine handles up to %u", n->con.max_in,
                 kMaxInList - 1);
   // stencil-window phase 1: on for large networks; DPSNN_WINDOW=1 forces
   // it on (tests of small networks), 0 off.  A connectome whose stencil

@@ -143,7 +143,7 @@ def batch_mix(count=64, seed=20181203):
 def serve_jobs(rate_per_s=20.0, window_s=120.0, seed=20181204):
     """The serving workload: Poisson arrivals at rate_per_s over window_s;
     each job a network of side 1..16 columns (1250 neurons each), 0.5-5 s
-    simulated, STDP with probability 0.3, the bath of configs[0]."""
+    simulated, STDP with probability 0.3, the bath of the 2x2-column network."""
     rng = np.random.default_rng(seed)
     t, jobs = 0.0, []
     while True:

@@ -1,12 +1,12 @@
 """Benchmark of the MI355X engine on DPSNN's per-millisecond LIFCA step.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--config c1]
-    python bench.py --mode batch [--steps K] [--warmup W]   (configs[3])
-    python bench.py --mode serve [--serve-window S]          (configs[4])
+    python bench.py --mode batch [--steps K] [--warmup W]   (the batching mix)
+    python bench.py --mode serve [--serve-window S]          (the serving stream)
     python bench.py --mode copy                              (counter calibration)
 
 One "step" = one simulated millisecond of the whole local network on one GPU
-(the reference's completeTimeStep_ms).  Default workload: configs[1], a
+(the reference's completeTimeStep_ms).  Default workload: the 24x24-column network, a
 24x24 grid of 1250-neuron columns (720,000 neurons, ~810 M synapses),
 on-device Philox external drive, no raster files; synthetic network from
 the reference generator (no external data).  Prints one JSON line:
@@ -22,7 +22,7 @@ the reference generator (no external data).  Prints one JSON line:
   cpu_baseline  the oracle (scalar C restatement of DPSNN's step) on one host
                 core, on the benchmarked network continued from the
                 engine's state after the timed steps (a few steps); beside
-                it configs[0] on the host: one oracle process per core of
+                it the 2x2-column network on the host: one oracle process per core of
                 this process's share, and the reference itself (built from
                 its sources into oracle/_ref) under mpirun -np 1
 """
@@ -94,7 +94,7 @@ def cpu_baseline_port(pkg, cfg, ckpt, backward, t_ckpt, steps):
 
 
 def cpu_baseline_c0(pkg, ms):
-    """configs[0] on the host: (a) the oracle's driver, one process per host
+    """the 2x2-column network on the host: (a) the oracle's driver, one process per host
     core of this process's share, all at once; (b) the reference itself
     (oracle/_ref, built from the reference's sources) at MPI world size 1,
     where that binary and mpirun are present.  Both run `ms` simulated ms;
@@ -127,7 +127,7 @@ def cpu_baseline_c0(pkg, ms):
     assert all(r[1] == ev for r in runs)
     wall = max(r[0] for r in runs)
     res = {"per_core": {"value": P * ev / wall, "unit": "synaptic events/s", "cores": P, "kind": "port",
-                        "sample": f"{P} concurrent oracle processes (dpsnn_cpu_ref), each configs[0] "
+                        "sample": f"{P} concurrent oracle processes (dpsnn_cpu_ref), each on the 2x2-column network "
                                   f"(5000 neurons) for {ms} ms after init; slowest step loop {wall:.1f} s"}}
     ref = os.path.join(ROOT, "oracle", "_ref", "LIFCA-DPSNN.out")
     mpirun = "/opt/conda/bin/mpirun"
@@ -145,7 +145,7 @@ def cpu_baseline_c0(pkg, ms):
             fire_s = float(line[0].rsplit("total fire time =", 1)[1].split()[0])
             res["reference"] = {"value": ev / fire_s, "unit": "synaptic events/s", "cores": 1, "kind": "reference",
                                 "sample": f"the reference (oracle/_ref, built from its sources) under mpirun -np 1 on "
-                                          f"configs[0] for {ms} ms: {fire_s:.1f} s of localNetProcess_fire "
+                                          f"the 2x2-column network for {ms} ms: {fire_s:.1f} s of localNetProcess_fire "
                                           f"(its own CHRONO line), events as counted by the oracle on the same "
                                           f"raster"}
     else:
@@ -158,7 +158,7 @@ def log(msg):
 
 
 def bench_batch(pkg, args):
-    """64 independent networks (grid side 2..12, own seeds, bath
+    """the batching mix: 64 independent networks (grid side 2..12, own seeds, bath
     2-6 Hz per input, STDP on half) advancing in one batched step; then the
     same networks stepped one at a time for the same number of steps."""
     import tempfile
@@ -262,7 +262,7 @@ def bench_batch(pkg, args):
         "ms_per_step": 1000.0 * wall / args.steps,
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
         "data": "synthetic (reference generator, seeded)",
-        "config": {"workload": f"configs[3] batching: {len(mix)} independent LIFCA networks, grid side 2-12 "
+        "config": {"workload": f"batching: {len(mix)} independent LIFCA networks, grid side 2-12 "
                                f"columns x 1250 neurons, own seeds, bath 2-6 Hz/input, STDP on "
                                f"{sum(x['stdp'] for x in mix)}; one batched step = 1 ms of every network",
                    "networks": len(mix), "neurons": neurons, "synapses": synapses, "device_bytes": dev_bytes,
@@ -320,7 +320,7 @@ def run_server(pkg, jobs, dirs, envs, args, max_running):
 
 
 def bench_serve(pkg, args):
-    """Poisson job arrivals (20 jobs/s for 120 s), each job a
+    """the serving stream: Poisson job arrivals (20 jobs/s for 120 s), each job a
     LIFCA network of 1-16 x 1-16 columns, 0.5-5 s simulated, STDP with
     probability 0.3, admitted under a 200 GB HBM budget with continuous
     batching; the same arrival stream served one job at a time is the plain
@@ -350,7 +350,7 @@ def bench_serve(pkg, args):
         "ms_per_step": None,
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
         "data": "synthetic (reference generator, seeded)",
-        "config": {"workload": f"configs[4] serving: Poisson arrivals {args.serve_rate:g} jobs/s for "
+        "config": {"workload": f"serving: Poisson arrivals {args.serve_rate:g} jobs/s for "
                                f"{args.serve_window:g} s, grid side 1-16 x 1250 neurons, 0.5-5 s simulated, "
                                f"STDP p=0.3, {args.serve_budget_gb:g} GB HBM admission budget, continuous batching",
                    "jobs_arrived": len(jobs), "offered_events_per_s": offered, "drive": "philox",
@@ -382,7 +382,7 @@ def traffic_from_profile(config):
     """HBM bytes per step of this workload from the committed counter summary
     (two rocprofv3 --pmc passes, FETCH_SIZE then WRITE_SIZE, scaled by the
     same counters read for a plain copy) -- measured by a profiled run of
-    this build, not by this run -- with where and over which steps it was
+    the engine, not by this run -- with where and over which steps it was
     measured; or None."""
     path = os.path.join(ROOT, "profiles", f"step_{config}_traffic.json")
     if not os.path.exists(path):
@@ -415,11 +415,11 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=2,
                     help="steps of the oracle on the benchmarked network (from the engine's checkpoint)")
     ap.add_argument("--cpu-c0-ms", type=int, default=40,
-                    help="simulated ms of the configs[0] host runs (one process per core; the reference)")
+                    help="simulated ms of the host runs of the 2x2-column network (one process per core; the reference)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
-        print("bench.py: this engine runs one network per GPU and this build measures one GPU; "
+        print("bench.py: the engine runs one network per GPU and measures one GPU; "
               "--gpus must be 1 (multi-GPU sharding is a design note, not a runnable mode)",
               file=sys.stderr)
         return 2
@@ -519,7 +519,7 @@ def main():
         os.remove(ck)
         os.rmdir(ck_dir)
         log(f"cpu baseline: {base['sample']}")
-        log(f"cpu baseline: configs[0], {args.cpu_c0_ms} ms, one process per core and the reference ...")
+        log(f"cpu baseline: the 2x2-column network, {args.cpu_c0_ms} ms, one process per core and the reference ...")
         base.update(cpu_baseline_c0(pkg, args.cpu_c0_ms))
         line["cpu_baseline"] = base
     print(json.dumps(line), flush=True)

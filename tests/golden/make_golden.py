@@ -8,11 +8,11 @@ small fixtures:
 
   c0_synapses.json  the BACKWARD synapse dump (delivery order): sha256, line
                     count, per-delay counts, head/tail and a strided sample
-  c0_raster.json    spikes-sec0 raster of configs[0] (1 s): sha256 of the
+  c0_raster.json    spikes-sec0 raster of the 2x2-column network (1 s): sha256 of the
                     whole file, line counts and sha256 of each prefix that
                     ends at 100/200/500/1000 ms, per-ms spike counts
   c0_ratesPerPop-sec0.dat   the reference's ratesPerPop file, verbatim
-  c2small_*.json / .dat     the same for configs[2]'s STDP at configs[0]'s
+  c2small_*.json / .dat     the same for the STDP network's plasticity at the 2x2-column
                     size (1.2 s), plus sha256 of the synPeriodicProbe A/B
                     dumps written at the end-of-second weight commit
 

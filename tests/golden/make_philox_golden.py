@@ -1,4 +1,4 @@
-"""Writes c0_philox.json: the raster, rates and counters of configs[0]'s
+"""Writes c0_philox.json: the raster, rates and counters of the 2x2-column network's
 network under the on-device Philox drive (seed 20181201), 500 ms.  The
 Philox drive has no counterpart in the reference (it replaces the host's
 ran3 bath with a counter-based stream of the same statistics), so this

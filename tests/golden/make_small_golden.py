@@ -4,7 +4,7 @@ The reference (DPSNN, built from its own sources by oracle/ref.mk into
 oracle/_ref/LIFCA-DPSNN.out) runs at MPI world size 1 on two networks that
 the other fixtures leave open:
 
-  djext       configs[0] (2 x 2 columns of 1250 neurons, seed 1) with the
+  djext       the 2x2-column network (2 x 2 columns of 1250 neurons, seed 1) with the
               bath efficacy spread DJExt = 0.1 in every sub-population
               (column.txt field 4), 100 ms: the Gaussian bath LUT's
               DJExt > 0 branch (erflib.c:141-169, used by the external drive
@@ -12,9 +12,9 @@ the other fixtures leave open:
               50 and 100 ms) and the ratesPerPop file verbatim.
               Also the connection-probability matrices the reference
               prints at init (probConnectivityMatrix, DPSNN_LIFCAconnectome
-              .cpp:181-192, "%f", rows y, columns x) for configs[0]'s
+              .cpp:181-192, "%f", rows y, columns x) for the 2x2-column network's
               connectivity: the values writeIniFiles puts in PerseoConFile.
-  stdp_small  configs[2]'s STDP on a 2 x 2 grid of 125-neuron columns for
+  stdp_small  the STDP network's plasticity on a 2 x 2 grid of 125-neuron columns for
               1100 ms, across the end-of-second weight commit, with the
               synPeriodicProbe dumps on: raster summaries of both seconds,
               both rates files verbatim, sha256 of synPeriodicProbe A/B.
@@ -60,7 +60,7 @@ def _copy_ini(src, d, names):
 
 
 def write_djext(d):
-    """configs[0] with DJExt = 0.1; returns the run's env settings."""
+    """the 2x2-column network with DJExt = 0.1; returns the run's env settings."""
     _copy_ini("c0", d, ("connectivity.txt", "bathEfficacy.txt"))
     rows = []
     for line in open(os.path.join(CONFIGS, "c0", "column.txt")):
@@ -76,7 +76,7 @@ def write_djext(d):
 
 
 def write_stdp_small(d):
-    """configs[2]'s STDP on 2 x 2 columns of 125 neurons; returns the env."""
+    """the STDP network's plasticity on 2 x 2 columns of 125 neurons; returns the env."""
     _copy_ini("c2_small", d, ("connectivity.txt", "bathEfficacy.txt", "plasticity.txt"))
     with open(os.path.join(d, "column.txt"), "w") as f:
         f.write("0 50 0.5 0.0 3 20 20 15 2 1 500 0.05\n"

@@ -1,7 +1,7 @@
 """Writes tests/golden/stdpevent_small.json from a run of the reference.
 
 The reference (DPSNN, built from its own sources by oracle/ref.mk into
-oracle/_ref/LIFCA-DPSNN.out) runs configs[2]'s STDP on a 2 x 2 grid of
+oracle/_ref/LIFCA-DPSNN.out) runs the STDP network's plasticity on a 2 x 2 grid of
 125-neuron columns for 100 ms at MPI world size 1 with the STDPevent file on
 (env_statSTDPevent_writeDisable=0) and the reference's debug window at 60 ms
 (env_debugPrintEnable_ms=60: contributions of ms 60..80 are written,

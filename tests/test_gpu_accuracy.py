@@ -9,8 +9,8 @@ process loc_h = k would draw) -- against R Philox runs of different seeds.
 A statistic passes when the Philox mean is within 1% of the replay mean or,
 where the runs themselves spread more than that, within three standard
 errors of the difference of the two means (the spread is measured, not
-assumed).  configs[0] (5000 neurons, the full second) is dominated by that
-spread; on configs[1] (720,000 neurons, ms 100-300) the rates agree within
+assumed).  the 2x2-column network (5000 neurons, the full second) is dominated by that
+spread; on the 24x24-column network (720,000 neurons, ms 100-300) the rates agree within
 1% and the ISI CVs within a few percent.
 """
 import importlib
@@ -91,11 +91,11 @@ def _compare(name, pkg, cfg, t0, t1, runs, monkeypatch):
 def test_philox_matches_replay_on_c0_full_second(pkg, monkeypatch):
     cfg = pkg.config_dir("c0")
     pkg.apply_env(cfg, env_totalSimTime_ms=1001)
-    _compare("configs[0]", pkg, cfg, 0, 1000, 6, monkeypatch)
+    _compare("the 2x2-column network", pkg, cfg, 0, 1000, 6, monkeypatch)
 
 
 def test_philox_matches_replay_on_c1_window(pkg, monkeypatch):
     """ms 100-300, past the start-up transient."""
     cfg = pkg.config_dir("c1")
     pkg.apply_env(cfg, env_totalSimTime_ms=301)
-    _compare("configs[1]", pkg, cfg, 100, 300, 3, monkeypatch)
+    _compare("the 24x24-column network", pkg, cfg, 100, 300, 3, monkeypatch)

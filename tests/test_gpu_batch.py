@@ -104,7 +104,7 @@ def test_batch_members_match_alone(pkg, tmp_path, monkeypatch, window):
             n.close()
     for i, (a, g) in enumerate(zip(alone, got)):
         _same(a, g)
-    # member 2 is configs[0] (2x2 columns, seed 1, replay drive) cut at 320 ms:
+    # member 2 is the 2x2-column network (seed 1, replay drive) cut at 320 ms:
     # its batched raster and rates are the reference's
     g = golden("c0_raster.json")
     assert prefix_sha(outs[2] / "spikes-sec0-C4-H1-h000.dat", g["prefix"]["200"]["lines"]) == \

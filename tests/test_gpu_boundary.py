@@ -1,6 +1,6 @@
 """The drop-in boundary itself on the device: the reference's main loop
 (DPSNN_mainUnderMPI.c:33-102) as the built program lib/dpsnn_hip_main in a
-configs[0] run directory, and localNetProcess_init/fire/finish
+the 2x2-column network run directory, and localNetProcess_init/fire/finish
 (localNetProcess.c:40-173) called through ctypes.  Outputs must be the
 reference's files (golden fixtures)."""
 import os
@@ -37,7 +37,7 @@ def test_main_program_reproduces_reference_files(pkg, tmp_path):
     main = os.path.join(os.path.dirname(pkg.LIB_PATH), "dpsnn_hip_main")
     assert os.access(main, os.X_OK), main
     d, env = _run_dir(pkg, tmp_path, 500)
-    print("dpsnn_hip_main: configs[0], 500 ms", flush=True)
+    print("dpsnn_hip_main: the 2x2-column network, 500 ms", flush=True)
     proc = subprocess.run([main], cwd=str(d), env=env, stdout=subprocess.PIPE, stderr=subprocess.STDOUT,
                           text=True, timeout=300)
     assert proc.returncode == 0, proc.stdout[-2000:]
@@ -61,7 +61,7 @@ def test_process_interface_through_ctypes(pkg, tmp_path):
         os.environ.clear()
         os.environ.update(env)
         proc = pkg.LocalNetProcess(inidir=str(d), output_dir=str(d))
-        print("localNetProcess via ctypes: configs[0], 100 ms", flush=True)
+        print("localNetProcess via ctypes: the 2x2-column network, 100 ms", flush=True)
         proc.init()
         assert proc.parameters.totalSimTime_ms == 100 and proc.parameters.globCFT == 4
         codes = []

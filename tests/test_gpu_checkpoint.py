@@ -22,7 +22,7 @@ def test_c0_resumed_at_500ms_is_the_reference_second(pkg, tmp_path):
     cfg = pkg.config_dir("c0")
     pkg.apply_env(cfg, env_totalSimTime_ms=1000)
     ck = tmp_path / "c0.ckpt"
-    print("checkpoint: configs[0] 500 ms, checkpoint, destroy", flush=True)
+    print("checkpoint: the 2x2-column network 500 ms, checkpoint, destroy", flush=True)
     with pkg.Network(cfg, output_dir=str(tmp_path)) as net:
         net.step(500)
         net.checkpoint(ck)
@@ -42,7 +42,7 @@ def test_c0_resumed_at_500ms_is_the_reference_second(pkg, tmp_path):
 
 
 def test_stdp_resumed_across_the_commit(pkg, tmp_path):
-    """configs[2]'s STDP at configs[0]'s size, checkpointed at 700 ms and
+    """the STDP network's plasticity at the 2x2-column size, checkpointed at 700 ms and
     resumed: both seconds' rasters and the synapse probes around the
     t = 999 ms commit are the reference's."""
     g = golden("c2small_golden.json")

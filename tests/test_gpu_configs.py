@@ -1,11 +1,11 @@
 """The engine at the workloads' own sizes.
 
-* configs[0] and configs[1] (720,000 neurons, 777 M synapses): the oracle,
+* the 2x2-column network and the 24x24-column network (720,000 neurons, 810 M synapses): the oracle,
   built from the engine's backward synapse list, gives the engine's first
   steps; handed an engine checkpoint taken in the steady state (state,
   spike history, ran3 stream), it continues with the engine's next steps,
   state and spikes bit for bit.
-* configs[1] for 60 ms: the raster's invariants (ascending ids per step,
+* the 24x24-column network for 60 ms: the raster's invariants (ascending ids per step,
   emission times inside their step, refractory period), the engine's
   synaptic-event counter recomputed from the backward list and the spikes,
   and the ratesPerPop file recomputed from the raster.
@@ -81,7 +81,7 @@ def test_c0_checkpoint_continues_identically_in_oracle(pkg, oracle, tmp_path):
 
 
 def test_c1_steady_state_bit_exact_vs_oracle(pkg, oracle, tmp_path):
-    """configs[1] at full size: its first 3 steps and, past the start-up
+    """the 24x24-column network at full size: its first 3 steps and, past the start-up
     transient, the 2 steps after a checkpoint at ms 80 (stencil-window
     phase 1) are the oracle's."""
     cfg = pkg.config_dir("c1")
@@ -97,7 +97,7 @@ def test_c1_raster_invariants_and_event_count(pkg, tmp_path):
     out.mkdir()
     records = []
     with pkg.Network(cfg, drive=pkg.DRIVE_PHILOX, philox_seed=3, keep_spikes=True, output_dir=str(out)) as net:
-        print("configs[1]: 60 ms", flush=True)
+        print("the 24x24-column network: 60 ms", flush=True)
         for m in range(T):
             net.step(1)
             records.append(net.last_spikes())
@@ -109,7 +109,7 @@ def test_c1_raster_invariants_and_event_count(pkg, tmp_path):
     assert N == 720000 and st["synapses"] == len(syn["pre"]) and st["steps"] == T
     total = check_raster(records, N)
     assert total == st["spikes"] and total > 0
-    print(f"configs[1]: {total} spikes; event count from the backward list", flush=True)
+    print(f"the 24x24-column network: {total} spikes; event count from the backward list", flush=True)
     assert delivered_events(syn, N, [r[0] for r in records]) == st["synaptic_events"]
     del syn
     # ratesPerPop (DPSNN_stat.c:225-273): the line of step t (t % sampling

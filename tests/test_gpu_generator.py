@@ -118,7 +118,7 @@ def _rate(pkg, cfg, generator, ms, t0):
 @pytest.mark.parametrize("name,ms,t0,tol", [("c0", 400, 100, 0.10), ("c1", 200, 100, 0.03)])
 def test_device_generated_network_fires_like_the_reference_one(pkg, name, ms, t0, tol):
     """Population rate over ms t0..ms of a network from each generator (both
-    Philox-driven): within tol (the configs[0] network's own run-to-run
+    Philox-driven): within tol (the 2x2-column network's own run-to-run
     spread is about 6%)."""
     cfg = pkg.config_dir(name)
     pkg.apply_env(cfg, env_totalSimTime_ms=ms)

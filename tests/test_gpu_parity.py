@@ -12,7 +12,7 @@ pytestmark = pytest.mark.gpu
 
 
 def _tiny_config(tmp, pkg, counts=(50, 50, 25), grid=2, nu_ext=3.0):
-    """configs[0]'s parameters with smaller columns (the stencil keeps the
+    """the 2x2-column network's parameters with smaller columns (the stencil keeps the
     same exp(-d) shape and 0.9 total connection probability)."""
     src = pkg.config_dir("c0")
     d = tmp / "tiny"
@@ -168,7 +168,7 @@ def test_philox_drive_statistics(pkg, tmp_path):
 
 
 def test_stdp_c2small_identical_to_reference(pkg, tmp_path):
-    """configs[2]'s STDP at configs[0]'s size, 1.2 s: raster and rates of
+    """the STDP network's plasticity at the 2x2-column size, 1.2 s: raster and rates of
     both seconds and the synapse probes around the t = 999 ms weight commit
     are the reference's, byte for byte."""
     g = golden("c2small_golden.json")
@@ -195,8 +195,8 @@ def test_stdp_c2small_identical_to_reference(pkg, tmp_path):
 def test_phase1_general_path_identical_to_reference(pkg, tmp_path, monkeypatch, dense_max):
     """Phase 1's general path (register batches, gathered twice), for every
     neuron (0) or for those with more than 400 delivered events in a step
-    (400, both paths in one step): configs[0]'s first second and configs[2]'s
-    STDP run at configs[0]'s size are still the reference's, byte for byte."""
+    (400, both paths in one step): the 2x2-column network's first second and the STDP network's
+    STDP run at the 2x2-column size are still the reference's, byte for byte."""
     monkeypatch.setenv("DPSNN_PHASE1_DENSE_MAX", str(dense_max))
     g = golden("c0_raster.json")
     st = _run_c0(pkg, tmp_path, 1000)
@@ -235,7 +235,7 @@ def test_stdp_rejects_sub_ms_refractory(pkg, tmp_path):
 
 
 def test_philox_drive_regression(pkg, tmp_path):
-    """The on-device Philox bath reproduces its committed vector (configs[0],
+    """The on-device Philox bath reproduces its committed vector (the 2x2-column network,
     seed 20181201, 500 ms): raster, rates, counters and final state."""
     import importlib.util
     spec = importlib.util.spec_from_file_location("make_philox_golden",

@@ -1,5 +1,5 @@
 """The engine against the reference runs of small_golden.json: the bath
-LUT's DJExt > 0 branch on configs[0] and configs[2]'s STDP on a tiny grid
+LUT's DJExt > 0 branch on the 2x2-column network and the STDP network's plasticity on a tiny grid
 across the end-of-second commit, on both phase-1 forms."""
 import os
 

@@ -1,6 +1,6 @@
 """GPU parity of the stencil-window form of phase 1 (k_collect + k_sort,
 or the sort fused into k_collect; steps t >= 32 without LTD): forced onto
-configs[0] and small networks it must give the reference's raster and the
+the 2x2-column network and small networks it must give the reference's raster and the
 oracle's state bit for bit, like the gather form."""
 import os
 
@@ -36,7 +36,7 @@ def _columns(tmp, pkg, name, counts, src="c0", nu_ext=3.0):
                                                   ("1", "200", "1"), ("0", None, "0")])
 def test_c0_full_second_identical_to_reference(pkg, tmp_path, monkeypatch, window, sort_cap, fused):
     """sort_cap 200: targets with more than 200 records of a step (nearly all
-    of configs[0]'s) are sorted by the second sort launch.  fused 1: the sort
+    of the 2x2-column network's) are sorted by the second sort launch.  fused 1: the sort
     runs inside k_collect, per unit."""
     monkeypatch.setenv("DPSNN_WINDOW", window)
     monkeypatch.setenv("DPSNN_FUSED_SORT", fused)
@@ -45,7 +45,7 @@ def test_c0_full_second_identical_to_reference(pkg, tmp_path, monkeypatch, windo
     g = golden("c0_raster.json")
     cfg = pkg.config_dir("c0")
     pkg.apply_env(cfg, env_totalSimTime_ms=1000)
-    print(f"window={window} sort_cap={sort_cap} fused={fused}: configs[0] 1000 ms", flush=True)
+    print(f"window={window} sort_cap={sort_cap} fused={fused}: the 2x2-column network 1000 ms", flush=True)
     with pkg.Network(cfg, output_dir=str(tmp_path)) as net:
         while not net.step(100):
             pass
@@ -90,7 +90,7 @@ def test_window_state_trace_bit_exact_vs_oracle(pkg, oracle, tmp_path, monkeypat
 
 @pytest.mark.parametrize("window,fused", [("1", "0"), ("1", "1"), ("0", "0")])
 def test_grid24_small_columns_bit_exact_vs_oracle(pkg, oracle, tmp_path, monkeypatch, window, fused):
-    """configs[1]'s 24 x 24 grid and stencil (corner, edge and interior
+    """The 24 x 24 grid and stencil (corner, edge and interior
     columns; 576 columns through the work queues) with 40-neuron columns,
     past the packed-scratch step: state and spikes are the oracle's."""
     monkeypatch.setenv("DPSNN_WINDOW", window)

@@ -286,7 +286,7 @@ def test_window_layout_on_grid_edges(pkg, tmp_path):
 
 
 def test_perseo_ini_files(pkg, tmp_path):
-    """writeIniFiles (DPSNN_writeIniFiles.c:31-158) on configs[0]: every
+    """writeIniFiles (DPSNN_writeIniFiles.c:31-158) on the 2x2-column network: every
     PerseoConFile probability equals the matrix the reference prints at
     init (its stdout, "%f"; rows y, columns x), every column pair within the
     stencil appears once per sub-population pair, sources in column order

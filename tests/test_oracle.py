@@ -136,7 +136,7 @@ int main(void){ long bad=0,n=0; srand48(3);
 
 @pytest.mark.skipif(not os.environ.get("DPSNN_SLOW"), reason="5 minutes of CPU; set DPSNN_SLOW=1")
 def test_oracle_stdp_c2small_identical_to_reference(pkg, oracle, tmp_path):
-    """configs[2]'s STDP at configs[0]'s size, 1.2 s, with the synapse
+    """the STDP network's plasticity at the 2x2-column size, 1.2 s, with the synapse
     probes on: every output file equals the reference's."""
     g = golden("c2small_golden.json")
     cfg = pkg.config_dir("c2_small")
@@ -191,7 +191,7 @@ def _small(oracle, kind, tmp_path, monkeypatch):
 
 
 def test_oracle_djext_bath_matches_reference(oracle, tmp_path, monkeypatch):
-    """The Gaussian bath LUT's DJExt > 0 branch (erflib.c:141-169): configs[0]
+    """The Gaussian bath LUT's DJExt > 0 branch (erflib.c:141-169): the 2x2-column network
     with DJExt = 0.1, raster and rates of the first 50 ms."""
     g = golden("small_golden.json")["djext"]
     net, out = _small(oracle, "djext", tmp_path, monkeypatch)
@@ -205,7 +205,7 @@ def test_oracle_djext_bath_matches_reference(oracle, tmp_path, monkeypatch):
 
 
 def test_oracle_stdp_small_across_commit_matches_reference(oracle, tmp_path, monkeypatch):
-    """configs[2]'s STDP on 2 x 2 columns of 125 neurons for 1.1 s: both
+    """the STDP network's plasticity on 2 x 2 columns of 125 neurons for 1.1 s: both
     seconds' rasters and rates and the synapse probes around the
     end-of-second commit (DPSNN_localNet_sim.c:255-262) equal the
     reference's."""

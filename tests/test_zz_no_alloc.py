@@ -48,7 +48,7 @@ def _compare(pkg, inidir, steps, env):
 def test_noalloc_step_matches_step_then_smaller_network(pkg, tmp_path):
     src = pkg.config_dir("c0")
     pkg.apply_env(src)
-    # configs[0] (5000 neurons) past the packed-scratch step, then a network
+    # the 2x2-column network (5000 neurons) past the packed-scratch step, then a network
     # a quarter of its size on the same device
     big = _compare(pkg, src, 60, {"env_totalSimTime_ms": 60})
     small_dir = _config(tmp_path, pkg, "small", (250, 250, 125), 2)
@@ -62,7 +62,7 @@ def test_noalloc_step_refuses_what_it_cannot_hold(pkg, tmp_path):
     with pkg.Network(src, drive=pkg.DRIVE_REPLAY, env={"env_totalSimTime_ms": 10}) as net:
         with pytest.raises(pkg.DpsnnError, match="Philox"):
             net.step_noalloc(1)
-    # a 3x3 grid of configs[0]'s columns needs more key scratch than 128 MiB
+    # a 3x3 grid of 1250-neuron columns needs more key scratch than 128 MiB
     big_dir = _config(tmp_path, pkg, "big", (500, 500, 250), 3)
     with pkg.Network(big_dir, drive=pkg.DRIVE_PHILOX, env={"env_CFX": 3, "env_CFY": 3,
                                                           "env_totalSimTime_ms": 10}) as net:
