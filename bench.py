@@ -469,6 +469,9 @@ def main():
         del syn
     net.close()
     traffic = traffic_from_profile(args.config)
+    par = pkg.get_parameters(cfg)
+    grid, npc = f"{par['globCFX']}x{par['globCFY']}", par["neuronsPerCM"]
+    plastic = os.path.exists(os.path.join(cfg, "plasticity.txt"))
     events = b["synaptic_events"] - a["synaptic_events"]
     ext = b["external_events"] - a["external_events"]
     spikes = b["spikes"] - a["spikes"]
@@ -490,9 +493,10 @@ def main():
         "vs_baseline": None,
         "dtype": "f64",
         "data": "synthetic (reference generator, seeded)",
-        "config": {"workload": f"configs[{args.config[1:]}] plain LIFCA step, "
-                               f"{st0['neurons']} neurons, {st0['synapses']} synapses, "
-                               f"{args.drive} external drive",
+        "config": {"workload": f"plain LIFCA step, {grid} grid of {npc}-neuron columns "
+                               f"({st0['neurons']} neurons, {st0['synapses']} synapses)"
+                               f"{', pair-based STDP' if plastic else ''}, {args.drive} external drive "
+                               f"(config directory {args.config})",
                    "neurons": neurons, "synapses": st0["synapses"], "drive": args.drive,
                    "generator": args.generator,
                    "spikes_per_step": spikes / args.steps,
