@@ -131,3 +131,22 @@ def test_bad_job_fails_alone(pkg, tmp_path):
         b, g = srv.job(bad), srv.job(good)
     assert b["state"] == "failed" and b["status"] == pkg.ERR_CONFIG and b["message"]
     assert g["state"] == "done" and g["stats"]["steps"] == 50
+
+
+def test_host_backlog_bound_serves_every_job(pkg, tmp_path):
+    """With a host backlog bound of one byte a builder starts a build only
+    when no built job waits for its upload: the server does not stall, and
+    every job finishes with its counters."""
+    W = __import__("importlib").import_module("201812realtimecortsim_amd.workloads")
+    dirs, envs = [], []
+    for i in range(5):
+        d = tmp_path / f"b{i}"
+        envs.append(W.write_instance(str(d), _spec(1 + i % 2, 40 + i, 60, False)))
+        dirs.append(str(d))
+    with pkg.Server(build_threads=2, host_backlog_bytes=1) as srv:
+        ids = [srv.submit(dirs[i], drive=pkg.DRIVE_PHILOX, philox_seed=200 + i, env=envs[i]) for i in range(5)]
+        assert srv.wait_all(300.0)
+        infos = [srv.job(j) for j in ids]
+    for inf in infos:
+        assert inf["state"] == "done", inf["message"]
+        assert inf["stats"]["steps"] == 60 and inf["stats"]["synaptic_events"] > 0

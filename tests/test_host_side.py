@@ -344,3 +344,27 @@ def test_perseo_ini_files(pkg, tmp_path):
         # JExt DJExt | NuExt Tau Theta H Tarp AlphaC TauC gC
         for got, want_ in zip(f[1:3] + f[4:12], c[2:4] + c[4:12]):
             assert abs(float(got) - float(want_)) <= 1e-6 * max(1.0, abs(float(want_))), (ln, got, want_)
+
+
+def test_prepared_network_reports_host_and_device_bytes(pkg):
+    """A network built on the host only (dpsnn_prepare, no device call)
+    reports the host bytes its synapse and delivery lists hold -- what the
+    server's backlog bound counts -- and the device bytes its upload would
+    take; it can be destroyed without ever being uploaded."""
+    cfg = pkg.config_dir("c0")
+    pkg.apply_env(cfg)
+    lib = pkg.library()
+    opts = pkg.Options()
+    opts.drive = pkg.DRIVE_REPLAY
+    h = ctypes.c_void_p()
+    rc = lib.dpsnn_prepare(cfg.encode(), ctypes.byref(opts), ctypes.byref(h))
+    assert rc == 0, lib.dpsnn_last_error().decode()
+    try:
+        S = 5621357  # the reference's synapse count for this network
+        host = lib.dpsnn_host_bytes(h)
+        # backward list (12 B per synapse) plus the in-lists (10 B per item
+        # and a 2-byte local index) before upload
+        assert 22 * S <= host <= 40 * S, host
+        assert lib.dpsnn_device_bytes_needed(h) > 6 * S
+    finally:
+        lib.dpsnn_destroy(h)
