@@ -619,10 +619,16 @@ constexpr int kChunk = 8;  // internal events per lane per LDS-staged chunk
 // steps keep the weight in its own array.
 constexpr uint32_t kPackT = 32;
 constexpr int kRun = 16;   // events per neuron run in the blocked scratch
-// LDS of one wave in phase 2: one chunk of 64 lanes' ticks (the wave has
-// read all of a chunk before it stores the next one over it), then the bath
-// queue (times, values: [kQueue][64] doubles each)
-constexpr size_t kStageBytes = kChunk * kWave * sizeof(uint64_t) + 2 * kQueue * kWave * sizeof(double);
+// LDS of one wave in phase 2: one chunk of 64 lanes' ticks, a row of
+// kStageRow per event (the wave has read all of a chunk before it stores the
+// next one over it), then the bath queue (times, values: [kQueue][64]
+// doubles each).  A row is 64 ticks plus 8 of padding: the transposing
+// stores write 8 rows x 8 neurons per instruction, which a 64-wide row would
+// put on one bank per neuron (8-way conflicts); 72 spreads them over all 64
+// banks.
+constexpr uint32_t kStageRow = kWave + 8;
+constexpr size_t kStageKBytes = kChunk * kStageRow * sizeof(uint64_t);
+constexpr size_t kStageBytes = kStageKBytes + 2 * kQueue * kWave * sizeof(double);
 
 // A buffer view of `bytes` (< 2^31) bytes at base (raw: stride 0, byte
 // offsets; the gfx9 descriptor word).  An access at kDropOffset is out of
@@ -667,7 +673,7 @@ __device__ inline void load_chunk(const __amdgpu_buffer_rsrc_t &tileK, const int
 __device__ inline void store_chunk(uint64_t *stageK, const uint64_t (&rk)[kChunk]) {
   const uint32_t lane = threadIdx.x & 63, e = lane & 7;
 #pragma unroll
-  for (int r = 0; r < kChunk; ++r) stageK[e * kWave + (uint32_t)r * 8 + (lane >> 3)] = rk[r];
+  for (int r = 0; r < kChunk; ++r) stageK[e * kStageRow + (uint32_t)r * 8 + (lane >> 3)] = rk[r];
 }
 
 // Phase 2, one lane per neuron: lane q integrates neuron j0 + q (q < m) over
@@ -722,7 +728,7 @@ __device__ __attribute__((always_inline)) inline uint32_t integrate_wave(const D
     }
   }
   if (!REPLAY) {
-    lds_f64 *q = (lds_f64 *)(stage + kChunk * kWave * sizeof(uint64_t));
+    lds_f64 *q = (lds_f64 *)(stage + kStageKBytes);
     xq.qt = q + lane;
     xq.qv = q + kQueue * kWave + lane;
     xq.fill(on);
@@ -733,7 +739,7 @@ __device__ __attribute__((always_inline)) inline uint32_t integrate_wave(const D
   // lane's internal events are used up
   const uint32_t chunks = (nmax + kChunk - 1) / kChunk;
   const uint32_t iters = chunks ? chunks : 1u;
-  uint64_t *stageK = (uint64_t *)stage;  // [kChunk][64]
+  uint64_t *stageK = (uint64_t *)stage;  // [kChunk][kStageRow]
   uint64_t rk[kChunk];
   uint4 wCur = make_uint4(0, 0, 0, 0), wNext = wCur;
   const bool packed = PK == 2 ? t >= kPackT : PK == 1;
@@ -773,7 +779,7 @@ __device__ __attribute__((always_inline)) inline uint32_t integrate_wave(const D
         at = kI;
         val = (double)(((float)w16) * P.w2f);
         ++e;
-        if (e < left) kv = curK[e * kWave + lane];
+        if (e < left) kv = curK[e * kStageRow + lane];
       } else {
         break;
       }
