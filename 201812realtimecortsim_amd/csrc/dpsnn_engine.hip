@@ -1331,7 +1331,10 @@ __global__ __launch_bounds__(kWave) void k_gather(DevNet P, uint32_t t, int ltd)
 // ---------------------------------------------------------------------------
 constexpr int kCollectThreads = kWindowWaves * kWave;
 constexpr uint32_t kWindowMinNeurons = 1u << 16;  // the window form by default from this size
-constexpr uint32_t kWindowUnitsWanted = 2304;     // units per step: about nine per CU (a short queue tail)
+// Targets per unit: enough items per pass to amortise the pass's window
+// load, units small enough for a short queue tail (1250-neuron columns: 4
+// units of 313; measured best on grids of 8x8, 12x12 and 24x24 columns)
+constexpr uint32_t kWindowUnitTargets = 320;
 constexpr uint32_t kWindowMaxEntries = 12 * 1024;  // window entries: kWinPerThread per thread of k_collect
 
 // Block-wide work queue: the next unit for the whole workgroup (one unit
@@ -3003,10 +3006,12 @@ int dpsnn_prepare(const char *inidir, const dpsnn_options *opt, dpsnn_net **out)
     bool want = n->con.N >= kWindowMinNeurons;
     if (const char *e = std::getenv("DPSNN_WINDOW")) want = std::atoi(e) != 0;
     if (want) {
-      // DPSNN_WINDOW_UNITS: units wanted per step (tuning; results do not change)
+      // DPSNN_WINDOW_UNITS: units wanted per step instead (tuning; results do
+      // not change)
       const char *ue = getenv("DPSNN_WINDOW_UNITS");
-      const uint32_t wanted = ue && atoi(ue) > 0 ? (uint32_t)atoi(ue) : kWindowUnitsWanted;
-      const uint32_t unitsPerCol = std::max<uint32_t>(1, (wanted + q.globCFT - 1) / q.globCFT);
+      const uint32_t unitsPerCol =
+          ue && atoi(ue) > 0 ? std::max<uint32_t>(1, ((uint32_t)atoi(ue) + q.globCFT - 1) / q.globCFT)
+                             : std::max<uint32_t>(1, (q.neuronsPerCM + kWindowUnitTargets - 1) / kWindowUnitTargets);
       build_window(n->par, n->con, unitsPerCol, kWindowMaxEntries, n->window, n->opt.host_threads);
     }
   }
