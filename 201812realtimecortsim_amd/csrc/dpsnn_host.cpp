@@ -10,6 +10,9 @@
 
 #include <algorithm>
 #include <atomic>
+#include <condition_variable>
+#include <functional>
+#include <mutex>
 #include <thread>
 #include <cerrno>
 #include <cmath>
@@ -499,30 +502,111 @@ uint32_t sub_pop_in_column(const Params &p, uint32_t inCM) {
   return 0;
 }
 
-uint32_t geometric_gap(Ran3 &rng, float prob, uint32_t maxN) {
-  const double q = (double)prob, r = rng.next();
-  double P = q, C = P;
-  uint32_t n = 1;
-  while (C < r && n <= maxN) {
-    ++n;
-    P *= 1 - q;
-    C += P;
+// getEmptyRandomSynapses (DPSNN_LIFCAconnectome.cpp:1079-1133) draws r and
+// returns the first n with C_n >= r, or maxN + 1 if there is none, where
+// P_1 = C_1 = q, P_{n+1} = P_n * (1 - q) and C_{n+1} = C_n + P_{n+1}, each
+// rounded in double.  C_n depends on (q, n) only, and rounding keeps it
+// non-decreasing, so that scan is a lower_bound over C_1..C_maxN accumulated
+// once per (q, maxN) with the same operations in the same order: the same n
+// for every r, in log2(maxN) compares instead of one dependent multiply-add
+// per candidate neuron skipped.
+class GapTables {
+ public:
+  explicit GapTables(const Params &p)
+      : ns_(p.pub.subPopNumber), sx_(p.stencilX), sy_(p.stencilY),
+        slot_(size_t(ns_) * ns_ * sx_ * sy_, kNone) {
+    std::map<std::pair<uint32_t, uint32_t>, uint32_t> made;  // (prob bits, maxN) -> table
+    for (uint32_t s = 0; s < ns_; ++s)
+      for (uint32_t t = 0; t < ns_; ++t)
+        for (uint32_t x = 0; x < sx_; ++x)
+          for (uint32_t y = 0; y < sy_; ++y) {
+            const float prob = p.prob_at(s, t, x, y);
+            if (!(prob > 0.00001)) continue;  // the scan never runs for these
+            const uint32_t maxN = p.sp[t].count + 1;
+            uint32_t bits;
+            std::memcpy(&bits, &prob, sizeof bits);
+            const auto ins = made.emplace(std::make_pair(bits, maxN), (uint32_t)start_.size());
+            if (ins.second) {
+              start_.push_back(cum_.size());
+              const double q = (double)prob;
+              double P = q, C = P;
+              for (uint32_t n = 1; n <= maxN; ++n) {
+                cum_.push_back(C);
+                P *= 1 - q;
+                C += P;
+              }
+            }
+            slot_[index(s, t, x, y)] = ins.first->second;
+          }
   }
-  return n;
+  // C_1.. of source sub-population s, target t, stencil cell (x, y).
+  const double *at(uint32_t s, uint32_t t, uint32_t x, uint32_t y) const {
+    return cum_.data() + start_[slot_[index(s, t, x, y)]];
+  }
+
+ private:
+  static constexpr uint32_t kNone = ~0u;
+  size_t index(uint32_t s, uint32_t t, uint32_t x, uint32_t y) const {
+    return ((size_t(s) * ns_ + t) * sx_ + x) * sy_ + y;
+  }
+  uint32_t ns_, sx_, sy_;
+  std::vector<uint32_t> slot_;
+  std::vector<size_t> start_;
+  std::vector<double> cum_;
+};
+
+// The gap to the next target of a scan whose remaining candidates number
+// `rem` (the distance to the end of the target sub-population).  A gap of rem
+// or more ends the scan wherever it lands, so it is returned as rem: only
+// C_1..C_{rem-1} are searched.  The first few are checked in order (a dense
+// stencil cell's gaps are short), the rest by binary search.
+uint32_t geometric_gap(Ran3 &rng, const double *cum, uint32_t rem) {
+  const double r = rng.next();
+  const uint32_t lim = rem - 1;  // the largest gap that stays inside
+  const uint32_t head = std::min<uint32_t>(lim, 16);
+  for (uint32_t i = 0; i < head; ++i)
+    if (!(cum[i] < r)) return i + 1;
+  return (uint32_t)(std::lower_bound(cum + head, cum + lim, r) - cum) + 1;
 }
 
-uint32_t uniform_delay(const Params &p, Ran3 &rng, uint32_t s, uint32_t t) {
-  float DMin = (float)p.sp[s].DMin[t], DMax = (float)p.sp[s].DMax[t];
-  float lo = (float)p.pub.delayMin, hi = (float)p.pub.delayMax;
-  const float levels = (float)kDelaySlots;
-  const float step = (hi - lo) / levels;
-  hi -= step / 2;
-  lo += step / 2;
-  DMax = round_half_up((DMax - lo) / step) * step + lo;
-  DMin = round_half_up((DMin - lo) / step) * step + lo;
-  const float level = ((rng.next() * (DMax - DMin) + DMin - lo) / step);
-  return round_half_up(level * step);
-}
+// getRandomDelay_UNI (DPSNN_LIFCAconnectome.cpp:299-338) for one (source,
+// target) sub-population pair.  The float bounds of the delay levels depend
+// on the pair only, so they are rounded once; a draw is then the reference's
+// product, sum and division in double, narrowed to float and rounded.
+struct DelayRule {
+  float DMin = 0, range = 0, lo = 0, step = 0;
+  DelayRule() = default;
+  DelayRule(const Params &p, uint32_t s, uint32_t t) {
+    float dmin = (float)p.sp[s].DMin[t], dmax = (float)p.sp[s].DMax[t];
+    lo = (float)p.pub.delayMin;
+    const float hi = (float)p.pub.delayMax, levels = (float)kDelaySlots;
+    step = (hi - lo) / levels;
+    lo += step / 2;
+    dmax = round_half_up((dmax - lo) / step) * step + lo;
+    dmin = round_half_up((dmin - lo) / step) * step + lo;
+    DMin = dmin;
+    range = dmax - dmin;
+  }
+  uint32_t draw(Ran3 &rng) const {
+    const float level = ((rng.next() * range + DMin - lo) / step);
+    return round_half_up(level * step);
+  }
+};
+
+// What every column's generation reads and no draw changes: the gap tables,
+// the delay rule of each sub-population pair and each in-column index's
+// sub-population.
+struct ColumnRules {
+  GapTables gaps;
+  std::vector<DelayRule> delay;  // [source][target]
+  std::vector<uint8_t> subOf;    // [index in column]
+  explicit ColumnRules(const Params &p) : gaps(p) {
+    const uint32_t ns = p.pub.subPopNumber;
+    for (uint32_t s = 0; s < ns; ++s)
+      for (uint32_t t = 0; t < ns; ++t) delay.emplace_back(p, s, t);
+    for (uint32_t k = 0; k < p.pub.neuronsPerCM; ++k) subOf.push_back((uint8_t)sub_pop_in_column(p, k));
+  }
+};
 
 }  // namespace
 
@@ -546,35 +630,73 @@ struct ColumnOut {
 
 namespace {
 
+// An empty column that keeps `old`'s array memory, for the next column.
+ColumnOut recycled(ColumnOut &&old) {
+  ColumnOut o;
+  o.count = std::move(old.count);
+  o.post = std::move(old.post);
+  o.delay = std::move(old.delay);
+  o.w = std::move(old.w);
+  o.post.clear();
+  o.delay.clear();
+  o.w.clear();
+  return o;
+}
+
 // Generates the neurons of column `col` from `rng` (already positioned at the
 // column's first draw).  If `placeholderEntry`, the first NormDev value is
 // the cached entry value, unknown yet: it is assumed to pass the weight
 // bounds (it does unless |x| >= 1/DJ) and the weight is patched later.
-void generate_column(const Params &p, uint32_t col, Ran3 &rng, bool placeholderEntry, Connectome &c,
-                     ColumnOut &out, bool writeNeuronInfo) {
+void generate_column(const Params &p, const ColumnRules &rules, uint32_t col, Ran3 &rng, bool placeholderEntry,
+                     Connectome &c, ColumnOut &out, bool writeNeuronInfo) {
   const dpsnn_parameters &q = p.pub;
   const uint32_t npc = q.neuronsPerCM, cfx = col % q.globCFX, cfy = col / q.globCFX;
+  // the column of stencil cell (x, y), false if it lies off the grid
+  auto cell_column = [&](uint32_t x, uint32_t y, uint32_t &colBase) {
+    int32_t tx, ty;
+    if (q.overallConnectivity == 0) {
+      tx = (int32_t)cfx + (int32_t)x - (int32_t)p.maxModDeltaX;
+      ty = (int32_t)cfy + (int32_t)y - (int32_t)p.maxModDeltaY;
+    } else {
+      tx = (int32_t)x;
+      ty = (int32_t)y;
+    }
+    if (tx < 0 || tx >= (int32_t)q.globCFX || ty < 0 || ty >= (int32_t)q.globCFY) return false;
+    colBase = (uint32_t)tx * npc + (uint32_t)ty * npc * q.globCFX;
+    return true;
+  };
+  {
+    // room for the expected synapse count plus a margin far above its
+    // binomial spread: the arrays are neither regrown nor oversized
+    double expect = 0;
+    uint32_t colBase;
+    for (uint32_t s = 0; s < q.subPopNumber; ++s)
+      for (uint32_t ts = 0; ts < q.subPopNumber; ++ts)
+        for (uint32_t x = 0; x < p.stencilX; ++x)
+          for (uint32_t y = 0; y < p.stencilY; ++y) {
+            const float prob = p.prob_at(s, ts, x, y);
+            if (prob > 0 && cell_column(x, y, colBase))
+              expect += (double)p.sp[s].count * prob * p.sp[ts].count;
+          }
+    const size_t cap = (size_t)(expect * 1.02) + 4096;
+    out.post.reserve(cap);
+    out.delay.reserve(cap);
+    out.w.reserve(cap);
+  }
   std::vector<Target> targets;
   targets.reserve(kMaxOutDegree);
   out.count.assign(npc, 0);
   bool pendingPlaceholder = placeholderEntry;
   for (uint32_t inCM = 0; inCM < npc; ++inCM) {
     const uint32_t i = col * npc + inCM;
-    const uint32_t src = sub_pop_in_column(p, inCM);
+    const uint32_t src = rules.subOf[inCM];
     if (writeNeuronInfo) c.subPop[i] = (uint8_t)src;
     targets.clear();
     for (uint32_t ts = 0; ts < q.subPopNumber; ++ts)
       for (uint32_t x = 0; x < p.stencilX; ++x)
         for (uint32_t y = 0; y < p.stencilY; ++y) {
-          int32_t tx, ty;
-          if (q.overallConnectivity == 0) {
-            tx = (int32_t)cfx + (int32_t)x - (int32_t)p.maxModDeltaX;
-            ty = (int32_t)cfy + (int32_t)y - (int32_t)p.maxModDeltaY;
-          } else {
-            tx = (int32_t)x;
-            ty = (int32_t)y;
-          }
-          if (tx < 0 || tx >= (int32_t)q.globCFX || ty < 0 || ty >= (int32_t)q.globCFY) continue;
+          uint32_t colBase;
+          if (!cell_column(x, y, colBase)) continue;
           const float prob = p.prob_at(src, ts, x, y);
           if (!(prob > 0)) continue;
           if (!(prob > 0.00001)) {
@@ -583,20 +705,19 @@ void generate_column(const Params &p, uint32_t col, Ran3 &rng, bool placeholderE
             return;
           }
           const uint32_t first = p.sp[ts].offset, end = first + p.sp[ts].count;
-          const uint32_t colBase = (uint32_t)tx * npc + (uint32_t)ty * npc * q.globCFX;
           uint32_t cur = first - 1;  // wraps like the reference's uint32 arithmetic
-          const uint32_t span = end - cur;
-          while ((cur += geometric_gap(rng, prob, span)) < end) {
+          const double *cum = rules.gaps.at(src, ts, x, y);
+          while ((cur += geometric_gap(rng, cum, end - cur)) < end) {
             if (targets.size() + 1 >= kMaxOutDegree) {
               out.ok = false;
               out.err = "a neuron would get 7500 or more synapses (DSD__maxM)";
               return;
             }
-            targets.push_back({colBase + cur, (uint8_t)sub_pop_in_column(p, cur)});
+            targets.push_back({colBase + cur, rules.subOf[cur]});
           }
         }
     for (const Target &tg : targets) {
-      const uint32_t d = uniform_delay(p, rng, src, tg.sub);
+      const uint32_t d = rules.delay[src * q.subPopNumber + tg.sub].draw(rng);
       const float mean = (float)p.sp[src].J[tg.sub];
       const float sigma = (float)(p.sp[src].DJ[tg.sub] * mean);
       float w;
@@ -653,103 +774,229 @@ void connectome_neurons(const Params &p, Connectome &c) {
   }
 }
 
+namespace {
+
+// Sizes the backward list's five arrays to S entries.  Growing zero-fills
+// (and first touches) them, so each array gets a thread of its own.
+void grow_backward(Connectome &c, uint64_t S, unsigned threads) {
+  std::function<void()> grow[] = {[&] { c.pre.resize(S); }, [&] { c.post.resize(S); },
+                                  [&] { c.weight.resize(S); }, [&] { c.delay.resize(S); },
+                                  [&] { c.kind.resize(S); }};
+  const unsigned n = std::max(1u, std::min<unsigned>(threads, 5));
+  std::vector<std::thread> pool;
+  for (unsigned k = 0; k < n; ++k)
+    pool.emplace_back([&, k]() {
+      for (unsigned j = k; j < 5; j += n) grow[j]();
+    });
+  for (auto &t : pool) t.join();
+}
+
+// Runs fn(k) for k in [0, n) on up to `threads` threads, k handed out in order.
+template <class Fn>
+void for_each_index(uint32_t n, unsigned threads, Fn fn) {
+  std::atomic<uint32_t> next{0};
+  std::vector<std::thread> pool;
+  for (unsigned t = 0; t < std::max(1u, std::min<unsigned>(threads, n)); ++t)
+    pool.emplace_back([&]() {
+      for (uint32_t k; (k = next.fetch_add(1)) < n;) fn(k);
+    });
+  for (auto &t : pool) t.join();
+}
+
+// The backward list straight from the columns' generation-order synapses:
+// inflate_backward's stable counting sort by delay with the columns as the
+// chunks, which are contiguous neuron ranges in order.  Each column's arrays
+// are released once scattered.
+void inflate_columns(Connectome &c, std::vector<ColumnOut> &col, uint32_t npc, unsigned threads) {
+  if (threads == 0) threads = default_threads();
+  const uint32_t ncol = (uint32_t)col.size();
+  uint64_t S = 0;
+  for (const ColumnOut &o : col) S += o.post.size();
+  c.S = S;
+  grow_backward(c, S, threads);
+  std::vector<uint64_t> cnt(size_t(ncol) * kDelaySlots, 0);
+  for_each_index(ncol, threads, [&](uint32_t k) {
+    uint64_t *cc = &cnt[size_t(k) * kDelaySlots];
+    for (const uint8_t d : col[k].delay) ++cc[d];
+  });
+  // bucket-major, column-minor offsets
+  uint64_t run = 0;
+  uint32_t maxd = 0;
+  for (uint32_t d = 0; d < kDelaySlots; ++d)
+    for (uint32_t k = 0; k < ncol; ++k) {
+      const uint64_t v = cnt[size_t(k) * kDelaySlots + d];
+      if (v) maxd = d;
+      cnt[size_t(k) * kDelaySlots + d] = run;
+      run += v;
+    }
+  for_each_index(ncol, threads, [&](uint32_t k) {
+    uint64_t *pos = &cnt[size_t(k) * kDelaySlots];
+    const ColumnOut &o = col[k];
+    uint64_t s = 0;
+    for (uint32_t j = 0; j < npc; ++j) {
+      const uint32_t i = k * npc + j;
+      const uint8_t kind = c.subPop[i];
+      for (const uint64_t e = s + o.count[j]; s < e; ++s) {
+        const uint64_t b = pos[o.delay[s]]++;
+        c.pre[b] = i;
+        c.post[b] = o.post[s];
+        c.delay[b] = o.delay[s];
+        c.kind[b] = kind;
+        c.weight[b] = o.w[s];
+      }
+    }
+    col[k] = ColumnOut();
+  });
+  c.max_delay = maxd;
+}
+
+}  // namespace
+
 bool generate_connectome(const Params &p, Ran3 &rng, Connectome &c, std::string &err, unsigned threads) {
   const dpsnn_parameters &q = p.pub;
-  const uint32_t N = q.globN, npc = q.neuronsPerCM, cols = q.globCFT;
+  const uint32_t npc = q.neuronsPerCM, cols = q.globCFT;
   connectome_neurons(p, c);
   if (q.tabNeuron != npc) {
     err = "reseeding period (tabNeuron) differs from the column size";
     return false;
   }
   if (threads == 0) threads = default_threads();
+  const ColumnRules rules(p);
   // column 0 continues the stream seeded with env_globalSeed: SetRandomSeed(0)
   // only advances it (DPSNN_LIFCAcorticalModuleGen.cpp:96-107, randdev.c:477-489)
-  std::vector<uint64_t> gen_off(N + 1, 0);
-  std::vector<uint32_t> g_post;
-  std::vector<uint8_t> g_delay;
-  std::vector<int16_t> g_w;
-  const size_t guess = size_t(N) * 1200;
-  g_post.reserve(guess); g_delay.reserve(guess); g_w.reserve(guess);
-  auto append = [&](uint32_t col, const ColumnOut &o) {
-    g_post.insert(g_post.end(), o.post.begin(), o.post.end());
-    g_delay.insert(g_delay.end(), o.delay.begin(), o.delay.end());
-    g_w.insert(g_w.end(), o.w.begin(), o.w.end());
-    for (uint32_t k = 0; k < npc; ++k) {
-      const uint32_t i = col * npc + k;
-      gen_off[i + 1] = gen_off[i] + o.count[k];
-    }
-  };
+  std::vector<ColumnOut> kept(cols);  // each column's synapses, in generation order
   {
     ColumnOut o;
     rng.seed(0);
-    generate_column(p, 0, rng, false, c, o, true);
+    generate_column(p, rules, 0, rng, false, c, o, true);
     if (!o.ok) { err = o.err; return false; }
-    append(0, o);
+    kept[0] = std::move(o);
   }
   int32_t set = rng.norm_set();
   double cached = rng.norm_cached();
-  // columns 1..: each restarts ran3 from its own index; two variants per
-  // column (entry cache empty / full), chosen and patched in column order
-  const uint32_t batch = std::max<uint32_t>(threads, 8);
-  for (uint32_t c0 = 1; c0 < cols; c0 += batch) {
-    const uint32_t c1 = std::min(cols, c0 + batch);
-    const uint32_t nb = c1 - c0;
-    std::vector<ColumnOut> outA(nb), outB(nb);
-    std::vector<std::thread> pool;
-    std::atomic<uint32_t> nextJob{0};
-    auto worker = [&]() {
-      for (;;) {
-        const uint32_t job = nextJob.fetch_add(1);
-        if (job >= 2 * nb) return;
-        const uint32_t k = job / 2, col = c0 + k;
-        const bool variantB = job & 1;
-        Ran3 r;
-        r.seed((int32_t)col);
-        r.set_norm_state(0, 0.0);
-        generate_column(p, col, r, variantB, c, variantB ? outB[k] : outA[k], !variantB);
-      }
-    };
-    for (unsigned th = 0; th < std::min<unsigned>(threads, 2 * nb); ++th) pool.emplace_back(worker);
-    for (auto &th : pool) th.join();
-    for (uint32_t k = 0; k < nb; ++k) {
-      const uint32_t col = c0 + k;
-      ColumnOut *o = set ? &outB[k] : &outA[k];
-      if (set && o->ok) {
-        if (!o->usedNorm) {
-          // nothing drew a Gaussian: the cache passes through unchanged
-          o->exitSet = 1;
-          o->exitCached = cached;
-        } else {
-          const float w = (float)(o->patchMean + cached * o->patchSigma);
-          const bool inRange = o->patchMean > 0.0 ? !(w < 0.0 || w >= 2.0 * o->patchMean)
-                                                  : !(w <= 2.0 * o->patchMean || w > 0);
-          if (inRange && !(w < q.minSynWeight_f || w > -q.minSynWeight_f)) {
-            o->w[o->patchAt] = (int16_t)(w * q.factorFloat_2_weightType);
-          } else {
-            // the cached value is rejected by the weight bounds: regenerate
-            // the column sequentially from the true entry state
-            ColumnOut redo;
-            Ran3 r;
-            r.seed((int32_t)col);
-            r.set_norm_state(1, cached);
-            generate_column(p, col, r, false, c, redo, false);
-            outB[k] = std::move(redo);
-            o = &outB[k];
-          }
+  // columns 1..: each restarts ran3 from its own index, while NormDev's
+  // cached value carries over from the column before
+  auto finish = [&](uint32_t col, ColumnOut &o) {
+    set = o.exitSet;
+    cached = o.exitCached;
+    if (col == cols - 1) {
+      // the simulation stream continues from where the last column ended
+      rng = o.exitRng;
+      rng.set_norm_state(set, cached);
+    }
+    kept[col] = std::move(o);
+  };
+  if (threads <= 2) {
+    // one thread: each column from its true entry state, in order
+    for (uint32_t col = 1; col < cols; ++col) {
+      ColumnOut o;
+      Ran3 r;
+      r.seed((int32_t)col);
+      r.set_norm_state(set, cached);
+      generate_column(p, rules, col, r, false, c, o, true);
+      if (!o.ok) { err = o.err; return false; }
+      finish(col, o);
+    }
+    inflate_columns(c, kept, npc, threads);
+    return true;
+  }
+  // several threads: both variants of a column (entry cache empty / full)
+  // are generated ahead by a pool, at most `window` columns past the last
+  // one resolved; this thread picks, patches and appends them in order
+  struct Variants {
+    ColumnOut a, b;
+    int done = 0;
+  };
+  std::vector<Variants> var(cols);
+  std::vector<ColumnOut> spare;  // variants not taken, their memory reused
+  const uint32_t jobs = 2 * (cols - 1), window = threads + 2;
+  uint32_t nextJob = 0, resolved = 0;
+  bool stop = false;
+  std::mutex mu;
+  std::condition_variable cv;
+  auto worker = [&]() {
+    for (;;) {
+      uint32_t job;
+      {
+        std::unique_lock<std::mutex> lk(mu);
+        cv.wait(lk, [&] { return stop || nextJob >= jobs || 1 + nextJob / 2 <= resolved + window; });
+        if (stop || nextJob >= jobs) return;
+        job = nextJob++;
+        if (!spare.empty()) {
+          (job & 1 ? var[1 + job / 2].b : var[1 + job / 2].a) = std::move(spare.back());
+          spare.pop_back();
         }
       }
-      if (!o->ok) { err = o->err; return false; }
-      append(col, *o);
-      set = o->exitSet;
-      cached = o->exitCached;
-      if (col == cols - 1) {
-        // the simulation stream continues from where the last column ended
-        rng = o->exitRng;
-        rng.set_norm_state(set, cached);
+      const uint32_t col = 1 + job / 2;
+      const bool variantB = job & 1;
+      Ran3 r;
+      r.seed((int32_t)col);
+      r.set_norm_state(0, 0.0);
+      generate_column(p, rules, col, r, variantB, c, variantB ? var[col].b : var[col].a, !variantB);
+      {
+        std::lock_guard<std::mutex> lk(mu);
+        ++var[col].done;
+      }
+      cv.notify_all();
+    }
+  };
+  std::vector<std::thread> pool;
+  for (unsigned th = 0; th < std::min<unsigned>(threads, jobs); ++th) pool.emplace_back(worker);
+  auto drain = [&]() {
+    {
+      std::lock_guard<std::mutex> lk(mu);
+      stop = true;
+    }
+    cv.notify_all();
+    for (auto &th : pool) th.join();
+  };
+  for (uint32_t col = 1; col < cols; ++col) {
+    {
+      std::unique_lock<std::mutex> lk(mu);
+      cv.wait(lk, [&] { return var[col].done == 2; });
+    }
+    ColumnOut *o = set ? &var[col].b : &var[col].a;
+    if (set && o->ok) {
+      if (!o->usedNorm) {
+        // nothing drew a Gaussian: the cache passes through unchanged
+        o->exitSet = 1;
+        o->exitCached = cached;
+      } else {
+        const float w = (float)(o->patchMean + cached * o->patchSigma);
+        const bool inRange = o->patchMean > 0.0 ? !(w < 0.0 || w >= 2.0 * o->patchMean)
+                                                : !(w <= 2.0 * o->patchMean || w > 0);
+        if (inRange && !(w < q.minSynWeight_f || w > -q.minSynWeight_f)) {
+          o->w[o->patchAt] = (int16_t)(w * q.factorFloat_2_weightType);
+        } else {
+          // the cached value is rejected by the weight bounds: regenerate
+          // the column sequentially from the true entry state
+          ColumnOut redo;
+          Ran3 r;
+          r.seed((int32_t)col);
+          r.set_norm_state(1, cached);
+          generate_column(p, rules, col, r, false, c, redo, false);
+          var[col].b = std::move(redo);
+        }
       }
     }
+    if (!o->ok) {
+      err = o->err;
+      drain();
+      return false;
+    }
+    finish(col, *o);
+    ColumnOut unused = recycled(std::move(set ? var[col].a : var[col].b));
+    var[col].a = ColumnOut();
+    var[col].b = ColumnOut();
+    {
+      std::lock_guard<std::mutex> lk(mu);
+      spare.push_back(std::move(unused));
+      resolved = col;
+    }
+    cv.notify_all();
   }
-  inflate_backward(c, gen_off, g_post, g_delay, g_w, threads);
+  drain();
+  inflate_columns(c, kept, npc, threads);
   return true;
 }
 
@@ -762,7 +1009,7 @@ void inflate_backward(Connectome &c, const std::vector<uint64_t> &gen_off, const
   if (threads == 0) threads = default_threads();
   const uint64_t S = g_post.size();
   c.S = S;
-  c.pre.resize(S); c.post.resize(S); c.delay.resize(S); c.kind.resize(S); c.weight.resize(S);
+  grow_backward(c, S, threads);
   const unsigned chunks = std::max(1u, std::min<unsigned>(threads, (N + 4095) / 4096));
   std::vector<uint64_t> cnt(size_t(chunks) * kDelaySlots, 0);
   auto chunk_lo = [&](unsigned k) { return (uint32_t)((uint64_t)N * k / chunks); };
@@ -821,64 +1068,116 @@ void build_delivery(const Params &p, Connectome &c, bool with_incoming, unsigned
   const uint32_t N = c.N, D = std::min<uint32_t>(p.pub.D, kDelaySlots);
   const uint64_t S = c.S;
   if (threads == 0) threads = default_threads();
-  threads = std::max(1u, std::min<unsigned>(threads, (N + 1023) / 1024));
-  // bucket boundaries of the backward list
+  // bucket boundaries of the backward list, which is sorted by delay
   std::vector<uint64_t> bstart(kDelaySlots + 1, 0);
-  for (uint64_t s = 0; s < S; ++s) ++bstart[c.delay[s] + 1];
-  for (uint32_t d = 0; d < kDelaySlots; ++d) bstart[d + 1] += bstart[d];
-  // Visits every delivered (run delay, source, synapse) in the reference's
-  // order, calling f for those whose target lies in [lo, hi).
-  auto visit = [&](uint32_t lo, uint32_t hi, auto &&f) {
-    for (uint32_t d = D; d-- > 0;) {
-      uint64_t s = bstart[d];
-      while (s < bstart[d + 1]) {
+  for (uint32_t d = 0; d <= kDelaySlots; ++d)
+    bstart[d] = (uint64_t)(std::lower_bound(c.delay.begin(), c.delay.end(), d,
+                                            [](uint8_t a, uint32_t b) { return a < b; }) - c.delay.begin());
+  // The reference's delivery order: buckets by descending delay, in each the
+  // runs of one source, a run possibly spilling over into the next bucket.
+  // It is cut into `parts` contiguous pieces at run starts; piece k's items
+  // for a target then follow those of the pieces before it.  A cut is
+  // (di, s): the run starting at s in bucket D - 1 - di; (D, 0) is the end.
+  struct Cut {
+    uint32_t di;
+    uint64_t s;
+  };
+  const unsigned parts = std::max(1u, threads);
+  std::vector<Cut> cut(parts + 1);
+  auto bucket_start = [&](uint32_t di) { return di < D ? Cut{di, bstart[D - 1 - di]} : Cut{D, 0}; };
+  cut[0] = bucket_start(0);
+  cut[parts] = Cut{D, 0};
+  const uint64_t total = bstart[D];
+  for (unsigned k = 1; k < parts; ++k) {
+    uint64_t v = total * k / parts;
+    uint32_t di = 0;
+    for (; di < D; ++di) {
+      const uint32_t d = D - 1 - di;
+      if (v < bstart[d + 1] - bstart[d]) break;
+      v -= bstart[d + 1] - bstart[d];
+    }
+    Cut at = bucket_start(di);
+    if (di < D) {
+      const uint32_t d = D - 1 - di;
+      uint64_t s = bstart[d] + v;
+      while (s > bstart[d] && s < bstart[d + 1] && c.pre[s] == c.pre[s - 1]) ++s;  // to a run start
+      at = s < bstart[d + 1] ? Cut{di, s} : bucket_start(di + 1);
+    }
+    const Cut &prev = cut[k - 1];
+    if (at.di < prev.di || (at.di == prev.di && at.s < prev.s)) at = prev;
+    cut[k] = at;
+  }
+  // Visits piece k's delivered (run delay, source, synapse, target) in order.
+  auto visit = [&](unsigned k, auto &&f) {
+    const Cut a = cut[k], b = cut[k + 1];
+    for (uint32_t di = a.di; di < D && di <= b.di; ++di) {
+      const uint32_t d = D - 1 - di;
+      uint64_t s = di == a.di ? a.s : bstart[d];
+      const uint64_t se = di == b.di ? b.s : bstart[d + 1];
+      while (s < se) {
         const uint32_t src = c.pre[s];
         uint64_t e = s;
         while (e < S && c.pre[e] == src) {  // the run, possibly spilling over
-          const uint32_t tgt = c.post[e];
-          if (tgt >= lo && tgt < hi) f(d, src, e, tgt);
+          f(d, src, e, c.post[e]);
           ++e;
         }
         s = e < bstart[d + 1] ? e : bstart[d + 1];
       }
     }
   };
-  std::vector<uint64_t> cnt(N + 1, 0);
-  auto parallel = [&](auto &&body) {
-    std::vector<std::thread> pool;
-    for (unsigned th = 0; th < threads; ++th) {
-      const uint32_t lo = (uint32_t)((uint64_t)N * th / threads), hi = (uint32_t)((uint64_t)N * (th + 1) / threads);
-      pool.emplace_back([&, lo, hi]() { body(lo, hi); });
+  // per piece and target: the count, then where the piece's next item goes
+  std::vector<uint64_t> pos(size_t(parts) * N, 0);
+  // turns the counts into positions in a list laid out by offsets `off`
+  auto place = [&](std::vector<uint64_t> &off) {
+    off.assign(N + 1, 0);
+    for (uint32_t j = 0; j < N; ++j) {
+      uint64_t at = off[j];
+      for (unsigned k = 0; k < parts; ++k) {
+        const uint64_t n = pos[size_t(k) * N + j];
+        pos[size_t(k) * N + j] = at;
+        at += n;
+      }
+      off[j + 1] = at;
     }
-    for (auto &t : pool) t.join();
   };
-  parallel([&](uint32_t lo, uint32_t hi) {
-    visit(lo, hi, [&](uint32_t, uint32_t, uint64_t, uint32_t tgt) { ++cnt[tgt]; });
+  for_each_index(parts, threads, [&](uint32_t k) {
+    uint64_t *cnt = &pos[size_t(k) * N];
+    visit(k, [&](uint32_t, uint32_t, uint64_t, uint32_t tgt) { ++cnt[tgt]; });
   });
-  c.in_off.assign(N + 1, 0);
-  for (uint32_t j = 0; j < N; ++j) c.in_off[j + 1] = c.in_off[j] + cnt[j];
+  place(c.in_off);
   c.max_in = 0;
-  for (uint32_t j = 0; j < N; ++j) c.max_in = std::max<uint32_t>(c.max_in, (uint32_t)cnt[j]);
+  for (uint32_t j = 0; j < N; ++j) c.max_in = std::max<uint32_t>(c.max_in, (uint32_t)(c.in_off[j + 1] - c.in_off[j]));
   const uint64_t items = c.in_off[N];
-  c.in_pre_d.resize(items); c.in_w.resize(items); c.in_syn.resize(items);
-  std::fill(cnt.begin(), cnt.end(), 0);
-  parallel([&](uint32_t lo, uint32_t hi) {
-    visit(lo, hi, [&](uint32_t d, uint32_t src, uint64_t e, uint32_t tgt) {
-      const uint64_t at = c.in_off[tgt] + cnt[tgt]++;
-      c.in_pre_d[at] = src | (d << 24);
-      c.in_w[at] = c.weight[e];
-      c.in_syn[at] = (uint32_t)e;
+  {
+    std::thread a([&] { c.in_pre_d.resize(items); }), b([&] { c.in_w.resize(items); });
+    c.in_syn.resize(items);
+    a.join();
+    b.join();
+  }
+  for_each_index(parts, threads, [&](uint32_t k) {
+    uint64_t *at = &pos[size_t(k) * N];
+    visit(k, [&](uint32_t d, uint32_t src, uint64_t e, uint32_t tgt) {
+      const uint64_t i = at[tgt]++;
+      c.in_pre_d[i] = src | (d << 24);
+      c.in_w[i] = c.weight[e];
+      c.in_syn[i] = (uint32_t)e;
     });
   });
   if (!with_incoming) return;
   // incoming synapses per target in backward-list order (LTP list,
-  // DPSNN_localNet_init.c:225-237)
-  c.inc_off.assign(N + 1, 0);
-  for (uint64_t s = 0; s < S; ++s) ++c.inc_off[c.post[s] + 1];
-  for (uint32_t j = 0; j < N; ++j) c.inc_off[j + 1] += c.inc_off[j];
+  // DPSNN_localNet_init.c:225-237): the list cut into `parts` equal pieces
+  std::fill(pos.begin(), pos.end(), 0);
+  auto piece = [&](uint32_t k) { return S * k / parts; };
+  for_each_index(parts, threads, [&](uint32_t k) {
+    uint64_t *cnt = &pos[size_t(k) * N];
+    for (uint64_t s = piece(k); s < piece(k + 1); ++s) ++cnt[c.post[s]];
+  });
+  place(c.inc_off);
   c.inc_syn.resize(S);
-  std::vector<uint64_t> fill(c.inc_off.begin(), c.inc_off.end() - 1);
-  for (uint64_t s = 0; s < S; ++s) c.inc_syn[fill[c.post[s]]++] = (uint32_t)s;
+  for_each_index(parts, threads, [&](uint32_t k) {
+    uint64_t *at = &pos[size_t(k) * N];
+    for (uint64_t s = piece(k); s < piece(k + 1); ++s) c.inc_syn[at[c.post[s]]++] = (uint32_t)s;
+  });
 }
 
 // --------------------------------------------------------------------------

@@ -187,24 +187,43 @@ def _small_grid_config(tmp_path, pkg, grid=4):
     return str(d)
 
 
-def test_column_parallel_generator_matches_oracle_on_many_columns(pkg, oracle, tmp_path):
-    """36 columns: the column-parallel generator resolves NormDev's cached
-    value across column boundaries exactly like the sequential reference."""
+def test_column_parallel_generator_matches_oracle_on_many_columns(pkg, oracle, tmp_path, monkeypatch):
+    """36 columns: the generator resolves NormDev's cached value across column
+    boundaries exactly like the sequential reference, on one thread (each
+    column from its true entry state), on three (a small pool generating both
+    entry variants ahead) and on all the host's threads; the in-lists, built
+    from pieces of the delivery order on as many threads, keep that order."""
     cfg = _small_grid_config(tmp_path, pkg, grid=6)
-    host = pkg.HostConnectome(cfg)
     ora = OracleNet(oracle, cfg)
-    a, b = host.synapses(), ora.backward()
-    for k in ("pre", "post", "delay", "weight"):
-        assert np.array_equal(a[k], b[k]), k
+    b = ora.backward()
+    want = np.array(_reference_delivery_order(b, pkg.get_parameters(cfg)["D"]), dtype=np.int64)
     # the stream after generation continues identically (external drive)
     ora.record_external(True)
+    ext = []
     for _ in range(3):
-        off, t, v = host.external_step()
         ora.fire()
         cnt, ot, ov = ora.last_external()
-        assert np.array_equal(t, ot) and np.array_equal(v, ov)
-    host.close()
+        ext.append((ot.copy(), ov.copy()))
     ora.close()
+    for threads in ("1", "3", None):
+        if threads is None:
+            monkeypatch.delenv("OMP_NUM_THREADS", raising=False)
+        else:
+            monkeypatch.setenv("OMP_NUM_THREADS", threads)
+        host = pkg.HostConnectome(cfg)
+        a = host.synapses()
+        for k in ("pre", "post", "delay", "weight"):
+            assert np.array_equal(a[k], b[k]), (threads, k)
+        il = host.inlists()
+        tgt = np.repeat(np.arange(host.n), np.diff(il["off"]).astype(np.int64))
+        assert np.array_equal(tgt, want[:, 0]), threads
+        assert np.array_equal(il["delay"].astype(np.int64), want[:, 1]), threads
+        assert np.array_equal(il["pre"].astype(np.int64), want[:, 2]), threads
+        assert np.array_equal(il["syn"].astype(np.int64), want[:, 3]), threads
+        for ot, ov in ext:
+            off, t, v = host.external_step()
+            assert np.array_equal(t, ot) and np.array_equal(v, ov), threads
+        host.close()
     pkg.apply_env(pkg.config_dir("c0"))
 
 
