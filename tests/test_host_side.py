@@ -227,6 +227,41 @@ def test_column_parallel_generator_matches_oracle_on_many_columns(pkg, oracle, t
     pkg.apply_env(pkg.config_dir("c0"))
 
 
+def test_generator_matches_oracle_on_a_sparse_stencil(pkg, oracle, tmp_path, monkeypatch):
+    """36 columns whose stencil mixes a dense centre (q = 0.3) with sparse
+    sides (0.01) and a diagonal barely above the reference's 1e-5 floor
+    (0.0003): most gap draws land past the sub-population's end, so the
+    generator's search of the accumulated gap distribution runs over its
+    whole range, against the oracle's linear scan."""
+    cfg = _small_grid_config(tmp_path, pkg, grid=6)
+    rows = []
+    for line in open(os.path.join(cfg, "connectivity.txt")):
+        v = line.split()
+        if len(v) >= 10:
+            rows.append(" ".join(v[:6] + ["0.3", "0.01", "0.01", "0.0003"]))
+    (tmp_path / "grid6" / "connectivity.txt").write_text("\n".join(rows) + "\n")
+    ora = OracleNet(oracle, cfg)
+    b = ora.backward()
+    ora.record_external(True)
+    ora.fire()
+    _, ot, ov = ora.last_external()
+    ora.close()
+    for threads in ("1", None):
+        if threads is None:
+            monkeypatch.delenv("OMP_NUM_THREADS", raising=False)
+        else:
+            monkeypatch.setenv("OMP_NUM_THREADS", threads)
+        host = pkg.HostConnectome(cfg)
+        a = host.synapses()
+        assert len(a["pre"]) > 0
+        for k in ("pre", "post", "delay", "weight"):
+            assert np.array_equal(a[k], b[k]), (threads, k)
+        _, t, v = host.external_step()
+        assert np.array_equal(t, ot) and np.array_equal(v, ov), threads
+        host.close()
+    pkg.apply_env(pkg.config_dir("c0"))
+
+
 def _check_window(pkg, host, units_per_col):
     """The stencil-window layout streams, per (unit, pass, wave), each owned
     target's in-list items of the pass's run delay in in-list order, padded
