@@ -509,7 +509,9 @@ uint32_t sub_pop_in_column(const Params &p, uint32_t inCM) {
 // non-decreasing, so that scan is a lower_bound over C_1..C_maxN accumulated
 // once per (q, maxN) with the same operations in the same order: the same n
 // for every r, in log2(maxN) compares instead of one dependent multiply-add
-// per candidate neuron skipped.
+// per candidate neuron skipped.  Tables are built until they hold kMaxEntries
+// values (large stencils with many distinct probabilities); the cells left
+// over accumulate C_n as they go, as the reference does.
 class GapTables {
  public:
   explicit GapTables(const Params &p)
@@ -525,7 +527,9 @@ class GapTables {
             const uint32_t maxN = p.sp[t].count + 1;
             uint32_t bits;
             std::memcpy(&bits, &prob, sizeof bits);
-            const auto ins = made.emplace(std::make_pair(bits, maxN), (uint32_t)start_.size());
+            const auto key = std::make_pair(bits, maxN);
+            if (!made.count(key) && cum_.size() + maxN > kMaxEntries) continue;
+            const auto ins = made.emplace(key, (uint32_t)start_.size());
             if (ins.second) {
               start_.push_back(cum_.size());
               const double q = (double)prob;
@@ -539,13 +543,16 @@ class GapTables {
             slot_[index(s, t, x, y)] = ins.first->second;
           }
   }
-  // C_1.. of source sub-population s, target t, stencil cell (x, y).
+  // C_1.. of source sub-population s, target t, stencil cell (x, y), or
+  // nullptr for a cell without a table.
   const double *at(uint32_t s, uint32_t t, uint32_t x, uint32_t y) const {
-    return cum_.data() + start_[slot_[index(s, t, x, y)]];
+    const uint32_t k = slot_[index(s, t, x, y)];
+    return k == kNone ? nullptr : cum_.data() + start_[k];
   }
 
  private:
   static constexpr uint32_t kNone = ~0u;
+  static constexpr size_t kMaxEntries = size_t(1) << 25;  // 256 MiB of doubles
   size_t index(uint32_t s, uint32_t t, uint32_t x, uint32_t y) const {
     return ((size_t(s) * ns_ + t) * sx_ + x) * sy_ + y;
   }
@@ -560,9 +567,19 @@ class GapTables {
 // or more ends the scan wherever it lands, so it is returned as rem: only
 // C_1..C_{rem-1} are searched.  The first few are checked in order (a dense
 // stencil cell's gaps are short), the rest by binary search.
-uint32_t geometric_gap(Ran3 &rng, const double *cum, uint32_t rem) {
+uint32_t geometric_gap(Ran3 &rng, const double *cum, float prob, uint32_t rem) {
   const double r = rng.next();
   const uint32_t lim = rem - 1;  // the largest gap that stays inside
+  if (!cum) {  // no table: C_n accumulated as the reference does
+    const double q = (double)prob;
+    double P = q, C = P;
+    for (uint32_t n = 1; n <= lim; ++n) {
+      if (!(C < r)) return n;
+      P *= 1 - q;
+      C += P;
+    }
+    return rem;
+  }
   const uint32_t head = std::min<uint32_t>(lim, 16);
   for (uint32_t i = 0; i < head; ++i)
     if (!(cum[i] < r)) return i + 1;
@@ -707,7 +724,7 @@ void generate_column(const Params &p, const ColumnRules &rules, uint32_t col, Ra
           const uint32_t first = p.sp[ts].offset, end = first + p.sp[ts].count;
           uint32_t cur = first - 1;  // wraps like the reference's uint32 arithmetic
           const double *cum = rules.gaps.at(src, ts, x, y);
-          while ((cur += geometric_gap(rng, cum, end - cur)) < end) {
+          while ((cur += geometric_gap(rng, cum, prob, end - cur)) < end) {
             if (targets.size() + 1 >= kMaxOutDegree) {
               out.ok = false;
               out.err = "a neuron would get 7500 or more synapses (DSD__maxM)";
