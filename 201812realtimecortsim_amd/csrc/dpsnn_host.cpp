@@ -919,7 +919,7 @@ bool generate_connectome(const Params &p, Ran3 &rng, Connectome &c, std::string 
   }
   // several threads: both variants of a column (entry cache empty / full)
   // are generated ahead by a pool, at most `window` columns past the last
-  // one resolved; this thread picks, patches and appends them in order
+  // one resolved; this thread picks and patches them in column order
   struct Variants {
     ColumnOut a, b;
     int done = 0;
