@@ -71,7 +71,10 @@ typedef struct {
                                 unavailable */
   uint32_t host_threads;     /* host threads of the connectome generator and
                                 delivery layout (0 = the hardware threads,
-                                capped by OMP_NUM_THREADS where set) */
+                                capped by OMP_NUM_THREADS where set).  The
+                                generator draws each column once on 1 or 2
+                                threads and both of its entry variants on
+                                3 or more; the output is the same */
   uint32_t generator;        /* dpsnn_generator: who draws the connectome */
   uint64_t generator_seed;   /* seed of the device generator */
 } dpsnn_options;
