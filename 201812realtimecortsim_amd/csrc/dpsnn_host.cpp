@@ -29,8 +29,8 @@ namespace dpsnn {
 // first initialisation only advances the stream), :584-606 (polar
 // Box-Muller whose second value is cached across reseeds).
 // --------------------------------------------------------------------------
-double Ran3::next() {
-  constexpr int32_t kBig = 1000000000, kSeed = 161803398;
+int32_t Ran3::next_int() {
+  constexpr int32_t kSeed = 161803398;
   if (idum_ < 0 || iff_ == 0) {
     iff_ = 1;
     int32_t mj = kSeed - (idum_ < 0 ? -idum_ : idum_);
@@ -60,7 +60,7 @@ double Ran3::next() {
   if (mj < 0) mj += kBig;
   ma_[inext_] = mj;
   ++count_;
-  return mj * (1.0 / (double)kBig);
+  return mj;
 }
 
 void Ran3::seed(int32_t s) {
@@ -588,10 +588,17 @@ uint32_t geometric_gap(Ran3 &rng, const double *cum, float prob, uint32_t rem) {
 
 // getRandomDelay_UNI (DPSNN_LIFCAconnectome.cpp:299-338) for one (source,
 // target) sub-population pair.  The float bounds of the delay levels depend
-// on the pair only, so they are rounded once; a draw is then the reference's
-// product, sum and division in double, narrowed to float and rounded.
+// on the pair only, so they are rounded once.  A draw is then the reference's
+// product, sum and division in double, narrowed to float, scaled and rounded:
+// with step > 0 and range >= 0 every one of these operations is monotone
+// (IEEE rounding is), so the delay is a non-decreasing step function of
+// ran3's integer draw mj.  Its rise points are found once by bisection on
+// that same formula, and a draw looks its delay up among them.
 struct DelayRule {
   float DMin = 0, range = 0, lo = 0, step = 0;
+  uint32_t base = 0;          // the delay of mj = 0
+  std::vector<int32_t> rise;  // rise[k]: the smallest mj whose delay is base + k + 1
+  bool stepped = false;       // false: every draw evaluates the formula
   DelayRule() = default;
   DelayRule(const Params &p, uint32_t s, uint32_t t) {
     float dmin = (float)p.sp[s].DMin[t], dmax = (float)p.sp[s].DMax[t];
@@ -603,10 +610,35 @@ struct DelayRule {
     dmin = round_half_up((dmin - lo) / step) * step + lo;
     DMin = dmin;
     range = dmax - dmin;
+    const int32_t last = Ran3::kBig - 1;
+    if (!(step > 0) || !(range >= 0) || !(scaled(0) >= 0) || !(scaled(last) < 256)) return;
+    base = round_half_up(scaled(0));
+    for (uint32_t v = base + 1; v <= round_half_up(scaled(last)); ++v) {
+      int32_t a = 0, b = last;  // value(b) >= v > value(a)
+      while (b - a > 1) {
+        const int32_t m = a + (b - a) / 2;
+        (value(m) >= v ? b : a) = m;
+      }
+      rise.push_back(b);
+    }
+    stepped = true;
   }
-  uint32_t draw(Ran3 &rng) const {
-    const float level = ((rng.next() * range + DMin - lo) / step);
-    return round_half_up(level * step);
+  // the level before rounding, for the draw mj
+  double scaled(int32_t mj) const {
+    const double r = mj * (1.0 / (double)Ran3::kBig);
+    const float level = ((r * range + DMin - lo) / step);
+    return level * step;
+  }
+  uint32_t value(int32_t mj) const { return round_half_up(scaled(mj)); }
+  uint32_t draw(Ran3 &rng) const { return at(rng.next_int()); }
+  // value(mj), from the rise points where there are some
+  uint32_t at(int32_t mj) const {
+    if (!stepped) return value(mj);
+    const uint32_t n = (uint32_t)rise.size();
+    uint32_t k = (uint32_t)((uint64_t)mj * n / Ran3::kBig);  // near the answer
+    while (k > 0 && mj < rise[k - 1]) --k;
+    while (k < n && mj >= rise[k]) ++k;
+    return base + k;
   }
 };
 

@@ -60,7 +60,10 @@ bool read_params(const std::string &dir, Params &p, std::string &err, const EnvM
 // (randdev.c:116-189, :477-502, :584-606), one instance per network.
 class Ran3 {
  public:
-  double next();
+  static constexpr int32_t kBig = 1000000000;
+  // one draw: the integer mj in [0, kBig), or next() = mj / kBig
+  int32_t next_int();
+  double next() { return next_int() * (1.0 / (double)kBig); }
   void seed(int32_t s);
   double norm();
   uint64_t draws() const { return count_; }
