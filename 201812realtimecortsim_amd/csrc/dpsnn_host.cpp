@@ -1131,7 +1131,10 @@ void build_delivery(const Params &p, Connectome &c, bool with_incoming, unsigned
     uint32_t di;
     uint64_t s;
   };
-  const unsigned parts = std::max(1u, threads);
+  // one piece per thread, as long as the pieces' per-target positions
+  // (8 bytes per piece and target) stay within 1 GiB
+  const unsigned parts = (unsigned)std::max<uint64_t>(
+      1, std::min<uint64_t>(threads, (uint64_t(1) << 30) / (8 * uint64_t(std::max(N, 1u)))));
   std::vector<Cut> cut(parts + 1);
   auto bucket_start = [&](uint32_t di) { return di < D ? Cut{di, bstart[D - 1 - di]} : Cut{D, 0}; };
   cut[0] = bucket_start(0);
